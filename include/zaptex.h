@@ -1,0 +1,95 @@
+/* zaptex.h — C ABI of libzaptex, the gfx950 (MI355X) implementation of zap_tex.h's encoders and mips.
+ *
+ * Plain C types only; every function returns a status (ZAPTEX_OK = 0) and never falls back to the CPU:
+ * without a usable HIP device the call fails with ZAPTEX_E_DEVICE and zaptex_last_error() says why.
+ * Output bytes equal the reference encoder's (WatchDogStudios/zap 1.3.0, zap_tex.h) at rdo = 0.
+ *
+ * Two families of entry points:
+ *   host  — pointers are ordinary (pageable) host memory, exactly like zap_tex.h; the call is synchronous.
+ *           Each replaces the zap_tex.h function cited beside it.  Thread-safe (calls are serialised).
+ *   _dev  — pointers are device memory on the current HIP device; work is enqueued on `stream`
+ *           (a hipStream_t, NULL = the default stream) and the call returns without waiting.
+ */
+#ifndef ZAPTEX_H
+#define ZAPTEX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ZAPTEX_OK = 0,
+    ZAPTEX_E_ARG = 1,         /* invalid argument (size, format, alignment, workspace too small) */
+    ZAPTEX_E_DEVICE = 2,      /* no HIP device, or a HIP runtime / kernel launch error */
+    ZAPTEX_E_NOMEM = 3,       /* device or pinned allocation failed */
+    ZAPTEX_E_UNSUPPORTED = 4  /* a reference feature outside this library: rdo > 0 (rate-distortion mode) */
+};
+
+/* formats: zap_bc_format values (zap_tex.h:32), plus the two non-BCn encoders */
+enum { ZAPTEX_BC1 = 0, ZAPTEX_BC3 = 1, ZAPTEX_BC4 = 2, ZAPTEX_BC5 = 3, ZAPTEX_BC7 = 4, ZAPTEX_BC6H = 5, ZAPTEX_ASTC = 6 };
+
+/* options */
+enum { ZAPTEX_OPT_BC7_M1K = 1 /* mode-1 partitions fully fitted after ranking, 1..64 (zap_tex.h:750-752, default 4) */ };
+
+/* ---------------- host entry points (drop-in bodies of zap_tex.h) */
+
+/* zap_tex.h:1474 zap_bc_encode. rdo > 0 returns ZAPTEX_E_UNSUPPORTED. */
+int zaptex_bc_encode(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, void *out);
+/* zap_tex.h:1526 zap_bc_decode */
+int zaptex_bc_decode(const void *bc, int w, int h, int fmt, uint8_t *rgba, size_t stride);
+/* zap_tex.h:946 zap_bc6h_encode (stride in floats per row) */
+int zaptex_bc6h_encode(const float *rgba, int w, int h, size_t stride, void *out);
+/* zap_tex.h:963 zap_bc6h_decode (stride in uint16 per row) */
+int zaptex_bc6h_decode(const void *bc, int w, int h, uint16_t *rgba, size_t stride);
+/* zap_tex.h:1291 zap_astc_encode */
+int zaptex_astc_encode(const uint8_t *rgba, int w, int h, size_t stride, void *out);
+/* zap_tex.h:1344 zap_astc_decode */
+int zaptex_astc_decode(const void *bc, int w, int h, uint8_t *rgba, size_t stride);
+/* zap_tex.h:1442 zap_mip_next */
+int zaptex_mip_next(const uint8_t *src, int w, int h, size_t stride, int srgb, uint8_t *dst, size_t dst_stride);
+/* zap_tex.h:1460 zap_mip_next_f (strides in floats per row) */
+int zaptex_mip_next_f(const float *src, int w, int h, size_t stride, float *dst, size_t dst_stride);
+/* zap_tex.h:1570-1571 zap_bc_split / zap_bc_merge */
+int zaptex_bc_split(const void *bc, size_t n, int fmt, void *out);
+int zaptex_bc_merge(const void *planes, size_t n, int fmt, void *out);
+
+/* ---------------- device-resident entry points (no reference counterpart: the reference is CPU-only;
+ * these are what the host entry points run, exposed for callers whose data already lives in HBM) */
+
+/* One image to encode: RGBA8 pixels (RGBA float for ZAPTEX_BC6H), `stride` bytes per row; blocks written
+ * row-major to dst.  src rows must be 4-byte aligned (16 for BC6H), dst 16-byte aligned. */
+typedef struct {
+    const void *src;
+    void *dst;
+    size_t stride;
+    int w, h, fmt;
+} zaptex_image;
+
+/* workspace bytes zaptex_encode_dev needs for these images */
+size_t zaptex_workspace_size(const zaptex_image *imgs, int n);
+/* Encode n (1..16) images, any mix of formats, on `stream`.  The workspace is device memory of at least
+ * zaptex_workspace_size bytes, owned by the caller; calls sharing a workspace must not overlap. */
+int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream);
+/* zap_mip_next on device memory */
+int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream);
+/* zap_mip_next_f on device memory (strides in floats) */
+int zaptex_mip_next_f_dev(const void *src, int w, int h, size_t stride, void *dst, size_t dst_stride, void *stream);
+/* Whole chain: levels[i] receives level i+1 (tightly packed rows), i < zap_mip_levels(w, h) - 1. */
+int zaptex_mip_chain_dev(const void *base, int w, int h, size_t stride, int srgb, void *const *levels, int nlevels, void *stream);
+/* zap_bc_decode on device memory (out rows 4-byte aligned) */
+int zaptex_bc_decode_dev(const void *bc, int w, int h, int fmt, void *rgba, size_t stride, void *stream);
+
+/* ---------------- library */
+int zaptex_set_option(int key, int value);
+int zaptex_get_option(int key);
+const char *zaptex_last_error(void);  /* message of this thread's last failed call */
+int zaptex_device_count(void);
+const char *zaptex_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+"""Writes tests/golden/golden.json: SHA-256 of the reference encoder's outputs on the seeded workloads.
+
+Runs the compiled reference (oracle/_ref, built from zap_tex.h with -std=c11 -O2 -ffp-contract=off) in this
+container only; the GPU tests compare against the committed hashes and never need the reference at run time.
+Usage: python tests/golden/make_golden.py   (about a minute on 8 cores)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from oracle import loader  # noqa: E402
+from zap_amd import synth  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json")
+SMALL_SIZES = [(1, 1), (3, 2), (5, 3), (37, 21), (67, 45), (64, 64)]
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def banded_bc(lib, img, fmt, threads=8):
+    """Reference encode split into 8-block-row bands (byte-identical at rdo = 0, as the viewer does)."""
+    h, w = img.shape[:2]
+    bh = (h + 3) // 4
+    bs = loader.BLOCK_BYTES[loader.FMT[fmt]]
+    out = np.zeros(((w + 3) // 4) * bh * bs, dtype=np.uint8)
+    band = 8
+    def run(b0):
+        y0 = b0 * 4
+        sub = img[y0:min(h, y0 + band * 4)]
+        o = lib.bc_encode(sub, fmt)
+        out[b0 * ((w + 3) // 4) * bs: b0 * ((w + 3) // 4) * bs + o.size] = o
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(run, range(0, bh, band)))
+    return out
+
+
+def banded_rows(lib, img, kind, threads=8):
+    h, w = img.shape[:2]
+    bh = (h + 3) // 4
+    out = np.zeros(((w + 3) // 4) * bh * 16, dtype=np.uint8)
+    band = 8
+    def run(b0):
+        y0 = b0 * 4
+        sub = img[y0:min(h, y0 + band * 4)]
+        o = lib.bc6h_encode(sub) if kind == "bc6h" else lib.astc_encode(sub)
+        off = b0 * ((w + 3) // 4) * 16
+        out[off:off + o.size] = o
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(run, range(0, bh, band)))
+    return out
+
+
+def small_image(w, h, seed):
+    img = synth.c2_bc7(max(w, h, 4), seed)[:h, :w]
+    return np.ascontiguousarray(img)
+
+
+def main():
+    ref = loader.reference()
+    ref64 = loader.reference(m1k64=True)
+    t0 = time.time()
+    g = {"meta": {"reference": "zap_tex.h 1.3.0 sha256 " + hashlib.sha256(open("/root/reference/zap_tex.h", "rb").read()).hexdigest(),
+                  "build": "gcc 11.4 -std=c11 -O2 -ffp-contract=off", "generator": "zap_amd.synth"},
+         "inputs": {}, "outputs": {}, "small": {}}
+    # C1: BC1 (and BC3/BC4/BC5 on the same image) 1024^2
+    c1 = synth.c1_bc1(1024, 1)
+    g["inputs"]["c1_1024"] = sha(c1)
+    for f in ["bc1", "bc3", "bc4", "bc5"]:
+        g["outputs"][f"c1_1024_{f}"] = sha(banded_bc(ref, c1, f))
+    nm = synth.normal_map(1024, 1024, 11)
+    g["inputs"]["normal_1024"] = sha(nm)
+    g["outputs"]["normal_1024_bc5"] = sha(banded_bc(ref, nm, "bc5"))
+    # C2: BC7 4096^2 + 13-level box chain (linear; sRGB chain hashed too)
+    c2 = synth.c2_bc7(4096, 2)
+    g["inputs"]["c2_4096"] = sha(c2)
+    for srgb in (0, 1):
+        chain = ref.mip_chain(c2, srgb)
+        g["outputs"][f"c2_mips_srgb{srgb}"] = [sha(l) for l in chain]
+        if srgb == 0:
+            g["outputs"]["c2_bc7_chain"] = [sha(banded_bc(ref, l, "bc7")) for l in chain]
+            g["outputs"]["c2_bc7_1024_m1k64"] = sha(banded_bc(ref64, chain[2], "bc7"))
+            g["outputs"]["c2_bc1_chain"] = [sha(banded_bc(ref, l, "bc1")) for l in chain[2:]]
+    print(f"c2 done {time.time() - t0:.1f}s", flush=True)
+    # C3: BC6H 2048^2 and ASTC 2048^2
+    hdr = synth.c3_hdr(2048, 5)
+    g["inputs"]["c3_hdr_2048"] = sha(hdr)
+    g["outputs"]["c3_bc6h_2048"] = sha(banded_rows(ref, hdr, "bc6h"))
+    a = synth.c3_astc(2048, 3)
+    g["inputs"]["c3_astc_2048"] = sha(a)
+    g["outputs"]["c3_astc_2048"] = sha(banded_rows(ref, a, "astc"))
+    fl = hdr[:512, :512]
+    g["outputs"]["c3_hdr_mipf_512"] = sha(ref.mip_next_f(fl))
+    print(f"c3 done {time.time() - t0:.1f}s", flush=True)
+    # adversarial blocks through every encoder
+    blocks = synth.blocks_to_image(synth.random_blocks(8192, 7))
+    g["inputs"]["blocks_8192"] = sha(blocks)
+    for f in ["bc1", "bc3", "bc4", "bc5", "bc7"]:
+        g["outputs"][f"blocks_8192_{f}"] = sha(ref.bc_encode(blocks, f))
+    g["outputs"]["blocks_8192_bc7_m1k64"] = sha(ref64.bc_encode(blocks, "bc7"))
+    g["outputs"]["blocks_8192_astc"] = sha(ref.astc_encode(blocks))
+    # small full-byte fixtures (edge clamps, partial blocks, odd mips)
+    for (w, h) in SMALL_SIZES:
+        img = small_image(w, h, 40 + w * 7 + h)
+        key = f"{w}x{h}"
+        ent = {"input": img.tobytes().hex()}
+        for f in ["bc1", "bc3", "bc4", "bc5", "bc7"]:
+            ent[f] = ref.bc_encode(img, f).tobytes().hex()
+        ent["astc"] = ref.astc_encode(img).tobytes().hex()
+        hf = (img.astype(np.float32) / 16.0) ** 2
+        hf[..., 3] = 1.0
+        ent["hdr_input"] = hf.astype(np.float32).tobytes().hex()
+        ent["bc6h"] = ref.bc6h_encode(hf).tobytes().hex()
+        ent["mip"] = ref.mip_next(img, 0).tobytes().hex()
+        ent["mip_srgb"] = ref.mip_next(img, 1).tobytes().hex()
+        g["small"][key] = ent
+    with open(OUT, "w") as fh:
+        json.dump(g, fh, indent=1, sort_keys=True)
+    print(f"wrote {OUT} in {time.time() - t0:.1f}s")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,230 @@
+"""ctypes binding of libzaptex (include/zaptex.h), mirroring zap_tex.h's API with numpy and torch buffers.
+
+Host functions take and return numpy arrays and keep the reference's argument meaning (zap_tex.h:4-23):
+RGBA8 images are HxWx4 uint8, BC6H input is HxWx4 float32, decoded BC6H is HxWx4 uint16 half bits.
+Device functions take torch tensors that live on the GPU and enqueue on the current torch stream.
+There is no CPU fallback: a missing library or device raises ZapTexError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libzaptex.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "zaptex.h")
+
+BC1, BC3, BC4, BC5, BC7, BC6H, ASTC = range(7)
+FORMATS = {"bc1": BC1, "bc3": BC3, "bc4": BC4, "bc5": BC5, "bc7": BC7, "bc6h": BC6H, "astc": ASTC}
+OPT_BC7_M1K = 1
+STATUS = {0: "ok", 1: "bad argument", 2: "device error", 3: "out of memory", 4: "unsupported"}
+
+
+class ZapTexError(RuntimeError):
+    pass
+
+
+def fmt_id(fmt) -> int:
+    return FORMATS[fmt.lower()] if isinstance(fmt, str) else int(fmt)
+
+
+def bc_block_bytes(fmt) -> int:  # zap_tex.h:34
+    return 8 if fmt_id(fmt) in (BC1, BC4) else 16
+
+
+def bc_size(w: int, h: int, fmt) -> int:  # zap_tex.h:35-37 (BC6H / ASTC use 16-byte blocks)
+    return ((w + 3) // 4) * ((h + 3) // 4) * bc_block_bytes(fmt)
+
+
+def mip_levels(w: int, h: int) -> int:  # zap_tex.h:1435-1439
+    n = 1
+    while w > 1 or h > 1:
+        w, h = max(1, w // 2), max(1, h // 2)
+        n += 1
+    return n
+
+
+def mip_sizes(w: int, h: int) -> list[tuple[int, int]]:
+    out = [(w, h)]
+    while w > 1 or h > 1:
+        w, h = max(1, w // 2), max(1, h // 2)
+        out.append((w, h))
+    return out
+
+
+class ZapImage(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("stride", C.c_size_t),
+                ("w", C.c_int), ("h", C.c_int), ("fmt", C.c_int)]
+
+
+def declared_functions(header: str = HEADER) -> list[str]:
+    """Names of the functions include/zaptex.h declares."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s+\*?\s*(zaptex_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Library:
+    def __init__(self, path: str = LIB_PATH):
+        if not os.path.exists(path):
+            raise ZapTexError(f"{path} is not built (run __graft_entry__.build() / python zap_amd/build.py)")
+        self.path = path
+        self.lib = L = C.CDLL(path)
+        vp, i, sz, f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
+        sig = {
+            "zaptex_bc_encode": [vp, i, i, sz, i, f, vp], "zaptex_bc_decode": [vp, i, i, i, vp, sz],
+            "zaptex_bc6h_encode": [vp, i, i, sz, vp], "zaptex_bc6h_decode": [vp, i, i, vp, sz],
+            "zaptex_astc_encode": [vp, i, i, sz, vp], "zaptex_astc_decode": [vp, i, i, vp, sz],
+            "zaptex_mip_next": [vp, i, i, sz, i, vp, sz], "zaptex_mip_next_f": [vp, i, i, sz, vp, sz],
+            "zaptex_bc_split": [vp, sz, i, vp], "zaptex_bc_merge": [vp, sz, i, vp],
+            "zaptex_encode_dev": [C.POINTER(ZapImage), i, vp, sz, vp],
+            "zaptex_mip_next_dev": [vp, i, i, sz, i, vp, sz, vp], "zaptex_mip_next_f_dev": [vp, i, i, sz, vp, sz, vp],
+            "zaptex_mip_chain_dev": [vp, i, i, sz, i, C.POINTER(vp), i, vp],
+            "zaptex_bc_decode_dev": [vp, i, i, i, vp, sz, vp],
+            "zaptex_set_option": [i, i], "zaptex_get_option": [i],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
+        L.zaptex_workspace_size.restype = C.c_size_t
+        L.zaptex_last_error.restype = C.c_char_p
+        L.zaptex_version.restype = C.c_char_p
+        L.zaptex_device_count.restype = C.c_int
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self.lib.zaptex_last_error().decode(errors="replace")
+            raise ZapTexError(f"{what}: {STATUS.get(rc, rc)}: {msg}")
+
+    # ------------------------------------------------------------------ host API (zap_tex.h)
+    def bc_encode(self, img: np.ndarray, fmt, rdo: float = 0.0, w: int | None = None, h: int | None = None) -> np.ndarray:
+        f = fmt_id(fmt)
+        img = np.ascontiguousarray(img) if img.strides[-1] != 1 or img.strides[-2] != 4 else img
+        h = img.shape[0] if h is None else h
+        w = img.shape[1] if w is None else w
+        out = np.zeros(bc_size(w, h, f), dtype=np.uint8)
+        self._check(self.lib.zaptex_bc_encode(_ptr(img), w, h, img.strides[0], f, rdo, _ptr(out)), "zap_bc_encode")
+        return out
+
+    def bc_decode(self, bc: np.ndarray, w: int, h: int, fmt) -> np.ndarray:
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        self._check(self.lib.zaptex_bc_decode(_ptr(np.ascontiguousarray(bc)), w, h, fmt_id(fmt), _ptr(out), w * 4), "zap_bc_decode")
+        return out
+
+    def bc6h_encode(self, img: np.ndarray) -> np.ndarray:
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        h, w = img.shape[:2]
+        out = np.zeros(bc_size(w, h, BC7), dtype=np.uint8)
+        self._check(self.lib.zaptex_bc6h_encode(_ptr(img), w, h, img.strides[0] // 4, _ptr(out)), "zap_bc6h_encode")
+        return out
+
+    def bc6h_decode(self, bc: np.ndarray, w: int, h: int) -> np.ndarray:
+        out = np.zeros((h, w, 4), dtype=np.uint16)
+        self._check(self.lib.zaptex_bc6h_decode(_ptr(np.ascontiguousarray(bc)), w, h, _ptr(out), w * 4), "zap_bc6h_decode")
+        return out
+
+    def astc_encode(self, img: np.ndarray) -> np.ndarray:
+        img = np.ascontiguousarray(img)
+        h, w = img.shape[:2]
+        out = np.zeros(bc_size(w, h, BC7), dtype=np.uint8)
+        self._check(self.lib.zaptex_astc_encode(_ptr(img), w, h, img.strides[0], _ptr(out)), "zap_astc_encode")
+        return out
+
+    def astc_decode(self, bc: np.ndarray, w: int, h: int) -> np.ndarray:
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        self._check(self.lib.zaptex_astc_decode(_ptr(np.ascontiguousarray(bc)), w, h, _ptr(out), w * 4), "zap_astc_decode")
+        return out
+
+    def mip_next(self, img: np.ndarray, srgb: int = 0) -> np.ndarray:
+        img = np.ascontiguousarray(img)
+        h, w = img.shape[:2]
+        nw, nh = max(1, w // 2), max(1, h // 2)
+        out = np.zeros((nh, nw, 4), dtype=np.uint8)
+        self._check(self.lib.zaptex_mip_next(_ptr(img), w, h, img.strides[0], srgb, _ptr(out), nw * 4), "zap_mip_next")
+        return out
+
+    def mip_next_f(self, img: np.ndarray) -> np.ndarray:
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        h, w = img.shape[:2]
+        nw, nh = max(1, w // 2), max(1, h // 2)
+        out = np.zeros((nh, nw, 4), dtype=np.float32)
+        self._check(self.lib.zaptex_mip_next_f(_ptr(img), w, h, w * 4, _ptr(out), nw * 4), "zap_mip_next_f")
+        return out
+
+    def bc_split(self, bc: np.ndarray, fmt) -> np.ndarray:
+        bc = np.ascontiguousarray(bc)
+        out = np.zeros_like(bc)
+        self._check(self.lib.zaptex_bc_split(_ptr(bc), bc.size, fmt_id(fmt), _ptr(out)), "zap_bc_split")
+        return out
+
+    def bc_merge(self, planes: np.ndarray, fmt) -> np.ndarray:
+        planes = np.ascontiguousarray(planes)
+        out = np.zeros_like(planes)
+        self._check(self.lib.zaptex_bc_merge(_ptr(planes), planes.size, fmt_id(fmt), _ptr(out)), "zap_bc_merge")
+        return out
+
+    def set_option(self, key: int, value: int) -> None:
+        self._check(self.lib.zaptex_set_option(key, value), "zaptex_set_option")
+
+    def get_option(self, key: int) -> int:
+        return self.lib.zaptex_get_option(key)
+
+    def set_m1k(self, k: int) -> None:
+        self.set_option(OPT_BC7_M1K, k)
+
+    # ------------------------------------------------------------------ device API (torch tensors on the GPU)
+    @staticmethod
+    def _stream(stream=None) -> int:
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        return s.cuda_stream
+
+    @staticmethod
+    def images(specs) -> C.Array:
+        """specs: iterable of (src_tensor, dst_tensor, w, h, row_stride_bytes, fmt)."""
+        specs = list(specs)
+        arr = (ZapImage * len(specs))()
+        for k, (src, dst, w, h, stride, fmt) in enumerate(specs):
+            arr[k] = ZapImage(src.data_ptr(), dst.data_ptr(), stride, w, h, fmt_id(fmt))
+        return arr
+
+    def workspace_size(self, imgs) -> int:
+        return int(self.lib.zaptex_workspace_size(imgs, len(imgs)))
+
+    def encode_dev(self, imgs, workspace=None, stream=None) -> None:
+        ws_ptr = workspace.data_ptr() if workspace is not None else None
+        ws_len = workspace.numel() * workspace.element_size() if workspace is not None else 0
+        self._check(self.lib.zaptex_encode_dev(imgs, len(imgs), ws_ptr, ws_len, self._stream(stream)), "zaptex_encode_dev")
+
+    def mip_chain_dev(self, base, w: int, h: int, stride: int, levels: list, srgb: int = 0, stream=None) -> None:
+        ptrs = (C.c_void_p * len(levels))(*[t.data_ptr() for t in levels])
+        self._check(self.lib.zaptex_mip_chain_dev(base.data_ptr(), w, h, stride, srgb, ptrs, len(levels), self._stream(stream)),
+                    "zaptex_mip_chain_dev")
+
+    def bc_decode_dev(self, bc, w: int, h: int, fmt, out, stream=None) -> None:
+        self._check(self.lib.zaptex_bc_decode_dev(bc.data_ptr(), w, h, fmt_id(fmt), out.data_ptr(), w * 4, self._stream(stream)),
+                    "zaptex_bc_decode_dev")
+
+    def version(self) -> str:
+        return self.lib.zaptex_version().decode()
+
+    def device_count(self) -> int:
+        return int(self.lib.zaptex_device_count())
+
+
+_LIB: Library | None = None
+
+
+def load() -> Library:
+    global _LIB
+    if _LIB is None:
+        _LIB = Library()
+    return _LIB

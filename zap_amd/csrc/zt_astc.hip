@@ -1,0 +1,406 @@
+// zt_astc.hip — ASTC 4x4 LDR encoder and decoder for gfx950, one lane per block (zap_tex.h:977-1356).
+//
+// Per block: three 1-partition trial configurations (CEM 8 for opaque blocks, CEM 12 otherwise), each a PCA
+// start, quantisation through the configuration's 8-bit -> ISE map, a weight argmin and <= 3 least-squares
+// refinements; opaque blocks with SSE > 192 then try the 4 stored 2-partition patterns nearest to a 2-means
+// split.  Configurations are compile-time (their colour ranges are fixed by the spec rule, checked at set-up),
+// so every per-texel loop unrolls into registers.
+#include "zt_astc_setup.h"
+#include "zt_common.h"
+#include "zt_kernels.h"
+#include "zt_pca.h"
+
+namespace {
+
+__device__ zt_astc_setup g_astc;
+
+// trial configuration K: CEM, weight bits, partitions, and its colour range (trits, quints, bits)
+template <int K> struct AC;
+template <> struct AC<0> { static constexpr int CEM = 8, WB = 4, PARTS = 1, TRIT = 1, QUINT = 0, BITS = 6; };
+template <> struct AC<1> { static constexpr int CEM = 8, WB = 3, PARTS = 1, TRIT = 0, QUINT = 0, BITS = 8; };
+template <> struct AC<2> { static constexpr int CEM = 8, WB = 5, PARTS = 1, TRIT = 0, QUINT = 0, BITS = 5; };
+template <> struct AC<3> { static constexpr int CEM = 12, WB = 4, PARTS = 1, TRIT = 1, QUINT = 0, BITS = 4; };
+template <> struct AC<4> { static constexpr int CEM = 12, WB = 3, PARTS = 1, TRIT = 1, QUINT = 0, BITS = 6; };
+template <> struct AC<5> { static constexpr int CEM = 12, WB = 2, PARTS = 1, TRIT = 0, QUINT = 0, BITS = 8; };
+template <> struct AC<6> { static constexpr int CEM = 8, WB = 3, PARTS = 2, TRIT = 0, QUINT = 0, BITS = 4; };
+template <> struct AC<7> { static constexpr int CEM = 8, WB = 2, PARTS = 2, TRIT = 0, QUINT = 1, BITS = 3; };
+
+ZT_DEV int lerp(int a, int b, int w) { return (((a * 257) * (64 - w) + (b * 257) * w + 32) >> 6) >> 8; }
+
+// decoded endpoints of one partition (zap_tex.h:1125-1129); packed RGBA8 in e0/e1
+template <int CEM> ZT_DEV void endpoints(const int *v, int e0[4], int e1[4]) {
+    int a0 = CEM == 12 ? v[6] : 255, a1 = CEM == 12 ? v[7] : 255;
+    if (v[1] + v[3] + v[5] >= v[0] + v[2] + v[4]) {
+        e0[0] = v[0]; e0[1] = v[2]; e0[2] = v[4]; e0[3] = a0; e1[0] = v[1]; e1[1] = v[3]; e1[2] = v[5]; e1[3] = a1;
+    } else {
+        e0[0] = (v[1] + v[5]) >> 1; e0[1] = (v[3] + v[5]) >> 1; e0[2] = v[5]; e0[3] = a1;
+        e1[0] = (v[0] + v[4]) >> 1; e1[1] = (v[2] + v[4]) >> 1; e1[2] = v[4]; e1[3] = a0;
+    }
+}
+
+// float endpoints -> ISE values and their unquantised values (zap_tex.h:1183-1194)
+template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int *ise, int *uv) {
+    constexpr int NV = AC<K>::CEM == 8 ? 6 : 8;
+    float s0 = lo[0] + lo[1] + lo[2], s1 = hi[0] + hi[1] + hi[2];
+    const bool keep = s1 >= s0;
+#pragma unroll
+    for (int k = 0; k < NV / 2; k++) {
+        float a = keep ? lo[k] : hi[k], b = keep ? hi[k] : lo[k];
+        int x0 = clampi(cvt_i32(a + 0.5f), 0, 255), x1 = clampi(cvt_i32(b + 0.5f), 0, 255);
+        ise[2 * k] = g_astc.q[K][x0]; ise[2 * k + 1] = g_astc.q[K][x1];
+        uv[2 * k] = g_astc.unq[K][ise[2 * k]]; uv[2 * k + 1] = g_astc.unq[K][ise[2 * k + 1]];
+    }
+    if (uv[1] + uv[3] + uv[5] < uv[0] + uv[2] + uv[4]) {
+#pragma unroll
+        for (int k = 0; k < NV / 2; k++) {
+            int t = ise[2 * k]; ise[2 * k] = ise[2 * k + 1]; ise[2 * k + 1] = t;
+            t = uv[2 * k]; uv[2 * k] = uv[2 * k + 1]; uv[2 * k + 1] = t;
+        }
+    }
+}
+
+// weights for fixed endpoints (zap_tex.h:1197-1210); pm: bit i = texel i in partition 1
+template <int K> ZT_DEV int fit_weights(const uint32_t px[16], const int *uv, uint32_t pm, int w[16]) {
+    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, L = 1 << WB, NC = CEM == 12 ? 4 : 3, NV = CEM == 8 ? 6 : 8;
+    constexpr uint32_t CM = NC == 4 ? 0xFFFFFFFFu : 0x00FFFFFFu;
+    uint32_t pal[P][L];
+    int Kk[P][L];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        int e0[4], e1[4];
+        endpoints<CEM>(uv + k * NV, e0, e1);
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+            const int wt = zt_weight_unquant(j, WB);
+            uint32_t v = 0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) v |= (uint32_t)lerp(e0[c], e1[c], wt) << (8 * c);
+            pal[k][j] = v & CM;
+            Kk[k][j] = 15 - j - 16 * (int)dot4(pal[k][j], pal[k][j]);
+        }
+    }
+    int err = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t p = px[i] & CM;
+        const bool second = P == 2 && ((pm >> i) & 1);
+        int best = -0x7FFFFFFF;
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+            uint32_t pj = (P == 2 && second) ? pal[P - 1][j] : pal[0][j];
+            int kj = (P == 2 && second) ? Kk[P - 1][j] : Kk[0][j];
+            best = max(best, (int)(dot4(p, pj) << 5) + kj);
+        }
+        w[i] = 15 - (best & 15);
+        err += (int)dot4(p, p) - (best >> 4);
+    }
+    return err;
+}
+
+// one trial configuration (zap_tex.h:1212-1244)
+template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12], int w[16]) {
+    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NC = CEM == 12 ? 4 : 3, NV = CEM == 8 ? 6 : 8;
+    int uv[12];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        float lo[4], hi[4];
+        if (P == 1) pca<NC, true>(px, 0xFFFFu, 16, lo, hi);
+        else {
+            uint32_t m = k ? pm : (~pm & 0xFFFFu);
+            int cnt = __popc(m);
+            if (!cnt) { m = 1; cnt = 1; }
+            pca<NC, false>(px, m, cnt, lo, hi);
+        }
+        quantise<K>(lo, hi, ise + k * NV, uv + k * NV);
+    }
+    int err = fit_weights<K>(px, uv, pm, w);
+    for (int it = 0; it < 3 && err > 0; it++) {
+        int ti[12], tuv[12], tw[16];
+#pragma unroll
+        for (int v = 0; v < NV * P; v++) { ti[v] = ise[v]; tuv[v] = uv[v]; }
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+            const uint32_t m = P == 1 ? 0xFFFFu : k ? pm : (~pm & 0xFFFFu);
+            float aa = 0.0f, ab = 0.0f, bb = 0.0f, xa[NC], xb[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) { xa[c] = 0.0f; xb[c] = 0.0f; }
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                if (!((m >> i) & 1)) continue;
+                float f = (float)zt_weight_unquant(w[i], WB) / 64.0f, g = 1 - f;
+                aa += g * g; ab += g * f; bb += f * f;
+#pragma unroll
+                for (int c = 0; c < NC; c++) { xa[c] += g * chanf(px[i], c); xb[c] += f * chanf(px[i], c); }
+            }
+            float det = aa * bb - ab * ab;
+            if (det < 1e-6f && det > -1e-6f) continue;
+            float nlo[4], nhi[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (c < NC) { nlo[c] = (xa[c] * bb - xb[c] * ab) / det; nhi[c] = (xb[c] * aa - xa[c] * ab) / det; }
+                else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
+            }
+            quantise<K>(nlo, nhi, ti + k * NV, tuv + k * NV);
+        }
+        int e = fit_weights<K>(px, tuv, pm, tw);
+        if (e >= err) break;
+        err = e;
+#pragma unroll
+        for (int v = 0; v < NV * P; v++) { ise[v] = ti[v]; uv[v] = tuv[v]; }
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = tw[i];
+    }
+    return err;
+}
+
+// integer-sequence encode of N values (zap_tex.h:1080-1095)
+template <int K, int N> ZT_DEV void ise_put(Bits128 &b, int pos, const int *v) {
+    constexpr int TR = AC<K>::TRIT, QU = AC<K>::QUINT, BITS = AC<K>::BITS;
+    constexpr int GS = TR ? 5 : QU ? 3 : 1, BASE = TR ? 3 : 5;
+    const int xb[5] = { TR ? 2 : 3, 2, TR ? 1 : 2, 2, 1 };
+#pragma unroll
+    for (int g = 0; g < N; g += GS) {
+        const int cnt = N - g < GS ? N - g : GS;
+        int X = 0, xp = 0;
+        if (GS > 1) {
+            int id = 0;
+#pragma unroll
+            for (int k = GS - 1; k >= 0; k--) id = id * BASE + (k < cnt ? v[g + k] >> BITS : 0);
+            X = TR ? g_astc.tenc[id] : g_astc.qenc[id];
+        }
+#pragma unroll
+        for (int k = 0; k < cnt; k++) {
+            if (BITS) { b.put(pos, (uint64_t)(v[g + k] & ((1 << BITS) - 1)), BITS); pos += BITS; }
+            if (GS > 1) { b.put(pos, (uint64_t)((X >> xp) & ((1 << xb[k]) - 1)), xb[k]); pos += xb[k]; xp += xb[k]; }
+        }
+    }
+}
+
+template <int K> ZT_DEV void pack(int seed, const int *ise, const int w[16], uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:1246-1260
+    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NV = CEM == 8 ? 6 : 8;
+    constexpr int R = WB == 2 ? 4 : WB == 3 ? 7 : WB == 4 ? 4 : 7, H = WB >= 4 ? 1 : 0;
+    constexpr uint32_t mode = (R >> 1 & 1) | (R >> 2 & 1) << 1 | (R & 1) << 4 | 2 << 5 | H << 9;
+    Bits128 b{ mode | (uint64_t)(P - 1) << 11, 0 };
+    int pos;
+    if (P == 2) { b.put(13, (uint64_t)seed, 10); b.put(23, (uint64_t)CEM << 2, 6); pos = 29; }
+    else { b.put(13, (uint64_t)CEM, 4); pos = 17; }
+    ise_put<K, NV * P>(b, pos, ise);
+    Bits128 wb{ 0, 0 };
+#pragma unroll
+    for (int i = 0; i < 16; i++) wb.put(i * WB, (uint64_t)w[i], WB);
+    // weight bit i lands at block bit 127 - i
+    olo = b.lo | __brevll(wb.hi);
+    ohi = b.hi | __brevll(wb.lo);
+}
+
+ZT_DEV uint32_t two_means(const uint32_t px[16]) {  // zap_tex.h:1263-1283
+    int lo = 0, hi = 0, slo = chan(px[0], 0) + chan(px[0], 1) + chan(px[0], 2), shi = slo;
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+        int s = chan(px[i], 0) + chan(px[i], 1) + chan(px[i], 2);
+        if (s < slo) { lo = i; slo = s; }
+        if (s > shi) { hi = i; shi = s; }
+    }
+    float c[2][3];
+    uint32_t plo = 0, phi = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) { if (i == lo) plo = px[i]; if (i == hi) phi = px[i]; }
+#pragma unroll
+    for (int k = 0; k < 3; k++) { c[0][k] = chanf(plo, k); c[1][k] = chanf(phi, k); }
+    uint32_t m = 0;
+    for (int it = 0; it < 4; it++) {
+        float sum[2][3] = { { 0.0f, 0.0f, 0.0f }, { 0.0f, 0.0f, 0.0f } };
+        int n[2] = { 0, 0 };
+        m = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            float d0 = 0.0f, d1 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) { float a = chanf(px[i], k) - c[0][k], b = chanf(px[i], k) - c[1][k]; d0 += a * a; d1 += b * b; }
+            const bool s = d1 < d0;
+            m |= (uint32_t)s << i;
+            if (s) { n[1]++; } else { n[0]++; }
+#pragma unroll
+            for (int k = 0; k < 3; k++) { if (s) sum[1][k] += chanf(px[i], k); else sum[0][k] += chanf(px[i], k); }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; s++) if (n[s]) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[s][k] = sum[s][k] / (float)n[s];
+        }
+    }
+    return m;
+}
+
+template <int K> ZT_DEV void try_cfg(const uint32_t px[16], uint32_t pm, int seed, int &best, int &bc, int &bseed, int bi[12], int bw[16]) {
+    int ise[12], w[16];
+    int e = trial<K>(px, pm, ise, w);
+    if (e < best) {
+        best = e; bc = K; bseed = seed;
+#pragma unroll
+        for (int v = 0; v < 12; v++) bi[v] = ise[v];
+#pragma unroll
+        for (int i = 0; i < 16; i++) bw[i] = w[i];
+    }
+}
+
+ZT_DEV void encode_block(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {  // zap_tex.h:1318-1340
+    uint32_t amin = 255;
+#pragma unroll
+    for (int i = 0; i < 16; i++) amin = min(amin, px[i] >> 24);
+    const bool opaque = amin == 255;
+    int best = 1 << 30, bc = 0, bseed = 0, bi[12], bw[16];
+#pragma unroll
+    for (int v = 0; v < 12; v++) bi[v] = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) bw[i] = 0;
+    if (opaque) {
+        try_cfg<0>(px, 0, 0, best, bc, bseed, bi, bw);
+        try_cfg<1>(px, 0, 0, best, bc, bseed, bi, bw);
+        try_cfg<2>(px, 0, 0, best, bc, bseed, bi, bw);
+    } else {
+        try_cfg<3>(px, 0, 0, best, bc, bseed, bi, bw);
+        try_cfg<4>(px, 0, 0, best, bc, bseed, bi, bw);
+        try_cfg<5>(px, 0, 0, best, bc, bseed, bi, bw);
+    }
+    if (opaque && best > 16 * 3 * 4) {
+        uint32_t km = two_means(px);
+        int cand[4] = { -1, -1, -1, -1 }, cd[4] = { 99, 99, 99, 99 };
+        const int np = g_astc.np;
+        for (int pi = 0; pi < np && cd[3]; pi++) {  // stops once 4 exact matches are found
+            int dd = __popc((uint32_t)g_astc.pmask[pi] ^ km), di = 16 - dd, dist = dd < di ? dd : di;
+            if (dist < cd[3]) {  // insertion into the sorted 4-list (strict <: earlier patterns stay first)
+                if (dist < cd[0]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = cd[0]; cand[1] = cand[0]; cd[0] = dist; cand[0] = pi; }
+                else if (dist < cd[1]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = dist; cand[1] = pi; }
+                else if (dist < cd[2]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = dist; cand[2] = pi; }
+                else { cd[3] = dist; cand[3] = pi; }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (cand[j] < 0) break;
+            uint32_t pm = g_astc.pmask[cand[j]];
+            int seed = g_astc.pseed[cand[j]];
+            try_cfg<6>(px, pm, seed, best, bc, bseed, bi, bw);
+            try_cfg<7>(px, pm, seed, best, bc, bseed, bi, bw);
+        }
+    }
+    switch (bc) {
+    case 0: pack<0>(bseed, bi, bw, lo, hi); break;
+    case 1: pack<1>(bseed, bi, bw, lo, hi); break;
+    case 2: pack<2>(bseed, bi, bw, lo, hi); break;
+    case 3: pack<3>(bseed, bi, bw, lo, hi); break;
+    case 4: pack<4>(bseed, bi, bw, lo, hi); break;
+    case 5: pack<5>(bseed, bi, bw, lo, hi); break;
+    case 6: pack<6>(bseed, bi, bw, lo, hi); break;
+    default: pack<7>(bseed, bi, bw, lo, hi); break;
+    }
+}
+
+// ---- decoder (zap_tex.h:1135-1168): the encoder's subset plus void-extent blocks; anything else is magenta
+ZT_DEV void decode_block(const uint8_t *blk, uint32_t out[16]) {
+    BitReader R = reader_at(blk);
+    const int mode = R.take(11);
+    for (int i = 0; i < 16; i++) out[i] = 0xFFFF00FFu;
+    if ((mode & 0x1FF) == 0x1FC) {
+        uint32_t v = 0;
+        for (int c = 0; c < 4; c++) v |= (uint32_t)((int)get_bits(R.lo, R.hi, 64 + 16 * c, 16) >> 8) << (8 * c);
+        for (int i = 0; i < 16; i++) out[i] = v;
+        return;
+    }
+    const int Rr = (mode & 1) << 1 | (mode >> 1 & 1) << 2 | (mode >> 4 & 1), H = mode >> 9 & 1;
+    if (!(mode & 3) || (mode >> 2 & 3) || (mode >> 5 & 3) != 2 || (mode >> 7 & 3) || (mode >> 10 & 1)) return;
+    const int WL0[8] = { 0, 0, 2, 3, 4, 5, 6, 8 }, WL1[8] = { 0, 0, 10, 12, 16, 20, 24, 32 };
+    const int levels = H ? WL1[Rr] : WL0[Rr];
+    const int wb = levels == 2 ? 1 : levels == 4 ? 2 : levels == 8 ? 3 : levels == 16 ? 4 : levels == 32 ? 5 : 0;
+    const int parts = R.take(2) + 1;
+    int seed = 0, cem;
+    if (!wb || parts > 2) return;
+    if (parts == 2) {
+        seed = R.take(10);
+        int cf = R.take(6);
+        if (cf & 3) return;
+        cem = cf >> 2;
+    } else cem = R.take(4);
+    const int nv = cem == 8 ? 6 : cem == 12 ? 8 : 0;
+    if (!nv) return;
+    const int ri = zt_colour_range(nv * parts, 128 - R.pos - 16 * wb);
+    if (ri < 0) return;
+    const zt_ise_range r = zt_ise(ri);
+    int v[16];
+    const int gs = r.trit ? 5 : r.quint ? 3 : 1, n = nv * parts;
+    for (int g = 0; g < n; g += gs) {
+        int cnt = n - g < gs ? n - g : gs, m[5], X = 0, xp = 0, t[5];
+        for (int k = 0; k < cnt; k++) {
+            m[k] = R.take(r.bits);
+            if (gs > 1) {
+                int xb = r.trit ? (k == 0 || k == 1 || k == 3 ? 2 : 1) : (k == 0 ? 3 : 2);
+                X |= R.take(xb) << xp; xp += xb;
+            }
+        }
+        if (r.trit) zt_trits(X, t);
+        else if (r.quint) zt_quints(X, t);
+        for (int k = 0; k < cnt; k++) v[g + k] = gs > 1 ? t[k] << r.bits | m[k] : m[k];
+    }
+    for (int i = 0; i < n; i++) v[i] = zt_colour_unquant(r, v[i]);
+    int e[2][2][4];
+    for (int k = 0; k < parts; k++) {
+        const int *q = v + k * nv;
+        int a0 = cem == 12 ? q[6] : 255, a1 = cem == 12 ? q[7] : 255;
+        if (q[1] + q[3] + q[5] >= q[0] + q[2] + q[4]) {
+            e[k][0][0] = q[0]; e[k][0][1] = q[2]; e[k][0][2] = q[4]; e[k][0][3] = a0;
+            e[k][1][0] = q[1]; e[k][1][1] = q[3]; e[k][1][2] = q[5]; e[k][1][3] = a1;
+        } else {
+            e[k][0][0] = (q[1] + q[5]) >> 1; e[k][0][1] = (q[3] + q[5]) >> 1; e[k][0][2] = q[5]; e[k][0][3] = a1;
+            e[k][1][0] = (q[0] + q[4]) >> 1; e[k][1][1] = (q[2] + q[4]) >> 1; e[k][1][2] = q[4]; e[k][1][3] = a0;
+        }
+    }
+    for (int i = 0; i < 16; i++) {
+        int w = 0, k = parts == 2 ? zt_astc_partition(seed, i & 3, i >> 2, 2) : 0;
+        for (int j = 0; j < wb; j++) { int bit = 127 - (i * wb + j); w |= (int)get_bits(R.lo, R.hi, bit, 1) << j; }
+        w = zt_weight_unquant(w, wb);
+        uint32_t o = 0;
+        for (int c = 0; c < 4; c++) o |= (uint32_t)lerp(e[k][0][c], e[k][1][c], w) << (8 * c);
+        out[i] = o;
+    }
+}
+
+}  // namespace
+
+hipError_t zt_astc_upload_tables(const void *setup, size_t bytes) {
+    const zt_astc_setup *s = (const zt_astc_setup *)setup;
+    // the kernels' compile-time colour ranges must be the ones the spec rule picks
+    const int tr[8] = { AC<0>::TRIT, AC<1>::TRIT, AC<2>::TRIT, AC<3>::TRIT, AC<4>::TRIT, AC<5>::TRIT, AC<6>::TRIT, AC<7>::TRIT };
+    const int qu[8] = { AC<0>::QUINT, AC<1>::QUINT, AC<2>::QUINT, AC<3>::QUINT, AC<4>::QUINT, AC<5>::QUINT, AC<6>::QUINT, AC<7>::QUINT };
+    const int bi[8] = { AC<0>::BITS, AC<1>::BITS, AC<2>::BITS, AC<3>::BITS, AC<4>::BITS, AC<5>::BITS, AC<6>::BITS, AC<7>::BITS };
+    for (int k = 0; k < 8; k++)
+        if (s->cfg[k].trit != tr[k] || s->cfg[k].quint != qu[k] || s->cfg[k].bits != bi[k]) return hipErrorInvalidValue;
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_astc), setup, bytes);
+}
+
+extern "C" __global__ void __launch_bounds__(256) zt_astc_encode(zt_jobs jobs, uint32_t total) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        uint32_t b = g - J.first;
+        int bx = (int)(b % (uint32_t)J.bw), by = (int)(b / (uint32_t)J.bw);
+        uint32_t px[16];
+        gather_block(J.src, J.w, J.h, J.stride, bx, by, px);
+        uint64_t lo, hi;
+        encode_block(px, lo, hi);
+        store16(J.dst + (uint64_t)b * 16, lo, hi);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) zt_astc_decode(const uint8_t *bc, int w, int h, uint8_t *rgba, uint64_t stride) {
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= bw * bh) return;
+    const int bx = g % bw, by = g / bw;
+    uint32_t px[16];
+    decode_block(bc + (uint64_t)g * 16, px);
+    for (int i = 0; i < 16; i++) {
+        int x = bx * 4 + (i & 3), y = by * 4 + (i >> 2);
+        if (x < w && y < h) *(uint32_t *)(rgba + (uint64_t)y * stride + (uint64_t)x * 4) = px[i];
+    }
+}

@@ -1,0 +1,136 @@
+// zt_common.h — device helpers shared by the gfx950 texture kernels.
+//
+// Exactness rules every kernel follows (the output bytes must equal the reference encoder's):
+//   * all TUs are compiled with -ffp-contract=off and HIP's correctly rounded fp32 division, so each
+//     float expression rounds exactly as the reference's C does (zap_tex.h is pinned contraction-off);
+//   * float accumulations run lane-serially in the reference's pixel order; integer SSE sums and
+//     argmins are order-free, with ties going to the lowest index (the reference's strict `<`);
+//   * (int)float casts of unbounded values use x86 cvttss2si semantics (cvt_i32), not the AMDGPU
+//     saturating convert.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ZT_DEV static __device__ __forceinline__
+
+// One unit of encode work: a block-compressed image (or one mip level of one).  Blocks of all jobs
+// are numbered consecutively; job j owns global blocks [first, first + bw*bh).
+struct zt_job {
+    const uint8_t *src;   // RGBA8 (or RGBA float for BC6H) rows
+    uint8_t *dst;         // blocks, row-major by block
+    uint64_t stride;      // bytes per source row
+    uint32_t first;       // global index of this job's first block
+    int32_t w, h, bw;     // pixels, pixels, blocks per row
+    int32_t fmt;          // zap_bc_format (0 BC1, 1 BC3, 2 BC4, 3 BC5, 4 BC7); 5 BC6H, 6 ASTC
+    int32_t pad;
+};
+
+// Job tables travel by value in the kernel arguments (up to ZT_INLINE_JOBS) or as a device array (`ext`).
+#define ZT_INLINE_JOBS 16
+struct zt_jobs {
+    const zt_job *ext;    // device table, or null to use `inl`
+    int32_t n, pad;
+    zt_job inl[ZT_INLINE_JOBS];
+};
+ZT_DEV zt_job job_at(const zt_jobs &J, int i) { return J.ext ? J.ext[i] : J.inl[i]; }
+ZT_DEV uint32_t job_first(const zt_jobs &J, int i) { return J.ext ? J.ext[i].first : J.inl[i].first; }
+
+// job owning global block g (jobs sorted by `first`); binary search
+ZT_DEV int zt_find_job(const zt_jobs &J, uint32_t g) {
+    int lo = 0, hi = J.n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (job_first(J, mid) <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// x86 (int)f: out-of-range or NaN gives INT_MIN (zap_tex.h:46, :842, :1187 cast unclamped values)
+ZT_DEV int cvt_i32(float f) {
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : (int)0x80000000u;
+}
+
+ZT_DEV int chan(uint32_t p, int c) { return (int)((p >> (8 * c)) & 255u); }
+ZT_DEV float chanf(uint32_t p, int c) { return (float)chan(p, c); }
+ZT_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+ZT_DEV uint32_t dot4(uint32_t a, uint32_t b) { return __builtin_amdgcn_udot4(a, b, 0u, false); }
+
+// 16 pixels of block (bx, by), edge-clamped like zap_tex.h:1482-1488, packed RGBA8 (R in the low byte)
+ZT_DEV void gather_block(const uint8_t *src, int w, int h, uint64_t stride, int bx, int by, uint32_t px[16]) {
+    int x0 = bx * 4;
+    bool full_x = x0 + 3 < w;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int y = by * 4 + r;
+        if (y >= h) y = h - 1;
+        const uint8_t *row = src + (uint64_t)y * stride;
+        if (full_x && (((uintptr_t)(row + x0 * 4)) & 15) == 0) {
+            uint4 v = *(const uint4 *)(row + x0 * 4);
+            px[r * 4 + 0] = v.x; px[r * 4 + 1] = v.y; px[r * 4 + 2] = v.z; px[r * 4 + 3] = v.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                int x = x0 + c;
+                if (x >= w) x = w - 1;
+                const uint8_t *p = row + x * 4;
+                px[r * 4 + c] = (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+            }
+        }
+    }
+}
+
+// 128-bit little-endian bit writer whose positions fold to constants when fully unrolled
+struct Bits128 {
+    uint64_t lo, hi;
+    __device__ __forceinline__ void put(int pos, uint64_t v, int n) {  // v < 2^n, n <= 32
+        if (pos >= 64) hi |= v << (pos - 64);
+        else if (pos + n <= 64) lo |= v << pos;
+        else { lo |= v << pos; hi |= v >> (64 - pos); }
+    }
+};
+ZT_DEV uint64_t get_bits(uint64_t lo, uint64_t hi, int pos, int n) {
+    uint64_t v;
+    if (pos >= 64) v = hi >> (pos - 64);
+    else if (pos + n <= 64) v = lo >> pos;
+    else v = (lo >> pos) | (hi << (64 - pos));
+    return n >= 64 ? v : v & ((1ull << n) - 1);
+}
+ZT_DEV void store16(uint8_t *dst, uint64_t lo, uint64_t hi) {
+    uint4 v;
+    v.x = (uint32_t)lo; v.y = (uint32_t)(lo >> 32); v.z = (uint32_t)hi; v.w = (uint32_t)(hi >> 32);
+    *(uint4 *)dst = v;
+}
+ZT_DEV void store8(uint8_t *dst, uint64_t v) {
+    uint2 t; t.x = (uint32_t)v; t.y = (uint32_t)(v >> 32);
+    *(uint2 *)dst = t;
+}
+
+// wave-level stream compaction: returns this lane's slot in the list (or -1), one atomic per wave
+ZT_DEV int wave_append(bool pred, unsigned *counter) {
+    uint64_t mask = __ballot(pred);
+    if (!mask) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!pred) return -1;
+    uint64_t below = lane ? (mask & ((1ull << lane) - 1)) : 0ull;
+    return (int)(base + __popcll(below));
+}
+
+// LSB-first reader over a 128-bit block (zap_tex.h:367-371)
+struct BitReader {
+    uint64_t lo, hi;
+    int pos;
+    __device__ __forceinline__ int take(int n) {
+        if (n == 0) return 0;
+        int v = (int)get_bits(lo, hi, pos, n);
+        pos += n;
+        return v;
+    }
+};
+ZT_DEV BitReader reader_at(const uint8_t *b) {
+    uint4 raw = *(const uint4 *)b;
+    return BitReader{ (uint64_t)raw.x | (uint64_t)raw.y << 32, (uint64_t)raw.z | (uint64_t)raw.w << 32, 0 };
+}
