@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""BC7 encode throughput on one MI355X: the 4096x4096 RGBA8 texture plus its 13-level box mip chain.
+
+One step = generate the 12 mip levels from the resident 4096^2 base (zap_mip_next, linear) and BC7-encode all
+13 levels (zap_bc_encode's search: mode 6, then mode 5 and mode 1 where the reference takes them), inputs
+already in HBM.  value = 22,369,621 source pixels per step / step time, in Mpix/s.  The step's output is
+checked against the reference's golden hashes before timing.
+
+Also reported, measured inside this run:
+  roofline      the dominant kernel (BC7 mode-6 stage) against the VALU lane-op peak: algorithmic ops per
+                launch (tests/golden/ops.json, counted by the oracle on this exact input) / its average
+                duration from HIP events on its launch stream; the mip chain's HBM GB/s is given beside it
+  cpu_baseline  the reference encoder (oracle/_ref, built from zap_tex.h) on this host's cores, banded into
+                8-block rows like the reference viewer, over the same 13-level chain
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz: one scalar op per lane per clock
+PEAK_HBM_GBS = 8000.0
+MIP_CHAIN_BYTES = 89_478_484                 # read the 4096^2 base once + write every generated level once
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(levels_np, threads: int, repeats: int = 3) -> dict:
+    """Reference zap_tex.h encoder (oracle/_ref; the oracle restatement if that is absent) on host cores."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import loader
+    kind = "reference" if loader.reference_available() else "port"
+    lib = loader.reference() if kind == "reference" else loader.oracle()
+    bands = []
+    for lvl in levels_np:
+        h = lvl.shape[0]
+        for y0 in range(0, h, 32):
+            bands.append(lvl[y0:min(h, y0 + 32)])
+    npx = sum(l.shape[0] * l.shape[1] for l in levels_np)
+    times = []
+    with ThreadPoolExecutor(threads) as ex:
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            list(ex.map(lambda b: lib.bc_encode(b, "bc7"), bands))
+            times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    one = levels_np[2]  # 1024^2, single thread
+    t1 = time.perf_counter()
+    lib.bc_encode(one, "bc7")
+    t1 = time.perf_counter() - t1
+    return {"value": round(npx / t / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": kind,
+            "sample": f"BC7 of the full 13-level 4096^2 chain ({npx} px), 8-block-row bands on {threads} threads, median of {repeats}",
+            "single_thread_mpix_s": round(one.shape[0] * one.shape[1] / t1 / 1e6, 3),
+            "cpu": _cpu_model()}
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py measures one GPU: this build runs the hot path on a single MI355X (no multi-GPU path); use --gpus 1")
+        return 2
+
+    import numpy as np
+    import torch
+
+    import zap_amd
+    from zap_amd import binding, synth
+
+    lib = zap_amd.load()
+    torch.cuda.set_device(0)
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    ops = json.load(open(os.path.join(ROOT, "tests", "golden", "ops.json")))["c2_bc7_chain"]
+
+    log("generating the 4096^2 texture ...")
+    c2 = synth.c2_bc7(4096, 2)
+    W = H = 4096
+    sizes = binding.mip_sizes(W, H)
+    npx = sum(w * h for w, h in sizes)
+    base = torch.from_numpy(c2).cuda()
+    levels = [torch.empty((h, w, 4), dtype=torch.uint8, device="cuda") for (w, h) in sizes[1:]]
+    src = [base] + levels
+    outs = [torch.empty(binding.bc_size(w, h, "bc7"), dtype=torch.uint8, device="cuda") for (w, h) in sizes]
+    imgs = lib.images((src[i], outs[i], w, h, w * 4, "bc7") for i, (w, h) in enumerate(sizes))
+    ws = torch.empty(lib.workspace_size(imgs), dtype=torch.uint8, device="cuda")
+
+    def step():
+        lib.mip_chain_dev(base, W, H, W * 4, levels, srgb=0)
+        lib.encode_dev(imgs, ws)
+
+    step()
+    torch.cuda.synchronize()
+    got = [hashlib.sha256(o.cpu().numpy().tobytes()).hexdigest() for o in outs]
+    exact = got == golden["outputs"]["c2_bc7_chain"]
+    log(f"output byte-identical to the reference on all 13 levels: {exact}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    value = npx / (ms / 1e3) / 1e6
+    log(f"{args.steps} steps: {ms:.3f} ms/step (wall {wall / args.steps * 1e3:.3f}) -> {value:.1f} Mpix/s")
+
+    # per-kernel durations: HIP events recorded by the library on the launch stream
+    lib.profile(True)
+    lib.profile_reset()
+    for _ in range(args.profile_steps):
+        step()
+    torch.cuda.synchronize()
+    kern = {}
+    for k in ["zt_mip_level", "zt_zero_u32", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_stage1"]:
+        t, n = lib.profile_get(k)
+        kern[k] = {"launches_per_step": n // args.profile_steps, "ms_per_launch": round(t / max(1, n), 5),
+                   "ms_per_step": round(t / args.profile_steps, 5)}
+    lib.profile(False)
+    s6 = kern["zt_bc7_stage6"]["ms_per_launch"]
+    achieved = ops["ops_stage6"] / (s6 / 1e3) / 1e12
+    mip_ms = kern["zt_mip_level"]["ms_per_step"]
+    mip_gbs = MIP_CHAIN_BYTES / (mip_ms / 1e3) / 1e9
+    stage_tops = {k: round(ops[f"ops_{k[7:]}"] / (kern[k]["ms_per_launch"] / 1e3) / 1e12, 3)
+                  for k in ["zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_stage1"] if kern[k]["ms_per_launch"] > 0}
+    log(json.dumps(kern))
+
+    cpu = None if args.no_cpu else cpu_baseline([c2] + [l.cpu().numpy() for l in levels], args.cpu_threads)
+    line = {
+        "metric": "BC7 Mpix/s on 1 MI355X (plain search, 4096^2 + 13-level mip chain)",
+        "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/int32/f32 (bit-exact)", "data": "synthetic (seeded value noise + rectangles + independent alpha, 25% opaque blocks)",
+        "config": {"workload": "BC7 plain search of a 4096x4096 RGBA8 texture and its 13-level box mip chain "
+                               "(mip generation + encode per step; 1,398,103 blocks, 22,369,621 px)",
+                   "global_batch": 1, "seq_len": None, "parallelism": "single GPU", "bc7_m1k": lib.get_option(1),
+                   "byte_exact_vs_reference": exact},
+        "roofline": {"bound": "valu", "kernel": "zt_bc7_stage6", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 2),
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": None,
+                     "ops_per_launch": ops["ops_stage6"], "ms_per_launch": s6, "stage_tops": stage_tops,
+                     "mips": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                              "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "bytes_per_chain": MIP_CHAIN_BYTES}},
+        "kernels": kern,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if exact else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -32,7 +32,10 @@ enum {
 enum { ZAPTEX_BC1 = 0, ZAPTEX_BC3 = 1, ZAPTEX_BC4 = 2, ZAPTEX_BC5 = 3, ZAPTEX_BC7 = 4, ZAPTEX_BC6H = 5, ZAPTEX_ASTC = 6 };
 
 /* options */
-enum { ZAPTEX_OPT_BC7_M1K = 1 /* mode-1 partitions fully fitted after ranking, 1..64 (zap_tex.h:750-752, default 4) */ };
+enum {
+    ZAPTEX_OPT_BC7_M1K = 1,  /* mode-1 partitions fully fitted after ranking, 1..64 (zap_tex.h:750-752, default 4) */
+    ZAPTEX_OPT_PROFILE = 2   /* 1: time every kernel with HIP events on its launch stream (zaptex_profile_*) */
+};
 
 /* ---------------- host entry points (drop-in bodies of zap_tex.h) */
 
@@ -88,6 +91,11 @@ int zaptex_get_option(int key);
 const char *zaptex_last_error(void);  /* message of this thread's last failed call */
 int zaptex_device_count(void);
 const char *zaptex_version(void);
+
+/* ---------------- kernel timing (no reference counterpart; for benchmarks) */
+int zaptex_profile_collect(void);  /* wait for the recorded events and fold them into per-kernel totals */
+int zaptex_profile_get(const char *kernel, double *ms_total, int *launches);
+void zaptex_profile_reset(void);
 
 #ifdef __cplusplus
 }

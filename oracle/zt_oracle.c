@@ -21,12 +21,18 @@
 #include <string.h>
 
 #ifdef ZTO_COUNT
-static _Thread_local unsigned long long g_ops;
+static _Thread_local unsigned long long g_ops, g_stage[4];
 #define OPS(n) (g_ops += (unsigned long long)(n))
+#define STAGE_BEGIN() unsigned long long stage_t0_ = g_ops
+#define STAGE_END(k) (g_stage[k] += g_ops - stage_t0_, stage_t0_ = g_ops)
 unsigned long long zto_ops_get(void) { return g_ops; }
-void zto_ops_reset(void) { g_ops = 0; }
+void zto_ops_reset(void) { g_ops = 0; g_stage[0] = g_stage[1] = g_stage[2] = g_stage[3] = 0; }
+/* BC7 ops by search stage: [0] mode 6 + opaque test, [1] mode 5, [2] mode-1 ranking + fits, [3] blocks reaching mode 1 */
+void zto_stage_ops(unsigned long long out[4]) { for (int k = 0; k < 4; k++) out[k] = g_stage[k]; }
 #else
 #define OPS(n) ((void)0)
+#define STAGE_BEGIN() ((void)0)
+#define STAGE_END(k) ((void)0)
 #endif
 
 typedef uint8_t px4[4];
@@ -619,15 +625,21 @@ static int g_m1k = 4; /* zap_tex.h:750-752 (ZAP__BC7_M1K, compile-time there) */
 
 /* zap_tex.h:755-784 */
 static int bc7_block(const px4 *px, uint8_t out[16]) {
+    STAGE_BEGIN();
     int err = bc7_mode6(px, out), opaque = 1;
     for (int i = 0; i < 16; i++) opaque &= px[i][3] == 255;
     OPS(32 + 1);
+    STAGE_END(0);
     if (err <= 16 * 4) return err;
     uint8_t m5[16];
     bc7_mode5(px, m5);
     int e5 = bc7_decoded_sse(px, m5);
     if (e5 < err) { memcpy(out, m5, 16); err = e5; }
+    STAGE_END(1);
     if (!opaque) return err;
+#ifdef ZTO_COUNT
+    g_stage[3]++;
+#endif
     float mo[16][9], tot[9] = { 0 }, ce[64];
     int cand[64], bp = -1, be = err, K = g_m1k;
     for (int i = 0; i < 16; i++) {
@@ -649,6 +661,7 @@ static int bc7_block(const px4 *px, uint8_t out[16]) {
         int e1 = bc7_decoded_sse(px, m1);
         if (e1 < err) { memcpy(out, m1, 16); err = e1; }
     }
+    STAGE_END(2);
     return err;
 }
 

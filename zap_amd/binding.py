@@ -20,6 +20,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "zaptex.h")
 BC1, BC3, BC4, BC5, BC7, BC6H, ASTC = range(7)
 FORMATS = {"bc1": BC1, "bc3": BC3, "bc4": BC4, "bc5": BC5, "bc7": BC7, "bc6h": BC6H, "astc": ASTC}
 OPT_BC7_M1K = 1
+OPT_PROFILE = 2
 STATUS = {0: "ok", 1: "bad argument", 2: "device error", 3: "out of memory", 4: "unsupported"}
 
 
@@ -93,6 +94,10 @@ class Library:
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
+        L.zaptex_profile_collect.restype = C.c_int
+        L.zaptex_profile_get.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.zaptex_profile_get.restype = C.c_int
+        L.zaptex_profile_reset.restype = None
         L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
         L.zaptex_workspace_size.restype = C.c_size_t
         L.zaptex_last_error.restype = C.c_char_p
@@ -212,6 +217,20 @@ class Library:
     def bc_decode_dev(self, bc, w: int, h: int, fmt, out, stream=None) -> None:
         self._check(self.lib.zaptex_bc_decode_dev(bc.data_ptr(), w, h, fmt_id(fmt), out.data_ptr(), w * 4, self._stream(stream)),
                     "zaptex_bc_decode_dev")
+
+    # ------------------------------------------------------------------ kernel timing
+    def profile(self, on: bool) -> None:
+        self.set_option(OPT_PROFILE, 1 if on else 0)
+
+    def profile_reset(self) -> None:
+        self.lib.zaptex_profile_reset()
+
+    def profile_get(self, kernel: str) -> tuple[float, int]:
+        """(total milliseconds, launches) of `kernel` since the last reset."""
+        self._check(self.lib.zaptex_profile_collect(), "zaptex_profile_collect")
+        ms, n = C.c_double(), C.c_int()
+        self._check(self.lib.zaptex_profile_get(kernel.encode(), C.byref(ms), C.byref(n)), "zaptex_profile_get")
+        return ms.value, n.value
 
     def version(self) -> str:
         return self.lib.zaptex_version().decode()

@@ -21,6 +21,14 @@ thread_local char g_err[512];
 std::mutex g_mu;     // serialises host entry points (shared staging buffers)
 int g_m1k = 4;
 
+// optional per-kernel timing: HIP events recorded on the launch stream around every kernel
+bool g_profile = false;
+std::mutex g_prof_mu;
+struct ProfPending { const char *name; hipEvent_t a, b; };
+struct ProfTotal { const char *name; double ms; long n; };
+std::vector<ProfPending> g_prof_pending;
+std::vector<ProfTotal> g_prof_total;
+
 int fail(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -33,6 +41,24 @@ int fail(int code, const char *fmt, ...) {
         hipError_t e_ = (x);                                                                    \
         if (e_ != hipSuccess) return fail(ZAPTEX_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
     } while (0)
+
+// launch `fn` on stream s; with profiling on, bracket it with events
+template <typename F> int launch(const char *name, hipStream_t s, F &&fn) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (g_profile) {
+        HIP_TRY(hipEventCreate(&a)); HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(a, s));
+    }
+    fn();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ZAPTEX_E_DEVICE, "launch of %s: %s", name, hipGetErrorString(e));
+    if (g_profile) {
+        HIP_TRY(hipEventRecord(b, s));
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        g_prof_pending.push_back({ name, a, b });
+    }
+    return ZAPTEX_OK;
+}
 
 size_t block_bytes(int fmt) { return fmt == ZAPTEX_BC1 || fmt == ZAPTEX_BC4 ? 8 : 16; }
 size_t out_bytes(int w, int h, int fmt) { return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * block_bytes(fmt); }
@@ -154,26 +180,30 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
         j.first = *t; j.w = im.w; j.h = im.h; j.bw = (im.w + 3) / 4; j.fmt = im.fmt; j.pad = 0;
         *t += nblocks(im);
     }
-    if (tot[0]) {
-        hipLaunchKernelGGL(zt_bc14_encode, dim3(grid_for(tot[0], 256)), dim3(256), 0, s, fam[0], tot[0]);
-        HIP_TRY(hipGetLastError());
-    }
-    if (tot_b6) {
-        hipLaunchKernelGGL(zt_bc6h_encode, dim3(grid_for(tot_b6, 256)), dim3(256), 0, s, b6, tot_b6);
-        HIP_TRY(hipGetLastError());
-    }
-    if (tot_astc) {
-        hipLaunchKernelGGL(zt_astc_encode, dim3(grid_for(tot_astc, 256)), dim3(256), 0, s, astc, tot_astc);
-        HIP_TRY(hipGetLastError());
-    }
+    int rc;
+    if (tot[0] && (rc = launch("zt_bc14_encode", s, [&] {
+            hipLaunchKernelGGL(zt_bc14_encode, dim3(grid_for(tot[0], 256)), dim3(256), 0, s, fam[0], tot[0]); })))
+        return rc;
+    if (tot_b6 && (rc = launch("zt_bc6h_encode", s, [&] {
+            hipLaunchKernelGGL(zt_bc6h_encode, dim3(grid_for(tot_b6, 256)), dim3(256), 0, s, b6, tot_b6); })))
+        return rc;
+    if (tot_astc && (rc = launch("zt_astc_encode", s, [&] {
+            hipLaunchKernelGGL(zt_astc_encode, dim3(grid_for(tot_astc, 256)), dim3(256), 0, s, astc, tot_astc); })))
+        return rc;
     if (tot[1]) {
         if (ws_bytes < ws_bytes_for(tot[1]) || !ws) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, ws_bytes_for(tot[1]));
         zt_bc7_lists L = lists_in(ws, tot[1]);
-        hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, L.counters, 64);
-        hipLaunchKernelGGL(zt_bc7_stage6, dim3(grid_for(tot[1], 256)), dim3(256), 0, s, fam[1], tot[1], L);
-        hipLaunchKernelGGL(zt_bc7_stage5, dim3(grid_for((uint64_t)tot[1] * 4, 256)), dim3(256), 0, s, fam[1], L);
-        hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot[1], 4)), dim3(256), 0, s, fam[1], L, g_m1k);
-        HIP_TRY(hipGetLastError());
+        if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, L.counters, 64); })))
+            return rc;
+        if ((rc = launch("zt_bc7_stage6", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_stage6, dim3(grid_for(tot[1], 256)), dim3(256), 0, s, fam[1], tot[1], L); })))
+            return rc;
+        if ((rc = launch("zt_bc7_stage5", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_stage5, dim3(grid_for((uint64_t)tot[1] * 4, 256)), dim3(256), 0, s, fam[1], L); })))
+            return rc;
+        if ((rc = launch("zt_bc7_stage1", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot[1], 4)), dim3(256), 0, s, fam[1], L, g_m1k); })))
+            return rc;
     }
     return ZAPTEX_OK;
 }
@@ -181,9 +211,9 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
 int enqueue_mip(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, hipStream_t s) {
     int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
     dim3 grid((nw + 63) / 64, (nh + 3) / 4), block(64, 4);
-    hipLaunchKernelGGL(zt_mip_level, grid, block, 0, s, (const uint8_t *)src, w, h, (uint64_t)stride, srgb, (uint8_t *)dst, (uint64_t)dst_stride);
-    HIP_TRY(hipGetLastError());
-    return ZAPTEX_OK;
+    return launch("zt_mip_level", s, [&] {
+        hipLaunchKernelGGL(zt_mip_level, grid, block, 0, s, (const uint8_t *)src, w, h, (uint64_t)stride, srgb, (uint8_t *)dst, (uint64_t)dst_stride);
+    });
 }
 
 // run one host-buffer image through the device: stage in, encode, copy out
@@ -419,6 +449,10 @@ int zaptex_bc_decode_dev(const void *bc, int w, int h, int fmt, void *rgba, size
 }
 
 int zaptex_set_option(int key, int value) {
+    if (key == ZAPTEX_OPT_PROFILE) {
+        g_profile = value != 0;
+        return ZAPTEX_OK;
+    }
     if (key == ZAPTEX_OPT_BC7_M1K) {
         if (value < 1 || value > 64) return fail(ZAPTEX_E_ARG, "BC7 M1K must be 1..64");
         g_m1k = value;
@@ -426,7 +460,34 @@ int zaptex_set_option(int key, int value) {
     }
     return fail(ZAPTEX_E_ARG, "unknown option %d", key);
 }
-int zaptex_get_option(int key) { return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : -1; }
+int zaptex_get_option(int key) { return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : -1; }
+
+int zaptex_profile_collect(void) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (const ProfPending &p : g_prof_pending) {
+        HIP_TRY(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+        auto it = std::find_if(g_prof_total.begin(), g_prof_total.end(), [&](const ProfTotal &t) { return !strcmp(t.name, p.name); });
+        if (it == g_prof_total.end()) g_prof_total.push_back({ p.name, ms, 1 });
+        else { it->ms += ms; it->n++; }
+        hipEventDestroy(p.a); hipEventDestroy(p.b);
+    }
+    g_prof_pending.clear();
+    return ZAPTEX_OK;
+}
+int zaptex_profile_get(const char *kernel, double *ms_total, int *launches) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (const ProfTotal &t : g_prof_total)
+        if (!strcmp(t.name, kernel)) { *ms_total = t.ms; *launches = (int)t.n; return ZAPTEX_OK; }
+    *ms_total = 0; *launches = 0;
+    return ZAPTEX_OK;
+}
+void zaptex_profile_reset(void) {
+    zaptex_profile_collect();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_total.clear();
+}
 const char *zaptex_last_error(void) { return g_err; }
 int zaptex_device_count(void) {
     int n = 0;
