@@ -70,6 +70,9 @@ class TexLib:
             L.zto_bc7_mode_errors.argtypes = [vp, i, vp]
             L.zto_bc7_estimate.argtypes = [vp, i]; L.zto_bc7_estimate.restype = f
             L.zto_set_m1k.argtypes = [i]
+            L.zto_bc7_candidate.argtypes = [vp, i, vp]; L.zto_bc7_candidate.restype = i
+            L.zto_bc7_full_block.argtypes = [vp, vp, C.POINTER(i)]; L.zto_bc7_full_block.restype = i
+            L.zto_bc7_full_encode.argtypes = [vp, i, i, sz, vp]
             L.zto_astc_partition.argtypes = [i, i, i, i]; L.zto_astc_partition.restype = i
             L.zto_ise_roundtrip.argtypes = [i, i, vp]; L.zto_ise_roundtrip.restype = i
             L.zto_bc_shuffle.argtypes = [vp, vp, sz, i, i]
@@ -95,6 +98,27 @@ class TexLib:
         else:
             self._enc(_ptr(img), w, h, stride, fmt, _ptr(out))
         return out
+
+    def bc7_full_encode(self, img: np.ndarray) -> np.ndarray:
+        """Exhaustive all-mode BC7 search (project-defined; oracle only)."""
+        img = np.ascontiguousarray(img)
+        h, w = img.shape[:2]
+        out = np.zeros(bc_size(w, h, 4), dtype=np.uint8)
+        self.lib.zto_bc7_full_encode(_ptr(img), w, h, img.strides[0], _ptr(out))
+        return out
+
+    def bc7_candidate(self, block16x4: np.ndarray, cand: int):
+        px = np.ascontiguousarray(block16x4, dtype=np.uint8)
+        out = np.zeros(16, np.uint8)
+        e = self.lib.zto_bc7_candidate(_ptr(px), cand, _ptr(out))
+        return e, out
+
+    def bc7_full_block(self, block16x4: np.ndarray):
+        px = np.ascontiguousarray(block16x4, dtype=np.uint8)
+        out = np.zeros(16, np.uint8)
+        c = C.c_int()
+        e = self.lib.zto_bc7_full_block(_ptr(px), _ptr(out), C.byref(c))
+        return e, c.value, out
 
     def bc_encode_rows(self, img, fmt, out, by0, by1):
         fmt = FMT.get(fmt, fmt)

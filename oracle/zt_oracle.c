@@ -665,6 +665,196 @@ static int bc7_block(const px4 *px, uint8_t out[16]) {
     return err;
 }
 
+static void gather(const uint8_t *rgba, int w, int h, size_t stride, int bx, int by, px4 *px);
+/* ======================================================================= exhaustive BC7 search (all 8 modes)
+ * Project-defined, with no reference counterpart: the reference encoder only tries modes 6, 5 and 1.  Candidates
+ * are numbered mode*64 + slot: slot = partition for modes 0 (0..15), 1, 2, 3, 7 (0..63); rot*2 + isb for mode 4;
+ * rot for mode 5; 0 for mode 6 (285 candidates).  Every subset is encoded the reference's way (PCA start, fit,
+ * <= 2 least-squares refinements); modes 6, 5 and 1 are exactly the reference's encoders, so the exhaustive
+ * result's SSE never exceeds the reference search's.  The SSE counts all four channels of the decoded block. */
+
+static int fit_subset_plain(const px4 *px, const int *sel, int cnt, int nc, int n, int ib, const float lo[4], const float hi[4],
+                            int q[2][4], int *idx) {
+    int e0[4], e1[4];
+    quant_plain(lo, nc, n, q[0], e0); quant_plain(hi, nc, n, q[1], e1);
+    int pal[16][4], np = 1 << ib, err = 0;
+    for (int j = 0; j < np; j++) for (int c = 0; c < nc; c++) { int w = weight(ib, j); pal[j][c] = ((64 - w) * e0[c] + w * e1[c] + 32) >> 6; }
+    for (int k = 0; k < cnt; k++) {
+        int bi = 0, be = 1 << 30;
+        for (int j = 0; j < np; j++) { int d = sse_n(px[sel[k]], pal[j], nc); if (d < be) { be = d; bi = j; } }
+        idx[k] = bi; err += be;
+    }
+    OPS(np * nc * 6 + cnt * (np * (3 * nc + 3) + 1));
+    return err;
+}
+/* subset without p-bits (modes 2 / 4-5 colour): PCA, plain n-bit quantisation, <= 2 least-squares refinements */
+static int encode_subset_plain(const px4 *px, const int *sel, int cnt, int nc, int n, int ib, int q[2][4], int *idx) {
+    float lo[4], hi[4];
+    pca_endpoints(px, sel, cnt, nc, lo, hi);
+    int err = fit_subset_plain(px, sel, cnt, nc, n, ib, lo, hi, q, idx);
+    for (int it = 0; it < 2 && err > 0; it++) {
+        float aa = 0, ab = 0, bb = 0, sa[4] = { 0, 0, 0, 0 }, sb[4] = { 0, 0, 0, 0 };
+        for (int k = 0; k < cnt; k++) {
+            float w = weight(ib, idx[k]) / 64.0f, v = 1 - w;
+            aa += v * v; ab += v * w; bb += w * w;
+            for (int c = 0; c < nc; c++) { sa[c] += v * px[sel[k]][c]; sb[c] += w * px[sel[k]][c]; }
+        }
+        float det = aa * bb - ab * ab;
+        if (det < 1e-6f && det > -1e-6f) break;
+        float nlo[4], nhi[4];
+        for (int c = 0; c < nc; c++) {
+            nlo[c] = (sa[c] * bb - sb[c] * ab) / det; nhi[c] = (sb[c] * aa - sa[c] * ab) / det;
+            nlo[c] = nlo[c] < 0 ? 0 : nlo[c] > 255 ? 255 : nlo[c]; nhi[c] = nhi[c] < 0 ? 0 : nhi[c] > 255 ? 255 : nhi[c];
+        }
+        for (int c = nc; c < 4; c++) nlo[c] = nhi[c] = 255;
+        int nq[2][4], ni[16];
+        int ne = fit_subset_plain(px, sel, cnt, nc, n, ib, nlo, nhi, nq, ni);
+        if (ne >= err) break;
+        err = ne; memcpy(q, nq, sizeof nq); memcpy(idx, ni, sizeof(int) * (size_t)cnt);
+    }
+    return err;
+}
+
+/* modes 0, 1, 2, 3, 6, 7: one encode per subset, anchor fix-ups, pack */
+static int p1_subsets(const px4 *px, int mode, int part, uint8_t out[16]) {
+    const bc7_layout *m = &L7[mode];
+    const int ns = m->ns, nc = m->ab ? 4 : 3, haspb = m->epb || m->spb, n = m->cb + haspb, ib = m->ib;
+    int sel[3][16], cnt[3] = { 0, 0, 0 }, q[3][2][4], pb[3][2] = { { 0, 0 }, { 0, 0 }, { 0, 0 } }, idx[3][16], full[16], err = 0;
+    for (int i = 0; i < 16; i++) { int sb = subset_of(ns, part, i); sel[sb][cnt[sb]++] = i; }
+    for (int sb = 0; sb < ns; sb++)
+        err += haspb ? encode_subset(px, sel[sb], cnt[sb], nc, n, ib, m->spb, q[sb], pb[sb], idx[sb])
+                     : encode_subset_plain(px, sel[sb], cnt[sb], nc, n, ib, q[sb], idx[sb]);
+    if (nc == 3) for (int i = 0; i < 16; i++) { int d = 255 - px[i][3]; err += d * d; }  /* decoded alpha is 255 */
+    for (int sb = 0; sb < ns; sb++) {
+        for (int k = 0; k < cnt[sb]; k++) full[sel[sb][k]] = idx[sb][k];
+        int anc = anchor_of(ns, part, sb), top = 1 << (ib - 1), mx = (1 << ib) - 1;
+        if (full[anc] & top) {
+            for (int c = 0; c < 4; c++) { int t = q[sb][0][c]; q[sb][0][c] = q[sb][1][c]; q[sb][1][c] = t; }
+            if (m->epb) { int t = pb[sb][0]; pb[sb][0] = pb[sb][1]; pb[sb][1] = t; }
+            for (int k = 0; k < cnt[sb]; k++) full[sel[sb][k]] = mx - full[sel[sb][k]];
+        }
+    }
+    int pos = 0;
+    memset(out, 0, 16);
+    putbits(out, &pos, 1u << mode, mode + 1);
+    putbits(out, &pos, (unsigned)part, m->pb);
+    for (int c = 0; c < 3; c++) for (int sb = 0; sb < ns; sb++) for (int e = 0; e < 2; e++) putbits(out, &pos, (unsigned)q[sb][e][c], m->cb);
+    if (m->ab) for (int sb = 0; sb < ns; sb++) for (int e = 0; e < 2; e++) putbits(out, &pos, (unsigned)q[sb][e][3], m->ab);
+    if (m->epb) for (int sb = 0; sb < ns; sb++) for (int e = 0; e < 2; e++) putbits(out, &pos, (unsigned)pb[sb][e], 1);
+    if (m->spb) for (int sb = 0; sb < ns; sb++) putbits(out, &pos, (unsigned)pb[sb][0], 1);
+    for (int i = 0; i < 16; i++) putbits(out, &pos, (unsigned)full[i], ib - (i == anchor_of(ns, part, subset_of(ns, part, i))));
+    return err;
+}
+
+/* n-bit / ib-bit part of a rotation mode: plain quantisation, argmin, <= 2 refinements (generalises zap_tex.h:644-657) */
+static int fit_nbit(const px4 *r, int c0, int nc, int ib, const int *e0, const int *e1, int idx[16]) {
+    int pal[8][4], err = 0, np = 1 << ib;
+    for (int j = 0; j < np; j++) for (int c = 0; c < nc; c++) { int w = weight(ib, j); pal[j][c] = ((64 - w) * e0[c] + w * e1[c] + 32) >> 6; }
+    for (int i = 0; i < 16; i++) {
+        int bi = 0, be = 1 << 30;
+        for (int j = 0; j < np; j++) {
+            int d = 0;
+            for (int c = 0; c < nc; c++) { int x = r[i][c0 + c] - pal[j][c]; d += x * x; }
+            if (d < be) { be = d; bi = j; }
+        }
+        idx[i] = bi; err += be;
+    }
+    OPS(np * nc * 6 + 16 * (np * (3 * nc + 3) + 1));
+    return err;
+}
+static int lsq_nbit(const px4 *r, int c0, int nc, int ib, const int idx[16], float lo[4], float hi[4]) {
+    float aa = 0, ab = 0, bb = 0, sa[4] = { 0, 0, 0, 0 }, sb[4] = { 0, 0, 0, 0 };
+    for (int i = 0; i < 16; i++) {
+        float w = weight(ib, idx[i]) / 64.0f, v = 1 - w;
+        aa += v * v; ab += v * w; bb += w * w;
+        for (int c = 0; c < nc; c++) { sa[c] += v * r[i][c0 + c]; sb[c] += w * r[i][c0 + c]; }
+    }
+    float det = aa * bb - ab * ab;
+    if (det < 1e-6f && det > -1e-6f) return 0;
+    for (int c = 0; c < nc; c++) {
+        lo[c] = (sa[c] * bb - sb[c] * ab) / det; hi[c] = (sb[c] * aa - sa[c] * ab) / det;
+        lo[c] = lo[c] < 0 ? 0 : lo[c] > 255 ? 255 : lo[c]; hi[c] = hi[c] < 0 ? 0 : hi[c] > 255 ? 255 : hi[c];
+    }
+    return 1;
+}
+static int part_nbit(const px4 *r, int c0, int nc, int n, int ib, const float lo[4], const float hi[4], int q[2][3], int idx[16]) {
+    int e0[3], e1[3], ti[16], tq[2][3], te0[3], te1[3];
+    quant_plain(lo, nc, n, q[0], e0); quant_plain(hi, nc, n, q[1], e1);
+    int err = fit_nbit(r, c0, nc, ib, e0, e1, idx);
+    for (int it = 0; it < 2 && err > 0; it++) {
+        float nlo[4], nhi[4];
+        if (!lsq_nbit(r, c0, nc, ib, idx, nlo, nhi)) break;
+        quant_plain(nlo, nc, n, tq[0], te0); quant_plain(nhi, nc, n, tq[1], te1);
+        int e = fit_nbit(r, c0, nc, ib, te0, te1, ti);
+        if (e >= err) break;
+        err = e; memcpy(q, tq, sizeof tq); memcpy(idx, ti, sizeof ti);
+    }
+    return err;
+}
+/* modes 4 and 5: rotation rot, index selector isb (mode 4) */
+static int p1_rotation(const px4 *px, int mode, int rot, int isb, uint8_t out[16]) {
+    const bc7_layout *m = &L7[mode];
+    const int ibc = mode == 4 && isb ? 3 : 2, iba = mode == 4 && !isb ? 3 : 2;
+    px4 r[16];
+    int sel[16], cq[2][3], aq[2][3], ci[16], ai[16];
+    memcpy(r, px, sizeof r);
+    for (int i = 0; i < 16; i++) { sel[i] = i; if (rot) { uint8_t t = r[i][3]; r[i][3] = r[i][rot - 1]; r[i][rot - 1] = t; } }
+    float lo[4], hi[4], alo[4] = { 255 }, ahi[4] = { 0 };
+    pca_endpoints((const px4 *)r, sel, 16, 3, lo, hi);
+    int err = part_nbit((const px4 *)r, 0, 3, m->cb, ibc, lo, hi, cq, ci);
+    for (int i = 0; i < 16; i++) { if (r[i][3] < alo[0]) alo[0] = r[i][3]; if (r[i][3] > ahi[0]) ahi[0] = r[i][3]; }
+    err += part_nbit((const px4 *)r, 3, 1, m->ab, iba, alo, ahi, aq, ai);
+    if (ci[0] & (1 << (ibc - 1))) { for (int c = 0; c < 3; c++) { int t = cq[0][c]; cq[0][c] = cq[1][c]; cq[1][c] = t; } for (int i = 0; i < 16; i++) ci[i] = (1 << ibc) - 1 - ci[i]; }
+    if (ai[0] & (1 << (iba - 1))) { int t = aq[0][0]; aq[0][0] = aq[1][0]; aq[1][0] = t; for (int i = 0; i < 16; i++) ai[i] = (1 << iba) - 1 - ai[i]; }
+    int pos = 0;
+    memset(out, 0, 16);
+    putbits(out, &pos, 1u << mode, mode + 1);
+    putbits(out, &pos, (unsigned)rot, 2);
+    if (mode == 4) putbits(out, &pos, (unsigned)isb, 1);
+    for (int c = 0; c < 3; c++) for (int e = 0; e < 2; e++) putbits(out, &pos, (unsigned)cq[e][c], m->cb);
+    for (int e = 0; e < 2; e++) putbits(out, &pos, (unsigned)aq[e][0], m->ab);
+    const int *first = isb ? ai : ci, *second = isb ? ci : ai;   /* the 2-bit set is stored first */
+    for (int i = 0; i < 16; i++) putbits(out, &pos, (unsigned)first[i], m->ib - (i == 0));
+    for (int i = 0; i < 16; i++) putbits(out, &pos, (unsigned)second[i], m->ib2 - (i == 0));
+    return err;
+}
+
+static int p1_valid(int cand) {
+    int mode = cand >> 6, slot = cand & 63;
+    return mode == 0 ? slot < 16 : mode == 4 ? slot < 8 : mode == 5 ? slot < 4 : mode == 6 ? slot == 0 : 1;
+}
+/* one candidate: SSE of its decoded block, block bytes in out; -1 for an unused slot */
+int zto_bc7_candidate(const uint8_t px64[64], int cand, uint8_t out[16]) {
+    const px4 *px = (const px4 *)px64;
+    p3_init();
+    if (cand < 0 || cand >= 512 || !p1_valid(cand)) return -1;
+    int mode = cand >> 6, slot = cand & 63;
+    if (mode == 4) return p1_rotation(px, 4, slot >> 1, slot & 1, out);
+    if (mode == 5) return p1_rotation(px, 5, slot, 0, out);
+    return p1_subsets(px, mode, slot, out);
+}
+/* exhaustive search: best candidate (smallest SSE, then lowest candidate number) */
+int zto_bc7_full_block(const uint8_t px64[64], uint8_t out[16], int *best_cand) {
+    int best = 1 << 30, bc = -1;
+    uint8_t tmp[16];
+    for (int c = 0; c < 512; c++) {
+        int e = zto_bc7_candidate(px64, c, tmp);
+        if (e >= 0 && e < best) { best = e; bc = c; memcpy(out, tmp, 16); }
+    }
+    if (best_cand) *best_cand = bc;
+    return best;
+}
+void zto_bc7_full_encode(const uint8_t *rgba, int w, int h, size_t stride, void *out_) {
+    uint8_t *out = (uint8_t *)out_;
+    int bw = (w + 3) / 4, bh = (h + 3) / 4;
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            px4 px[16];
+            gather(rgba, w, h, stride, bx, by, px);
+            zto_bc7_full_block((const uint8_t *)px, out + ((size_t)by * (size_t)bw + (size_t)bx) * 16, NULL);
+        }
+}
+
 /* ======================================================================= BC6H (zap_tex.h:817-975) */
 
 static uint16_t half_of(float f) { /* zap_tex.h:820-827 */

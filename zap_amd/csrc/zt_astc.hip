@@ -59,9 +59,12 @@ template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int 
     }
 }
 
-// weights for fixed endpoints (zap_tex.h:1197-1210); pm: bit i = texel i in partition 1
+// weights for fixed endpoints (zap_tex.h:1197-1210); pm: bit i = texel i in partition 1.
+// key_j = 2^(IB+1) dot(p, pal_j) - 2^IB |pal_j|^2 + (2^IB - 1 - j): the max is the min SSE, lowest j on ties;
+// IB index bits hold up to 32 entries (5-bit weights), keys stay below 2^25.
 template <int K> ZT_DEV int fit_weights(const uint32_t px[16], const int *uv, uint32_t pm, int w[16]) {
     constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, L = 1 << WB, NC = CEM == 12 ? 4 : 3, NV = CEM == 8 ? 6 : 8;
+    constexpr int IB = WB > 4 ? WB : 4, IM = (1 << IB) - 1;
     constexpr uint32_t CM = NC == 4 ? 0xFFFFFFFFu : 0x00FFFFFFu;
     uint32_t pal[P][L];
     int Kk[P][L];
@@ -76,7 +79,7 @@ template <int K> ZT_DEV int fit_weights(const uint32_t px[16], const int *uv, ui
 #pragma unroll
             for (int c = 0; c < 4; c++) v |= (uint32_t)lerp(e0[c], e1[c], wt) << (8 * c);
             pal[k][j] = v & CM;
-            Kk[k][j] = 15 - j - 16 * (int)dot4(pal[k][j], pal[k][j]);
+            Kk[k][j] = IM - j - (1 << IB) * (int)dot4(pal[k][j], pal[k][j]);
         }
     }
     int err = 0;
@@ -89,10 +92,10 @@ template <int K> ZT_DEV int fit_weights(const uint32_t px[16], const int *uv, ui
         for (int j = 0; j < L; j++) {
             uint32_t pj = (P == 2 && second) ? pal[P - 1][j] : pal[0][j];
             int kj = (P == 2 && second) ? Kk[P - 1][j] : Kk[0][j];
-            best = max(best, (int)(dot4(p, pj) << 5) + kj);
+            best = max(best, (int)(dot4(p, pj) << (IB + 1)) + kj);
         }
-        w[i] = 15 - (best & 15);
-        err += (int)dot4(p, p) - (best >> 4);
+        w[i] = IM - (best & IM);
+        err += (int)dot4(p, p) - (best >> IB);
     }
     return err;
 }

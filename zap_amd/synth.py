@@ -39,17 +39,18 @@ def _smooth_q16(t: np.ndarray) -> np.ndarray:
 
 def value_noise(h: int, w: int, cell: int, seed: int, chan: int) -> np.ndarray:
     """Value noise in Q16 [0, 65536): random lattice values every `cell` pixels, smoothstep-bilinear."""
-    y = np.arange(h, dtype=np.int64)[:, None]
-    x = np.arange(w, dtype=np.int64)[None, :]
+    y = np.arange(h, dtype=np.int64)
+    x = np.arange(w, dtype=np.int64)
     gx, gy = x // cell, y // cell
-    tx = _smooth_q16(((x % cell) << 16) // cell)
-    ty = _smooth_q16(((y % cell) << 16) // cell)
-
-    def lat(ix, iy):
-        return (hash32(seed, chan, cell, ix, iy) >> np.uint32(16)).astype(np.int64)  # 0..65535
-
-    top = lat(gx, gy) * (65536 - tx) + lat(gx + 1, gy) * tx
-    bot = lat(gx, gy + 1) * (65536 - tx) + lat(gx + 1, gy + 1) * tx
+    tx = _smooth_q16(((x % cell) << 16) // cell)[None, :]
+    ty = _smooth_q16(((y % cell) << 16) // cell)[:, None]
+    # lattice values (0..65535) hashed once per lattice point, then gathered per pixel
+    lx = np.arange(int(gx[-1]) + 2, dtype=np.int64)[None, :]
+    ly = np.arange(int(gy[-1]) + 2, dtype=np.int64)[:, None]
+    lat = (hash32(seed, chan, cell, lx, ly) >> np.uint32(16)).astype(np.int64)
+    r0, r1 = lat[gy], lat[gy + 1]
+    top = r0[:, gx] * (65536 - tx) + r0[:, gx + 1] * tx
+    bot = r1[:, gx] * (65536 - tx) + r1[:, gx + 1] * tx
     return ((top >> 16) * (65536 - ty) + (bot >> 16) * ty) >> 16
 
 
