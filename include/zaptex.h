@@ -76,6 +76,10 @@ size_t zaptex_workspace_size(const zaptex_image *imgs, int n);
 /* Encode n (1..16) images, any mix of formats, on `stream`.  The workspace is device memory of at least
  * zaptex_workspace_size bytes, owned by the caller; calls sharing a workspace must not overlap. */
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream);
+/* Exhaustive all-mode BC7 search (no reference counterpart: the reference tries modes 6, 5 and 1 only).
+ * 285 candidates per block (mode*64 + slot); never worse than zaptex_encode_dev's reference search.
+ * best_cand (device, one uint32 per block of all images in order, may be NULL) receives the chosen candidate. */
+int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, void *stream);
 /* zap_mip_next on device memory */
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream);
 /* zap_mip_next_f on device memory (strides in floats) */

@@ -76,6 +76,14 @@ class Library:
         if not os.path.exists(path):
             raise ZapTexError(f"{path} is not built (run __graft_entry__.build() / python zap_amd/build.py)")
         self.path = path
+        # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 and
+        # its libraries name them without the version suffix.  Loaded first, torch's copies carry the sonames
+        # libzaptex.so asks for, so both share them; loaded second, torch would start a second HSA runtime
+        # that cannot open the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         self.lib = L = C.CDLL(path)
         vp, i, sz, f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
         sig = {
@@ -88,6 +96,7 @@ class Library:
             "zaptex_mip_next_dev": [vp, i, i, sz, i, vp, sz, vp], "zaptex_mip_next_f_dev": [vp, i, i, sz, vp, sz, vp],
             "zaptex_mip_chain_dev": [vp, i, i, sz, i, C.POINTER(vp), i, vp],
             "zaptex_bc_decode_dev": [vp, i, i, i, vp, sz, vp],
+            "zaptex_bc7_full_dev": [C.POINTER(ZapImage), i, vp, vp],
             "zaptex_set_option": [i, i], "zaptex_get_option": [i],
         }
         for name, args in sig.items():
@@ -208,6 +217,11 @@ class Library:
         ws_ptr = workspace.data_ptr() if workspace is not None else None
         ws_len = workspace.numel() * workspace.element_size() if workspace is not None else 0
         self._check(self.lib.zaptex_encode_dev(imgs, len(imgs), ws_ptr, ws_len, self._stream(stream)), "zaptex_encode_dev")
+
+    def bc7_full_dev(self, imgs, best_cand=None, stream=None) -> None:
+        """Exhaustive all-mode BC7 search; best_cand: optional int32 tensor, one entry per block."""
+        self._check(self.lib.zaptex_bc7_full_dev(imgs, len(imgs), best_cand.data_ptr() if best_cand is not None else None,
+                                                 self._stream(stream)), "zaptex_bc7_full_dev")
 
     def mip_chain_dev(self, base, w: int, h: int, stride: int, levels: list, srgb: int = 0, stream=None) -> None:
         ptrs = (C.c_void_p * len(levels))(*[t.data_ptr() for t in levels])

@@ -10,388 +10,9 @@
 // The modes-5/1 scores are the fit SSEs: for those encoders the decoded block's SSE equals the fit SSE
 // (endpoint swap with index inversion is symmetric for W2/W3, rotation permutes channels back), so the
 // reference's decode-and-measure (zap_tex.h:761, :780) compares the same integers.
-#include "zt_common.h"
+#include "zt_bc7_core.h"
 #include "zt_kernels.h"
-#include "zt_pca.h"
 
-namespace {
-
-// BC7 format-specification tables (2-subset partitions as bit masks, second-subset anchors, weights)
-__constant__ uint16_t kP2[64] = {
-    0xCCCC, 0x8888, 0xEEEE, 0xECC8, 0xC880, 0xFEEC, 0xFEC8, 0xEC80, 0xC800, 0xFFEC, 0xFE80, 0xE800, 0xFFE8, 0xFF00, 0xFFF0, 0xF000,
-    0xF710, 0x008E, 0x7100, 0x08CE, 0x008C, 0x7310, 0x3100, 0x8CCE, 0x088C, 0x3110, 0x6666, 0x366C, 0x17E8, 0x0FF0, 0x718E, 0x399C,
-    0xAAAA, 0xF0F0, 0x5A5A, 0x33CC, 0x3C3C, 0x55AA, 0x9696, 0xA55A, 0x73CE, 0x13C8, 0x324C, 0x3BDC, 0x6996, 0xC33C, 0x9966, 0x0660,
-    0x0272, 0x04E4, 0x4E40, 0x2720, 0xC936, 0x936C, 0x39C6, 0x639C, 0x9336, 0x9CC6, 0x817E, 0xE718, 0xCCF0, 0x0FCC, 0x7744, 0xEE22 };
-__constant__ uint8_t kA2[64] = {
-    15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 2, 8, 2, 2, 8, 8, 15, 2, 8, 2, 2, 8, 8, 2, 2,
-    15, 15, 6, 8, 2, 8, 15, 15, 2, 8, 2, 2, 2, 15, 15, 6, 6, 2, 6, 8, 15, 15, 2, 2, 15, 15, 15, 15, 15, 2, 2, 15 };
-
-// BC7 weights: W2 {0,21,43,64}, W3 {0,9,..,64}, W4 {0,4,..,64} are round(64 j / (2^IB - 1))
-template <int IB> ZT_DEV int wgt(int j) {
-    constexpr int D = (1 << IB) - 1;
-    return (j * 64 + D / 2) / D;
-}
-ZT_DEV int expand_bits(int v, int n) { v <<= 8 - n; return v | v >> n; }
-
-// channel-masked pixel: NC channels starting at C0, packed from byte 0
-template <int NC, int C0> ZT_DEV uint32_t sub_px(uint32_t p) {
-    uint32_t v = p >> (8 * C0);
-    return v & (uint32_t)((1ull << (8 * NC)) - 1ull);
-}
-
-// ---- p-bit quantiser for one channel (zap_tex.h:426-438); N = bits including the p-bit
-template <int N> ZT_DEV int quant_pbit1(float v, int p, int &q) {
-    const int mx = (1 << (N - 1)) - 1;
-    int g = (int)((v / 255.0f * (float)((1 << N) - 1) - (float)p) / 2.0f + 0.5f), bq = 0, be = 1 << 30;
-#pragma unroll
-    for (int dt = -1; dt <= 1; dt++) {
-        int t = g + dt, tq = t < 0 ? 0 : t > mx ? mx : t, x = expand_bits(tq << 1 | p, N);
-        float d = (float)x - v;
-        int e = (int)(d * d);
-        if (e < be) { be = e; bq = tq; }
-    }
-    q = bq;
-    return be;
-}
-
-// ---- argmin over a palette of NP packed entries: returns SSE, writes the index (ties -> lowest index).
-// key_j = 16*(2 p.pal_j - |pal_j|^2) + (15 - j): maximising it minimises |p - pal_j|^2 exactly.
-template <int NP> ZT_DEV int best_entry(uint32_t p, const uint32_t pal[NP], const int K[NP], int &idx) {
-    int best = (int)(dot4(p, pal[0]) << 5) + K[0];
-#pragma unroll
-    for (int j = 1; j < NP; j++) best = max(best, (int)(dot4(p, pal[j]) << 5) + K[j]);
-    idx = 15 - (best & 15);
-    return (int)dot4(p, p) - (best >> 4);
-}
-template <int NP> ZT_DEV void palette_keys(const uint32_t pal[NP], int K[NP]) {
-#pragma unroll
-    for (int j = 0; j < NP; j++) K[j] = 15 - j - 16 * (int)dot4(pal[j], pal[j]);
-}
-
-// ---- one subset: quantised endpoints + indices (zap_tex.h:466-486).  idx: 4 bits per pixel at 4*i.
-template <int NC, int N, int IB, bool SHARED>
-ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], const float hi[4], int q[2][4], int pb[2],
-                      uint64_t &idx) {
-    int ql[2][NC], qh[2][NC], el[2], eh[2];
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-        el[p] = 0; eh[p] = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) { el[p] += quant_pbit1<N>(lo[c], p, ql[p][c]); eh[p] += quant_pbit1<N>(hi[c], p, qh[p][c]); }
-    }
-    int best = 1 << 30, b0 = 0, b1 = 0;
-#pragma unroll
-    for (int p0 = 0; p0 < 2; p0++)
-#pragma unroll
-        for (int p1 = 0; p1 < 2; p1++) {
-            if (SHARED && p0 != p1) continue;
-            int e = el[p0] + eh[p1];
-            if (e < best) { best = e; b0 = p0; b1 = p1; }
-        }
-    pb[0] = b0; pb[1] = b1;
-    int e0[NC], e1[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        q[0][c] = b0 ? ql[1][c] : ql[0][c]; q[1][c] = b1 ? qh[1][c] : qh[0][c];
-        e0[c] = expand_bits(q[0][c] << 1 | b0, N); e1[c] = expand_bits(q[1][c] << 1 | b1, N);
-    }
-    constexpr int NP = 1 << IB;
-    uint32_t pal[NP];
-    int K[NP];
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-        const int w = wgt<IB>(j);
-        uint32_t v = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
-        pal[j] = v;
-    }
-    palette_keys<NP>(pal, K);
-    int err = 0;
-    uint64_t bits = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (!((mask >> k) & 1)) continue;
-        int j;
-        err += best_entry<NP>(sub_px<NC, 0>(px[k]), pal, K, j);
-        bits |= (uint64_t)j << (4 * k);
-    }
-    idx = bits;
-    return err;
-}
-
-// ---- PCA start, fit, at most 2 least-squares refinements (zap_tex.h:489-515)
-template <int NC, int N, int IB, bool SHARED, bool FULL>
-ZT_DEV int encode_subset(const uint32_t px[16], uint32_t mask, int cnt, int q[2][4], int pb[2], uint64_t &idx) {
-    float lo[4], hi[4];
-    pca<NC, FULL>(px, mask, cnt, lo, hi);
-    int err = fit_subset<NC, N, IB, SHARED>(px, mask, lo, hi, q, pb, idx);
-    for (int it = 0; it < 2 && err > 0; it++) {
-        float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
-#pragma unroll
-        for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            if (!((mask >> k) & 1)) continue;
-            float w = (float)wgt<IB>((int)((idx >> (4 * k)) & 15)) / 64.0f, v = 1 - w;
-            aa += v * v; ab += v * w; bb += w * w;
-#pragma unroll
-            for (int c = 0; c < NC; c++) { sa[c] += v * chanf(px[k], c); sb[c] += w * chanf(px[k], c); }
-        }
-        float det = aa * bb - ab * ab;
-        if (det < 1e-6f && det > -1e-6f) break;
-        float nlo[4], nhi[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (c < NC) {
-                float l = (sa[c] * bb - sb[c] * ab) / det, h = (sb[c] * aa - sa[c] * ab) / det;
-                nlo[c] = l < 0 ? 0 : l > 255 ? 255 : l; nhi[c] = h < 0 ? 0 : h > 255 ? 255 : h;
-            } else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
-        }
-        int nq[2][4], np[2];
-        uint64_t ni;
-        int ne = fit_subset<NC, N, IB, SHARED>(px, mask, nlo, nhi, nq, np, ni);
-        if (ne >= err) break;
-        err = ne; pb[0] = np[0]; pb[1] = np[1]; idx = ni;
-#pragma unroll
-        for (int c = 0; c < NC; c++) { q[0][c] = nq[0][c]; q[1][c] = nq[1][c]; }
-    }
-    return err;
-}
-
-// ---- mode 6 (zap_tex.h:582-593, pack :520-527)
-ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
-    int q[2][4], pb[2];
-    uint64_t idx;
-    int err = encode_subset<4, 8, 4, false, true>(px, 0xFFFFu, 16, q, pb, idx);
-    if (idx & 8) {  // anchor MSB set: swap endpoints and p-bits, invert indices
-#pragma unroll
-        for (int c = 0; c < 4; c++) { int t = q[0][c]; q[0][c] = q[1][c]; q[1][c] = t; }
-        int t = pb[0]; pb[0] = pb[1]; pb[1] = t;
-        idx ^= 0xFFFFFFFFFFFFFFFFull;
-    }
-    uint64_t l = 0x40;
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-        for (int e = 0; e < 2; e++) l |= (uint64_t)q[e][c] << (7 + 7 * (2 * c + e));
-    l |= (uint64_t)pb[0] << 63;
-    uint64_t field = (idx & 7) | ((idx >> 1) & ~7ull);  // pixel 0 keeps 3 bits
-    lo = l;
-    hi = (uint64_t)pb[1] | field << 1;
-    return err;
-}
-
-// ---- mode 5 pieces (zap_tex.h:597-657); NC = 3 (colour, 7 bits) or 1 (alpha, 8 bits)
-template <int NC> ZT_DEV void quant_plain(const float *v, int n, int q[], int e[]) {
-    const int mx = (1 << n) - 1;
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        int g = (int)(v[c] * (float)mx / 255.0f + 0.5f), bq = 0, bd = 1 << 30;
-#pragma unroll
-        for (int dt = -1; dt <= 1; dt++) {
-            int t = g + dt, tq = t < 0 ? 0 : t > mx ? mx : t, x = expand_bits(tq, n);
-            float df = (float)x - v[c];
-            int d = (int)(df < 0 ? v[c] - (float)x : df);
-            if (d < bd) { bd = d; bq = tq; }
-        }
-        q[c] = bq; e[c] = expand_bits(bq, n);
-    }
-}
-template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], const int e0[], const int e1[], uint32_t &idx) {
-    uint32_t pal[4];
-    int K[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int w = wgt<2>(j);
-        uint32_t v = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
-        pal[j] = v;
-    }
-    palette_keys<4>(pal, K);
-    int err = 0;
-    uint32_t bits = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        int j;
-        err += best_entry<4>(sub_px<NC, C0>(r[i]), pal, K, j);
-        bits |= (uint32_t)j << (2 * i);
-    }
-    idx = bits;
-    return err;
-}
-template <int NC, int C0> ZT_DEV bool lsq_2bit(const uint32_t r[16], uint32_t idx, float lo[], float hi[]) {
-    float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        float w = (float)wgt<2>((int)((idx >> (2 * i)) & 3)) / 64.0f, v = 1 - w;
-        aa += v * v; ab += v * w; bb += w * w;
-#pragma unroll
-        for (int c = 0; c < NC; c++) { sa[c] += v * chanf(r[i], C0 + c); sb[c] += w * chanf(r[i], C0 + c); }
-    }
-    float det = aa * bb - ab * ab;
-    if (det < 1e-6f && det > -1e-6f) return false;
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        float l = (sa[c] * bb - sb[c] * ab) / det, h = (sb[c] * aa - sa[c] * ab) / det;
-        lo[c] = l < 0 ? 0 : l > 255 ? 255 : l; hi[c] = h < 0 ? 0 : h > 255 ? 255 : h;
-    }
-    return true;
-}
-template <int NC, int C0> ZT_DEV int part_mode5(const uint32_t r[16], int n, const float lo[], const float hi[], int q[2][3], uint32_t &idx) {
-    int e0[NC], e1[NC];
-    quant_plain<NC>(lo, n, q[0], e0); quant_plain<NC>(hi, n, q[1], e1);
-    int err = fit_2bit<NC, C0>(r, e0, e1, idx);
-    for (int it = 0; it < 2 && err > 0; it++) {
-        float nlo[NC], nhi[NC];
-        if (!lsq_2bit<NC, C0>(r, idx, nlo, nhi)) break;
-        int tq[2][3], te0[NC], te1[NC];
-        quant_plain<NC>(nlo, n, tq[0], te0); quant_plain<NC>(nhi, n, tq[1], te1);
-        uint32_t ti;
-        int e = fit_2bit<NC, C0>(r, te0, te1, ti);
-        if (e >= err) break;
-        err = e; idx = ti;
-#pragma unroll
-        for (int c = 0; c < NC; c++) { q[0][c] = tq[0][c]; q[1][c] = tq[1][c]; }
-    }
-    return err;
-}
-// swap alpha with channel rot-1 (rot 0 = identity)
-ZT_DEV uint32_t rotate_px(uint32_t p, int rot) {
-    if (!rot) return p;
-    int s = 8 * (rot - 1);
-    uint32_t a = p >> 24, c = (p >> s) & 255u;
-    p &= ~(0xFFu << s) & 0x00FFFFFFu;
-    return p | a << s | c << 24;
-}
-ZT_DEV int mode5_rotation(const uint32_t px[16], int rot, int cq[2][3], int aq[2][3], uint32_t &ci, uint32_t &ai) {
-    uint32_t r[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) r[i] = rotate_px(px[i], rot);
-    float lo[4], hi[4], alo[1] = { 255.0f }, ahi[1] = { 0.0f };
-    pca<3, true>(r, 0xFFFFu, 16, lo, hi);
-    int err = part_mode5<3, 0>(r, 7, lo, hi, cq, ci);
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        float a = (float)(r[i] >> 24);
-        if (a < alo[0]) alo[0] = a;
-        if (a > ahi[0]) ahi[0] = a;
-    }
-    err += part_mode5<1, 3>(r, 8, alo, ahi, aq, ai);
-    return err;
-}
-ZT_DEV void pack_mode5(int rot, int cq[2][3], int aq[2][3], uint32_t ci, uint32_t ai, uint64_t &lo, uint64_t &hi) {
-    if (ci & 2) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) { int t = cq[0][c]; cq[0][c] = cq[1][c]; cq[1][c] = t; }
-        ci ^= 0xFFFFFFFFu;
-    }
-    if (ai & 2) { int t = aq[0][0]; aq[0][0] = aq[1][0]; aq[1][0] = t; ai ^= 0xFFFFFFFFu; }
-    Bits128 b{ 0x20ull | (uint64_t)rot << 6, 0 };
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int e = 0; e < 2; e++) b.put(8 + 7 * (2 * c + e), (uint64_t)cq[e][c], 7);
-    b.put(50, (uint64_t)aq[0][0], 8); b.put(58, (uint64_t)aq[1][0], 8);
-    uint64_t cf = (ci & 1) | ((ci >> 1) & ~1u), af = (ai & 1) | ((ai >> 1) & ~1u);  // pixel 0 keeps 1 bit
-    b.put(66, cf, 31); b.put(97, af, 31);
-    lo = b.lo; hi = b.hi;
-}
-
-// ---- mode-1 partition estimate (zap_tex.h:722-748)
-ZT_DEV float mode1_estimate(const uint32_t px[16], const float tot[9], int part) {
-    float s1[9];
-    int n1 = 0;
-    const uint32_t m = kP2[part];
-#pragma unroll
-    for (int j = 0; j < 9; j++) s1[j] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        if (!((m >> i) & 1)) continue;
-        float r = chanf(px[i], 0), g = chanf(px[i], 1), b = chanf(px[i], 2);
-        float v[9] = { r, g, b, r * r, g * g, b * b, r * g, r * b, g * b };
-        n1++;
-#pragma unroll
-        for (int j = 0; j < 9; j++) s1[j] += v[j];
-    }
-    float err = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        float v[9];
-#pragma unroll
-        for (int j = 0; j < 9; j++) v[j] = k ? s1[j] : tot[j] - s1[j];
-        float n = (float)(k ? n1 : 16 - n1);
-        if (n < 2) continue;
-        float S[3][3];
-        S[0][0] = v[3] - v[0] * v[0] / n; S[0][1] = v[6] - v[0] * v[1] / n; S[0][2] = v[7] - v[0] * v[2] / n;
-        S[1][1] = v[4] - v[1] * v[1] / n; S[1][2] = v[8] - v[1] * v[2] / n; S[2][2] = v[5] - v[2] * v[2] / n;
-        S[1][0] = S[0][1]; S[2][0] = S[0][2]; S[2][1] = S[1][2];
-        float tr = S[0][0] + S[1][1] + S[2][2], ax[3];
-        int r0 = S[0][0] >= S[1][1] && S[0][0] >= S[2][2] ? 0 : S[1][1] >= S[2][2] ? 1 : 2;
-#pragma unroll
-        for (int c = 0; c < 3; c++) ax[c] = r0 == 0 ? S[0][c] : r0 == 1 ? S[1][c] : S[2][c];
-        for (int it = 0; it < 3; it++) {
-            float t[3], mx = 1e-12f;
-#pragma unroll
-            for (int c = 0; c < 3; c++) { t[c] = S[c][0] * ax[0] + S[c][1] * ax[1] + S[c][2] * ax[2]; mx = t[c] > mx ? t[c] : -t[c] > mx ? -t[c] : mx; }
-#pragma unroll
-            for (int c = 0; c < 3; c++) ax[c] = t[c] / mx;
-        }
-        float vv = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2], vsv = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; c++) vsv += ax[c] * (S[c][0] * ax[0] + S[c][1] * ax[1] + S[c][2] * ax[2]);
-        float lam = vv > 1e-12f ? vsv / vv : 0;
-        err += tr - lam * (48.0f / 49.0f);
-    }
-    return err;
-}
-
-// ---- mode 1 fit of one partition (zap_tex.h:687-708); returns SSE, optionally packs
-ZT_DEV int mode1(const uint32_t px[16], int part, bool pack, uint64_t &lo, uint64_t &hi) {
-    const uint32_t m1 = kP2[part], m0 = ~m1 & 0xFFFFu;
-    int q[2][2][4], pb[2][2];
-    uint64_t idx[2];
-    int err = encode_subset<3, 7, 3, true, false>(px, m0, __popc(m0), q[0], pb[0], idx[0]);
-    err += encode_subset<3, 7, 3, true, false>(px, m1, __popc(m1), q[1], pb[1], idx[1]);
-    if (!pack) return err;
-    const int anc = kA2[part];
-    uint64_t full = idx[0] | idx[1];  // disjoint pixel sets, 4 bits per pixel
-    // anchor fix per subset: swap its endpoints and invert its indices (7 - i = i ^ 7 on the subset's pixels)
-    uint64_t sel0 = 0, sel1 = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) { if ((m0 >> i) & 1) sel0 |= 7ull << (4 * i); else sel1 |= 7ull << (4 * i); }
-    if (full & 4) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) { int t = q[0][0][c]; q[0][0][c] = q[0][1][c]; q[0][1][c] = t; }
-        full ^= sel0;
-    }
-    if ((full >> (4 * anc)) & 4) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) { int t = q[1][0][c]; q[1][0][c] = q[1][1][c]; q[1][1][c] = t; }
-        full ^= sel1;
-    }
-    Bits128 b{ 2ull | (uint64_t)part << 2, 0 };
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int s = 0; s < 2; s++)
-#pragma unroll
-            for (int e = 0; e < 2; e++) b.put(8 + 6 * (4 * c + 2 * s + e), (uint64_t)q[s][e][c], 6);
-    b.put(80, (uint64_t)pb[0][0], 1); b.put(81, (uint64_t)pb[1][0], 1);
-    int pos = 82;
-    for (int i = 0; i < 16; i++) {
-        int nb = 3 - (i == 0 || i == anc);
-        b.put(pos, (full >> (4 * i)) & 7, nb);
-        pos += nb;
-    }
-    lo = b.lo; hi = b.hi;
-    return err;
-}
-
-}  // namespace
 
 // ===================================================================================== kernels
 

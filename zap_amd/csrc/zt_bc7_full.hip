@@ -1,0 +1,347 @@
+// zt_bc7_full.hip — exhaustive all-mode BC7 search for gfx950, and the per-candidate encoder the learned
+// proposal uses.  Project-defined (the reference encoder only tries modes 6, 5 and 1): 285 candidates per
+// block, numbered mode*64 + slot (partition for modes 0-3 and 7, rot*2 + index selector for mode 4,
+// rotation for mode 5, 0 for mode 6).  Every subset is encoded the reference's way (PCA start, fit, <= 2
+// least-squares refinements); modes 6, 5 and 1 are the reference encoders, so the exhaustive result is never
+// worse than the reference search.  SSEs count all four channels of the decoded block, exactly.
+#include "zt_bc7_core.h"
+#include "zt_bc7_tables.h"
+#include "zt_kernels.h"
+
+namespace {
+
+// mode table (BC7 specification): subsets, partition bits, colour bits, alpha bits, endpoint p-bit, shared
+// p-bit, index bits
+template <int M> struct B7;
+template <> struct B7<0> { static constexpr int NS = 3, PB = 4, CB = 4, AB = 0, EPB = 1, SPB = 0, IB = 3; };
+template <> struct B7<1> { static constexpr int NS = 2, PB = 6, CB = 6, AB = 0, EPB = 0, SPB = 1, IB = 3; };
+template <> struct B7<2> { static constexpr int NS = 3, PB = 6, CB = 5, AB = 0, EPB = 0, SPB = 0, IB = 2; };
+template <> struct B7<3> { static constexpr int NS = 2, PB = 6, CB = 7, AB = 0, EPB = 1, SPB = 0, IB = 2; };
+template <> struct B7<6> { static constexpr int NS = 1, PB = 0, CB = 7, AB = 7, EPB = 1, SPB = 0, IB = 4; };
+template <> struct B7<7> { static constexpr int NS = 2, PB = 6, CB = 5, AB = 5, EPB = 1, SPB = 0, IB = 2; };
+
+// subset without p-bits: plain n-bit endpoints, palette argmin (idx: 4 bits per pixel)
+template <int NC, int N, int IB>
+ZT_DEV int fit_subset_plain(const uint32_t px[16], uint32_t mask, const float lo[4], const float hi[4], int q[2][4], uint64_t &idx) {
+    int e0[NC], e1[NC];
+    quant_plain<NC>(lo, N, q[0], e0); quant_plain<NC>(hi, N, q[1], e1);
+    constexpr int NP = 1 << IB;
+    uint32_t pal[NP];
+    int K[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        const int w = wgt<IB>(j);
+        uint32_t v = 0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
+        pal[j] = v;
+    }
+    palette_keys<NP>(pal, K);
+    int err = 0;
+    uint64_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (!((mask >> k) & 1)) continue;
+        int j;
+        err += best_entry<NP>(sub_px<NC, 0>(px[k]), pal, K, j);
+        bits |= (uint64_t)j << (4 * k);
+    }
+    idx = bits;
+    return err;
+}
+template <int NC, int N, int IB>
+ZT_DEV int encode_subset_plain(const uint32_t px[16], uint32_t mask, int cnt, int q[2][4], uint64_t &idx) {
+    float lo[4], hi[4];
+    pca<NC, false>(px, mask, cnt, lo, hi);
+    int err = fit_subset_plain<NC, N, IB>(px, mask, lo, hi, q, idx);
+    for (int it = 0; it < 2 && err > 0; it++) {
+        float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (!((mask >> k) & 1)) continue;
+            float w = (float)wgt<IB>((int)((idx >> (4 * k)) & 15)) / 64.0f, v = 1 - w;
+            aa += v * v; ab += v * w; bb += w * w;
+#pragma unroll
+            for (int c = 0; c < NC; c++) { sa[c] += v * chanf(px[k], c); sb[c] += w * chanf(px[k], c); }
+        }
+        float det = aa * bb - ab * ab;
+        if (det < 1e-6f && det > -1e-6f) break;
+        float nlo[4], nhi[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (c < NC) {
+                float l = (sa[c] * bb - sb[c] * ab) / det, h = (sb[c] * aa - sa[c] * ab) / det;
+                nlo[c] = l < 0 ? 0 : l > 255 ? 255 : l; nhi[c] = h < 0 ? 0 : h > 255 ? 255 : h;
+            } else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
+        }
+        int nq[2][4];
+        uint64_t ni;
+        int ne = fit_subset_plain<NC, N, IB>(px, mask, nlo, nhi, nq, ni);
+        if (ne >= err) break;
+        err = ne; idx = ni;
+#pragma unroll
+        for (int c = 0; c < NC; c++) { q[0][c] = nq[0][c]; q[1][c] = nq[1][c]; }
+    }
+    return err;
+}
+
+ZT_DEV uint32_t subset_mask(int ns, int part, int s) {
+    if (ns == 1) return 0xFFFFu;
+    if (ns == 2) { uint32_t m = kP2[part]; return s ? m : (~m & 0xFFFFu); }
+    uint32_t p3 = kP3[part], m = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) m |= (uint32_t)(((p3 >> (2 * i)) & 3) == (uint32_t)s) << i;
+    return m;
+}
+ZT_DEV int anchor_pixel(int ns, int part, int s) {
+    return s == 0 ? 0 : ns == 2 ? (int)kA2[part] : s == 1 ? (int)kA3a[part] : (int)kA3b[part];
+}
+// nibble mask selecting the low `bits` bits of every pixel in `mask`
+ZT_DEV uint64_t nibble_fill(uint32_t mask, int bits) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) if ((mask >> i) & 1) r |= (uint64_t)((1 << bits) - 1) << (4 * i);
+    return r;
+}
+
+// modes 0, 1, 2, 3, 6, 7 for one partition: SSE, packed block
+template <int M> ZT_DEV int cand_subsets(const uint32_t px[16], int part, uint64_t &olo, uint64_t &ohi) {
+    using T = B7<M>;
+    constexpr int NC = T::AB ? 4 : 3, HASP = T::EPB || T::SPB, N = T::CB + HASP;
+    int q[T::NS][2][4], pb[T::NS][2];
+    uint64_t full = 0;
+    uint32_t masks[T::NS];
+    int err = 0;
+#pragma unroll
+    for (int s = 0; s < T::NS; s++) {
+        masks[s] = subset_mask(T::NS, part, s);
+        const int cnt = __popc(masks[s]);
+        uint64_t idx;
+        pb[s][0] = pb[s][1] = 0;
+        if (HASP) err += encode_subset<NC, N, T::IB, (bool)T::SPB, false>(px, masks[s], cnt, q[s], pb[s], idx);
+        else err += encode_subset_plain<NC, N, T::IB>(px, masks[s], cnt, q[s], idx);
+        full |= idx;
+    }
+    if (NC == 3) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) { int d = 255 - (int)(px[i] >> 24); err += d * d; }  // decoded alpha is 255
+    }
+    int anc[T::NS];
+#pragma unroll
+    for (int s = 0; s < T::NS; s++) {
+        anc[s] = anchor_pixel(T::NS, part, s);
+        if ((full >> (4 * anc[s])) & (1u << (T::IB - 1))) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) { int t = q[s][0][c]; q[s][0][c] = q[s][1][c]; q[s][1][c] = t; }
+            if (T::EPB) { int t = pb[s][0]; pb[s][0] = pb[s][1]; pb[s][1] = t; }
+            full ^= nibble_fill(masks[s], T::IB);
+        }
+    }
+    Bits128 b{ 1ull << M, 0 };
+    int pos = M + 1;
+    b.put(pos, (uint64_t)part, T::PB); pos += T::PB;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int s = 0; s < T::NS; s++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) { b.put(pos, (uint64_t)q[s][e][c], T::CB); pos += T::CB; }
+    if (T::AB) {
+#pragma unroll
+        for (int s = 0; s < T::NS; s++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) { b.put(pos, (uint64_t)q[s][e][3], T::AB); pos += T::AB; }
+    }
+    if (T::EPB) {
+#pragma unroll
+        for (int s = 0; s < T::NS; s++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) { b.put(pos, (uint64_t)pb[s][e], 1); pos += 1; }
+    }
+    if (T::SPB) {
+#pragma unroll
+        for (int s = 0; s < T::NS; s++) { b.put(pos, (uint64_t)pb[s][0], 1); pos += 1; }
+    }
+    for (int i = 0; i < 16; i++) {
+        bool is_anchor = i == 0;
+#pragma unroll
+        for (int s = 1; s < T::NS; s++) is_anchor |= i == anc[s];
+        const int nb = T::IB - (is_anchor ? 1 : 0);
+        b.put(pos, (full >> (4 * i)) & ((1u << nb) - 1), nb);
+        pos += nb;
+    }
+    olo = b.lo; ohi = b.hi;
+    return err;
+}
+
+// n-bit endpoints, IB-bit indices on channels [C0, C0 + NC) of the rotated block (generalises the mode-5 part)
+template <int NC, int C0, int IB> ZT_DEV int fit_nbit(const uint32_t r[16], const int e0[], const int e1[], uint64_t &idx) {
+    constexpr int NP = 1 << IB;
+    uint32_t pal[NP];
+    int K[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        const int w = wgt<IB>(j);
+        uint32_t v = 0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
+        pal[j] = v;
+    }
+    palette_keys<NP>(pal, K);
+    int err = 0;
+    uint64_t bits = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int j;
+        err += best_entry<NP>(sub_px<NC, C0>(r[i]), pal, K, j);
+        bits |= (uint64_t)j << (4 * i);
+    }
+    idx = bits;
+    return err;
+}
+template <int NC, int C0, int IB> ZT_DEV bool lsq_nbit(const uint32_t r[16], uint64_t idx, float lo[], float hi[]) {
+    float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        float w = (float)wgt<IB>((int)((idx >> (4 * i)) & 15)) / 64.0f, v = 1 - w;
+        aa += v * v; ab += v * w; bb += w * w;
+#pragma unroll
+        for (int c = 0; c < NC; c++) { sa[c] += v * chanf(r[i], C0 + c); sb[c] += w * chanf(r[i], C0 + c); }
+    }
+    float det = aa * bb - ab * ab;
+    if (det < 1e-6f && det > -1e-6f) return false;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        float l = (sa[c] * bb - sb[c] * ab) / det, h = (sb[c] * aa - sa[c] * ab) / det;
+        lo[c] = l < 0 ? 0 : l > 255 ? 255 : l; hi[c] = h < 0 ? 0 : h > 255 ? 255 : h;
+    }
+    return true;
+}
+template <int NC, int C0, int IB> ZT_DEV int part_nbit(const uint32_t r[16], int n, const float lo[], const float hi[], int q[2][3], uint64_t &idx) {
+    int e0[NC], e1[NC];
+    quant_plain<NC>(lo, n, q[0], e0); quant_plain<NC>(hi, n, q[1], e1);
+    int err = fit_nbit<NC, C0, IB>(r, e0, e1, idx);
+    for (int it = 0; it < 2 && err > 0; it++) {
+        float nlo[NC], nhi[NC];
+        if (!lsq_nbit<NC, C0, IB>(r, idx, nlo, nhi)) break;
+        int tq[2][3], te0[NC], te1[NC];
+        quant_plain<NC>(nlo, n, tq[0], te0); quant_plain<NC>(nhi, n, tq[1], te1);
+        uint64_t ti;
+        int e = fit_nbit<NC, C0, IB>(r, te0, te1, ti);
+        if (e >= err) break;
+        err = e; idx = ti;
+#pragma unroll
+        for (int c = 0; c < NC; c++) { q[0][c] = tq[0][c]; q[1][c] = tq[1][c]; }
+    }
+    return err;
+}
+
+// modes 4 (ISB = index selector) and 5: rotation `rot`
+template <int M, int ISB> ZT_DEV int cand_rotation(const uint32_t px[16], int rot, uint64_t &olo, uint64_t &ohi) {
+    constexpr int CB = M == 4 ? 5 : 7, AB = M == 4 ? 6 : 8, IBC = (M == 4 && ISB) ? 3 : 2, IBA = (M == 4 && !ISB) ? 3 : 2;
+    uint32_t r[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) r[i] = rotate_px(px[i], rot);
+    float lo[4], hi[4], alo[1] = { 255.0f }, ahi[1] = { 0.0f };
+    int cq[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } }, aq[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
+    uint64_t ci, ai;
+    pca<3, true>(r, 0xFFFFu, 16, lo, hi);
+    int err = part_nbit<3, 0, IBC>(r, CB, lo, hi, cq, ci);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        float a = (float)(r[i] >> 24);
+        if (a < alo[0]) alo[0] = a;
+        if (a > ahi[0]) ahi[0] = a;
+    }
+    err += part_nbit<1, 3, IBA>(r, AB, alo, ahi, aq, ai);
+    if (ci & (1u << (IBC - 1))) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { int t = cq[0][c]; cq[0][c] = cq[1][c]; cq[1][c] = t; }
+        ci ^= nibble_fill(0xFFFFu, IBC);
+    }
+    if (ai & (1u << (IBA - 1))) { int t = aq[0][0]; aq[0][0] = aq[1][0]; aq[1][0] = t; ai ^= nibble_fill(0xFFFFu, IBA); }
+    Bits128 b{ 1ull << M, 0 };
+    int pos = M + 1;
+    b.put(pos, (uint64_t)rot, 2); pos += 2;
+    if (M == 4) { b.put(pos, (uint64_t)ISB, 1); pos += 1; }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) { b.put(pos, (uint64_t)cq[e][c], CB); pos += CB; }
+#pragma unroll
+    for (int e = 0; e < 2; e++) { b.put(pos, (uint64_t)aq[e][0], AB); pos += AB; }
+    const uint64_t first = ISB ? ai : ci, second = ISB ? ci : ai;   // the 2-bit index set is stored first
+    constexpr int B1 = 2, B2 = M == 4 ? 3 : 2;
+#pragma unroll
+    for (int i = 0; i < 16; i++) { const int nb = B1 - (i == 0); b.put(pos, (first >> (4 * i)) & ((1u << nb) - 1), nb); pos += nb; }
+#pragma unroll
+    for (int i = 0; i < 16; i++) { const int nb = B2 - (i == 0); b.put(pos, (second >> (4 * i)) & ((1u << nb) - 1), nb); pos += nb; }
+    olo = b.lo; ohi = b.hi;
+    return err;
+}
+
+}  // namespace
+
+// one candidate of one block (the learned proposal's exact evaluation); SSE, or -1 for an unused slot
+__device__ int zt_bc7_eval_candidate(const uint32_t px[16], int cand, uint64_t &lo, uint64_t &hi) {
+    const int mode = cand >> 6, slot = cand & 63;
+    switch (mode) {
+    case 0: return slot < 16 ? cand_subsets<0>(px, slot, lo, hi) : -1;
+    case 1: return cand_subsets<1>(px, slot, lo, hi);
+    case 2: return cand_subsets<2>(px, slot, lo, hi);
+    case 3: return cand_subsets<3>(px, slot, lo, hi);
+    case 4: return slot >= 8 ? -1 : (slot & 1) ? cand_rotation<4, 1>(px, slot >> 1, lo, hi) : cand_rotation<4, 0>(px, slot >> 1, lo, hi);
+    case 5: return slot < 4 ? cand_rotation<5, 0>(px, slot, lo, hi) : -1;
+    case 6: return slot == 0 ? cand_subsets<6>(px, 0, lo, hi) : -1;
+    default: return cand_subsets<7>(px, slot, lo, hi);
+    }
+}
+
+namespace {
+ZT_DEV void keep(int err, int cand, uint64_t lo, uint64_t hi, uint32_t &bkey, uint64_t &blo, uint64_t &bhi) {
+    uint32_t key = (uint32_t)err << 9 | (uint32_t)cand;  // SSE < 2^22: smallest SSE, then lowest candidate
+    if (err >= 0 && key < bkey) { bkey = key; blo = lo; bhi = hi; }
+}
+}  // namespace
+
+// Exhaustive search: one wave per block; within each mode the lanes take the partitions / rotations, so a
+// wave runs one mode's code at a time; a wave min over (SSE, candidate) picks the block.
+// `list`/`nlist` (optional) restrict the search to listed global blocks (the learned path's fallback).
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_full(zt_jobs jobs, uint32_t total, const uint32_t *list, const unsigned *nlist,
+                                                                uint32_t *best_cand) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t count = list ? *nlist : total;
+    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < count; e += nw) {
+        const uint32_t g = list ? list[e] : e;
+        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        const uint32_t b = g - J.first;
+        uint32_t px[16];
+        gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+        uint32_t bkey = 0xFFFFFFFFu;
+        uint64_t blo = 0, bhi = 0, lo, hi;
+        int err;
+        if (lane < 16) { err = cand_subsets<0>(px, lane, lo, hi); keep(err, lane, lo, hi, bkey, blo, bhi); }
+        err = cand_subsets<1>(px, lane, lo, hi); keep(err, 64 + lane, lo, hi, bkey, blo, bhi);
+        err = cand_subsets<2>(px, lane, lo, hi); keep(err, 128 + lane, lo, hi, bkey, blo, bhi);
+        err = cand_subsets<3>(px, lane, lo, hi); keep(err, 192 + lane, lo, hi, bkey, blo, bhi);
+        if (lane < 8) {
+            err = (lane & 1) ? cand_rotation<4, 1>(px, lane >> 1, lo, hi) : cand_rotation<4, 0>(px, lane >> 1, lo, hi);
+            keep(err, 256 + lane, lo, hi, bkey, blo, bhi);
+        }
+        if (lane < 4) { err = cand_rotation<5, 0>(px, lane, lo, hi); keep(err, 320 + lane, lo, hi, bkey, blo, bhi); }
+        if (lane == 0) { err = cand_subsets<6>(px, 0, lo, hi); keep(err, 384, lo, hi, bkey, blo, bhi); }
+        err = cand_subsets<7>(px, lane, lo, hi); keep(err, 448 + lane, lo, hi, bkey, blo, bhi);
+        uint32_t m = bkey;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = min(m, (uint32_t)__shfl_xor(m, off));
+        if (bkey == m) {
+            store16(J.dst + (uint64_t)b * 16, blo, bhi);
+            if (best_cand) best_cand[g] = m & 511u;
+        }
+    }
+}

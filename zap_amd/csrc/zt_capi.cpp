@@ -389,6 +389,30 @@ int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t w
     return enqueue(imgs, n, workspace, ws_bytes, (hipStream_t)stream);
 }
 
+int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, void *stream) {
+    if (!imgs || n < 1 || n > ZT_INLINE_JOBS) return fail(ZAPTEX_E_ARG, "image count %d outside 1..%d", n, ZT_INLINE_JOBS);
+    zt_jobs J;
+    memset(&J, 0, sizeof J);
+    uint32_t total = 0;
+    for (int i = 0; i < n; i++) {
+        if (int rc = check_image(imgs[i])) return rc;
+        if (imgs[i].fmt != ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "the exhaustive search encodes BC7 only");
+        zt_job &j = J.inl[J.n++];
+        j.src = (const uint8_t *)imgs[i].src; j.dst = (uint8_t *)imgs[i].dst; j.stride = imgs[i].stride;
+        j.first = total; j.w = imgs[i].w; j.h = imgs[i].h; j.bw = (imgs[i].w + 3) / 4; j.fmt = ZAPTEX_BC7; j.pad = 0;
+        total += nblocks(imgs[i]);
+    }
+    DeviceState *st;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (int rc = ensure_ready(st)) return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    return launch("zt_bc7_full", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_full, dim3(grid_for(total, 4)), dim3(256), 0, s, J, total, (const uint32_t *)nullptr, (const unsigned *)nullptr, best_cand);
+    });
+}
+
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream) {
     if (w < 1 || h < 1 || !src || !dst || stride < (size_t)w * 4 || ((uintptr_t)src | stride | (uintptr_t)dst | dst_stride) % 4)
         return fail(ZAPTEX_E_ARG, "bad arguments (rows must be 4-byte aligned)");

@@ -2,46 +2,10 @@
 // split / merge (zap_tex.h:1549-1571) for gfx950, one lane per block.
 #include "zt_common.h"
 #include "zt_kernels.h"
+#include "zt_bc7_tables.h"
 
 namespace {
 
-// BC7 spec: 3-subset partitions, 16 subset digits per partition (pixel 0 first), packed 2 bits per pixel
-constexpr uint32_t pack3(const char *s) {
-    uint32_t v = 0;
-    for (int i = 0; i < 16; i++) v |= (uint32_t)(s[i] - '0') << (2 * i);
-    return v;
-}
-__constant__ uint32_t kP3[64] = {
-    pack3("0011001102212222"), pack3("0001001122112221"), pack3("0000200122112211"), pack3("0222002200110111"),
-    pack3("0000000011221122"), pack3("0011001100220022"), pack3("0022002211111111"), pack3("0011001122112211"),
-    pack3("0000000011112222"), pack3("0000111111112222"), pack3("0000111122222222"), pack3("0012001200120012"),
-    pack3("0112011201120112"), pack3("0122012201220122"), pack3("0011011211221222"), pack3("0011200122002220"),
-    pack3("0001001101121122"), pack3("0111001120012200"), pack3("0000112211221122"), pack3("0022002200221111"),
-    pack3("0111011102220222"), pack3("0001000122212221"), pack3("0000001101220122"), pack3("0000110022102210"),
-    pack3("0122012200110000"), pack3("0012001211222222"), pack3("0110122112210110"), pack3("0000011012211221"),
-    pack3("0022110211020022"), pack3("0110011020022222"), pack3("0011012201220011"), pack3("0000200022112221"),
-    pack3("0000000211221222"), pack3("0222002200120011"), pack3("0011001200220222"), pack3("0120012001200120"),
-    pack3("0000111122220000"), pack3("0120120120120120"), pack3("0120201212010120"), pack3("0011220011220011"),
-    pack3("0011112222000011"), pack3("0101010122222222"), pack3("0000000021212121"), pack3("0022112200221122"),
-    pack3("0022001100220011"), pack3("0220122102201221"), pack3("0101222222220101"), pack3("0000212121212121"),
-    pack3("0101010101012222"), pack3("0222011102220111"), pack3("0002111200021112"), pack3("0000211221122112"),
-    pack3("0222011101110222"), pack3("0002111211120002"), pack3("0110011001102222"), pack3("0000000021122112"),
-    pack3("0110011022222222"), pack3("0022001100110022"), pack3("0022112211220022"), pack3("0000000000002112"),
-    pack3("0002000100020001"), pack3("0222122202221222"), pack3("0101222222222222"), pack3("0111201122012220") };
-__constant__ uint16_t kP2[64] = {
-    0xCCCC, 0x8888, 0xEEEE, 0xECC8, 0xC880, 0xFEEC, 0xFEC8, 0xEC80, 0xC800, 0xFFEC, 0xFE80, 0xE800, 0xFFE8, 0xFF00, 0xFFF0, 0xF000,
-    0xF710, 0x008E, 0x7100, 0x08CE, 0x008C, 0x7310, 0x3100, 0x8CCE, 0x088C, 0x3110, 0x6666, 0x366C, 0x17E8, 0x0FF0, 0x718E, 0x399C,
-    0xAAAA, 0xF0F0, 0x5A5A, 0x33CC, 0x3C3C, 0x55AA, 0x9696, 0xA55A, 0x73CE, 0x13C8, 0x324C, 0x3BDC, 0x6996, 0xC33C, 0x9966, 0x0660,
-    0x0272, 0x04E4, 0x4E40, 0x2720, 0xC936, 0x936C, 0x39C6, 0x639C, 0x9336, 0x9CC6, 0x817E, 0xE718, 0xCCF0, 0x0FCC, 0x7744, 0xEE22 };
-__constant__ uint8_t kA2[64] = {
-    15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 2, 8, 2, 2, 8, 8, 15, 2, 8, 2, 2, 8, 8, 2, 2,
-    15, 15, 6, 8, 2, 8, 15, 15, 2, 8, 2, 2, 2, 15, 15, 6, 6, 2, 6, 8, 15, 15, 2, 2, 15, 15, 15, 15, 15, 2, 2, 15 };
-__constant__ uint8_t kA3a[64] = {
-    3, 3, 15, 15, 8, 3, 15, 15, 8, 8, 6, 6, 6, 5, 3, 3, 3, 3, 8, 15, 3, 3, 6, 10, 5, 8, 8, 6, 8, 5, 15, 15,
-    8, 15, 3, 5, 6, 10, 8, 15, 15, 3, 15, 5, 15, 15, 15, 15, 3, 15, 5, 5, 5, 8, 5, 10, 5, 10, 8, 13, 15, 12, 3, 3 };
-__constant__ uint8_t kA3b[64] = {
-    15, 8, 8, 3, 15, 15, 3, 8, 15, 15, 15, 15, 15, 15, 15, 8, 15, 8, 15, 3, 15, 8, 15, 8, 3, 15, 6, 10, 15, 15, 10, 8,
-    15, 3, 15, 10, 10, 8, 9, 10, 6, 15, 8, 15, 3, 6, 6, 8, 15, 3, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 3, 15, 15, 8 };
 // per mode: subsets, partition bits, rotation bits, index-selector bits, colour bits, alpha bits,
 // per-endpoint p-bit, shared p-bit, index bits, secondary index bits
 __constant__ uint8_t kMode[8][10] = {
@@ -82,8 +46,8 @@ ZT_DEV void bc7_decode(const uint8_t *b, uint32_t out[16]) {
         for (int c = 0; c < 3; c++) ep[e][c] = expand(ep[e][c], cb);
         if (ab) ep[e][3] = expand(ep[e][3], ab);
     }
-    auto subset = [&](int i) { return ns == 1 ? 0 : ns == 2 ? (int)((kP2[part] >> i) & 1) : (int)((kP3[part] >> (2 * i)) & 3); };
-    auto anchor = [&](int s) { return s == 0 ? 0 : ns == 2 ? (int)kA2[part] : s == 1 ? (int)kA3a[part] : (int)kA3b[part]; };
+    auto subset = [&](int i) { return ns == 1 ? 0 : ns == 2 ? (int)((kP2all[part] >> i) & 1) : (int)((kP3[part] >> (2 * i)) & 3); };
+    auto anchor = [&](int s) { return s == 0 ? 0 : ns == 2 ? (int)kA2all[part] : s == 1 ? (int)kA3a[part] : (int)kA3b[part]; };
     int i1[16], i2[16];
     for (int i = 0; i < 16; i++) i1[i] = R.take(ib - (i == anchor(subset(i))));
     for (int i = 0; i < 16; i++) i2[i] = ib2 ? R.take(ib2 - (i == 0)) : 0;

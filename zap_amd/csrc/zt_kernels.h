@@ -19,6 +19,8 @@ __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
 __global__ void zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L);
 __global__ void zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k);
 __global__ void zt_zero_u32(unsigned *p, int n);
+// exhaustive all-mode BC7 search (optionally restricted to listed global blocks); best_cand may be null
+__global__ void zt_bc7_full(zt_jobs jobs, uint32_t total, const uint32_t *list, const unsigned *nlist, uint32_t *best_cand);
 
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
