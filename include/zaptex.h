@@ -80,6 +80,31 @@ int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t w
  * 285 candidates per block (mode*64 + slot); never worse than zaptex_encode_dev's reference search.
  * best_cand (device, one uint32 per block of all images in order, may be NULL) receives the chosen candidate. */
 int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, void *stream);
+/* ---------------- learned BC7 proposal (no reference counterpart)
+ * A 64 -> 128 -> 128 -> 512 ReLU MLP (BF16 on MFMA, FP32 accumulate) scores the exhaustive search's candidate
+ * slots from the block's pixels; the top-k are encoded exactly and the best kept; blocks whose top-1
+ * probability is below `threshold` are searched exhaustively.  The network state is one device buffer of
+ * zaptex_mlp_state_size() bytes: fp32 parameters [W1 64x128 | b1 | W2 128x128 | b2 | W3 128x512 | b3],
+ * gradients, Adam moments, tables and the bf16 weight copies. */
+size_t zaptex_mlp_state_size(void);
+size_t zaptex_mlp_workspace_size(int batch);
+int zaptex_mlp_init(void *state, uint32_t seed, void *stream);
+/* load fp32 parameters (90,880 floats, host memory) into a state buffer */
+int zaptex_mlp_set_params(void *state, const float *host_params, void *stream);
+/* logits [batch x 512] for batch blocks of 16 RGBA8 pixels (64 bytes each) */
+int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *logits, void *ws, size_t ws_bytes, void *stream);
+/* mean cross-entropy gradients into the state's gradient area; stats (device, 2 floats) += (loss, top-1 hits) */
+int zaptex_mlp_gradients(void *state, const void *blocks, const uint32_t *labels, int batch, void *ws, size_t ws_bytes, float *stats,
+                         void *stream);
+/* gradients then one Adam step (step counts from 1) */
+int zaptex_mlp_train_step(void *state, const void *blocks, const uint32_t *labels, int batch, float lr, int step, void *ws,
+                          size_t ws_bytes, float *stats, void *stream);
+/* every block of the images as 64 contiguous bytes (training inputs) */
+int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *stream);
+size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int topk);
+int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, void *ws, size_t ws_bytes,
+                           void *stream);
+
 /* zap_mip_next on device memory */
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream);
 /* zap_mip_next_f on device memory (strides in floats) */

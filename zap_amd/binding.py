@@ -97,6 +97,12 @@ class Library:
             "zaptex_mip_chain_dev": [vp, i, i, sz, i, C.POINTER(vp), i, vp],
             "zaptex_bc_decode_dev": [vp, i, i, i, vp, sz, vp],
             "zaptex_bc7_full_dev": [C.POINTER(ZapImage), i, vp, vp],
+            "zaptex_mlp_init": [vp, C.c_uint32, vp], "zaptex_mlp_set_params": [vp, vp, vp],
+            "zaptex_mlp_forward": [vp, vp, i, vp, vp, sz, vp],
+            "zaptex_mlp_gradients": [vp, vp, vp, i, vp, sz, vp, vp],
+            "zaptex_mlp_train_step": [vp, vp, vp, i, f, i, vp, sz, vp, vp],
+            "zaptex_bc7_gather_dev": [C.POINTER(ZapImage), i, vp, vp],
+            "zaptex_bc7_learned_dev": [C.POINTER(ZapImage), i, vp, i, f, vp, sz, vp],
             "zaptex_set_option": [i, i], "zaptex_get_option": [i],
         }
         for name, args in sig.items():
@@ -107,6 +113,10 @@ class Library:
         L.zaptex_profile_get.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.zaptex_profile_get.restype = C.c_int
         L.zaptex_profile_reset.restype = None
+        L.zaptex_mlp_state_size.restype = C.c_size_t
+        L.zaptex_mlp_workspace_size.argtypes = [i]; L.zaptex_mlp_workspace_size.restype = C.c_size_t
+        L.zaptex_bc7_learned_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
+        L.zaptex_bc7_learned_workspace_size.restype = C.c_size_t
         L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
         L.zaptex_workspace_size.restype = C.c_size_t
         L.zaptex_last_error.restype = C.c_char_p
@@ -231,6 +241,60 @@ class Library:
     def bc_decode_dev(self, bc, w: int, h: int, fmt, out, stream=None) -> None:
         self._check(self.lib.zaptex_bc_decode_dev(bc.data_ptr(), w, h, fmt_id(fmt), out.data_ptr(), w * 4, self._stream(stream)),
                     "zaptex_bc_decode_dev")
+
+    # ------------------------------------------------------------------ learned proposal (torch tensors on the GPU)
+    MLP_PARAMS = 90880
+
+    def mlp_state(self, seed: int | None = 1, params: np.ndarray | None = None):
+        import torch
+        st = torch.zeros(int(self.lib.zaptex_mlp_state_size()), dtype=torch.uint8, device="cuda")
+        if params is not None:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            assert p.size == self.MLP_PARAMS
+            self._check(self.lib.zaptex_mlp_init(st.data_ptr(), 1, self._stream()), "zaptex_mlp_init")
+            self._check(self.lib.zaptex_mlp_set_params(st.data_ptr(), _ptr(p), self._stream()), "zaptex_mlp_set_params")
+        else:
+            self._check(self.lib.zaptex_mlp_init(st.data_ptr(), seed, self._stream()), "zaptex_mlp_init")
+        return st
+
+    @staticmethod
+    def mlp_params(state) -> np.ndarray:
+        import torch
+        return state[: Library.MLP_PARAMS * 4].view(torch.float32).cpu().numpy().copy()
+
+    @staticmethod
+    def mlp_grads(state) -> np.ndarray:
+        import torch
+        n = Library.MLP_PARAMS * 4
+        return state[n: 2 * n].view(torch.float32).cpu().numpy().copy()
+
+    def mlp_workspace(self, batch: int):
+        import torch
+        return torch.empty(int(self.lib.zaptex_mlp_workspace_size(batch)), dtype=torch.uint8, device="cuda")
+
+    def mlp_forward(self, state, blocks, batch: int, logits, ws) -> None:
+        self._check(self.lib.zaptex_mlp_forward(state.data_ptr(), blocks.data_ptr(), batch, logits.data_ptr(), ws.data_ptr(),
+                                                ws.numel(), self._stream()), "zaptex_mlp_forward")
+
+    def mlp_gradients(self, state, blocks, labels, batch: int, ws, stats) -> None:
+        self._check(self.lib.zaptex_mlp_gradients(state.data_ptr(), blocks.data_ptr(), labels.data_ptr(), batch, ws.data_ptr(), ws.numel(),
+                                                  stats.data_ptr(), self._stream()), "zaptex_mlp_gradients")
+
+    def mlp_train_step(self, state, blocks, labels, batch: int, lr: float, step: int, ws, stats) -> None:
+        self._check(self.lib.zaptex_mlp_train_step(state.data_ptr(), blocks.data_ptr(), labels.data_ptr(), batch, lr, step, ws.data_ptr(),
+                                                   ws.numel(), stats.data_ptr(), self._stream()), "zaptex_mlp_train_step")
+
+    def bc7_gather_dev(self, imgs, blocks) -> None:
+        self._check(self.lib.zaptex_bc7_gather_dev(imgs, len(imgs), blocks.data_ptr(), self._stream()), "zaptex_bc7_gather_dev")
+
+    def bc7_learned_workspace(self, imgs, topk: int):
+        import torch
+        n = int(self.lib.zaptex_bc7_learned_workspace_size(imgs, len(imgs), topk))
+        return torch.empty(n, dtype=torch.uint8, device="cuda")
+
+    def bc7_learned_dev(self, imgs, state, topk: int, threshold: float, ws, stream=None) -> None:
+        self._check(self.lib.zaptex_bc7_learned_dev(imgs, len(imgs), state.data_ptr(), topk, threshold, ws.data_ptr(), ws.numel(),
+                                                    self._stream(stream)), "zaptex_bc7_learned_dev")
 
     # ------------------------------------------------------------------ kernel timing
     def profile(self, on: bool) -> None:

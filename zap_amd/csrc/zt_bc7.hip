@@ -141,3 +141,8 @@ extern "C" __global__ void zt_zero_u32(unsigned *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
+
+extern "C" __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = v;
+}

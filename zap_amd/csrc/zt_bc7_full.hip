@@ -345,3 +345,88 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_full(zt_jobs jobs, uint
         }
     }
 }
+
+// ===================================================================================== learned proposal
+// The network proposes K candidates per block (zt_mlp.hip); they are bucketed by mode so that every wave of
+// the exact evaluation runs one mode's code, evaluated exactly, and the best of the K is kept.  Blocks whose
+// top-1 probability is below the threshold are searched exhaustively instead (zt_bc7_full over a list).
+
+// gather every block of the jobs into a dense array (16 RGBA8 pixels = 4 x uint4 per block)
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        const uint32_t b = g - J.first;
+        uint32_t px[16];
+        gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+        uint4 *d = blocks + (uint64_t)g * 4;
+        d[0] = make_uint4(px[0], px[1], px[2], px[3]); d[1] = make_uint4(px[4], px[5], px[6], px[7]);
+        d[2] = make_uint4(px[8], px[9], px[10], px[11]); d[3] = make_uint4(px[12], px[13], px[14], px[15]);
+    }
+}
+
+// pass 0: count proposals per mode (counters[0..7]); pass 1: scatter them into mode-contiguous runs starting
+// at counters[16 + mode] (prefix sums filled in between by zt_bc7_bucket_scan).  Low-confidence blocks go
+// to the fallback list (counters[8]) and get no proposals; their K keys are set to "none".
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *cand, const float *conf, uint32_t total, int K,
+                                                                  float threshold, unsigned *counters, uint4 *items, uint32_t pass,
+                                                                  uint32_t *fallback, uint32_t *keys) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += step) {
+        const uint32_t g = base + threadIdx.x;
+        const bool valid = g < total;
+        const bool fb = valid && conf[g] < threshold;
+        if (pass == 1) {
+            int slot = wave_append(fb, &counters[8]);
+            if (slot >= 0) {
+                fallback[slot] = g;
+                for (int k = 0; k < K; k++) keys[(uint64_t)g * K + k] = 0xFFFFFFFFu;
+            }
+        }
+        for (int k = 0; k < K; k++) {
+            const uint32_t c = valid && !fb ? cand[(uint64_t)g * K + k] : 0xFFFFFFFFu;
+            for (int m = 0; m < 8; m++) {
+                const bool p = c != 0xFFFFFFFFu && (int)(c >> 6) == m;
+                int slot = wave_append(p, pass == 0 ? &counters[m] : &counters[16 + m]);
+                if (pass == 1 && slot >= 0) items[slot] = make_uint4(g, (uint32_t)k, c, 0);
+            }
+        }
+    }
+}
+extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned s = 0;
+        for (int m = 0; m < 8; m++) { counters[24 + m] = s; counters[16 + m] = s; s += counters[m]; }
+    }
+}
+
+// exact evaluation of the proposals of one mode: key = SSE << 9 | candidate, and the block bits, per (block, k)
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_eval_items(const uint4 *blocks, const uint4 *items, const unsigned *counters, int mode,
+                                                                      uint32_t *keys, uint4 *bits, int K) {
+    const uint32_t begin = counters[24 + mode], n = counters[mode], step = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint4 it = items[begin + i];
+        uint32_t px[16];
+        const uint4 *s = blocks + (uint64_t)it.x * 4;
+#pragma unroll
+        for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+        uint64_t lo = 0, hi = 0;
+        int e = zt_bc7_eval_candidate(px, (int)it.z, lo, hi);
+        const uint64_t o = (uint64_t)it.x * K + it.y;
+        keys[o] = e < 0 ? 0xFFFFFFFFu : ((uint32_t)e << 9 | it.z);
+        bits[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    }
+}
+
+// best of the K evaluated proposals per block (smallest SSE, then lowest candidate); fallback blocks skipped
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_select(zt_jobs jobs, uint32_t total, const uint32_t *keys, const uint4 *bits, int K) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        uint32_t best = 0xFFFFFFFFu;
+        int bk = 0;
+        for (int k = 0; k < K; k++) { uint32_t v = keys[(uint64_t)g * K + k]; if (v < best) { best = v; bk = k; } }
+        if (best == 0xFFFFFFFFu) continue;
+        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        *(uint4 *)(J.dst + (uint64_t)(g - J.first) * 16) = bits[(uint64_t)g * K + bk];
+    }
+}

@@ -413,6 +413,292 @@ int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, vo
     });
 }
 
+// ===================================================================================== proposal network
+namespace {
+// parameters (fp32 master, weights stored [in x out]): W1 64x128, b1, W2 128x128, b2, W3 128x512, b3
+constexpr int kIn = 64, kH = 128, kOut = 512;
+constexpr int64_t oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH * kOut, kP = oB3 + kOut;
+// bf16 weight copies, stored transposed [out x in] so forward GEMM operands stream along k
+constexpr int64_t bW1 = 0, bW2 = bW1 + kIn * kH, bW3 = bW2 + kH * kH, kPb = bW3 + kH * kOut;
+struct MlpState {  // views into the caller's state buffer
+    float *p, *g, *m1, *m2, *scale;
+    int *bfi;
+    uint16_t *w;
+};
+MlpState mlp_view(void *state) {
+    uint8_t *b = (uint8_t *)state;
+    MlpState v;
+    v.p = (float *)b; v.g = v.p + kP; v.m1 = v.g + kP; v.m2 = v.m1 + kP; v.scale = v.m2 + kP;
+    v.bfi = (int *)(v.scale + kP);
+    v.w = (uint16_t *)(v.bfi + kP);
+    return v;
+}
+size_t mlp_state_bytes() { return (size_t)kP * 4 * 6 + (size_t)kPb * 2; }
+struct MlpWs { uint16_t *X, *H1, *H2, *dZ, *dH2, *dH1; float *Z, *stats; };
+size_t mlp_ws_bytes(int B) { return align_up((size_t)B * (kIn + 2 * kH + kOut + 2 * kH) * 2 + (size_t)B * kOut * 4 + 256, 256); }
+MlpWs mlp_ws(void *ws, int B) {
+    uint8_t *b = (uint8_t *)ws;
+    MlpWs v;
+    v.stats = (float *)b; b += 256;
+    v.Z = (float *)b; b += (size_t)B * kOut * 4;
+    v.X = (uint16_t *)b; b += (size_t)B * kIn * 2;
+    v.H1 = (uint16_t *)b; b += (size_t)B * kH * 2;
+    v.H2 = (uint16_t *)b; b += (size_t)B * kH * 2;
+    v.dZ = (uint16_t *)b; b += (size_t)B * kOut * 2;
+    v.dH2 = (uint16_t *)b; b += (size_t)B * kH * 2;
+    v.dH1 = (uint16_t *)b;
+    return v;
+}
+zt_gemm_args gemm(const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B, int64_t sbk, int64_t sbn, int M, int N, int K) {
+    zt_gemm_args g;
+    memset(&g, 0, sizeof g);
+    g.A = A; g.sam = sam; g.sak = sak; g.B = B; g.sbk = sbk; g.sbn = sbn; g.M = M; g.N = N; g.K = K;
+    return g;
+}
+int run_gemm(const zt_gemm_args &g, int ksplit, hipStream_t s) {
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, ksplit);
+    return launch("zt_gemm_bf16", s, [&] { hipLaunchKernelGGL(zt_gemm_bf16, grid, dim3(256), 0, s, g); });
+}
+// forward pass on B blocks (16 RGBA8 pixels each) -> Z logits (and X, H1, H2 kept for backward)
+int mlp_forward(const MlpState &st, const void *blocks, int B, const MlpWs &w, hipStream_t s) {
+    int rc;
+    if ((rc = launch("zt_mlp_features", s, [&] {
+            hipLaunchKernelGGL(zt_mlp_features, dim3((unsigned)(((int64_t)B * 64 + 255) / 256)), dim3(256), 0, s, (const uint8_t *)blocks, B, w.X); })))
+        return rc;
+    zt_gemm_args g = gemm(w.X, kIn, 1, st.w + bW1, 1, kIn, B, kH, kIn);
+    g.bias = st.p + oB1; g.relu = 1; g.Cb = w.H1; g.ldc = kH;
+    if ((rc = run_gemm(g, 1, s))) return rc;
+    g = gemm(w.H1, kH, 1, st.w + bW2, 1, kH, B, kH, kH);
+    g.bias = st.p + oB2; g.relu = 1; g.Cb = w.H2; g.ldc = kH;
+    if ((rc = run_gemm(g, 1, s))) return rc;
+    g = gemm(w.H2, kH, 1, st.w + bW3, 1, kH, B, kOut, kH);
+    g.bias = st.p + oB3; g.Cf = w.Z; g.ldc = kOut;
+    return run_gemm(g, 1, s);
+}
+// loss and gradients of the mean cross-entropy into st.g (zeroed first); stats = (loss, top-1 hits)
+int mlp_backward(const MlpState &st, const uint32_t *labels, int B, const MlpWs &w, hipStream_t s) {
+    int rc;
+    if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(256), dim3(256), 0, s, (uint32_t *)st.g, (uint64_t)kP, 0u); })))
+        return rc;
+    if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)w.stats, (uint64_t)64, 0u); })))
+        return rc;
+    if ((rc = launch("zt_mlp_xent", s, [&] { hipLaunchKernelGGL(zt_mlp_xent, dim3((B + 3) / 4), dim3(256), 0, s, w.Z, labels, B, w.dZ, w.stats); })))
+        return rc;
+    const int ks = std::max(1, std::min(64, B / 256));
+    auto wgrad = [&](const uint16_t *act, int in, const uint16_t *d, int out, int64_t off) {  // dW = act^T d
+        zt_gemm_args g = gemm(act, 1, in, d, out, 1, in, out, B);
+        g.Cf = st.g + off; g.ldc = out; g.accumulate = 1;
+        return run_gemm(g, ks, s);
+    };
+    auto bgrad = [&](const uint16_t *d, int out, int64_t off) {
+        return launch("zt_mlp_colsum", s, [&] {
+            hipLaunchKernelGGL(zt_mlp_colsum, dim3((out + 127) / 128, std::max(1, std::min(256, B / 64))), dim3(128), 0, s, d, B, out, st.g + off); });
+    };
+    auto dgrad = [&](const uint16_t *d, int out, const uint16_t *wt, int in, const uint16_t *act, uint16_t *dst) {  // (d W^T) * (act > 0)
+        zt_gemm_args g = gemm(d, out, 1, wt, in, 1, B, in, out);
+        g.Cb = dst; g.ldc = in; g.mask = act; g.ldm = in;
+        return run_gemm(g, 1, s);
+    };
+    if ((rc = wgrad(w.H2, kH, w.dZ, kOut, oW3))) return rc;
+    if ((rc = bgrad(w.dZ, kOut, oB3))) return rc;
+    if ((rc = dgrad(w.dZ, kOut, st.w + bW3, kH, w.H2, w.dH2))) return rc;
+    if ((rc = wgrad(w.H1, kH, w.dH2, kH, oW2))) return rc;
+    if ((rc = bgrad(w.dH2, kH, oB2))) return rc;
+    if ((rc = dgrad(w.dH2, kH, st.w + bW2, kH, w.H1, w.dH1))) return rc;
+    if ((rc = wgrad(w.X, kIn, w.dH1, kH, oW1))) return rc;
+    return bgrad(w.dH1, kH, oB1);
+}
+}  // namespace
+
+size_t zaptex_mlp_state_size(void) { return mlp_state_bytes(); }
+size_t zaptex_mlp_workspace_size(int batch) { return batch > 0 ? mlp_ws_bytes(batch) : 0; }
+
+int zaptex_mlp_init(void *state, uint32_t seed, void *stream) {
+    if (!state) return fail(ZAPTEX_E_ARG, "null state");
+    DeviceState *dst;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (int rc = ensure_ready(dst)) return rc;
+    }
+    MlpState st = mlp_view(state);
+    std::vector<int> bfi(kP, -1);
+    std::vector<float> scale(kP, 0.0f);
+    struct L { int64_t off, boff; int in, out; } layers[3] = { { oW1, bW1, kIn, kH }, { oW2, bW2, kH, kH }, { oW3, bW3, kH, kOut } };
+    for (const L &l : layers)
+        for (int i = 0; i < l.in; i++)
+            for (int o = 0; o < l.out; o++) {
+                bfi[l.off + (int64_t)i * l.out + o] = (int)(l.boff + (int64_t)o * l.in + i);
+                scale[l.off + (int64_t)i * l.out + o] = std::sqrt(6.0f / (float)l.in);
+            }
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(st.bfi, bfi.data(), kP * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(st.scale, scale.data(), kP * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int rc;
+    if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(512), dim3(256), 0, s, (uint32_t *)st.g, (uint64_t)kP * 3, 0u); })))
+        return rc;
+    return launch("zt_mlp_init", s, [&] {
+        hipLaunchKernelGGL(zt_mlp_init, dim3((unsigned)((kP + 255) / 256)), dim3(256), 0, s, st.p, (int)kP, seed, st.w, (const int *)st.bfi, (const float *)st.scale); });
+}
+
+int zaptex_mlp_set_params(void *state, const float *host_params, void *stream) {  // load fp32 master weights; refreshes bf16 copies
+    if (!state || !host_params) return fail(ZAPTEX_E_ARG, "null argument");
+    MlpState st = mlp_view(state);
+    {  // index tables (as zaptex_mlp_init writes them)
+        std::vector<int> bfi(kP, -1);
+        struct L { int64_t off, boff; int in, out; } layers[3] = { { oW1, bW1, kIn, kH }, { oW2, bW2, kH, kH }, { oW3, bW3, kH, kOut } };
+        for (const L &l : layers)
+            for (int i = 0; i < l.in; i++)
+                for (int o = 0; o < l.out; o++) bfi[l.off + (int64_t)i * l.out + o] = (int)(l.boff + (int64_t)o * l.in + i);
+        HIP_TRY(hipMemcpy(st.bfi, bfi.data(), kP * 4, hipMemcpyHostToDevice));
+    }
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(st.p, host_params, kP * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return launch("zt_mlp_refresh", s, [&] {
+        hipLaunchKernelGGL(zt_mlp_refresh, dim3((unsigned)((kP + 255) / 256)), dim3(256), 0, s, (const float *)st.p, (int)kP, st.w, (const int *)st.bfi);
+    });
+}
+
+int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *logits, void *ws, size_t ws_bytes, void *stream) {
+    if (!state || !blocks || batch < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
+    if (!ws || ws_bytes < mlp_ws_bytes(batch)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, mlp_ws_bytes(batch));
+    MlpState st = mlp_view((void *)state);
+    MlpWs w = mlp_ws(ws, batch);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = mlp_forward(st, blocks, batch, w, s)) return rc;
+    if (logits) HIP_TRY(hipMemcpyAsync(logits, w.Z, (size_t)batch * kOut * 4, hipMemcpyDeviceToDevice, s));
+    return ZAPTEX_OK;
+}
+
+int zaptex_mlp_gradients(void *state, const void *blocks, const uint32_t *labels, int batch, void *ws, size_t ws_bytes, float *stats, void *stream) {
+    if (!state || !blocks || !labels || batch < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
+    if (!ws || ws_bytes < mlp_ws_bytes(batch)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, mlp_ws_bytes(batch));
+    MlpState st = mlp_view(state);
+    MlpWs w = mlp_ws(ws, batch);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = mlp_forward(st, blocks, batch, w, s)) return rc;
+    if (int rc = mlp_backward(st, labels, batch, w, s)) return rc;
+    if (stats) HIP_TRY(hipMemcpyAsync(stats, w.stats, 8, hipMemcpyDeviceToDevice, s));
+    return ZAPTEX_OK;
+}
+
+int zaptex_mlp_train_step(void *state, const void *blocks, const uint32_t *labels, int batch, float lr, int step, void *ws, size_t ws_bytes,
+                          float *stats, void *stream) {
+    if (int rc = zaptex_mlp_gradients(state, blocks, labels, batch, ws, ws_bytes, stats, stream)) return rc;
+    MlpState st = mlp_view(state);
+    hipStream_t s = (hipStream_t)stream;
+    const float b1 = 0.9f, b2 = 0.999f, c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+    return launch("zt_mlp_adam", s, [&] {
+        hipLaunchKernelGGL(zt_mlp_adam, dim3((unsigned)((kP + 255) / 256)), dim3(256), 0, s, st.p, (const float *)st.g, st.m1, st.m2, (int)kP, lr,
+                           b1, b2, c1, c2, 1e-8f, st.w, (const int *)st.bfi);
+    });
+}
+
+int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *stream) {
+    if (!imgs || n < 1 || n > ZT_INLINE_JOBS || !blocks) return fail(ZAPTEX_E_ARG, "bad arguments");
+    zt_jobs J;
+    memset(&J, 0, sizeof J);
+    uint32_t total = 0;
+    for (int i = 0; i < n; i++) {
+        if (int rc = check_image(imgs[i])) return rc;
+        zt_job &j = J.inl[J.n++];
+        j.src = (const uint8_t *)imgs[i].src; j.dst = (uint8_t *)imgs[i].dst; j.stride = imgs[i].stride;
+        j.first = total; j.w = imgs[i].w; j.h = imgs[i].h; j.bw = (imgs[i].w + 3) / 4; j.fmt = imgs[i].fmt; j.pad = 0;
+        total += nblocks(imgs[i]);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    return launch("zt_bc7_gather", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, (uint4 *)blocks); });
+}
+
+// ---- learned-proposal BC7 encode
+namespace {
+constexpr int kChunk = 1 << 17;  // network rows per forward pass
+struct LearnedWs { uint4 *blocks, *items, *bits; uint32_t *cand, *fallback, *keys; float *conf; unsigned *counters; void *mlp; };
+size_t learned_ws_bytes(uint64_t total, int K) {
+    return 256 + total * (64 + 4 * K + 4 + 16 * K + 4 + 4 * K + 16 * K) + 1024 + mlp_ws_bytes((int)std::min<uint64_t>(total, kChunk));
+}
+LearnedWs learned_ws(void *ws, uint64_t total, int K) {
+    uint8_t *b = (uint8_t *)ws;
+    LearnedWs v;
+    v.counters = (unsigned *)b; b += 256;
+    v.blocks = (uint4 *)b; b += total * 64;
+    v.items = (uint4 *)b; b += total * 16 * K;
+    v.bits = (uint4 *)b; b += total * 16 * K;
+    v.cand = (uint32_t *)b; b += total * 4 * K;
+    v.keys = (uint32_t *)b; b += total * 4 * K;
+    v.conf = (float *)b; b += total * 4;
+    v.fallback = (uint32_t *)b; b += total * 4;
+    v.mlp = (void *)align_up((uintptr_t)b, 256);
+    return v;
+}
+}  // namespace
+
+size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int topk) {
+    uint64_t t = 0;
+    for (int i = 0; i < n; i++) t += nblocks(imgs[i]);
+    return learned_ws_bytes(t, topk);
+}
+
+int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, void *ws, size_t ws_bytes,
+                           void *stream) {
+    if (!imgs || n < 1 || n > ZT_INLINE_JOBS || !state) return fail(ZAPTEX_E_ARG, "bad arguments");
+    if (topk < 1 || topk > 16) return fail(ZAPTEX_E_ARG, "topk must be 1..16");
+    zt_jobs J;
+    memset(&J, 0, sizeof J);
+    uint32_t total = 0;
+    for (int i = 0; i < n; i++) {
+        if (int rc = check_image(imgs[i])) return rc;
+        if (imgs[i].fmt != ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "the learned proposal encodes BC7 only");
+        zt_job &j = J.inl[J.n++];
+        j.src = (const uint8_t *)imgs[i].src; j.dst = (uint8_t *)imgs[i].dst; j.stride = imgs[i].stride;
+        j.first = total; j.w = imgs[i].w; j.h = imgs[i].h; j.bw = (imgs[i].w + 3) / 4; j.fmt = ZAPTEX_BC7; j.pad = 0;
+        total += nblocks(imgs[i]);
+    }
+    if (!ws || ws_bytes < learned_ws_bytes(total, topk)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, learned_ws_bytes(total, topk));
+    DeviceState *dst;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (int rc = ensure_ready(dst)) return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    LearnedWs w = learned_ws(ws, total, topk);
+    MlpState st = mlp_view((void *)state);
+    int rc;
+    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, w.counters, 64); }))) return rc;
+    if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, w.blocks); })))
+        return rc;
+    for (uint32_t c0 = 0; c0 < total; c0 += kChunk) {
+        const int B = (int)std::min<uint32_t>(kChunk, total - c0);
+        MlpWs mw = mlp_ws(w.mlp, B);
+        if ((rc = mlp_forward(st, w.blocks + (uint64_t)c0 * 4, B, mw, s))) return rc;
+        if ((rc = launch("zt_mlp_topk", s, [&] {
+                hipLaunchKernelGGL(zt_mlp_topk, dim3((B + 3) / 4), dim3(256), 0, s, mw.Z, B, topk, w.cand + (uint64_t)c0 * topk, w.conf + c0); })))
+            return rc;
+    }
+    for (uint32_t pass = 0; pass < 2; pass++) {
+        if ((rc = launch("zt_bc7_bucket", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_bucket, dim3(grid_for(total, 256)), dim3(256), 0, s, (const uint32_t *)w.cand, (const float *)w.conf, total, topk,
+                                   threshold, w.counters, w.items, pass, w.fallback, w.keys); })))
+            return rc;
+        if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters); })))
+            return rc;
+    }
+    for (int m = 0; m < 8; m++)
+        if ((rc = launch("zt_bc7_eval_items", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_eval_items, dim3(grid_for((uint64_t)total * topk, 256)), dim3(256), 0, s, (const uint4 *)w.blocks,
+                                   (const uint4 *)w.items, (const unsigned *)w.counters, m, w.keys, w.bits, topk); })))
+            return rc;
+    if ((rc = launch("zt_bc7_select", s, [&] {
+            hipLaunchKernelGGL(zt_bc7_select, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, (const uint32_t *)w.keys, (const uint4 *)w.bits, topk); })))
+        return rc;
+    return launch("zt_bc7_full", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_full, dim3(grid_for(total, 4)), dim3(256), 0, s, J, total, (const uint32_t *)w.fallback, (const unsigned *)(w.counters + 8),
+                           (uint32_t *)nullptr);
+    });
+}
+
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream) {
     if (w < 1 || h < 1 || !src || !dst || stride < (size_t)w * 4 || ((uintptr_t)src | stride | (uintptr_t)dst | dst_stride) % 4)
         return fail(ZAPTEX_E_ARG, "bad arguments (rows must be 4-byte aligned)");

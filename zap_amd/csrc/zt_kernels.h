@@ -14,6 +14,19 @@ struct zt_bc7_lists {
     uint4 *meta1;        // (list-5 entry, current best SSE, 0, 0)
 };
 
+// bf16 GEMM (zt_mlp.hip): C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]
+struct zt_gemm_args {
+    const uint16_t *A, *B;
+    int64_t sam, sak, sbk, sbn;
+    float *Cf;              // fp32 output (or accumulation target), may be null
+    uint16_t *Cb;           // bf16 output, may be null
+    int64_t ldc;
+    const float *bias;      // per column, may be null
+    const uint16_t *mask;   // bf16 activation; output zeroed where it is 0 (ReLU backward), may be null
+    int64_t ldm;
+    int M, N, K, relu, accumulate, pad;
+};
+
 extern "C" {
 __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
 __global__ void zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L);
@@ -21,6 +34,25 @@ __global__ void zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k);
 __global__ void zt_zero_u32(unsigned *p, int n);
 // exhaustive all-mode BC7 search (optionally restricted to listed global blocks); best_cand may be null
 __global__ void zt_bc7_full(zt_jobs jobs, uint32_t total, const uint32_t *list, const unsigned *nlist, uint32_t *best_cand);
+
+__global__ void zt_gemm_bf16(zt_gemm_args g);
+__global__ void zt_mlp_features(const uint8_t *blocks, int n, uint16_t *X);
+__global__ void zt_mlp_xent(const float *Z, const uint32_t *labels, int B, uint16_t *dZ, float *stats);
+__global__ void zt_mlp_colsum(const uint16_t *src, int M, int N, float *out);
+__global__ void zt_mlp_adam(float *p, const float *gr, float *m1, float *m2, int n, float lr, float b1, float b2, float c1, float c2,
+                            float eps, uint16_t *wbf, const int *bf_index);
+__global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t *wbf, const int *bf_index, const float *scale);
+__global__ void zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *conf);
+__global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index);
+// learned-proposal BC7 (zt_learned.hip)
+__global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);
+__global__ void zt_bc7_bucket(const uint32_t *cand, const float *conf, uint32_t total, int K, float threshold, unsigned *counters,
+                              uint4 *items, uint32_t pass, uint32_t *fallback, uint32_t *keys);
+__global__ void zt_bc7_bucket_scan(unsigned *counters);
+__global__ void zt_bc7_eval_items(const uint4 *blocks, const uint4 *items, const unsigned *counters, int mode, uint32_t *keys, uint4 *bits,
+                                  int K);
+__global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
+__global__ void zt_bc7_select(zt_jobs jobs, uint32_t total, const uint32_t *keys, const uint4 *bits, int K);
 
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);

@@ -1,0 +1,228 @@
+// zt_mlp.hip — the BC7 proposal network on gfx950: 64 -> 128 -> 128 -> 512 MLP (ReLU), BF16 weights and
+// activations on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, trained here with its own
+// forward/backward/Adam kernels.  Input: the 4x4 RGBA8 block (64 values / 255); output: one logit per BC7
+// candidate slot mode*64 + slot, of which the 285 used slots are read.
+#include "zt_common.h"
+#include "zt_kernels.h"
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+ZT_DEV uint16_t to_bf16(float f) {  // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+ZT_DEV float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// candidate slot used by the exhaustive search (285 of 512)
+ZT_DEV bool cand_valid(int c) {
+    const int m = c >> 6, s = c & 63;
+    return m == 0 ? s < 16 : m == 4 ? s < 8 : m == 5 ? s < 4 : m == 6 ? s == 0 : true;
+}
+
+ZT_DEV bf16x4_t load_a4(const zt_gemm_args &g, int m, int k) {
+    bf16x4_t v = { 0, 0, 0, 0 };
+    if (m >= g.M) return v;
+    if (g.sak == 1 && k + 3 < g.K && (((uintptr_t)(g.A + (int64_t)m * g.sam + k)) & 7) == 0) {
+        uint2 t = *(const uint2 *)(g.A + (int64_t)m * g.sam + k);
+        v[0] = (short)(t.x & 0xFFFF); v[1] = (short)(t.x >> 16); v[2] = (short)(t.y & 0xFFFF); v[3] = (short)(t.y >> 16);
+        return v;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) if (k + i < g.K) v[i] = (short)g.A[(int64_t)m * g.sam + (int64_t)(k + i) * g.sak];
+    return v;
+}
+ZT_DEV bf16x4_t load_b4(const zt_gemm_args &g, int k, int n) {
+    bf16x4_t v = { 0, 0, 0, 0 };
+    if (n >= g.N) return v;
+    if (g.sbk == 1 && k + 3 < g.K && (((uintptr_t)(g.B + (int64_t)n * g.sbn + k)) & 7) == 0) {
+        uint2 t = *(const uint2 *)(g.B + (int64_t)n * g.sbn + k);
+        v[0] = (short)(t.x & 0xFFFF); v[1] = (short)(t.x >> 16); v[2] = (short)(t.y & 0xFFFF); v[3] = (short)(t.y >> 16);
+        return v;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) if (k + i < g.K) v[i] = (short)g.B[(int64_t)(k + i) * g.sbk + (int64_t)n * g.sbn];
+    return v;
+}
+
+}  // namespace
+
+// C(m, n) = sum_k A(m, k) B(k, n): 64x64 tile per workgroup, 32x32 per wave (2x2 MFMA 16x16x16 tiles).
+// MFMA lane layout: A row = lane % 16, k = 4 (lane / 16) + v; B col = lane % 16, same k; D col = lane % 16,
+// row = 4 (lane / 16) + v.  blockIdx.z splits K (accumulate mode adds into Cf atomically).
+extern "C" __global__ void __launch_bounds__(256) zt_gemm_bf16(zt_gemm_args g) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.y * 64 + (w >> 1) * 32, n0 = blockIdx.x * 64 + (w & 1) * 32;
+    const int kper = ((g.K + (int)gridDim.z - 1) / (int)gridDim.z + 15) / 16 * 16;
+    const int kb = blockIdx.z * kper, ke = min(g.K, kb + kper);
+    const int r = lane & 15, q = lane >> 4;
+    f32x4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+    for (int k = kb; k < ke; k += 16) {
+        bf16x4_t a[2], b[2];
+        const int kk = k + 4 * q;
+#pragma unroll
+        for (int i = 0; i < 2; i++) a[i] = kk < ke ? load_a4(g, m0 + 16 * i + r, kk) : bf16x4_t{ 0, 0, 0, 0 };
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[j] = kk < ke ? load_b4(g, kk, n0 + 16 * j + r) : bf16x4_t{ 0, 0, 0, 0 };
+        // zero the tail of a partial last K step on both operands
+        if (kk + 3 >= ke) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) if (kk + v >= ke) { a[0][v] = a[1][v] = 0; b[0][v] = b[1][v] = 0; }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int n = n0 + 16 * j + r;
+            if (n >= g.N) continue;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int m = m0 + 16 * i + 4 * q + v;
+                if (m >= g.M) continue;
+                float x = acc[i][j][v];
+                if (g.accumulate) { atomicAdd(g.Cf + (int64_t)m * g.ldc + n, x); continue; }
+                if (g.bias) x += g.bias[n];
+                if (g.relu && x < 0.0f) x = 0.0f;
+                if (g.mask && (g.mask[(int64_t)m * g.ldm + n] & 0x7FFF) == 0) x = 0.0f;  // ReLU backward: activation was 0
+                if (g.Cf) g.Cf[(int64_t)m * g.ldc + n] = x;
+                if (g.Cb) g.Cb[(int64_t)m * g.ldc + n] = to_bf16(x);
+            }
+        }
+}
+
+// block bytes (16 RGBA8 pixels) -> 64 bf16 features v / 255
+extern "C" __global__ void zt_mlp_features(const uint8_t *blocks, int n, uint16_t *X) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * 64) return;
+    X[i] = to_bf16((float)blocks[i] / 255.0f);
+}
+
+// softmax cross-entropy over the used candidate slots, one wave per row; dZ = (p - onehot) / B in bf16;
+// stats[0] += mean loss, stats[1] += top-1 hits
+extern "C" __global__ void __launch_bounds__(256) zt_mlp_xent(const float *Z, const uint32_t *labels, int B, uint16_t *dZ, float *stats) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= B) return;
+    const float *z = Z + (int64_t)row * 512;
+    float v[8], mx = -3.0e38f;
+    int am = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int c = lane + 64 * t;
+        v[t] = cand_valid(c) ? z[c] : -3.0e38f;
+        if (v[t] > mx) { mx = v[t]; am = c; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float om = __shfl_xor(mx, off);
+        int oa = __shfl_xor(am, off);
+        if (om > mx || (om == mx && oa < am)) { mx = om; am = oa; }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) if (cand_valid(lane + 64 * t)) s += __expf(v[t] - mx);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+    const int y = (int)labels[row];
+    const float invB = 1.0f / (float)B;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int c = lane + 64 * t;
+        float gr = cand_valid(c) ? (__expf(v[t] - mx) / s - (c == y ? 1.0f : 0.0f)) * invB : 0.0f;
+        dZ[(int64_t)row * 512 + c] = to_bf16(gr);
+    }
+    if (lane == 0) {
+        atomicAdd(stats, (mx + __logf(s) - z[y]) * invB);
+        if (am == y) atomicAdd(stats + 1, 1.0f);
+    }
+}
+
+// column sums of a bf16 [M x N] matrix into out[N] (bias gradients)
+extern "C" __global__ void zt_mlp_colsum(const uint16_t *src, int M, int N, float *out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int rows = (M + gridDim.y - 1) / gridDim.y, r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
+    float s = 0.0f;
+    for (int m = r0; m < r1; m++) s += from_bf16(src[(int64_t)m * N + n]);
+    atomicAdd(out + n, s);
+}
+
+// Adam on the fp32 master parameters; weights (not biases) are re-rounded into their bf16 copies
+extern "C" __global__ void zt_mlp_adam(float *p, const float *gr, float *m1, float *m2, int n, float lr, float b1, float b2,
+                                       float c1, float c2, float eps, uint16_t *wbf, const int *bf_index) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * g, b = b2 * m2[i] + (1.0f - b2) * g * g;
+    m1[i] = a; m2[i] = b;
+    const float x = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    p[i] = x;
+    const int j = bf_index[i];
+    if (j >= 0) wbf[j] = to_bf16(x);
+}
+
+// deterministic He-uniform initialisation (counter hash), biases zero; bf16 copies written
+extern "C" __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t *wbf, const int *bf_index, const float *scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (uint32_t)i * 0x9E3779B9u ^ seed;
+    h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
+    const float u = (float)(h >> 8) * (1.0f / 16777216.0f);
+    const float x = scale[i] * (2.0f * u - 1.0f);
+    p[i] = x;
+    const int j = bf_index[i];
+    if (j >= 0) wbf[j] = to_bf16(x);
+}
+
+// top-K used candidate slots per row (ties to the lower slot) and the top-1 softmax probability
+extern "C" __global__ void __launch_bounds__(256) zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *conf) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= B) return;
+    const float *z = Z + (int64_t)row * 512;
+    float v[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) v[t] = cand_valid(lane + 64 * t) ? z[lane + 64 * t] : -3.0e38f;
+    float top = 0.0f;
+    for (int k = 0; k < K; k++) {
+        float mx = -3.0e38f;
+        int am = 1 << 20;
+#pragma unroll
+        for (int t = 0; t < 8; t++) if (v[t] > mx) { mx = v[t]; am = lane + 64 * t; }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            float om = __shfl_xor(mx, off);
+            int oa = __shfl_xor(am, off);
+            if (om > mx || (om == mx && oa < am)) { mx = om; am = oa; }
+        }
+        if (k == 0) top = mx;
+        if (lane == 0) cand[(int64_t)row * K + k] = (uint32_t)am;
+#pragma unroll
+        for (int t = 0; t < 8; t++) if (lane + 64 * t == am) v[t] = -3.0e38f;
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) if (cand_valid(lane + 64 * t)) s += __expf(z[lane + 64 * t] - top);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) conf[row] = 1.0f / s;
+}
+
+// re-round the bf16 weight copies from the fp32 master weights (after loading parameters)
+extern "C" __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = bf_index[i];
+    if (j >= 0) wbf[j] = to_bf16(p[i]);
+}
