@@ -28,15 +28,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz: one scalar op per lane per clock
+PEAK_WAVE_INSTS = 256 * 2.4e9                # one wave64 VALU instruction per CU per clock (4 SIMD16 x 4 cycles)
 PEAK_HBM_GBS = 8000.0
 MIP_CHAIN_BYTES = 89_478_484                 # read the 4096^2 base once + write every generated level once
+# zt_bc7_stage6 on this workload, rocprofv3 --pmc per dispatch (profiles/zt_bc7_stage6_c2_chain.md):
+# FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the counter reads for a plain 256 MiB device copy
+# (FETCH 131,087 KiB -> x2.0, WRITE 262,144 KiB -> x1.0); SQ_INSTS_VALU in wave64 instructions.
+STAGE6_COUNTERS = {"fetch_kib": 43_997.0, "fetch_scale": 2.0, "write_kib": 47_723.2, "write_scale": 1.0,
+                   "sq_insts_valu": 202_885_748}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(levels_np, threads: int, repeats: int = 3) -> dict:
+def cpu_baseline(levels_np, threads: int, repeats: int = 5) -> dict:
     """Reference zap_tex.h encoder (oracle/_ref; the oracle restatement if that is absent) on host cores."""
     from concurrent.futures import ThreadPoolExecutor
 
@@ -63,7 +69,7 @@ def cpu_baseline(levels_np, threads: int, repeats: int = 3) -> dict:
     return {"value": round(npx / t / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": kind,
             "sample": f"BC7 of the full 13-level 4096^2 chain ({npx} px), 8-block-row bands on {threads} threads, median of {repeats}",
             "single_thread_mpix_s": round(one.shape[0] * one.shape[1] / t1 / 1e6, 3),
-            "cpu": _cpu_model()}
+            "cpu": _cpu_model(), "nproc": os.cpu_count()}
 
 
 def _cpu_model() -> str:
@@ -84,7 +90,18 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy (calibrates the profiler's HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x 268435456 B device copies")
+        return 0
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs the hot path on a single MI355X (no multi-GPU path); use --gpus 1")
         return 2
@@ -144,20 +161,41 @@ def main() -> int:
         step()
     torch.cuda.synchronize()
     kern = {}
-    for k in ["zt_mip_level", "zt_zero_u32", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_stage1"]:
+    for k in ["zt_mip_tiles", "zt_mip_tail", "zt_mip_level", "zt_zero_u32", "zt_bc7_stage6", "zt_bc7_stage5",
+              "zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_stage1"]:
         t, n = lib.profile_get(k)
-        kern[k] = {"launches_per_step": n // args.profile_steps, "ms_per_launch": round(t / max(1, n), 5),
-                   "ms_per_step": round(t / args.profile_steps, 5)}
+        if n:
+            kern[k] = {"launches_per_step": n // args.profile_steps, "ms_per_launch": round(t / n, 5),
+                       "ms_per_step": round(t / args.profile_steps, 5)}
     lib.profile(False)
+
+    def group_ms(names):
+        return sum(kern[k]["ms_per_step"] for k in names if k in kern)
+
+    stages = {"stage6": group_ms(["zt_bc7_stage6"]), "stage5": group_ms(["zt_bc7_stage5"]),
+              "stage1": group_ms(["zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_stage1"]),
+              "mips": group_ms(["zt_mip_tiles", "zt_mip_tail", "zt_mip_level"])}
     s6 = kern["zt_bc7_stage6"]["ms_per_launch"]
     achieved = ops["ops_stage6"] / (s6 / 1e3) / 1e12
-    mip_ms = kern["zt_mip_level"]["ms_per_step"]
-    mip_gbs = MIP_CHAIN_BYTES / (mip_ms / 1e3) / 1e9
-    stage_tops = {k: round(ops[f"ops_{k[7:]}"] / (kern[k]["ms_per_launch"] / 1e3) / 1e12, 3)
-                  for k in ["zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_stage1"] if kern[k]["ms_per_launch"] > 0}
+    mip_gbs = MIP_CHAIN_BYTES / (stages["mips"] / 1e3) / 1e9
+    stage_tops = {k: round(ops[f"ops_{k}"] / (v / 1e3) / 1e12, 3) for k, v in stages.items() if k != "mips" and v > 0}
     log(json.dumps(kern))
+    enc_ms = stages["stage6"] + stages["stage5"] + stages["stage1"] + group_ms(["zt_zero_u32"])
+    enc_tops = ops["ops_total"] / (enc_ms / 1e3) / 1e12
+    c = STAGE6_COUNTERS
+    traffic = int((c["fetch_kib"] * c["fetch_scale"] + c["write_kib"] * c["write_scale"]) * 1024)
+    valu_issue = c["sq_insts_valu"] / (s6 / 1e3) / PEAK_WAVE_INSTS
 
-    cpu = None if args.no_cpu else cpu_baseline([c2] + [l.cpu().numpy() for l in levels], args.cpu_threads)
+    # end to end through the drop-in's host entry (zap_bc_encode: host buffers in, PCIe both ways), per level
+    levels_np = [c2] + [l.cpu().numpy() for l in levels]
+    lib.bc_encode(levels_np[0], "bc7")
+    t0 = time.perf_counter()
+    for lvl in levels_np:
+        lib.bc_encode(lvl, "bc7")
+    e2e_mpix = npx / (time.perf_counter() - t0) / 1e6
+    log(f"host entry (H2D + encode + D2H per level): {e2e_mpix:.1f} Mpix/s")
+
+    cpu = None if args.no_cpu else cpu_baseline(levels_np, args.cpu_threads)
     line = {
         "metric": "BC7 Mpix/s on 1 MI355X (plain search, 4096^2 + 13-level mip chain)",
         "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -168,10 +206,16 @@ def main() -> int:
                    "global_batch": 1, "seq_len": None, "parallelism": "single GPU", "bc7_m1k": lib.get_option(1),
                    "byte_exact_vs_reference": exact},
         "roofline": {"bound": "valu", "kernel": "zt_bc7_stage6", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": None,
-                     "ops_per_launch": ops["ops_stage6"], "ms_per_launch": s6, "stage_tops": stage_tops,
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": traffic,
+                     "ops_per_launch": ops["ops_stage6"], "ms_per_launch": s6,
+                     "valu_issue_frac": round(valu_issue, 4), "stage_tops": stage_tops,
+                     "encoder": {"ops": ops["ops_total"], "ms": round(enc_ms, 5), "achieved": round(enc_tops, 3),
+                                 "frac": round(enc_tops / PEAK_VALU_TOPS, 4)},
                      "mips": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                              "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "bytes_per_chain": MIP_CHAIN_BYTES}},
+                              "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "bytes_per_chain": MIP_CHAIN_BYTES,
+                              "ms": round(stages["mips"], 5)}},
+        "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "host_entry_mpix_s": round(e2e_mpix, 1),
         "kernels": kern,
         "cpu_baseline": cpu,
     }

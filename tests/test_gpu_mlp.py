@@ -1,0 +1,122 @@
+"""The proposal network's own GPU kernels against the CPU oracle (oracle/mlp_ref.py): forward logits,
+gradients of every parameter tensor, a loss that falls under training, and the learned BC7 encode's
+contract (all-fallback = the exhaustive search; otherwise every block is the exact encoding of one of the
+proposed candidates, never better than the exhaustive optimum)."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+from zap_amd import binding, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(zt, B, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    params = (rng.standard_normal(mlp_ref.NPARAM) * 0.08).astype(np.float32)
+    blocks = synth.random_blocks(B, seed).reshape(B, 64)
+    labels = rng.choice(np.nonzero(mlp_ref.VALID)[0], B).astype(np.int32)
+    st = zt.mlp_state(params=params)
+    tb = torch.from_numpy(np.ascontiguousarray(blocks)).cuda()
+    tl = torch.from_numpy(labels).cuda()
+    return params, blocks, labels, st, tb, tl
+
+
+def test_forward_matches_oracle(zt):
+    import torch
+    B = 300
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 1)
+    ws = zt.mlp_workspace(B)
+    z = torch.empty((B, 512), dtype=torch.float32, device="cuda")
+    zt.mlp_forward(st, tb, B, z, ws)
+    torch.cuda.synchronize()
+    got = z.cpu().numpy().astype(np.float64)
+    exp = mlp_ref.forward(params, blocks)[3]
+    err = np.abs(got - exp).max() / np.abs(exp).max()
+    assert err < 2e-2, err
+
+
+def test_gradients_match_oracle(zt):
+    import torch
+    B = 512
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 2)
+    ws = zt.mlp_workspace(B)
+    stats = torch.zeros(2, dtype=torch.float32, device="cuda")
+    zt.mlp_gradients(st, tb, tl, B, ws, stats)
+    torch.cuda.synchronize()
+    g = zt.mlp_grads(st).astype(np.float64)
+    loss, ge = mlp_ref.loss_and_grads(params, blocks, labels)
+    assert abs(float(stats[0].item()) - loss) < 2e-2 * abs(loss)
+    for k, (o, s) in mlp_ref.OFF.items():
+        n = int(np.prod(s))
+        a, b = g[o:o + n], ge[o:o + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert rel < 5e-2, (k, rel)
+
+
+def test_training_reduces_loss(zt):
+    import torch
+    B = 1024
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 3)
+    labels = (np.arange(B) % 7 * 64).astype(np.int32)  # learnable: 7 classes
+    labels[labels == 256] = 320
+    tl = torch.from_numpy(labels).cuda()
+    st = zt.mlp_state(seed=5)
+    ws = zt.mlp_workspace(B)
+    losses = []
+    for step in range(1, 301):
+        stats = torch.zeros(2, dtype=torch.float32, device="cuda")
+        zt.mlp_train_step(st, tb, tl, B, 3e-3, step, ws, stats)
+        if step in (1, 300):
+            losses.append(float(stats[0].item()))
+    assert losses[1] < 0.5 * losses[0], losses
+
+
+def _decode_sse(oracle, bc, img):
+    h, w = img.shape[:2]
+    d = oracle.bc_decode(bc, w, h, "bc7").astype(np.int64) - img.astype(np.int64)
+    return (d * d).reshape(h // 4, 4, w // 4, 4, 4).sum(axis=(1, 3, 4)).reshape(-1)
+
+
+def _cand_of(block: np.ndarray) -> int:
+    b0 = int(block[0])
+    mode = (b0 & -b0).bit_length() - 1
+    bits = int.from_bytes(block.tobytes(), "little")
+    if mode in (0, 1, 2, 3, 7):
+        return mode * 64 + ((bits >> (mode + 1)) & (15 if mode == 0 else 63))
+    if mode == 4:
+        return 256 + ((bits >> 5) & 3) * 2 + ((bits >> 7) & 1)
+    if mode == 5:
+        return 320 + ((bits >> 6) & 3)
+    return 384
+
+
+def test_learned_encode_contract(zt, oracle):
+    import torch
+    img = synth.c2_bc7(64, 33)
+    src = torch.from_numpy(img).cuda()
+    outs = []
+    full = torch.zeros(binding.bc_size(64, 64, "bc7"), dtype=torch.uint8, device="cuda")
+    imgs_full = zt.images([(src, full, 64, 64, 256, "bc7")])
+    zt.bc7_full_dev(imgs_full)
+    st = zt.mlp_state(seed=9)
+    for thr in (2.0, 0.0):
+        dst = torch.zeros_like(full)
+        imgs = zt.images([(src, dst, 64, 64, 256, "bc7")])
+        ws = zt.bc7_learned_workspace(imgs, 4)
+        zt.bc7_learned_dev(imgs, st, 4, thr, ws)
+        torch.cuda.synchronize()
+        outs.append(dst.cpu().numpy())
+    f = full.cpu().numpy()
+    assert np.array_equal(outs[0], f)                      # everything fell back to the exhaustive search
+    learned = outs[1]
+    sse_l, sse_f = _decode_sse(oracle, learned, img), _decode_sse(oracle, f, img)
+    assert (sse_l >= sse_f).all()
+    blocks = synth.blocks_to_image  # noqa: F841
+    for b in range(0, 256, 5):
+        by, bx = divmod(b, 16)
+        blk = img[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4].reshape(16, 4)
+        c = _cand_of(learned[16 * b:16 * b + 16])
+        e, exp = oracle.bc7_candidate(blk, c)
+        assert np.array_equal(exp, learned[16 * b:16 * b + 16]) and e == sse_l[b], (b, c)

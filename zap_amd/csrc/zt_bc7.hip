@@ -146,3 +146,121 @@ extern "C" __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = v;
 }
+
+// ---- stage 1 split over three kernels for M1K <= 8 (every lane busy):
+//   rank  one wave per list-1 entry: partition estimates, wave rank, top-M1K partitions in rank order
+//   fit   one lane per (entry, candidate, subset): the exact subset fit; SSE and the subset's quantised state
+//   pick  one lane per entry: first candidate in rank order with the smallest SSE below the current one, packed
+// Capacity: entry e owns cand[8e..8e+7] and items [16e, 16e + 16).
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand) {
+    const uint32_t n1 = L.counters[1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < n1; e += nw) {
+        const uint4 m = L.meta1[e];
+        uint32_t px[16];
+        const uint4 *s = L.px5 + (uint64_t)m.x * 4;
+#pragma unroll
+        for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+        float tot[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) tot[j] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            float r = chanf(px[i], 0), g = chanf(px[i], 1), b = chanf(px[i], 2);
+            float v[9] = { r, g, b, r * r, g * g, b * b, r * g, r * b, g * b };
+#pragma unroll
+            for (int j = 0; j < 9; j++) tot[j] += v[j];
+        }
+        float est = mode1_estimate(px, tot, lane);
+        if (est == 0.0f) est = 0.0f;
+        uint32_t fb = __float_as_uint(est);
+        fb = (fb & 0x80000000u) ? ~fb : (fb | 0x80000000u);
+        int rank = 0;
+        for (int k = 0; k < 64; k++) {
+            uint32_t ofb = __shfl(fb, k);
+            rank += ofb < fb || (ofb == fb && k < lane);
+        }
+        if (rank < m1k) cand[(uint64_t)e * 8 + rank] = (uint8_t)lane;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state) {
+    const uint32_t n1 = L.counters[1], items = n1 * (uint32_t)m1k * 2, step = gridDim.x * blockDim.x;
+    for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < items; it += step) {
+        const uint32_t e = it / (2u * m1k), k = (it >> 1) % (uint32_t)m1k, sb = it & 1;
+        const uint4 m = L.meta1[e];
+        uint32_t px[16];
+        const uint4 *s = L.px5 + (uint64_t)m.x * 4;
+#pragma unroll
+        for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+        const int part = cand[(uint64_t)e * 8 + k];
+        const uint32_t m1 = kP2[part], mask = sb ? m1 : (~m1 & 0xFFFFu);
+        int q[2][4], pb[2];
+        uint64_t idx;
+        const int err = encode_subset<3, 7, 3, true, false>(px, mask, __popc(mask), q, pb, idx);
+        const uint64_t o = (uint64_t)e * 16 + k * 2 + sb;
+        sub_err[o] = err;
+        // the 6 endpoint values (6 bits each: endpoint e, channel c at bit 18e + 6c) and the shared p-bit at bit 36
+        const uint64_t qs = (uint64_t)q[0][0] | (uint64_t)q[0][1] << 6 | (uint64_t)q[0][2] << 12 | (uint64_t)q[1][0] << 18 |
+                            (uint64_t)q[1][1] << 24 | (uint64_t)q[1][2] << 30 | (uint64_t)pb[0] << 36;
+        sub_state[o] = make_uint4((uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)qs, (uint32_t)(qs >> 32));
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err,
+                                                                   const uint4 *sub_state) {
+    const uint32_t n1 = L.counters[1], step = gridDim.x * blockDim.x;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n1; e += step) {
+        const uint4 m = L.meta1[e];
+        int best = (int)m.y, bk = -1;
+        for (int k = 0; k < m1k; k++) {
+            const int err = sub_err[(uint64_t)e * 16 + 2 * k] + sub_err[(uint64_t)e * 16 + 2 * k + 1];
+            if (err < best) { best = err; bk = k; }
+        }
+        if (bk < 0) continue;
+        const int part = cand[(uint64_t)e * 8 + bk], anc = kA2[part];
+        const uint4 s0 = sub_state[(uint64_t)e * 16 + 2 * bk], s1 = sub_state[(uint64_t)e * 16 + 2 * bk + 1];
+        const uint64_t qs0 = (uint64_t)s0.z | (uint64_t)s0.w << 32, qs1 = (uint64_t)s1.z | (uint64_t)s1.w << 32;
+        int q[2][2][3], pb[2] = { (int)((qs0 >> 36) & 1), (int)((qs1 >> 36) & 1) };
+        uint64_t full = ((uint64_t)s0.x | (uint64_t)s0.y << 32) | ((uint64_t)s1.x | (uint64_t)s1.y << 32);
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                q[0][x][c] = (int)((qs0 >> (18 * x + 6 * c)) & 63);
+                q[1][x][c] = (int)((qs1 >> (18 * x + 6 * c)) & 63);
+            }
+        const uint32_t m1 = kP2[part], m0 = ~m1 & 0xFFFFu;
+        uint64_t sel0 = 0, sel1 = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) { if ((m0 >> i) & 1) sel0 |= 7ull << (4 * i); else sel1 |= 7ull << (4 * i); }
+        if (full & 4) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) { int t = q[0][0][c]; q[0][0][c] = q[0][1][c]; q[0][1][c] = t; }
+            full ^= sel0;
+        }
+        if ((full >> (4 * anc)) & 4) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) { int t = q[1][0][c]; q[1][0][c] = q[1][1][c]; q[1][1][c] = t; }
+            full ^= sel1;
+        }
+        Bits128 b{ 2ull | (uint64_t)part << 2, 0 };
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int x = 0; x < 2; x++) b.put(8 + 6 * (4 * c + 2 * s + x), (uint64_t)q[s][x][c], 6);
+        b.put(80, (uint64_t)pb[0], 1); b.put(81, (uint64_t)pb[1], 1);
+        int pos = 82;
+        for (int i = 0; i < 16; i++) {
+            int nb = 3 - (i == 0 || i == anc);
+            b.put(pos, (full >> (4 * i)) & 7, nb);
+            pos += nb;
+        }
+        const uint4 meta = L.meta5[m.x];
+        const zt_job J = job_at(jobs, (int)meta.z);
+        store16(J.dst + (uint64_t)(meta.x - J.first) * 16, b.lo, b.hi);
+    }
+}

@@ -143,8 +143,8 @@ int check_image(const zaptex_image &im) {
     return ZAPTEX_OK;
 }
 
-// workspace layout: [counters 256 B][meta5 16 B/blk][px5 64 B/blk][meta1 16 B/blk] over the BC7 blocks
-size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? 256 + bc7_blocks * 96 : 0; }
+// workspace layout over the BC7 blocks: [counters 256 B][meta5 16][px5 64][meta1 16][cand1 8][sub_err 64][sub_state 256] B/blk
+size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? 256 + bc7_blocks * 424 : 0; }
 
 zt_bc7_lists lists_in(void *ws, uint64_t n) {
     uint8_t *p = (uint8_t *)ws;
@@ -153,6 +153,9 @@ zt_bc7_lists lists_in(void *ws, uint64_t n) {
     L.meta5 = (uint4 *)(p + 256);
     L.px5 = (uint4 *)(p + 256 + n * 16);
     L.meta1 = (uint4 *)(p + 256 + n * 80);
+    L.cand1 = p + 256 + n * 96;
+    L.sub_err = (int *)(p + 256 + n * 104);
+    L.sub_state = (uint4 *)(p + 256 + n * 168);
     return L;
 }
 
@@ -199,10 +202,21 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
                 hipLaunchKernelGGL(zt_bc7_stage6, dim3(grid_for(tot[1], 256)), dim3(256), 0, s, fam[1], tot[1], L); })))
             return rc;
         if ((rc = launch("zt_bc7_stage5", s, [&] {
-                hipLaunchKernelGGL(zt_bc7_stage5, dim3(grid_for((uint64_t)tot[1] * 4, 256)), dim3(256), 0, s, fam[1], L); })))
+                hipLaunchKernelGGL(zt_bc7_stage5, dim3(std::min(2048, grid_for((uint64_t)tot[1] * 4, 256))), dim3(256), 0, s, fam[1], L); })))
             return rc;
-        if ((rc = launch("zt_bc7_stage1", s, [&] {
-                hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot[1], 4)), dim3(256), 0, s, fam[1], L, g_m1k); })))
+        if (g_m1k <= 8) {
+            const int gsz = std::max(1, std::min(1024, grid_for((uint64_t)tot[1] / 16 + 1, 4)));
+            if ((rc = launch("zt_bc7_s1_rank", s, [&] { hipLaunchKernelGGL(zt_bc7_s1_rank, dim3(gsz), dim3(256), 0, s, L, g_m1k, L.cand1); })))
+                return rc;
+            if ((rc = launch("zt_bc7_s1_fit", s, [&] {
+                    hipLaunchKernelGGL(zt_bc7_s1_fit, dim3(gsz), dim3(256), 0, s, L, g_m1k, (const uint8_t *)L.cand1, L.sub_err, L.sub_state); })))
+                return rc;
+            if ((rc = launch("zt_bc7_s1_pick", s, [&] {
+                    hipLaunchKernelGGL(zt_bc7_s1_pick, dim3(gsz), dim3(256), 0, s, fam[1], L, g_m1k, (const uint8_t *)L.cand1, (const int *)L.sub_err,
+                                       (const uint4 *)L.sub_state); })))
+                return rc;
+        } else if ((rc = launch("zt_bc7_stage1", s, [&] {
+                       hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot[1], 4)), dim3(256), 0, s, fam[1], L, g_m1k); })))
             return rc;
     }
     return ZAPTEX_OK;
@@ -214,6 +228,45 @@ int enqueue_mip(const void *src, int w, int h, size_t stride, int srgb, void *ds
     return launch("zt_mip_level", s, [&] {
         hipLaunchKernelGGL(zt_mip_level, grid, block, 0, s, (const uint8_t *)src, w, h, (uint64_t)stride, srgb, (uint8_t *)dst, (uint64_t)dst_stride);
     });
+}
+
+// whole chain: fused tile passes (up to 7 levels each) while the source is larger than 128 on a side, then
+// one workgroup for the rest; a level with a side of 1 and the other side above 128 goes level by level
+int enqueue_mip_chain(const void *base, int w, int h, size_t stride, int srgb, void *const *levels, int nlevels, hipStream_t s) {
+    std::vector<int> lw(1, w), lh(1, h);
+    for (int i = 0; i < nlevels; i++) { lw.push_back(lw.back() > 1 ? lw.back() / 2 : 1); lh.push_back(lh.back() > 1 ? lh.back() / 2 : 1); }
+    const void *src = base;
+    size_t ss = stride;
+    int l = 0, rc;
+    while (l < nlevels && (lw[l] > 1 || lh[l] > 1)) {
+        const int cw = lw[l], ch = lh[l];
+        zt_mip_levels L;
+        memset(&L, 0, sizeof L);
+        if ((cw > 128 || ch > 128) && cw >= 2 && ch >= 2) {
+            int k = 0;
+            while (k < 7 && l + k < nlevels && lw[l + k] >= 2 && lh[l + k] >= 2) {
+                L.p[k] = (uint8_t *)levels[l + k]; L.w[k] = lw[l + k + 1]; L.h[k] = lh[l + k + 1];
+                k++;
+            }
+            dim3 grid((lw[l + 1] + 63) / 64, (lh[l + 1] + 63) / 64);
+            if ((rc = launch("zt_mip_tiles", s, [&] {
+                    hipLaunchKernelGGL(zt_mip_tiles, grid, dim3(256), 0, s, (const uint8_t *)src, cw, ch, (uint64_t)ss, srgb, k, L); })))
+                return rc;
+            src = levels[l + k - 1]; ss = (size_t)lw[l + k] * 4; l += k;
+        } else if (cw <= 128 && ch <= 128) {
+            const int k = std::min(16, nlevels - l);
+            for (int i = 0; i < k; i++) { L.p[i] = (uint8_t *)levels[l + i]; L.w[i] = lw[l + i + 1]; L.h[i] = lh[l + i + 1]; }
+            if ((rc = launch("zt_mip_tail", s, [&] {
+                    hipLaunchKernelGGL(zt_mip_tail, dim3(1), dim3(1024), 0, s, (const uint8_t *)src, cw, ch, (uint64_t)ss, srgb, k, L); })))
+                return rc;
+            l += k;
+            src = levels[l - 1]; ss = (size_t)lw[l] * 4;
+        } else {
+            if ((rc = enqueue_mip(src, cw, ch, ss, srgb, levels[l], (size_t)lw[l + 1] * 4, s))) return rc;
+            src = levels[l]; ss = (size_t)lw[l + 1] * 4; l++;
+        }
+    }
+    return ZAPTEX_OK;
 }
 
 // run one host-buffer image through the device: stage in, encode, copy out
@@ -733,15 +786,9 @@ int zaptex_mip_chain_dev(const void *base, int w, int h, size_t stride, int srgb
         std::lock_guard<std::mutex> lk(g_mu);
         if (int rc = ensure_ready(st)) return rc;
     }
-    const void *src = base;
-    size_t ss = stride;
-    for (int i = 0; i < nlevels && (w > 1 || h > 1); i++) {
-        int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
+    for (int i = 0; i < nlevels; i++)
         if (!levels[i] || ((uintptr_t)levels[i]) % 4) return fail(ZAPTEX_E_ARG, "bad level pointer %d", i);
-        if (int rc = enqueue_mip(src, w, h, ss, srgb, levels[i], (size_t)nw * 4, (hipStream_t)stream)) return rc;
-        src = levels[i]; ss = (size_t)nw * 4; w = nw; h = nh;
-    }
-    return ZAPTEX_OK;
+    return enqueue_mip_chain(base, w, h, stride, srgb, levels, nlevels, (hipStream_t)stream);
 }
 
 int zaptex_bc_decode_dev(const void *bc, int w, int h, int fmt, void *rgba, size_t stride, void *stream) {

@@ -12,6 +12,9 @@ struct zt_bc7_lists {
     uint4 *meta5;        // (global block, mode-6 SSE, job, opaque)
     uint4 *px5;          // 4 x uint4 = the block's 16 RGBA8 pixels
     uint4 *meta1;        // (list-5 entry, current best SSE, 0, 0)
+    uint8_t *cand1;      // 8 ranked partitions per list-1 entry
+    int *sub_err;        // 16 subset fits per list-1 entry
+    uint4 *sub_state;
 };
 
 // bf16 GEMM (zt_mlp.hip): C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]
@@ -27,11 +30,20 @@ struct zt_gemm_args {
     int M, N, K, relu, accumulate, pad;
 };
 
+// destination levels of a fused mip launch (packed rows, 4 bytes per pixel)
+struct zt_mip_levels {
+    uint8_t *p[16];
+    int w[16], h[16];
+};
+
 extern "C" {
 __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
 __global__ void zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L);
 __global__ void zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k);
 __global__ void zt_zero_u32(unsigned *p, int n);
+__global__ void zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand);
+__global__ void zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state);
+__global__ void zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err, const uint4 *sub_state);
 // exhaustive all-mode BC7 search (optionally restricted to listed global blocks); best_cand may be null
 __global__ void zt_bc7_full(zt_jobs jobs, uint32_t total, const uint32_t *list, const unsigned *nlist, uint32_t *best_cand);
 
@@ -65,6 +77,8 @@ __global__ void zt_bc_shuffle(const uint8_t *in, uint8_t *out, uint64_t nb, int 
 
 // mips: one level (any size); tile chain (levels 1..k inside a workgroup tile); float level
 __global__ void zt_mip_level(const uint8_t *src, int w, int h, uint64_t stride, int srgb, uint8_t *dst, uint64_t dst_stride);
+__global__ void zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
+__global__ void zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
 __global__ void zt_mip_level_f(const float *src, int w, int h, uint64_t stride, float *dst, uint64_t dst_stride);
 }
 

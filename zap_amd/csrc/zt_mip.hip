@@ -73,3 +73,109 @@ extern "C" __global__ void __launch_bounds__(256) zt_mip_level_f(const float *sr
     o.w = 0.25f * (a.w + b.w + c.w + d.w);
     *(float4 *)(dst + (uint64_t)y * dst_stride + (uint64_t)x * 4) = o;
 }
+
+// ---------------------------------------------------------------- fused chain
+// Level l pixel (x, y) reads level l-1 pixels (2x..2x+1, 2y..2y+1), which exist without clamping while the
+// source level is at least 2 wide and 2 high; so a 2^k x 2^k tile of level 1 yields the matching tiles of
+// levels 2..k+1 from LDS alone.  One launch of zt_mip_tiles makes up to 7 levels; the tail (a source of at
+// most 128 x 128, and every level where a dimension reaches 1) runs in one workgroup (zt_mip_tail).
+namespace {
+ZT_DEV uint32_t box4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int srgb, const float *lin, const float *thr) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int pa = chan(a, k), pb = chan(b, k), pc = chan(c, k), pd = chan(d, k), v;
+        if (srgb && k < 3) {
+            float f = (lin[pa] + lin[pb] + lin[pc] + lin[pd]) * 0.25f;
+            int lo = 0, hi = 255;
+            while (lo < hi) { int mid = (lo + hi) >> 1; if (f < thr[mid]) hi = mid; else lo = mid + 1; }
+            v = lo;
+        } else v = (pa + pb + pc + pd + 2) >> 2;
+        o |= (uint32_t)v << (8 * k);
+    }
+    return o;
+}
+}  // namespace
+
+// grid: tiles of 64 x 64 level-1 pixels; block 256.  lv[i] / lw[i], lh[i]: level i+1 buffers (packed rows)
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
+                                                                 zt_mip_levels L) {
+    __shared__ float lin[256], thr[256];
+    __shared__ uint32_t t0[64 * 64], t1[32 * 32];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (srgb)
+        for (int i = tid; i < 256; i += nt) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
+    __syncthreads();
+    const int tx = blockIdx.x, ty = blockIdx.y;
+    // level 1: 64 x 64 pixels of the tile, 16 per thread, 2 x 8-byte loads per pixel pair row
+    {
+        const int w1 = L.w[0], h1 = L.h[0];
+        for (int p = tid; p < 64 * 64; p += nt) {
+            const int lx = p & 63, ly = p >> 6, x = tx * 64 + lx, y = ty * 64 + ly;
+            uint32_t o = 0;
+            if (x < w1 && y < h1) {
+                const uint8_t *r0 = src + (uint64_t)(2 * y) * stride + (uint64_t)(2 * x) * 4, *r1 = r0 + stride;
+                uint32_t a, b, c, d;
+                if ((((uintptr_t)r0 | (uintptr_t)r1) & 7) == 0) {
+                    uint2 u0 = *(const uint2 *)r0, u1 = *(const uint2 *)r1;
+                    a = u0.x; b = u0.y; c = u1.x; d = u1.y;
+                } else {
+                    a = *(const uint32_t *)r0; b = *(const uint32_t *)(r0 + 4); c = *(const uint32_t *)r1; d = *(const uint32_t *)(r1 + 4);
+                }
+                o = box4(a, b, c, d, srgb, lin, thr);
+                *(uint32_t *)(L.p[0] + ((uint64_t)y * w1 + x) * 4) = o;
+            }
+            t0[p] = o;
+        }
+    }
+    __syncthreads();
+    // levels 2..nlev inside the tile, ping-ponging between t0 (64 wide) and t1 (32 wide)
+    int side = 32;
+    for (int l = 1; l < nlev; l++, side >>= 1) {
+        uint32_t *srcT = (l & 1) ? t0 : t1, *dstT = (l & 1) ? t1 : t0;
+        const int sw = side * 2;
+        const int wl = L.w[l], hl = L.h[l];
+        for (int p = tid; p < side * side; p += nt) {
+            const int lx = p % side, ly = p / side, x = tx * side + lx, y = ty * side + ly;
+            uint32_t o = box4(srcT[(2 * ly) * sw + 2 * lx], srcT[(2 * ly) * sw + 2 * lx + 1], srcT[(2 * ly + 1) * sw + 2 * lx],
+                              srcT[(2 * ly + 1) * sw + 2 * lx + 1], srgb, lin, thr);
+            if (x < wl && y < hl) *(uint32_t *)(L.p[l] + ((uint64_t)y * wl + x) * 4) = o;
+            dstT[p] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// one workgroup: levels from a source of at most 128 x 128, with the reference's clamping when a side is 1
+extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
+                                                                 zt_mip_levels L) {
+    __shared__ float lin[256], thr[256];
+    __shared__ uint32_t a[64 * 64], b[32 * 64];
+    const int tid = threadIdx.x;
+    if (srgb)
+        for (int i = tid; i < 256; i += blockDim.x) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
+    __syncthreads();
+    int cw = w, ch = h;
+    for (int l = 0; l < nlev; l++) {
+        const int nw = cw > 1 ? cw / 2 : 1, nh = ch > 1 ? ch / 2 : 1;
+        uint32_t *dstT = (l & 1) ? b : a;
+        const uint32_t *srcT = (l & 1) ? a : b;
+        for (int p = tid; p < nw * nh; p += blockDim.x) {
+            const int x = p % nw, y = p / nw;
+            const int x0 = 2 * x < cw ? 2 * x : cw - 1, x1 = 2 * x + 1 < cw ? 2 * x + 1 : cw - 1;
+            const int y0 = 2 * y < ch ? 2 * y : ch - 1, y1 = 2 * y + 1 < ch ? 2 * y + 1 : ch - 1;
+            uint32_t q0, q1, q2, q3;
+            if (l == 0) {
+                q0 = *(const uint32_t *)(src + (uint64_t)y0 * stride + x0 * 4); q1 = *(const uint32_t *)(src + (uint64_t)y0 * stride + x1 * 4);
+                q2 = *(const uint32_t *)(src + (uint64_t)y1 * stride + x0 * 4); q3 = *(const uint32_t *)(src + (uint64_t)y1 * stride + x1 * 4);
+            } else {
+                q0 = srcT[y0 * cw + x0]; q1 = srcT[y0 * cw + x1]; q2 = srcT[y1 * cw + x0]; q3 = srcT[y1 * cw + x1];
+            }
+            const uint32_t o = box4(q0, q1, q2, q3, srgb, lin, thr);
+            dstT[p] = o;
+            *(uint32_t *)(L.p[l] + (uint64_t)p * 4) = o;
+        }
+        __syncthreads();
+        cw = nw; ch = nh;
+    }
+}
