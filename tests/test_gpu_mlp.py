@@ -113,7 +113,6 @@ def test_learned_encode_contract(zt, oracle):
     learned = outs[1]
     sse_l, sse_f = _decode_sse(oracle, learned, img), _decode_sse(oracle, f, img)
     assert (sse_l >= sse_f).all()
-    blocks = synth.blocks_to_image  # noqa: F841
     for b in range(0, 256, 5):
         by, bx = divmod(b, 16)
         blk = img[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4].reshape(16, 4)
