@@ -12,6 +12,9 @@ Also reported, measured inside this run:
                 duration from HIP events on its launch stream; the mip chain's HBM GB/s is given beside it
   cpu_baseline  the reference encoder (oracle/_ref, built from zap_tex.h) on this host's cores, banded into
                 8-block rows like the reference viewer, over the same 13-level chain
+  learned       the learned-proposal encode (zap_amd/models/bc7_proposal.npy, top-4 exact + exhaustive fallback at
+                the model's calibrated threshold) against the exhaustive all-mode search on the same chain:
+                Mpix/s of both, chain PSNR of both (tex_bench.c's RGBA definition) and the gap
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -72,6 +75,23 @@ def cpu_baseline(levels_np, threads: int, repeats: int = 5) -> dict:
             "cpu": _cpu_model(), "nproc": os.cpu_count()}
 
 
+def learned_leg(lib, c2, steps: int = 5) -> dict:
+    """Learned proposal vs exhaustive all-mode search on the 4096^2 chain (mip chain regenerated on the GPU)."""
+    from zap_amd import learned
+    state, meta = learned.load(lib)
+    chain = learned.Chain(lib, c2)
+    r = learned.evaluate(lib, chain, state, meta["topk"], [meta["threshold"]], steps)
+    lr, full, plain = r["learned"][0], r["full"], r["plain"]
+    out = {"threshold": meta["threshold"], "topk": meta["topk"], "features": meta["features"], "train_blocks": meta["blocks"],
+           "mpix_s": round(lr["mpix_s"], 1), "exhaustive_mpix_s": round(full["mpix_s"], 1),
+           "speedup_vs_exhaustive": round(lr["mpix_s"] / full["mpix_s"], 3), "psnr_db": round(lr["psnr"], 4),
+           "exhaustive_psnr_db": round(full["psnr"], 4), "gap_db": round(lr["gap_db"], 4), "fallback": round(lr["fallback"], 4),
+           "plain_psnr_db": round(plain["psnr"], 4), "ms": round(lr["ms"], 3), "exhaustive_ms": round(full["ms"], 3)}
+    log(f"learned: {out['mpix_s']} Mpix/s ({out['speedup_vs_exhaustive']}x exhaustive {out['exhaustive_mpix_s']}), "
+        f"gap {out['gap_db']} dB, fallback {out['fallback']}")
+    return out
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -89,6 +109,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the profiler's HBM byte counters)")
@@ -195,14 +216,16 @@ def main() -> int:
     e2e_mpix = npx / (time.perf_counter() - t0) / 1e6
     log(f"host entry (H2D + encode + D2H per level): {e2e_mpix:.1f} Mpix/s")
 
+    learned_res = None if args.no_learned else learned_leg(lib, c2)
     cpu = None if args.no_cpu else cpu_baseline(levels_np, args.cpu_threads)
     line = {
-        "metric": "BC7 Mpix/s on 1 MI355X (plain search, 4096^2 + 13-level mip chain)",
+        "metric": "BC7 Mpix/s on 1 MI355X: plain vs learned-proposal at \u22640.05 dB PSNR gap",
         "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8/int32/f32 (bit-exact)", "data": "synthetic (seeded value noise + rectangles + independent alpha, 25% opaque blocks)",
         "config": {"workload": "BC7 plain search of a 4096x4096 RGBA8 texture and its 13-level box mip chain "
                                "(mip generation + encode per step; 1,398,103 blocks, 22,369,621 px)",
+                   "value_is": "the plain search (the reference's mode 6/5/1 search, byte-exact); learned-proposal figures in 'learned'",
                    "global_batch": 1, "seq_len": None, "parallelism": "single GPU", "bc7_m1k": lib.get_option(1),
                    "byte_exact_vs_reference": exact},
         "roofline": {"bound": "valu", "kernel": "zt_bc7_stage6", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 2),
@@ -217,6 +240,7 @@ def main() -> int:
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
         "host_entry_mpix_s": round(e2e_mpix, 1),
         "kernels": kern,
+        "learned": learned_res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

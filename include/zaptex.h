@@ -78,26 +78,50 @@ size_t zaptex_workspace_size(const zaptex_image *imgs, int n);
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream);
 /* Exhaustive all-mode BC7 search (no reference counterpart: the reference tries modes 6, 5 and 1 only).
  * 285 candidates per block (mode*64 + slot); never worse than zaptex_encode_dev's reference search.
- * best_cand (device, one uint32 per block of all images in order, may be NULL) receives the chosen candidate. */
-int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, void *stream);
+ * best_cand (device, one uint32 per block of all images in order, may be NULL) receives the chosen candidate;
+ * deltas (device, 512 uint16 per block in candidate-slot order, may be NULL) receives every candidate's
+ * min(65535, SSE - best SSE), 65535 in the 227 unused slots.  The workspace (device memory of at least
+ * zaptex_bc7_full_workspace_size bytes, with_deltas = whether deltas will be asked for) is the caller's. */
+size_t zaptex_bc7_full_workspace_size(const zaptex_image *imgs, int n, int with_deltas);
+int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, uint16_t *deltas, void *ws, size_t ws_bytes, void *stream);
 /* ---------------- learned BC7 proposal (no reference counterpart)
  * A 64 -> 128 -> 128 -> 512 ReLU MLP (BF16 on MFMA, FP32 accumulate) scores the exhaustive search's candidate
- * slots from the block's pixels; the top-k are encoded exactly and the best kept; blocks whose top-1
- * probability is below `threshold` are searched exhaustively.  The network state is one device buffer of
- * zaptex_mlp_state_size() bytes: fp32 parameters [W1 64x128 | b1 | W2 128x128 | b2 | W3 128x512 | b3],
- * gradients, Adam moments, tables and the bf16 weight copies. */
+ * slots from the block's pixels; the top-k are encoded exactly and the best kept.  One unused output slot (385)
+ * predicts log(1 + the SSE the top-k give away against the exhaustive optimum); blocks whose predicted SSE
+ * exceeds `threshold` are searched exhaustively instead (threshold < 0: every block; huge: none).  The network
+ * state is one device buffer of zaptex_mlp_state_size() bytes: fp32 parameters
+ * [W1 64x128 | b1 | W2 128x128 | b2 | W3 128x512 | b3], gradients, Adam moments, tables and the bf16 weight
+ * copies. */
 size_t zaptex_mlp_state_size(void);
 size_t zaptex_mlp_workspace_size(int batch);
 int zaptex_mlp_init(void *state, uint32_t seed, void *stream);
 /* load fp32 parameters (90,880 floats, host memory) into a state buffer */
 int zaptex_mlp_set_params(void *state, const float *host_params, void *stream);
+/* input normalisation of the network (kept in the state; zaptex_mlp_init sets 0), per 4x4 block, m_c = channel
+ * mean, R = largest channel range: 0 v/255; 1 (v - m_c)/64; 2 (v - m_c)/max(1, R); 3 (v - m_c)/sqrt(64 max(1, R));
+ * 4 colour as 3 with R over R, G, B only, alpha (a - 255)/255 */
+int zaptex_mlp_set_features(void *state, int mode, void *stream);
 /* logits [batch x 512] for batch blocks of 16 RGBA8 pixels (64 bytes each) */
 int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *logits, void *ws, size_t ws_bytes, void *stream);
-/* mean cross-entropy gradients into the state's gradient area; stats (device, 2 floats) += (loss, top-1 hits) */
-int zaptex_mlp_gradients(void *state, const void *blocks, const uint32_t *labels, int batch, void *ws, size_t ws_bytes, float *stats,
-                         void *stream);
+/* Training targets (device arrays, one row per block):
+ *   labels       uint32 slot per block: one-hot cross-entropy targets (used when temperature <= 0);
+ *   deltas       zaptex_bc7_full_dev's 512 uint16 per block (may be NULL without temperature or risk_k);
+ *   temperature  > 0: soft targets softmax(-delta / temperature) over the used slots instead of the labels;
+ *   risk_k       1..16: the risk slot also learns log(1 + min delta over the current top-risk_k slots), squared
+ *                error weighted by risk_weight; 0: off. */
+typedef struct {
+    const uint32_t *labels;
+    const uint16_t *deltas;
+    float temperature;
+    int risk_k;
+    float risk_weight;
+} zaptex_mlp_targets;
+/* Loss gradients (mean over the batch) into the state's gradient area; stats (device, 2 floats, may be NULL)
+ * receive this batch's (mean loss, top-1 hits), a hit being the label, or with soft targets a slot of delta 0. */
+int zaptex_mlp_gradients(void *state, const void *blocks, const zaptex_mlp_targets *targets, int batch, void *ws, size_t ws_bytes,
+                         float *stats, void *stream);
 /* gradients then one Adam step (step counts from 1) */
-int zaptex_mlp_train_step(void *state, const void *blocks, const uint32_t *labels, int batch, float lr, int step, void *ws,
+int zaptex_mlp_train_step(void *state, const void *blocks, const zaptex_mlp_targets *targets, int batch, float lr, int step, void *ws,
                           size_t ws_bytes, float *stats, void *stream);
 /* every block of the images as 64 contiguous bytes (training inputs) */
 int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *stream);

@@ -11,28 +11,29 @@ from zap_amd import binding, synth
 pytestmark = pytest.mark.gpu
 
 
-def _setup(zt, B, seed):
+def _setup(zt, B, seed, feat=0):
     import torch
     rng = np.random.default_rng(seed)
     params = (rng.standard_normal(mlp_ref.NPARAM) * 0.08).astype(np.float32)
     blocks = synth.random_blocks(B, seed).reshape(B, 64)
     labels = rng.choice(np.nonzero(mlp_ref.VALID)[0], B).astype(np.int32)
-    st = zt.mlp_state(params=params)
+    st = zt.mlp_state(params=params, features=feat)
     tb = torch.from_numpy(np.ascontiguousarray(blocks)).cuda()
     tl = torch.from_numpy(labels).cuda()
     return params, blocks, labels, st, tb, tl
 
 
-def test_forward_matches_oracle(zt):
+@pytest.mark.parametrize("feat", [0, 1, 2, 3, 4])
+def test_forward_matches_oracle(zt, feat):
     import torch
     B = 300
-    params, blocks, labels, st, tb, tl = _setup(zt, B, 1)
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 1, feat)
     ws = zt.mlp_workspace(B)
     z = torch.empty((B, 512), dtype=torch.float32, device="cuda")
     zt.mlp_forward(st, tb, B, z, ws)
     torch.cuda.synchronize()
     got = z.cpu().numpy().astype(np.float64)
-    exp = mlp_ref.forward(params, blocks)[3]
+    exp = mlp_ref.forward(params, blocks, feat=feat)[3]
     err = np.abs(got - exp).max() / np.abs(exp).max()
     assert err < 2e-2, err
 
@@ -53,6 +54,54 @@ def test_gradients_match_oracle(zt):
         a, b = g[o:o + n], ge[o:o + n]
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
         assert rel < 5e-2, (k, rel)
+
+
+def test_soft_target_gradients_match_oracle(zt):
+    import torch
+    B = 384
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 4)
+    rng = np.random.default_rng(4)
+    deltas = rng.integers(0, 600, (B, 512)).astype(np.uint16)
+    deltas[:, ~mlp_ref.VALID] = 65535
+    deltas[np.arange(B), labels] = 0
+    deltas[:7, 128:192] = 65535
+    td = torch.from_numpy(deltas.view(np.int16)).cuda()
+    ws = zt.mlp_workspace(B)
+    stats = torch.zeros(2, dtype=torch.float32, device="cuda")
+    zt.mlp_gradients(st, tb, None, B, ws, stats, deltas=td, temperature=50.0)
+    torch.cuda.synchronize()
+    g = zt.mlp_grads(st).astype(np.float64)
+    loss, ge = mlp_ref.loss_and_grads(params, blocks, None, deltas=deltas, T=50.0)
+    assert abs(float(stats[0].item()) - loss) < 2e-2 * abs(loss)
+    for k, (o, s) in mlp_ref.OFF.items():
+        n = int(np.prod(s))
+        a, b = g[o:o + n], ge[o:o + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert rel < 5e-2, (k, rel)
+
+
+def test_risk_head_gradients_match_oracle(zt):
+    import torch
+    B = 512
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 6, feat=4)
+    rng = np.random.default_rng(6)
+    deltas = rng.integers(0, 4000, (B, 512)).astype(np.uint16)
+    deltas[:, ~mlp_ref.VALID] = 65535
+    td = torch.from_numpy(deltas.view(np.int16)).cuda()
+    ws = zt.mlp_workspace(B)
+    stats = torch.zeros(2, dtype=torch.float32, device="cuda")
+    zt.mlp_gradients(st, tb, tl, B, ws, stats, deltas=td, risk_k=4, risk_weight=0.3)
+    torch.cuda.synchronize()
+    g = zt.mlp_grads(st).astype(np.float64)
+    loss, ge = mlp_ref.loss_and_grads(params, blocks, labels, deltas=deltas, risk_k=4, risk_w=0.3, feat=4)
+    assert abs(float(stats[0].item()) - loss) < 2e-2 * abs(loss)
+    for k, (o, s) in mlp_ref.OFF.items():
+        n = int(np.prod(s))
+        a, b = g[o:o + n], ge[o:o + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert rel < 5e-2, (k, rel)
+    rb = mlp_ref.OFF["b3"][0] + mlp_ref.RISK
+    assert abs(g[rb] - ge[rb]) < 2e-2 * abs(ge[rb])
 
 
 def test_training_reduces_loss(zt):
@@ -101,7 +150,7 @@ def test_learned_encode_contract(zt, oracle):
     imgs_full = zt.images([(src, full, 64, 64, 256, "bc7")])
     zt.bc7_full_dev(imgs_full)
     st = zt.mlp_state(seed=9)
-    for thr in (2.0, 0.0):
+    for thr in (-1.0, 1e30):      # every block falls back; no block falls back
         dst = torch.zeros_like(full)
         imgs = zt.images([(src, dst, 64, 64, 256, "bc7")])
         ws = zt.bc7_learned_workspace(imgs, 4)

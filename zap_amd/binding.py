@@ -61,6 +61,11 @@ class ZapImage(C.Structure):
                 ("w", C.c_int), ("h", C.c_int), ("fmt", C.c_int)]
 
 
+class MlpTargets(C.Structure):
+    _fields_ = [("labels", C.c_void_p), ("deltas", C.c_void_p), ("temperature", C.c_float), ("risk_k", C.c_int),
+                ("risk_weight", C.c_float)]
+
+
 def declared_functions(header: str = HEADER) -> list[str]:
     """Names of the functions include/zaptex.h declares."""
     text = open(header).read()
@@ -96,11 +101,12 @@ class Library:
             "zaptex_mip_next_dev": [vp, i, i, sz, i, vp, sz, vp], "zaptex_mip_next_f_dev": [vp, i, i, sz, vp, sz, vp],
             "zaptex_mip_chain_dev": [vp, i, i, sz, i, C.POINTER(vp), i, vp],
             "zaptex_bc_decode_dev": [vp, i, i, i, vp, sz, vp],
-            "zaptex_bc7_full_dev": [C.POINTER(ZapImage), i, vp, vp],
+            "zaptex_bc7_full_dev": [C.POINTER(ZapImage), i, vp, vp, vp, sz, vp],
             "zaptex_mlp_init": [vp, C.c_uint32, vp], "zaptex_mlp_set_params": [vp, vp, vp],
+            "zaptex_mlp_set_features": [vp, i, vp],
             "zaptex_mlp_forward": [vp, vp, i, vp, vp, sz, vp],
-            "zaptex_mlp_gradients": [vp, vp, vp, i, vp, sz, vp, vp],
-            "zaptex_mlp_train_step": [vp, vp, vp, i, f, i, vp, sz, vp, vp],
+            "zaptex_mlp_gradients": [vp, vp, C.POINTER(MlpTargets), i, vp, sz, vp, vp],
+            "zaptex_mlp_train_step": [vp, vp, C.POINTER(MlpTargets), i, f, i, vp, sz, vp, vp],
             "zaptex_bc7_gather_dev": [C.POINTER(ZapImage), i, vp, vp],
             "zaptex_bc7_learned_dev": [C.POINTER(ZapImage), i, vp, i, f, vp, sz, vp],
             "zaptex_set_option": [i, i], "zaptex_get_option": [i],
@@ -117,6 +123,8 @@ class Library:
         L.zaptex_mlp_workspace_size.argtypes = [i]; L.zaptex_mlp_workspace_size.restype = C.c_size_t
         L.zaptex_bc7_learned_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
         L.zaptex_bc7_learned_workspace_size.restype = C.c_size_t
+        L.zaptex_bc7_full_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
+        L.zaptex_bc7_full_workspace_size.restype = C.c_size_t
         L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
         L.zaptex_workspace_size.restype = C.c_size_t
         L.zaptex_last_error.restype = C.c_char_p
@@ -228,9 +236,17 @@ class Library:
         ws_len = workspace.numel() * workspace.element_size() if workspace is not None else 0
         self._check(self.lib.zaptex_encode_dev(imgs, len(imgs), ws_ptr, ws_len, self._stream(stream)), "zaptex_encode_dev")
 
-    def bc7_full_dev(self, imgs, best_cand=None, stream=None) -> None:
-        """Exhaustive all-mode BC7 search; best_cand: optional int32 tensor, one entry per block."""
+    def bc7_full_workspace(self, imgs, with_deltas: bool = False):
+        import torch
+        n = int(self.lib.zaptex_bc7_full_workspace_size(imgs, len(imgs), int(with_deltas)))
+        return torch.empty(n, dtype=torch.uint8, device="cuda")
+
+    def bc7_full_dev(self, imgs, best_cand=None, deltas=None, ws=None, stream=None) -> None:
+        """Exhaustive all-mode search; best_cand: int32 [blocks], deltas: int16/uint16 [blocks, 512] (optional)."""
+        if ws is None:
+            ws = self.bc7_full_workspace(imgs, deltas is not None)
         self._check(self.lib.zaptex_bc7_full_dev(imgs, len(imgs), best_cand.data_ptr() if best_cand is not None else None,
+                                                 deltas.data_ptr() if deltas is not None else None, ws.data_ptr(), ws.numel(),
                                                  self._stream(stream)), "zaptex_bc7_full_dev")
 
     def mip_chain_dev(self, base, w: int, h: int, stride: int, levels: list, srgb: int = 0, stream=None) -> None:
@@ -245,7 +261,7 @@ class Library:
     # ------------------------------------------------------------------ learned proposal (torch tensors on the GPU)
     MLP_PARAMS = 90880
 
-    def mlp_state(self, seed: int | None = 1, params: np.ndarray | None = None):
+    def mlp_state(self, seed: int | None = 1, params: np.ndarray | None = None, features: int = 0):
         import torch
         st = torch.zeros(int(self.lib.zaptex_mlp_state_size()), dtype=torch.uint8, device="cuda")
         if params is not None:
@@ -255,6 +271,7 @@ class Library:
             self._check(self.lib.zaptex_mlp_set_params(st.data_ptr(), _ptr(p), self._stream()), "zaptex_mlp_set_params")
         else:
             self._check(self.lib.zaptex_mlp_init(st.data_ptr(), seed, self._stream()), "zaptex_mlp_init")
+        self._check(self.lib.zaptex_mlp_set_features(st.data_ptr(), features, self._stream()), "zaptex_mlp_set_features")
         return st
 
     @staticmethod
@@ -276,12 +293,22 @@ class Library:
         self._check(self.lib.zaptex_mlp_forward(state.data_ptr(), blocks.data_ptr(), batch, logits.data_ptr(), ws.data_ptr(),
                                                 ws.numel(), self._stream()), "zaptex_mlp_forward")
 
-    def mlp_gradients(self, state, blocks, labels, batch: int, ws, stats) -> None:
-        self._check(self.lib.zaptex_mlp_gradients(state.data_ptr(), blocks.data_ptr(), labels.data_ptr(), batch, ws.data_ptr(), ws.numel(),
+    @staticmethod
+    def _targets(labels, deltas, temperature: float, risk_k: int, risk_weight: float) -> MlpTargets:
+        return MlpTargets(labels.data_ptr() if labels is not None else None, deltas.data_ptr() if deltas is not None else None,
+                          temperature, risk_k, risk_weight)
+
+    def mlp_gradients(self, state, blocks, labels, batch: int, ws, stats, deltas=None, temperature: float = 0.0, risk_k: int = 0,
+                      risk_weight: float = 0.0) -> None:
+        """labels: int32 [batch]; deltas: int16 [batch, 512] (soft targets at temperature > 0, and/or the risk slot)."""
+        t = self._targets(labels, deltas, temperature, risk_k, risk_weight)
+        self._check(self.lib.zaptex_mlp_gradients(state.data_ptr(), blocks.data_ptr(), C.byref(t), batch, ws.data_ptr(), ws.numel(),
                                                   stats.data_ptr(), self._stream()), "zaptex_mlp_gradients")
 
-    def mlp_train_step(self, state, blocks, labels, batch: int, lr: float, step: int, ws, stats) -> None:
-        self._check(self.lib.zaptex_mlp_train_step(state.data_ptr(), blocks.data_ptr(), labels.data_ptr(), batch, lr, step, ws.data_ptr(),
+    def mlp_train_step(self, state, blocks, labels, batch: int, lr: float, step: int, ws, stats, deltas=None,
+                       temperature: float = 0.0, risk_k: int = 0, risk_weight: float = 0.0) -> None:
+        t = self._targets(labels, deltas, temperature, risk_k, risk_weight)
+        self._check(self.lib.zaptex_mlp_train_step(state.data_ptr(), blocks.data_ptr(), C.byref(t), batch, lr, step, ws.data_ptr(),
                                                    ws.numel(), stats.data_ptr(), self._stream()), "zaptex_mlp_train_step")
 
     def bc7_gather_dev(self, imgs, blocks) -> None:

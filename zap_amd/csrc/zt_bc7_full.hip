@@ -302,54 +302,101 @@ __device__ int zt_bc7_eval_candidate(const uint32_t px[16], int cand, uint64_t &
 }
 
 namespace {
-ZT_DEV void keep(int err, int cand, uint64_t lo, uint64_t hi, uint32_t &bkey, uint64_t &blo, uint64_t &bhi) {
-    uint32_t key = (uint32_t)err << 9 | (uint32_t)cand;  // SSE < 2^22: smallest SSE, then lowest candidate
-    if (err >= 0 && key < bkey) { bkey = key; blo = lo; bhi = hi; }
+// the slots of mode M handled by one launch (mode 4 is split by its index selector ISB: slots 2 r + ISB)
+template <int M, int ISB> constexpr int kSlots = M == 0 ? 16 : (M == 4 || M == 5) ? 4 : M == 6 ? 1 : 64;
+template <int M, int ISB> ZT_DEV int slot_of(int s) { return M == 4 ? 2 * s + ISB : s; }
+template <int M, int ISB> ZT_DEV int eval_mode(const uint32_t px[16], int slot, uint64_t &lo, uint64_t &hi) {
+    if constexpr (M == 4) return cand_rotation<4, ISB>(px, slot >> 1, lo, hi);
+    else if constexpr (M == 5) return cand_rotation<5, 0>(px, slot, lo, hi);
+    else return cand_subsets<M>(px, slot, lo, hi);
 }
+ZT_DEV void load_block(const uint4 *blocks, uint32_t g, uint32_t px[16]) {
+    const uint4 *s = blocks + (uint64_t)g * 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+}
+// SSE first, then the lowest candidate; SSE < 2^22 for a 4x4 RGBA8 block
+ZT_DEV uint32_t cand_key(int err, int cand) { return err < 0 ? 0xFFFFFFFFu : (uint32_t)err << 9 | (uint32_t)cand; }
 }  // namespace
 
-// Exhaustive search: one wave per block; within each mode the lanes take the partitions / rotations, so a
-// wave runs one mode's code at a time; a wave min over (SSE, candidate) picks the block.
-// `list`/`nlist` (optional) restrict the search to listed global blocks (the learned path's fallback).
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_full(zt_jobs jobs, uint32_t total, const uint32_t *list, const unsigned *nlist,
-                                                                uint32_t *best_cand) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t count = list ? *nlist : total;
-    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < count; e += nw) {
-        const uint32_t g = list ? list[e] : e;
-        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
-        const uint32_t b = g - J.first;
-        uint32_t px[16];
-        gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
-        uint32_t bkey = 0xFFFFFFFFu;
-        uint64_t blo = 0, bhi = 0, lo, hi;
-        int err;
-        if (lane < 16) { err = cand_subsets<0>(px, lane, lo, hi); keep(err, lane, lo, hi, bkey, blo, bhi); }
-        err = cand_subsets<1>(px, lane, lo, hi); keep(err, 64 + lane, lo, hi, bkey, blo, bhi);
-        err = cand_subsets<2>(px, lane, lo, hi); keep(err, 128 + lane, lo, hi, bkey, blo, bhi);
-        err = cand_subsets<3>(px, lane, lo, hi); keep(err, 192 + lane, lo, hi, bkey, blo, bhi);
-        if (lane < 8) {
-            err = (lane & 1) ? cand_rotation<4, 1>(px, lane >> 1, lo, hi) : cand_rotation<4, 0>(px, lane >> 1, lo, hi);
-            keep(err, 256 + lane, lo, hi, bkey, blo, bhi);
+// Exhaustive search, one mode (mode 4: one index selector) per launch.  Lane = (block, slot), the S slots of a
+// block in adjacent lanes of one wave, so a wave never mixes modes and each launch holds only its mode's code;
+// the S keys reduce inside the wave and one atomicMin per block folds them into key[g] (initialised to ~0).
+// `list`/`nlist` (optional) restrict the search to listed blocks (the learned path's fallback); `errs`
+// (optional, 512 per block, pre-filled with ~0) receive every candidate's SSE.
+template <int M, int ISB>
+ZT_DEV void exh_body(const uint4 *blocks, uint32_t total, const uint32_t *list, const unsigned *nlist, uint32_t *key, uint32_t *errs) {
+    constexpr int S = kSlots<M, ISB>;
+    const uint64_t count = (uint64_t)(list ? *nlist : total) * S;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < count; base += step) {
+        const uint64_t i = base + threadIdx.x;
+        const bool valid = i < count;  // count is a multiple of S, so a block's S lanes are all valid or all not
+        uint32_t k = 0xFFFFFFFFu, g = 0;
+        if (valid) {
+            const uint32_t e = (uint32_t)(i / S);
+            const int slot = slot_of<M, ISB>((int)(i % S));
+            g = list ? list[e] : e;
+            uint32_t px[16];
+            load_block(blocks, g, px);
+            uint64_t lo, hi;
+            const int err = eval_mode<M, ISB>(px, slot, lo, hi);
+            k = cand_key(err, M * 64 + slot);
+            if (errs) errs[(uint64_t)g * 512 + M * 64 + slot] = err < 0 ? 0xFFFFFFFFu : (uint32_t)err;
         }
-        if (lane < 4) { err = cand_rotation<5, 0>(px, lane, lo, hi); keep(err, 320 + lane, lo, hi, bkey, blo, bhi); }
-        if (lane == 0) { err = cand_subsets<6>(px, 0, lo, hi); keep(err, 384, lo, hi, bkey, blo, bhi); }
-        err = cand_subsets<7>(px, lane, lo, hi); keep(err, 448 + lane, lo, hi, bkey, blo, bhi);
-        uint32_t m = bkey;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) m = min(m, (uint32_t)__shfl_xor(m, off));
-        if (bkey == m) {
-            store16(J.dst + (uint64_t)b * 16, blo, bhi);
-            if (best_cand) best_cand[g] = m & 511u;
-        }
+        for (int off = 1; off < S; off <<= 1) k = min(k, (uint32_t)__shfl_xor(k, off));
+        if (valid && i % S == 0) atomicMin(key + g, k);
+    }
+}
+#define ZT_EXH(NAME, M, ISB)                                                                                       \
+    extern "C" __global__ void __launch_bounds__(256) NAME(const uint4 *blocks, uint32_t total, const uint32_t *list,          \
+                                                           const unsigned *nlist, uint32_t *key, uint32_t *errs) {            \
+        exh_body<M, ISB>(blocks, total, list, nlist, key, errs);                                                               \
+    }
+ZT_EXH(zt_bc7_exh_m0, 0, 0)
+ZT_EXH(zt_bc7_exh_m1, 1, 0)
+ZT_EXH(zt_bc7_exh_m2, 2, 0)
+ZT_EXH(zt_bc7_exh_m3, 3, 0)
+ZT_EXH(zt_bc7_exh_m4i0, 4, 0)
+ZT_EXH(zt_bc7_exh_m4i1, 4, 1)
+ZT_EXH(zt_bc7_exh_m5, 5, 0)
+ZT_EXH(zt_bc7_exh_m6, 6, 0)
+ZT_EXH(zt_bc7_exh_m7, 7, 0)
+#undef ZT_EXH
+
+// The winner of every (listed) block: its candidate re-encoded (one candidate per block, so the all-mode switch
+// costs little), the 16 bytes stored, and the candidate slot written to best_cand (optional).
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key,
+                                                                  uint32_t *best_cand) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const uint32_t k = key[g];
+        if (k == 0xFFFFFFFFu) continue;
+        uint32_t px[16];
+        load_block(blocks, g, px);
+        uint64_t lo = 0, hi = 0;
+        zt_bc7_eval_candidate(px, (int)(k & 511u), lo, hi);
+        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        store16(J.dst + (uint64_t)(g - J.first) * 16, lo, hi);
+        if (best_cand) best_cand[g] = k & 511u;
+    }
+}
+
+// deltas[g][c] = min(65535, errs[g][c] - best SSE of block g), 65535 where the slot has no candidate
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_deltas(const uint32_t *errs, const uint32_t *key, uint32_t total, uint16_t *deltas) {
+    const uint64_t n = (uint64_t)total * 512, step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint32_t e = errs[i], best = key[i >> 9] >> 9;
+        deltas[i] = e == 0xFFFFFFFFu ? (uint16_t)65535 : (uint16_t)min(65535u, e - best);
     }
 }
 
 // ===================================================================================== learned proposal
 // The network proposes K candidates per block (zt_mlp.hip); they are bucketed by mode so that every wave of
-// the exact evaluation runs one mode's code, evaluated exactly, and the best of the K is kept.  Blocks whose
-// top-1 probability is below the threshold are searched exhaustively instead (zt_bc7_full over a list).
+// the exact evaluation runs one mode's code, and each evaluation folds its key into the block's key.  Blocks
+// whose predicted SSE cost of trusting the K (the network's risk output) exceeds the threshold get the
+// exhaustive launches over a list instead; zt_bc7_finish then encodes every block's winner.
 
 // gather every block of the jobs into a dense array (16 RGBA8 pixels = 4 x uint4 per block)
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks) {
@@ -366,31 +413,39 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_gather(zt_jobs jobs, ui
 }
 
 // pass 0: count proposals per mode (counters[0..7]); pass 1: scatter them into mode-contiguous runs starting
-// at counters[16 + mode] (prefix sums filled in between by zt_bc7_bucket_scan).  Low-confidence blocks go
-// to the fallback list (counters[8]) and get no proposals; their K keys are set to "none".
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *cand, const float *conf, uint32_t total, int K,
-                                                                  float threshold, unsigned *counters, uint4 *items, uint32_t pass,
-                                                                  uint32_t *fallback, uint32_t *keys) {
+// at counters[16 + mode] (prefix sums filled in between by zt_bc7_bucket_scan), and high-risk blocks into the
+// fallback list (counters[8]; they get no proposals).  Each workgroup tallies in LDS and reserves its ranges with
+// one global atomic per mode, so the global counters see a few atomics per workgroup, not one per proposal.
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K,
+                                                                  float threshold, unsigned *counters, uint2 *items, uint32_t pass,
+                                                                  uint32_t *fallback) {
+    __shared__ unsigned cnt[9], base[9];
     const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += step) {
-        const uint32_t g = base + threadIdx.x;
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < total; b0 += step) {
+        if (threadIdx.x < 9) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t g = b0 + threadIdx.x;
         const bool valid = g < total;
-        const bool fb = valid && conf[g] < threshold;
-        if (pass == 1) {
-            int slot = wave_append(fb, &counters[8]);
-            if (slot >= 0) {
-                fallback[slot] = g;
-                for (int k = 0; k < K; k++) keys[(uint64_t)g * K + k] = 0xFFFFFFFFu;
-            }
-        }
+        const bool fb = valid && risk[g] > threshold;
+        uint32_t c[16];
+        unsigned local[16], lfb = 0;
         for (int k = 0; k < K; k++) {
-            const uint32_t c = valid && !fb ? cand[(uint64_t)g * K + k] : 0xFFFFFFFFu;
-            for (int m = 0; m < 8; m++) {
-                const bool p = c != 0xFFFFFFFFu && (int)(c >> 6) == m;
-                int slot = wave_append(p, pass == 0 ? &counters[m] : &counters[16 + m]);
-                if (pass == 1 && slot >= 0) items[slot] = make_uint4(g, (uint32_t)k, c, 0);
-            }
+            c[k] = valid && !fb ? cand[(uint64_t)g * K + k] : 0xFFFFFFFFu;
+            if (c[k] != 0xFFFFFFFFu) local[k] = atomicAdd(&cnt[c[k] >> 6], 1u);
         }
+        if (pass == 1 && fb) lfb = atomicAdd(&cnt[8], 1u);
+        __syncthreads();
+        if (threadIdx.x < 9 && cnt[threadIdx.x]) {
+            if (pass == 0) { if (threadIdx.x < 8) atomicAdd(&counters[threadIdx.x], cnt[threadIdx.x]); }
+            else base[threadIdx.x] = atomicAdd(&counters[threadIdx.x == 8 ? 8 : 16 + threadIdx.x], cnt[threadIdx.x]);
+        }
+        __syncthreads();
+        if (pass == 1) {
+            for (int k = 0; k < K; k++)
+                if (c[k] != 0xFFFFFFFFu) items[base[c[k] >> 6] + local[k]] = make_uint2(g, c[k]);
+            if (fb) fallback[base[8] + lfb] = g;
+        }
+        __syncthreads();
     }
 }
 extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
@@ -400,33 +455,36 @@ extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
     }
 }
 
-// exact evaluation of the proposals of one mode: key = SSE << 9 | candidate, and the block bits, per (block, k)
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_eval_items(const uint4 *blocks, const uint4 *items, const unsigned *counters, int mode,
-                                                                      uint32_t *keys, uint4 *bits, int K) {
-    const uint32_t begin = counters[24 + mode], n = counters[mode], step = gridDim.x * blockDim.x;
+// exact evaluation of the proposals of mode M (lane per proposal), folded into the blocks' keys
+template <int M>
+ZT_DEV void prop_body(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key) {
+    const uint32_t begin = counters[24 + M], n = counters[M], step = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-        const uint4 it = items[begin + i];
+        const uint2 it = items[begin + i];
         uint32_t px[16];
-        const uint4 *s = blocks + (uint64_t)it.x * 4;
-#pragma unroll
-        for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
-        uint64_t lo = 0, hi = 0;
-        int e = zt_bc7_eval_candidate(px, (int)it.z, lo, hi);
-        const uint64_t o = (uint64_t)it.x * K + it.y;
-        keys[o] = e < 0 ? 0xFFFFFFFFu : ((uint32_t)e << 9 | it.z);
-        bits[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+        load_block(blocks, it.x, px);
+        uint64_t lo, hi;
+        const int slot = (int)(it.y & 63u);
+        int err = -1;
+        if constexpr (M == 4) {
+            if (slot < 8) err = (slot & 1) ? eval_mode<4, 1>(px, slot, lo, hi) : eval_mode<4, 0>(px, slot, lo, hi);
+        } else if (slot < kSlots<M, 0>) {
+            err = eval_mode<M, 0>(px, slot, lo, hi);
+        }
+        atomicMin(key + it.x, cand_key(err, (int)it.y));
     }
 }
-
-// best of the K evaluated proposals per block (smallest SSE, then lowest candidate); fallback blocks skipped
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_select(zt_jobs jobs, uint32_t total, const uint32_t *keys, const uint4 *bits, int K) {
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
-        uint32_t best = 0xFFFFFFFFu;
-        int bk = 0;
-        for (int k = 0; k < K; k++) { uint32_t v = keys[(uint64_t)g * K + k]; if (v < best) { best = v; bk = k; } }
-        if (best == 0xFFFFFFFFu) continue;
-        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
-        *(uint4 *)(J.dst + (uint64_t)(g - J.first) * 16) = bits[(uint64_t)g * K + bk];
+#define ZT_PROP(M)                                                                                                   \
+    extern "C" __global__ void __launch_bounds__(256) zt_bc7_prop_m##M(const uint4 *blocks, const uint2 *items,               \
+                                                                        const unsigned *counters, uint32_t *key) {           \
+        prop_body<M>(blocks, items, counters, key);                                                                            \
     }
-}
+ZT_PROP(0)
+ZT_PROP(1)
+ZT_PROP(2)
+ZT_PROP(3)
+ZT_PROP(4)
+ZT_PROP(5)
+ZT_PROP(6)
+ZT_PROP(7)
+#undef ZT_PROP

@@ -442,28 +442,79 @@ int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t w
     return enqueue(imgs, n, workspace, ws_bytes, (hipStream_t)stream);
 }
 
-int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, void *stream) {
+namespace {
+// job table of BC7 images (all blocks numbered in image order); total blocks out
+int bc7_jobs(const zaptex_image *imgs, int n, zt_jobs &J, uint32_t &total, const char *what) {
     if (!imgs || n < 1 || n > ZT_INLINE_JOBS) return fail(ZAPTEX_E_ARG, "image count %d outside 1..%d", n, ZT_INLINE_JOBS);
-    zt_jobs J;
     memset(&J, 0, sizeof J);
-    uint32_t total = 0;
+    uint64_t t = 0;
     for (int i = 0; i < n; i++) {
         if (int rc = check_image(imgs[i])) return rc;
-        if (imgs[i].fmt != ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "the exhaustive search encodes BC7 only");
+        if (imgs[i].fmt != ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "%s encodes BC7 only", what);
         zt_job &j = J.inl[J.n++];
         j.src = (const uint8_t *)imgs[i].src; j.dst = (uint8_t *)imgs[i].dst; j.stride = imgs[i].stride;
-        j.first = total; j.w = imgs[i].w; j.h = imgs[i].h; j.bw = (imgs[i].w + 3) / 4; j.fmt = ZAPTEX_BC7; j.pad = 0;
-        total += nblocks(imgs[i]);
+        j.first = (uint32_t)t; j.w = imgs[i].w; j.h = imgs[i].h; j.bw = (imgs[i].w + 3) / 4; j.fmt = ZAPTEX_BC7; j.pad = 0;
+        t += nblocks(imgs[i]);
     }
+    if (t >= (1ull << 26)) return fail(ZAPTEX_E_ARG, "too many blocks in one call (%llu)", (unsigned long long)t);
+    total = (uint32_t)t;
+    return ZAPTEX_OK;
+}
+// the nine per-mode exhaustive launches over `total` dense blocks, or over the device list (list, *nlist <= total)
+int enqueue_exhaustive(const uint4 *blocks, uint32_t total, const uint32_t *list, const unsigned *nlist, uint32_t *key, uint32_t *errs,
+                       hipStream_t s) {
+    typedef void (*Exh)(const uint4 *, uint32_t, const uint32_t *, const unsigned *, uint32_t *, uint32_t *);
+    struct K { const char *name; Exh fn; int slots; } ks[9] = {
+        { "zt_bc7_exh_m0", zt_bc7_exh_m0, 16 }, { "zt_bc7_exh_m1", zt_bc7_exh_m1, 64 }, { "zt_bc7_exh_m2", zt_bc7_exh_m2, 64 },
+        { "zt_bc7_exh_m3", zt_bc7_exh_m3, 64 }, { "zt_bc7_exh_m4i0", zt_bc7_exh_m4i0, 4 }, { "zt_bc7_exh_m4i1", zt_bc7_exh_m4i1, 4 },
+        { "zt_bc7_exh_m5", zt_bc7_exh_m5, 4 }, { "zt_bc7_exh_m6", zt_bc7_exh_m6, 1 }, { "zt_bc7_exh_m7", zt_bc7_exh_m7, 64 } };
+    for (const K &k : ks)
+        if (int rc = launch(k.name, s, [&] {
+                hipLaunchKernelGGL(k.fn, dim3(grid_for((uint64_t)total * k.slots, 256)), dim3(256), 0, s, blocks, total, list, nlist, key, errs); }))
+            return rc;
+    return ZAPTEX_OK;
+}
+size_t full_ws_bytes(uint64_t total, int with_deltas) { return 256 + total * (64 + 4) + (with_deltas ? total * 2048 + 256 : 0); }
+}  // namespace
+
+size_t zaptex_bc7_full_workspace_size(const zaptex_image *imgs, int n, int with_deltas) {
+    uint64_t t = 0;
+    for (int i = 0; i < n; i++) t += nblocks(imgs[i]);
+    return full_ws_bytes(t, with_deltas);
+}
+
+int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, uint16_t *deltas, void *ws, size_t ws_bytes, void *stream) {
+    zt_jobs J;
+    uint32_t total;
+    if (int rc = bc7_jobs(imgs, n, J, total, "the exhaustive search")) return rc;
+    if (!ws || ws_bytes < full_ws_bytes(total, deltas != nullptr))
+        return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, full_ws_bytes(total, deltas != nullptr));
     DeviceState *st;
     {
         std::lock_guard<std::mutex> lk(g_mu);
         if (int rc = ensure_ready(st)) return rc;
     }
     hipStream_t s = (hipStream_t)stream;
-    return launch("zt_bc7_full", s, [&] {
-        hipLaunchKernelGGL(zt_bc7_full, dim3(grid_for(total, 4)), dim3(256), 0, s, J, total, (const uint32_t *)nullptr, (const unsigned *)nullptr, best_cand);
-    });
+    uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
+    uint4 *blocks = (uint4 *)b; b += (uint64_t)total * 64;
+    uint32_t *key = (uint32_t *)b; b += (uint64_t)total * 4;
+    uint32_t *errs = deltas ? (uint32_t *)align_up((uintptr_t)b, 256) : nullptr;
+    int rc;
+    if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, blocks); })))
+        return rc;
+    if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(grid_for(total, 256)), dim3(256), 0, s, key, (uint64_t)total, 0xFFFFFFFFu); })))
+        return rc;
+    if (errs && (rc = launch("zt_fill_u32", s, [&] {
+                     hipLaunchKernelGGL(zt_fill_u32, dim3(grid_for((uint64_t)total * 512, 256)), dim3(256), 0, s, errs, (uint64_t)total * 512, 0xFFFFFFFFu); })))
+        return rc;
+    if ((rc = enqueue_exhaustive(blocks, total, nullptr, nullptr, key, errs, s))) return rc;
+    if ((rc = launch("zt_bc7_finish", s, [&] {
+            hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total, 256)), dim3(256), 0, s, J, (const uint4 *)blocks, total, (const uint32_t *)key, best_cand); })))
+        return rc;
+    if (!deltas) return ZAPTEX_OK;
+    return launch("zt_bc7_deltas", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_deltas, dim3(grid_for((uint64_t)total * 512, 256)), dim3(256), 0, s, (const uint32_t *)errs, (const uint32_t *)key, total,
+                           deltas); });
 }
 
 // ===================================================================================== proposal network
@@ -477,6 +528,7 @@ struct MlpState {  // views into the caller's state buffer
     float *p, *g, *m1, *m2, *scale;
     int *bfi;
     uint16_t *w;
+    int *cfg;  // [0] feature normalisation (zt_mlp_features)
 };
 MlpState mlp_view(void *state) {
     uint8_t *b = (uint8_t *)state;
@@ -484,9 +536,10 @@ MlpState mlp_view(void *state) {
     v.p = (float *)b; v.g = v.p + kP; v.m1 = v.g + kP; v.m2 = v.m1 + kP; v.scale = v.m2 + kP;
     v.bfi = (int *)(v.scale + kP);
     v.w = (uint16_t *)(v.bfi + kP);
+    v.cfg = (int *)(v.w + kPb);
     return v;
 }
-size_t mlp_state_bytes() { return (size_t)kP * 4 * 6 + (size_t)kPb * 2; }
+size_t mlp_state_bytes() { return (size_t)kP * 4 * 6 + (size_t)kPb * 2 + 256; }
 struct MlpWs { uint16_t *X, *H1, *H2, *dZ, *dH2, *dH1; float *Z, *stats; };
 size_t mlp_ws_bytes(int B) { return align_up((size_t)B * (kIn + 2 * kH + kOut + 2 * kH) * 2 + (size_t)B * kOut * 4 + 256, 256); }
 MlpWs mlp_ws(void *ws, int B) {
@@ -516,7 +569,7 @@ int run_gemm(const zt_gemm_args &g, int ksplit, hipStream_t s) {
 int mlp_forward(const MlpState &st, const void *blocks, int B, const MlpWs &w, hipStream_t s) {
     int rc;
     if ((rc = launch("zt_mlp_features", s, [&] {
-            hipLaunchKernelGGL(zt_mlp_features, dim3((unsigned)(((int64_t)B * 64 + 255) / 256)), dim3(256), 0, s, (const uint8_t *)blocks, B, w.X); })))
+            hipLaunchKernelGGL(zt_mlp_features, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, (const uint8_t *)blocks, B, (const int *)st.cfg, w.X); })))
         return rc;
     zt_gemm_args g = gemm(w.X, kIn, 1, st.w + bW1, 1, kIn, B, kH, kIn);
     g.bias = st.p + oB1; g.relu = 1; g.Cb = w.H1; g.ldc = kH;
@@ -528,14 +581,17 @@ int mlp_forward(const MlpState &st, const void *blocks, int B, const MlpWs &w, h
     g.bias = st.p + oB3; g.Cf = w.Z; g.ldc = kOut;
     return run_gemm(g, 1, s);
 }
-// loss and gradients of the mean cross-entropy into st.g (zeroed first); stats = (loss, top-1 hits)
-int mlp_backward(const MlpState &st, const uint32_t *labels, int B, const MlpWs &w, hipStream_t s) {
+// loss and gradients into st.g (zeroed first); stats = (loss, top-1 hits).  Targets: the one-hot labels, or with
+// deltas and a temperature the softmax of -delta / temperature over the candidate slots; with deltas and risk_k,
+// also the risk slot's regression.
+int mlp_backward(const MlpState &st, const zaptex_mlp_targets &t, int B, const MlpWs &w, hipStream_t s) {
     int rc;
     if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(256), dim3(256), 0, s, (uint32_t *)st.g, (uint64_t)kP, 0u); })))
         return rc;
     if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)w.stats, (uint64_t)64, 0u); })))
         return rc;
-    if ((rc = launch("zt_mlp_xent", s, [&] { hipLaunchKernelGGL(zt_mlp_xent, dim3((B + 3) / 4), dim3(256), 0, s, w.Z, labels, B, w.dZ, w.stats); })))
+    if ((rc = launch("zt_mlp_xent", s, [&] { hipLaunchKernelGGL(zt_mlp_xent, dim3((B + 3) / 4), dim3(256), 0, s, w.Z, t.labels, t.deltas,
+                               t.temperature > 0.0f ? 1.0f / t.temperature : 0.0f, t.deltas ? t.risk_k : 0, t.risk_weight, B, w.dZ, w.stats); })))
         return rc;
     const int ks = std::max(1, std::min(64, B / 256));
     auto wgrad = [&](const uint16_t *act, int in, const uint16_t *d, int out, int64_t off) {  // dW = act^T d
@@ -590,6 +646,8 @@ int zaptex_mlp_init(void *state, uint32_t seed, void *stream) {
     int rc;
     if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(512), dim3(256), 0, s, (uint32_t *)st.g, (uint64_t)kP * 3, 0u); })))
         return rc;
+    if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.cfg, (uint64_t)64, 0u); })))
+        return rc;
     return launch("zt_mlp_init", s, [&] {
         hipLaunchKernelGGL(zt_mlp_init, dim3((unsigned)((kP + 255) / 256)), dim3(256), 0, s, st.p, (int)kP, seed, st.w, (const int *)st.bfi, (const float *)st.scale); });
 }
@@ -613,6 +671,13 @@ int zaptex_mlp_set_params(void *state, const float *host_params, void *stream) {
     });
 }
 
+int zaptex_mlp_set_features(void *state, int mode, void *stream) {
+    if (!state || mode < 0 || mode > 4) return fail(ZAPTEX_E_ARG, "feature mode must be 0..4");
+    MlpState st = mlp_view(state);
+    hipStream_t s = (hipStream_t)stream;
+    return launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)st.cfg, (uint64_t)1, (uint32_t)mode); });
+}
+
 int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *logits, void *ws, size_t ws_bytes, void *stream) {
     if (!state || !blocks || batch < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
     if (!ws || ws_bytes < mlp_ws_bytes(batch)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, mlp_ws_bytes(batch));
@@ -624,21 +689,25 @@ int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *
     return ZAPTEX_OK;
 }
 
-int zaptex_mlp_gradients(void *state, const void *blocks, const uint32_t *labels, int batch, void *ws, size_t ws_bytes, float *stats, void *stream) {
-    if (!state || !blocks || !labels || batch < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
+int zaptex_mlp_gradients(void *state, const void *blocks, const zaptex_mlp_targets *targets, int batch, void *ws, size_t ws_bytes, float *stats,
+                         void *stream) {
+    if (!state || !blocks || !targets || batch < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
+    const zaptex_mlp_targets &t = *targets;
+    if (!t.labels && !(t.deltas && t.temperature > 0.0f)) return fail(ZAPTEX_E_ARG, "need labels, or deltas with a positive temperature");
+    if (t.risk_k < 0 || t.risk_k > 16 || (t.risk_k > 0 && !t.deltas)) return fail(ZAPTEX_E_ARG, "risk_k must be 0..16 and needs deltas");
     if (!ws || ws_bytes < mlp_ws_bytes(batch)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, mlp_ws_bytes(batch));
     MlpState st = mlp_view(state);
     MlpWs w = mlp_ws(ws, batch);
     hipStream_t s = (hipStream_t)stream;
     if (int rc = mlp_forward(st, blocks, batch, w, s)) return rc;
-    if (int rc = mlp_backward(st, labels, batch, w, s)) return rc;
+    if (int rc = mlp_backward(st, t, batch, w, s)) return rc;
     if (stats) HIP_TRY(hipMemcpyAsync(stats, w.stats, 8, hipMemcpyDeviceToDevice, s));
     return ZAPTEX_OK;
 }
 
-int zaptex_mlp_train_step(void *state, const void *blocks, const uint32_t *labels, int batch, float lr, int step, void *ws, size_t ws_bytes,
-                          float *stats, void *stream) {
-    if (int rc = zaptex_mlp_gradients(state, blocks, labels, batch, ws, ws_bytes, stats, stream)) return rc;
+int zaptex_mlp_train_step(void *state, const void *blocks, const zaptex_mlp_targets *targets, int batch, float lr, int step, void *ws,
+                          size_t ws_bytes, float *stats, void *stream) {
+    if (int rc = zaptex_mlp_gradients(state, blocks, targets, batch, ws, ws_bytes, stats, stream)) return rc;
     MlpState st = mlp_view(state);
     hipStream_t s = (hipStream_t)stream;
     const float b1 = 0.9f, b2 = 0.999f, c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
@@ -668,20 +737,19 @@ int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *s
 // ---- learned-proposal BC7 encode
 namespace {
 constexpr int kChunk = 1 << 17;  // network rows per forward pass
-struct LearnedWs { uint4 *blocks, *items, *bits; uint32_t *cand, *fallback, *keys; float *conf; unsigned *counters; void *mlp; };
+struct LearnedWs { uint4 *blocks; uint2 *items; uint32_t *cand, *fallback, *key; float *risk; unsigned *counters; void *mlp; };
 size_t learned_ws_bytes(uint64_t total, int K) {
-    return 256 + total * (64 + 4 * K + 4 + 16 * K + 4 + 4 * K + 16 * K) + 1024 + mlp_ws_bytes((int)std::min<uint64_t>(total, kChunk));
+    return 256 + 256 + total * (64 + 8 * K + 4 * K + 4 + 4 + 4) + 1024 + mlp_ws_bytes((int)std::min<uint64_t>(total, kChunk));
 }
 LearnedWs learned_ws(void *ws, uint64_t total, int K) {
-    uint8_t *b = (uint8_t *)ws;
+    uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
     LearnedWs v;
     v.counters = (unsigned *)b; b += 256;
     v.blocks = (uint4 *)b; b += total * 64;
-    v.items = (uint4 *)b; b += total * 16 * K;
-    v.bits = (uint4 *)b; b += total * 16 * K;
+    v.items = (uint2 *)b; b += total * 8 * K;
     v.cand = (uint32_t *)b; b += total * 4 * K;
-    v.keys = (uint32_t *)b; b += total * 4 * K;
-    v.conf = (float *)b; b += total * 4;
+    v.key = (uint32_t *)b; b += total * 4;
+    v.risk = (float *)b; b += total * 4;
     v.fallback = (uint32_t *)b; b += total * 4;
     v.mlp = (void *)align_up((uintptr_t)b, 256);
     return v;
@@ -696,19 +764,11 @@ size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int to
 
 int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, void *ws, size_t ws_bytes,
                            void *stream) {
-    if (!imgs || n < 1 || n > ZT_INLINE_JOBS || !state) return fail(ZAPTEX_E_ARG, "bad arguments");
+    if (!state) return fail(ZAPTEX_E_ARG, "null network state");
     if (topk < 1 || topk > 16) return fail(ZAPTEX_E_ARG, "topk must be 1..16");
     zt_jobs J;
-    memset(&J, 0, sizeof J);
-    uint32_t total = 0;
-    for (int i = 0; i < n; i++) {
-        if (int rc = check_image(imgs[i])) return rc;
-        if (imgs[i].fmt != ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "the learned proposal encodes BC7 only");
-        zt_job &j = J.inl[J.n++];
-        j.src = (const uint8_t *)imgs[i].src; j.dst = (uint8_t *)imgs[i].dst; j.stride = imgs[i].stride;
-        j.first = total; j.w = imgs[i].w; j.h = imgs[i].h; j.bw = (imgs[i].w + 3) / 4; j.fmt = ZAPTEX_BC7; j.pad = 0;
-        total += nblocks(imgs[i]);
-    }
+    uint32_t total;
+    if (int rc = bc7_jobs(imgs, n, J, total, "the learned proposal")) return rc;
     if (!ws || ws_bytes < learned_ws_bytes(total, topk)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, learned_ws_bytes(total, topk));
     DeviceState *dst;
     {
@@ -720,6 +780,8 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
     MlpState st = mlp_view((void *)state);
     int rc;
     if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, w.counters, 64); }))) return rc;
+    if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(grid_for(total, 256)), dim3(256), 0, s, w.key, (uint64_t)total, 0xFFFFFFFFu); })))
+        return rc;
     if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, w.blocks); })))
         return rc;
     for (uint32_t c0 = 0; c0 < total; c0 += kChunk) {
@@ -727,29 +789,30 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
         MlpWs mw = mlp_ws(w.mlp, B);
         if ((rc = mlp_forward(st, w.blocks + (uint64_t)c0 * 4, B, mw, s))) return rc;
         if ((rc = launch("zt_mlp_topk", s, [&] {
-                hipLaunchKernelGGL(zt_mlp_topk, dim3((B + 3) / 4), dim3(256), 0, s, mw.Z, B, topk, w.cand + (uint64_t)c0 * topk, w.conf + c0); })))
+                hipLaunchKernelGGL(zt_mlp_topk, dim3((B + 3) / 4), dim3(256), 0, s, mw.Z, B, topk, w.cand + (uint64_t)c0 * topk, w.risk + c0); })))
             return rc;
     }
     for (uint32_t pass = 0; pass < 2; pass++) {
         if ((rc = launch("zt_bc7_bucket", s, [&] {
-                hipLaunchKernelGGL(zt_bc7_bucket, dim3(grid_for(total, 256)), dim3(256), 0, s, (const uint32_t *)w.cand, (const float *)w.conf, total, topk,
-                                   threshold, w.counters, w.items, pass, w.fallback, w.keys); })))
+                hipLaunchKernelGGL(zt_bc7_bucket, dim3(grid_for(total, 256)), dim3(256), 0, s, (const uint32_t *)w.cand, (const float *)w.risk, total, topk,
+                                   threshold, w.counters, w.items, pass, w.fallback); })))
             return rc;
         if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters); })))
             return rc;
     }
+    typedef void (*Prop)(const uint4 *, const uint2 *, const unsigned *, uint32_t *);
+    const Prop props[8] = { zt_bc7_prop_m0, zt_bc7_prop_m1, zt_bc7_prop_m2, zt_bc7_prop_m3, zt_bc7_prop_m4, zt_bc7_prop_m5, zt_bc7_prop_m6, zt_bc7_prop_m7 };
+    static const char *prop_names[8] = { "zt_bc7_prop_m0", "zt_bc7_prop_m1", "zt_bc7_prop_m2", "zt_bc7_prop_m3",
+                                         "zt_bc7_prop_m4", "zt_bc7_prop_m5", "zt_bc7_prop_m6", "zt_bc7_prop_m7" };
     for (int m = 0; m < 8; m++)
-        if ((rc = launch("zt_bc7_eval_items", s, [&] {
-                hipLaunchKernelGGL(zt_bc7_eval_items, dim3(grid_for((uint64_t)total * topk, 256)), dim3(256), 0, s, (const uint4 *)w.blocks,
-                                   (const uint4 *)w.items, (const unsigned *)w.counters, m, w.keys, w.bits, topk); })))
+        if ((rc = launch(prop_names[m], s, [&] {
+                hipLaunchKernelGGL(props[m], dim3(grid_for((uint64_t)total * topk, 256)), dim3(256), 0, s, (const uint4 *)w.blocks, (const uint2 *)w.items,
+                                   (const unsigned *)w.counters, w.key); })))
             return rc;
-    if ((rc = launch("zt_bc7_select", s, [&] {
-            hipLaunchKernelGGL(zt_bc7_select, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, (const uint32_t *)w.keys, (const uint4 *)w.bits, topk); })))
-        return rc;
-    return launch("zt_bc7_full", s, [&] {
-        hipLaunchKernelGGL(zt_bc7_full, dim3(grid_for(total, 4)), dim3(256), 0, s, J, total, (const uint32_t *)w.fallback, (const unsigned *)(w.counters + 8),
-                           (uint32_t *)nullptr);
-    });
+    if ((rc = enqueue_exhaustive(w.blocks, total, w.fallback, w.counters + 8, w.key, nullptr, s))) return rc;
+    return launch("zt_bc7_finish", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total, 256)), dim3(256), 0, s, J, (const uint4 *)w.blocks, total, (const uint32_t *)w.key,
+                           (uint32_t *)nullptr); });
 }
 
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream) {

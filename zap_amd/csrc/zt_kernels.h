@@ -44,27 +44,38 @@ __global__ void zt_zero_u32(unsigned *p, int n);
 __global__ void zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand);
 __global__ void zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state);
 __global__ void zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err, const uint4 *sub_state);
-// exhaustive all-mode BC7 search (optionally restricted to listed global blocks); best_cand may be null
-__global__ void zt_bc7_full(zt_jobs jobs, uint32_t total, const uint32_t *list, const unsigned *nlist, uint32_t *best_cand);
+// exhaustive all-mode BC7 search: one launch per mode (mode 4: per index selector) over dense blocks, optionally a
+// list of them, folding (SSE << 9 | candidate) into key[block]; zt_bc7_finish encodes the winners
+#define ZT_EXH_DECL(NAME) __global__ void NAME(const uint4 *blocks, uint32_t total, const uint32_t *list, const unsigned *nlist, \
+                                              uint32_t *key, uint32_t *errs);
+ZT_EXH_DECL(zt_bc7_exh_m0) ZT_EXH_DECL(zt_bc7_exh_m1) ZT_EXH_DECL(zt_bc7_exh_m2) ZT_EXH_DECL(zt_bc7_exh_m3)
+ZT_EXH_DECL(zt_bc7_exh_m4i0) ZT_EXH_DECL(zt_bc7_exh_m4i1) ZT_EXH_DECL(zt_bc7_exh_m5) ZT_EXH_DECL(zt_bc7_exh_m6) ZT_EXH_DECL(zt_bc7_exh_m7)
+#undef ZT_EXH_DECL
+__global__ void zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key, uint32_t *best_cand);
+__global__ void zt_bc7_deltas(const uint32_t *errs, const uint32_t *key, uint32_t total, uint16_t *deltas);
 
 __global__ void zt_gemm_bf16(zt_gemm_args g);
-__global__ void zt_mlp_features(const uint8_t *blocks, int n, uint16_t *X);
-__global__ void zt_mlp_xent(const float *Z, const uint32_t *labels, int B, uint16_t *dZ, float *stats);
+__global__ void zt_mlp_features(const uint8_t *blocks, int n, const int *cfg, uint16_t *X);
+// the proposal network's unused output slot (mode 6, slot 1) that predicts log(1 + SSE given away by a top-k search)
+#define ZT_RISK_SLOT 385
+__global__ void zt_mlp_xent(const float *Z, const uint32_t *labels, const uint16_t *deltas, float inv_t, int risk_k, float risk_w, int B,
+                            uint16_t *dZ, float *stats);
 __global__ void zt_mlp_colsum(const uint16_t *src, int M, int N, float *out);
 __global__ void zt_mlp_adam(float *p, const float *gr, float *m1, float *m2, int n, float lr, float b1, float b2, float c1, float c2,
                             float eps, uint16_t *wbf, const int *bf_index);
 __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t *wbf, const int *bf_index, const float *scale);
-__global__ void zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *conf);
+__global__ void zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *risk);
 __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index);
-// learned-proposal BC7 (zt_learned.hip)
+// learned-proposal BC7 (zt_bc7_full.hip)
 __global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);
-__global__ void zt_bc7_bucket(const uint32_t *cand, const float *conf, uint32_t total, int K, float threshold, unsigned *counters,
-                              uint4 *items, uint32_t pass, uint32_t *fallback, uint32_t *keys);
+__global__ void zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K, float threshold, unsigned *counters,
+                              uint2 *items, uint32_t pass, uint32_t *fallback);
 __global__ void zt_bc7_bucket_scan(unsigned *counters);
-__global__ void zt_bc7_eval_items(const uint4 *blocks, const uint4 *items, const unsigned *counters, int mode, uint32_t *keys, uint4 *bits,
-                                  int K);
 __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
-__global__ void zt_bc7_select(zt_jobs jobs, uint32_t total, const uint32_t *keys, const uint4 *bits, int K);
+#define ZT_PROP_DECL(NAME) __global__ void NAME(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key);
+ZT_PROP_DECL(zt_bc7_prop_m0) ZT_PROP_DECL(zt_bc7_prop_m1) ZT_PROP_DECL(zt_bc7_prop_m2) ZT_PROP_DECL(zt_bc7_prop_m3)
+ZT_PROP_DECL(zt_bc7_prop_m4) ZT_PROP_DECL(zt_bc7_prop_m5) ZT_PROP_DECL(zt_bc7_prop_m6) ZT_PROP_DECL(zt_bc7_prop_m7)
+#undef ZT_PROP_DECL
 
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);

@@ -101,16 +101,49 @@ extern "C" __global__ void __launch_bounds__(256) zt_gemm_bf16(zt_gemm_args g) {
         }
 }
 
-// block bytes (16 RGBA8 pixels) -> 64 bf16 features v / 255
-extern "C" __global__ void zt_mlp_features(const uint8_t *blocks, int n, uint16_t *X) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n * 64) return;
-    X[i] = to_bf16((float)blocks[i] / 255.0f);
+// block bytes (16 RGBA8 pixels) -> 64 bf16 features, one thread per block; cfg[0] picks the normalisation:
+//   0: v / 255
+//   1: (v - m_c) / 64, m_c the block's channel mean
+//   2: (v - m_c) / max(1, R), R the block's largest channel range (max - min)
+//   3: (v - m_c) / sqrt(64 max(1, R))   (contrast-normalised, keeps a monotone cue of the block's contrast)
+//   4: colour as 3 with R over R, G, B only; alpha as (a - 255) / 255 (0 when opaque: modes 0-3 decode alpha 255)
+extern "C" __global__ void zt_mlp_features(const uint8_t *blocks, int n, const int *cfg, uint16_t *X) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const int mode = cfg[0];
+    uint32_t px[16];
+    const uint4 *src = (const uint4 *)(blocks + b * 64);
+#pragma unroll
+    for (int r = 0; r < 4; r++) { uint4 v = src[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+    float mean[4], inv = 1.0f / 255.0f;
+    int rng = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        int sum = 0, lo = 255, hi = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) { const int v = chan(px[k], c); sum += v; lo = min(lo, v); hi = max(hi, v); }
+        mean[c] = mode ? (float)sum / 16.0f : 0.0f;
+        if (mode != 4 || c < 3) rng = max(rng, hi - lo);
+    }
+    const float r = (float)max(1, rng);
+    if (mode == 1) inv = 1.0f / 64.0f;
+    else if (mode == 2) inv = 1.0f / r;
+    else if (mode >= 3) inv = 1.0f / sqrtf(64.0f * r);
+    float ainv = inv;
+    if (mode == 4) { mean[3] = 255.0f; ainv = 1.0f / 255.0f; }
+    uint16_t *x = X + b * 64;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[4 * k + c] = to_bf16(((float)chan(px[k], c) - mean[c]) * (c == 3 ? ainv : inv));
 }
 
-// softmax cross-entropy over the used candidate slots, one wave per row; dZ = (p - onehot) / B in bf16;
-// stats[0] += mean loss, stats[1] += top-1 hits
-extern "C" __global__ void __launch_bounds__(256) zt_mlp_xent(const float *Z, const uint32_t *labels, int B, uint16_t *dZ, float *stats) {
+// softmax cross-entropy over the used candidate slots, one wave per row; dZ = (p - q) / B in bf16 for targets q;
+// stats[0] += this row's loss / B, stats[1] += 1 for a top-1 hit.
+// With `deltas` and risk_k > 0, the unused slot ZT_RISK_SLOT also learns log(1 + the smallest delta among the row's
+// current top-risk_k slots) by squared error (weight risk_w): the SSE a top-k search would give away.
+extern "C" __global__ void __launch_bounds__(256) zt_mlp_xent(const float *Z, const uint32_t *labels, const uint16_t *deltas, float inv_t,
+                                                                int risk_k, float risk_w, int B, uint16_t *dZ, float *stats) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= B) return;
@@ -134,17 +167,68 @@ extern "C" __global__ void __launch_bounds__(256) zt_mlp_xent(const float *Z, co
     for (int t = 0; t < 8; t++) if (cand_valid(lane + 64 * t)) s += __expf(v[t] - mx);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-    const int y = (int)labels[row];
+    const uint16_t *d = deltas ? deltas + (int64_t)row * 512 : nullptr;
+    // targets q: one-hot at the label, or softmax(-delta / T) over the used slots (delta 0 = an optimal candidate)
+    float q[8], qs = 0.0f, qz = 0.0f;
+    bool hit;
+    if (d && inv_t > 0.0f) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = lane + 64 * t;
+            q[t] = cand_valid(c) && d[c] != 65535 ? __expf(-(float)d[c] * inv_t) : 0.0f;
+            qs += q[t];
+            qz += q[t] * (cand_valid(c) ? v[t] : 0.0f);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) { qs += __shfl_xor(qs, off); qz += __shfl_xor(qz, off); }
+        const float inv = 1.0f / qs;
+#pragma unroll
+        for (int t = 0; t < 8; t++) q[t] *= inv;
+        qz *= inv;
+        hit = d[am] == 0;
+    } else {
+        const int y = (int)labels[row];
+#pragma unroll
+        for (int t = 0; t < 8; t++) q[t] = lane + 64 * t == y ? 1.0f : 0.0f;
+        qz = z[y];
+        hit = am == y;
+    }
+    // risk target: the smallest delta among the top risk_k slots (largest logit first, lowest slot on ties)
+    float rgrad = 0.0f, rloss = 0.0f;
+    if (d && risk_k > 0) {
+        float w[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) w[t] = v[t];
+        int dmin = 65535;
+        for (int k = 0; k < risk_k; k++) {
+            float bm = -3.0e38f;
+            int ba = 1 << 20;
+#pragma unroll
+            for (int t = 0; t < 8; t++) if (w[t] > bm) { bm = w[t]; ba = lane + 64 * t; }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                float om = __shfl_xor(bm, off);
+                int oa = __shfl_xor(ba, off);
+                if (om > bm || (om == bm && oa < ba)) { bm = om; ba = oa; }
+            }
+            dmin = min(dmin, (int)d[ba]);
+#pragma unroll
+            for (int t = 0; t < 8; t++) if (lane + 64 * t == ba) w[t] = -3.0e38f;
+        }
+        const float e = z[ZT_RISK_SLOT] - __logf(1.0f + (float)dmin);
+        rgrad = 2.0f * risk_w * e;
+        rloss = risk_w * e * e;
+    }
     const float invB = 1.0f / (float)B;
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         const int c = lane + 64 * t;
-        float gr = cand_valid(c) ? (__expf(v[t] - mx) / s - (c == y ? 1.0f : 0.0f)) * invB : 0.0f;
+        float gr = cand_valid(c) ? (__expf(v[t] - mx) / s - q[t]) * invB : c == ZT_RISK_SLOT ? rgrad * invB : 0.0f;
         dZ[(int64_t)row * 512 + c] = to_bf16(gr);
     }
     if (lane == 0) {
-        atomicAdd(stats, (mx + __logf(s) - z[y]) * invB);
-        if (am == y) atomicAdd(stats + 1, 1.0f);
+        atomicAdd(stats, (mx + __logf(s) - qz + rloss) * invB);   // cross-entropy -sum q log p (targets sum to 1) + risk term
+        if (hit) atomicAdd(stats + 1, 1.0f);
     }
 }
 
@@ -186,7 +270,9 @@ extern "C" __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t 
 }
 
 // top-K used candidate slots per row (ties to the lower slot) and the top-1 softmax probability
-extern "C" __global__ void __launch_bounds__(256) zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *conf) {
+// top-K used slots per row (largest logit first, lowest slot on ties) and the row's predicted SSE cost of searching
+// only them, expm1 of the risk slot (clamped to [0, 2^20])
+extern "C" __global__ void __launch_bounds__(256) zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *risk) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= B) return;
@@ -194,7 +280,6 @@ extern "C" __global__ void __launch_bounds__(256) zt_mlp_topk(const float *Z, in
     float v[8];
 #pragma unroll
     for (int t = 0; t < 8; t++) v[t] = cand_valid(lane + 64 * t) ? z[lane + 64 * t] : -3.0e38f;
-    float top = 0.0f;
     for (int k = 0; k < K; k++) {
         float mx = -3.0e38f;
         int am = 1 << 20;
@@ -206,17 +291,14 @@ extern "C" __global__ void __launch_bounds__(256) zt_mlp_topk(const float *Z, in
             int oa = __shfl_xor(am, off);
             if (om > mx || (om == mx && oa < am)) { mx = om; am = oa; }
         }
-        if (k == 0) top = mx;
         if (lane == 0) cand[(int64_t)row * K + k] = (uint32_t)am;
 #pragma unroll
         for (int t = 0; t < 8; t++) if (lane + 64 * t == am) v[t] = -3.0e38f;
     }
-    float s = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 8; t++) if (cand_valid(lane + 64 * t)) s += __expf(z[lane + 64 * t] - top);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) conf[row] = 1.0f / s;
+    if (lane == 0) {
+        const float r = z[ZT_RISK_SLOT];
+        risk[row] = r <= 0.0f ? 0.0f : r >= 13.9f ? 1048576.0f : __expf(r) - 1.0f;
+    }
 }
 
 // re-round the bf16 weight copies from the fp32 master weights (after loading parameters)

@@ -1,15 +1,22 @@
-"""Learned BC7 proposal: training data, training loop and model file, all on the GPU through libzaptex.
+"""Learned BC7 proposal: training data, training loop, threshold calibration, evaluation and the model file, all
+on the GPU through libzaptex.
 
 Training blocks come from seeded textures of the benchmark's family (value noise + hard-edged rectangles +
 independent alpha, 25% opaque blocks) and their box mip chains, at seeds disjoint from the benchmark
 texture's.  Labels are the candidates the exhaustive all-mode search picks (zaptex_bc7_full_dev).  The network
 trains with the library's own MFMA forward/backward kernels and Adam (zaptex_mlp_train_step).
+
+Quality is the reference's PSNR (tex_bench.c:17-22: 10 log10(255^2 n / SSE) over R, G, B, A for BC7), summed
+over every level of a texture's chain; the learned encode is judged against the exhaustive search on the same
+chain.  The confidence threshold is calibrated on a held-out texture (its own seed), never on the benchmark's.
 """
 from __future__ import annotations
 
 import json
+import math
 import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -17,100 +24,216 @@ from . import binding, synth
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "models", "bc7_proposal.npy")
-META_PATH = os.path.join(HERE, "models", "bc7_proposal.json")
+TRAIN_SEED0 = 1000      # training textures use seeds TRAIN_SEED0 .. TRAIN_SEED0 + textures - 1
+CALIB_SEED = 500        # held-out texture for the threshold
 
 
-def training_set(lib, textures: int, size: int, seed0: int = 1000, log=print):
-    """(blocks uint8 [N, 64] cuda, labels int32 [N] cuda) from `textures` seeded size^2 textures + mip chains."""
+def _texture(args):
+    size, seed = args
+    return synth.c2_bc7(size, seed, rects=max(16, size // 16))
+
+
+class Chain:
+    """A texture and its box mip chain on the GPU, with one BC7 output buffer per level."""
+
+    def __init__(self, lib, img: np.ndarray, srgb: int = 0):
+        import torch
+        self.lib = lib
+        h, w = img.shape[:2]
+        self.sizes = binding.mip_sizes(w, h)
+        base = torch.from_numpy(np.ascontiguousarray(img)).cuda()
+        levels = [torch.empty((hh, ww, 4), dtype=torch.uint8, device="cuda") for (ww, hh) in self.sizes[1:]]
+        lib.mip_chain_dev(base, w, h, w * 4, levels, srgb=srgb)
+        self.srcs = [base] + levels
+        self.outs = [torch.empty(binding.bc_size(ww, hh, "bc7"), dtype=torch.uint8, device="cuda") for (ww, hh) in self.sizes]
+        self.pixels = sum(ww * hh for ww, hh in self.sizes)
+        self.blocks = sum(((ww + 3) // 4) * ((hh + 3) // 4) for ww, hh in self.sizes)
+
+    def images(self, lo: int = 0, hi: int | None = None):
+        hi = len(self.sizes) if hi is None else hi
+        return self.lib.images((self.srcs[i], self.outs[i], self.sizes[i][0], self.sizes[i][1], self.sizes[i][0] * 4, "bc7")
+                               for i in range(lo, hi))
+
+    def psnr(self) -> float:
+        """The reference's BC7 PSNR (RGBA) of the current outputs over the whole chain, decoded on the GPU."""
+        import torch
+        se = 0
+        for (w, h), src, out in zip(self.sizes, self.srcs, self.outs):
+            dec = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+            self.lib.bc_decode_dev(out, w, h, "bc7", dec)
+            d = dec.to(torch.int32) - src.to(torch.int32)
+            se += int((d * d).sum(dtype=torch.int64).item())
+        return 99.0 if se == 0 else 10 * math.log10(255.0 * 255.0 * 4 * self.pixels / se)
+
+
+def training_set(lib, textures: int, size: int, seed0: int = TRAIN_SEED0, threads: int = 16, log=print):
+    """(blocks uint8 [N, 64], labels int32 [N], deltas int16 [N, 512] holding uint16 SSE deltas), all on the GPU,
+    from `textures` seeded size^2 textures + mip chains; textures are generated on host threads while the GPU
+    labels the previous ones."""
     import torch
-    blocks, labels = [], []
+    blocks, labels, deltas = [], [], []
     t0 = time.time()
-    for t in range(textures):
-        img = synth.c2_bc7(size, seed0 + t, rects=max(16, size // 16))
-        base = torch.from_numpy(img).cuda()
-        sizes = binding.mip_sizes(size, size)
-        levels = [torch.empty((h, w, 4), dtype=torch.uint8, device="cuda") for (w, h) in sizes[1:]]
-        lib.mip_chain_dev(base, size, size, size * 4, levels, srgb=t & 1)
-        srcs = [base] + levels
-        for i0 in range(0, len(sizes), 16):
-            chunk = list(range(i0, min(len(sizes), i0 + 16)))
-            nb = sum(((sizes[i][0] + 3) // 4) * ((sizes[i][1] + 3) // 4) for i in chunk)
-            outs = [torch.empty(binding.bc_size(sizes[i][0], sizes[i][1], "bc7"), dtype=torch.uint8, device="cuda") for i in chunk]
-            imgs = lib.images((srcs[i], outs[k], sizes[i][0], sizes[i][1], sizes[i][0] * 4, "bc7") for k, i in enumerate(chunk))
-            blk = torch.empty((nb, 64), dtype=torch.uint8, device="cuda")
-            lab = torch.empty(nb, dtype=torch.int32, device="cuda")
-            lib.bc7_gather_dev(imgs, blk)
-            lib.bc7_full_dev(imgs, lab)
-            blocks.append(blk)
-            labels.append(lab)
-        torch.cuda.synchronize()
-        if log and (t + 1) % max(1, textures // 8) == 0:
-            log(f"  labelled {t + 1}/{textures} textures ({sum(b.shape[0] for b in blocks)} blocks, {time.time() - t0:.1f}s)")
-    return torch.cat(blocks), torch.cat(labels)
+    with ThreadPoolExecutor(threads) as ex:
+        for t, img in enumerate(ex.map(_texture, [(size, seed0 + t) for t in range(textures)])):
+            ch = Chain(lib, img, srgb=t & 1)
+            for i0 in range(0, len(ch.sizes), 16):
+                i1 = min(len(ch.sizes), i0 + 16)
+                nb = sum(((ch.sizes[i][0] + 3) // 4) * ((ch.sizes[i][1] + 3) // 4) for i in range(i0, i1))
+                imgs = ch.images(i0, i1)
+                blk = torch.empty((nb, 64), dtype=torch.uint8, device="cuda")
+                lab = torch.empty(nb, dtype=torch.int32, device="cuda")
+                dl = torch.empty((nb, 512), dtype=torch.int16, device="cuda")
+                lib.bc7_gather_dev(imgs, blk)
+                lib.bc7_full_dev(imgs, lab, dl)
+                blocks.append(blk)
+                labels.append(lab)
+                deltas.append(dl)
+            torch.cuda.synchronize()
+            if log and (t + 1) % max(1, textures // 8) == 0:
+                log(f"  labelled {t + 1}/{textures} textures ({sum(b.shape[0] for b in blocks)} blocks, {time.time() - t0:.1f}s)",
+                    flush=True)
+    return torch.cat(blocks), torch.cat(labels), torch.cat(deltas)
 
 
-def train(lib, blocks, labels, epochs: int = 4, batch: int = 8192, lr: float = 2e-3, seed: int = 1, log=print):
-    """Adam on shuffled minibatches; returns (state tensor, history)."""
+def train(lib, blocks, labels, deltas=None, temperature: float = 0.0, epochs: int = 4, batch: int = 8192, lr: float = 2e-3,
+          seed: int = 1, risk_k: int = 4, risk_weight: float = 0.2, features: int = 4, log=print):
+    """Adam on shuffled minibatches with a cosine learning rate; soft targets softmax(-delta / temperature) when
+    temperature > 0, else the one-hot labels; the risk slot learns the SSE the top-risk_k give away.  Returns
+    (state tensor, history)."""
     import torch
     n = blocks.shape[0]
-    st = lib.mlp_state(seed=seed)
+    st = lib.mlp_state(seed=seed, features=features)
     ws = lib.mlp_workspace(batch)
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     step, hist = 0, []
+    total_steps = max(1, epochs * (n // batch))
     for ep in range(epochs):
         perm = torch.randperm(n, device="cuda", generator=g)
-        stats = torch.zeros(2, dtype=torch.float32, device="cuda")
+        stats = torch.zeros(2, dtype=torch.float32, device="cuda")   # this step's (mean loss, top-1 hits)
+        total = torch.zeros(2, dtype=torch.float64, device="cuda")
         nb = 0
         for i in range(0, n - batch + 1, batch):
             idx = perm[i:i + batch]
             xb, yb = blocks[idx].contiguous(), labels[idx].contiguous()
+            db = deltas[idx].contiguous() if deltas is not None and (temperature > 0 or risk_k > 0) else None
             step += 1
-            rate = lr * (0.5 * (1 + np.cos(np.pi * step / max(1, epochs * (n // batch)))))
-            lib.mlp_train_step(st, xb, yb, batch, float(rate), step, ws, stats)
+            rate = lr * 0.5 * (1 + math.cos(math.pi * step / total_steps))
+            lib.mlp_train_step(st, xb, yb, batch, float(rate), step, ws, stats, deltas=db, temperature=temperature,
+                               risk_k=risk_k if db is not None else 0, risk_weight=risk_weight)
+            total += stats
             nb += 1
         torch.cuda.synchronize()
-        loss, acc = float(stats[0].item()) / max(1, nb), float(stats[1].item()) / max(1, nb * batch)
+        loss, acc = float(total[0].item()) / max(1, nb), float(total[1].item()) / max(1, nb * batch)
         hist.append({"epoch": ep + 1, "loss": round(loss, 4), "top1": round(acc, 4)})
         if log:
-            log(f"  epoch {ep + 1}: loss {loss:.4f} top-1 {acc:.4f}")
+            log(f"  epoch {ep + 1}: loss {loss:.4f} top-1 {acc:.4f}", flush=True)
     return st, hist
+
+
+def _time(fn, steps: int) -> float:
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def evaluate(lib, chain: Chain, state, topk: int, thresholds, steps: int = 5) -> dict:
+    """Encode time (ms, inputs resident) and chain PSNR for the plain search, the exhaustive search and the learned
+    proposal at each threshold; 'fallback' is the fraction of blocks the learned encode searched exhaustively."""
+    import torch
+    imgs = chain.images()
+    ws = torch.empty(lib.workspace_size(imgs), dtype=torch.uint8, device="cuda")
+    res = {"blocks": chain.blocks, "pixels": chain.pixels}
+    res["plain"] = {"ms": _time(lambda: lib.encode_dev(imgs, ws), steps)}
+    res["plain"]["psnr"] = chain.psnr()
+    fws = lib.bc7_full_workspace(imgs)
+    res["full"] = {"ms": _time(lambda: lib.bc7_full_dev(imgs, ws=fws), steps)}
+    res["full"]["psnr"] = chain.psnr()
+    lws = lib.bc7_learned_workspace(imgs, topk)
+    res["learned"] = []
+    for thr in thresholds:
+        ms = _time(lambda: lib.bc7_learned_dev(imgs, state, topk, float(thr), lws), steps)
+        fb = int(lws[:256].view(torch.int32)[8].item())
+        p = chain.psnr()
+        res["learned"].append({"threshold": float(thr), "ms": ms, "psnr": p, "fallback": fb / chain.blocks,
+                               "gap_db": res["full"]["psnr"] - p})
+    for r in [res["plain"], res["full"]] + res["learned"]:
+        r["mpix_s"] = chain.pixels / (r["ms"] / 1e3) / 1e6
+    return res
+
+
+def calibrate(lib, state, topk: int, size: int = 2048, max_gap_db: float = 0.04, log=print) -> dict:
+    """Fastest threshold (predicted SSE a block may give away before it is searched exhaustively) whose chain-PSNR
+    gap to the exhaustive search stays within max_gap_db on the held-out texture (seed CALIB_SEED); the margin
+    under 0.05 dB leaves room for texture-to-texture variation."""
+    chain = Chain(lib, _texture((size, CALIB_SEED)))
+    grid = [1e30, 64, 32, 16, 12, 8, 6, 5, 4, 3.5, 3, 2.5, 2, 1.75, 1.5, 1.25, 1, 0.75, 0.5, -1]
+    res = evaluate(lib, chain, state, topk, grid)
+    ok = [r for r in res["learned"] if r["gap_db"] <= max_gap_db]
+    best = max(ok, key=lambda r: r["mpix_s"])
+    if log:
+        for r in res["learned"]:
+            log(f"  thr {r['threshold']:8.4g}: {r['mpix_s']:9.1f} Mpix/s  gap {r['gap_db']:.4f} dB  fallback {r['fallback']:.3f}",
+                flush=True)
+        log(f"  exhaustive {res['full']['mpix_s']:.1f} Mpix/s -> threshold {best['threshold']}", flush=True)
+    return {"threshold": best["threshold"], "topk": topk, "calibration": res}
 
 
 def save(lib, state, meta: dict, path: str = MODEL_PATH) -> None:
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.save(path, lib.mlp_params(state).astype(np.float32), allow_pickle=False)
-    with open(META_PATH, "w") as fh:
+    with open(os.path.splitext(path)[0] + ".json", "w") as fh:
         json.dump(meta, fh, indent=1)
 
 
 def load(lib, path: str = MODEL_PATH):
+    """(state tensor, meta dict) of a trained model."""
     if not os.path.exists(path):
         raise binding.ZapTexError(f"no trained proposal model at {path} (train with python -m zap_amd.learned)")
     params = np.load(path, allow_pickle=False)
-    return lib.mlp_state(params=params)
+    with open(os.path.splitext(path)[0] + ".json") as fh:
+        meta = json.load(fh)
+    return lib.mlp_state(params=params, features=int(meta.get("features", 0))), meta
 
 
 if __name__ == "__main__":
     import argparse
 
-    import torch
     ap = argparse.ArgumentParser(description="train the BC7 proposal network on the GPU")
-    ap.add_argument("--textures", type=int, default=24)
+    ap.add_argument("--textures", type=int, default=128)
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--topk", type=int, default=4)
+    ap.add_argument("--temperature", type=float, default=0.0, help="soft-target temperature in SSE units (0: one-hot labels)")
+    ap.add_argument("--risk-weight", type=float, default=0.2)
+    ap.add_argument("--features", type=int, default=4, help="input normalisation (zaptex_mlp_set_features)")
+    ap.add_argument("--out", default=MODEL_PATH, help="model file (.npy; the .json beside it holds the metadata)")
     a = ap.parse_args()
     lib = binding.load()
     t0 = time.time()
-    xb, yb = training_set(lib, a.textures, a.size)
+    xb, yb, db = training_set(lib, a.textures, a.size)
     t_label = time.time() - t0
     print(f"training set: {xb.shape[0]} blocks labelled in {t_label:.1f}s", flush=True)
     t1 = time.time()
-    st, hist = train(lib, xb, yb, a.epochs, a.batch, a.lr)
+    st, hist = train(lib, xb, yb, db, a.temperature, a.epochs, a.batch, a.lr, risk_k=a.topk, risk_weight=a.risk_weight,
+                     features=a.features)
     t_train = time.time() - t1
-    save(lib, st, {"blocks": int(xb.shape[0]), "textures": a.textures, "size": a.size, "epochs": a.epochs, "batch": a.batch,
-                   "lr": a.lr, "label_seconds": round(t_label, 1), "train_seconds": round(t_train, 1), "history": hist,
-                   "seeds": f"1000..{1000 + a.textures - 1}"})
-    print(json.dumps(hist[-1]), flush=True)
+    print(f"trained in {t_train:.1f}s", flush=True)
+    n_blocks = int(xb.shape[0])
+    del xb, yb, db
+    cal = calibrate(lib, st, a.topk)
+    save(lib, st, {"blocks": n_blocks, "textures": a.textures, "size": a.size, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+                   "temperature": a.temperature, "risk_weight": a.risk_weight, "features": a.features,
+                   "label_seconds": round(t_label, 1), "train_seconds": round(t_train, 1), "history": hist,
+                   "train_seeds": f"{TRAIN_SEED0}..{TRAIN_SEED0 + a.textures - 1}", "calibration_seed": CALIB_SEED,
+                   "threshold": cal["threshold"], "topk": a.topk, "calibration": cal["calibration"]}, a.out)
+    print(json.dumps({"last_epoch": hist[-1], "threshold": cal["threshold"]}), flush=True)
