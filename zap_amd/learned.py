@@ -3,12 +3,14 @@ on the GPU through libzaptex.
 
 Training blocks come from seeded textures of the benchmark's family (value noise + hard-edged rectangles +
 independent alpha, 25% opaque blocks) and their box mip chains, at seeds disjoint from the benchmark
-texture's.  Labels are the candidates the exhaustive all-mode search picks (zaptex_bc7_full_dev).  The network
+texture's.  The family's noise scales with the texture (finest lattice cell = side / 512), so the training set
+mixes sizes (4096^2, 2048^2, 1024^2 and all their mip levels) to cover smooth and busy blocks alike.  Labels are the candidates the exhaustive all-mode search picks (zaptex_bc7_full_dev).  The network
 trains with the library's own MFMA forward/backward kernels and Adam (zaptex_mlp_train_step).
 
 Quality is the reference's PSNR (tex_bench.c:17-22: 10 log10(255^2 n / SSE) over R, G, B, A for BC7), summed
 over every level of a texture's chain; the learned encode is judged against the exhaustive search on the same
-chain.  The confidence threshold is calibrated on a held-out texture (its own seed), never on the benchmark's.
+chain.  The risk threshold is calibrated on held-out textures of three sizes (their own seeds), never on the
+benchmark's.
 """
 from __future__ import annotations
 
@@ -24,8 +26,9 @@ from . import binding, synth
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "models", "bc7_proposal.npy")
-TRAIN_SEED0 = 1000      # training textures use seeds TRAIN_SEED0 .. TRAIN_SEED0 + textures - 1
-CALIB_SEED = 500        # held-out texture for the threshold
+TRAIN_SEED0 = 1000                       # training textures use seeds TRAIN_SEED0, TRAIN_SEED0 + 1, ...
+TRAIN_MIX = ((4096, 4), (2048, 16), (1024, 32))  # (side, count): about 14M blocks with the mip chains
+CALIB = ((4096, 500), (1024, 501), (512, 502))   # held-out (side, seed) textures for the threshold
 
 
 def _texture(args):
@@ -66,15 +69,17 @@ class Chain:
         return 99.0 if se == 0 else 10 * math.log10(255.0 * 255.0 * 4 * self.pixels / se)
 
 
-def training_set(lib, textures: int, size: int, seed0: int = TRAIN_SEED0, threads: int = 16, log=print):
+def training_set(lib, mix=TRAIN_MIX, seed0: int = TRAIN_SEED0, threads: int = 16, log=print):
     """(blocks uint8 [N, 64], labels int32 [N], deltas int16 [N, 512] holding uint16 SSE deltas), all on the GPU,
-    from `textures` seeded size^2 textures + mip chains; textures are generated on host threads while the GPU
-    labels the previous ones."""
+    from the seeded textures of `mix` ((side, count) pairs, seeds counting up from seed0) and their mip chains;
+    textures are generated on host threads while the GPU labels the previous ones."""
     import torch
     blocks, labels, deltas = [], [], []
+    specs = [side for side, count in mix for _ in range(count)]
+    textures = len(specs)
     t0 = time.time()
     with ThreadPoolExecutor(threads) as ex:
-        for t, img in enumerate(ex.map(_texture, [(size, seed0 + t) for t in range(textures)])):
+        for t, img in enumerate(ex.map(_texture, [(side, seed0 + t) for t, side in enumerate(specs)])):
             ch = Chain(lib, img, srgb=t & 1)
             for i0 in range(0, len(ch.sizes), 16):
                 i1 = min(len(ch.sizes), i0 + 16)
@@ -169,21 +174,31 @@ def evaluate(lib, chain: Chain, state, topk: int, thresholds, steps: int = 5) ->
     return res
 
 
-def calibrate(lib, state, topk: int, size: int = 2048, max_gap_db: float = 0.04, log=print) -> dict:
+def calibrate(lib, state, topk: int, textures=CALIB, max_gap_db: float = 0.04, log=print) -> dict:
     """Fastest threshold (predicted SSE a block may give away before it is searched exhaustively) whose chain-PSNR
-    gap to the exhaustive search stays within max_gap_db on the held-out texture (seed CALIB_SEED); the margin
-    under 0.05 dB leaves room for texture-to-texture variation."""
-    chain = Chain(lib, _texture((size, CALIB_SEED)))
-    grid = [1e30, 64, 32, 16, 12, 8, 6, 5, 4, 3.5, 3, 2.5, 2, 1.75, 1.5, 1.25, 1, 0.75, 0.5, -1]
-    res = evaluate(lib, chain, state, topk, grid)
-    ok = [r for r in res["learned"] if r["gap_db"] <= max_gap_db]
+    gap to the exhaustive search stays within max_gap_db on every held-out texture; speed is pooled over them.
+    The margin under 0.05 dB leaves room for texture-to-texture variation."""
+    grid = [1e30, 64, 32, 16, 12, 8, 6, 5, 4, 3.5, 3, 2.5, 2, 1.75, 1.5, 1.25, 1, 0.75, 0.5, 0.25, -1]
+    per = []
+    for side, seed in textures:
+        per.append(evaluate(lib, Chain(lib, _texture((side, seed))), state, topk, grid))
+    px = sum(r["pixels"] for r in per)
+    full_ms = sum(r["full"]["ms"] for r in per)
+    rows = []
+    for i, thr in enumerate(grid):
+        ms = sum(r["learned"][i]["ms"] for r in per)
+        rows.append({"threshold": thr, "mpix_s": px / (ms / 1e3) / 1e6, "max_gap_db": max(r["learned"][i]["gap_db"] for r in per),
+                     "gaps_db": [round(r["learned"][i]["gap_db"], 4) for r in per],
+                     "fallback": sum(r["learned"][i]["fallback"] * r["blocks"] for r in per) / sum(r["blocks"] for r in per)})
+    ok = [r for r in rows if r["max_gap_db"] <= max_gap_db]
     best = max(ok, key=lambda r: r["mpix_s"])
+    full = px / (full_ms / 1e3) / 1e6
     if log:
-        for r in res["learned"]:
-            log(f"  thr {r['threshold']:8.4g}: {r['mpix_s']:9.1f} Mpix/s  gap {r['gap_db']:.4f} dB  fallback {r['fallback']:.3f}",
-                flush=True)
-        log(f"  exhaustive {res['full']['mpix_s']:.1f} Mpix/s -> threshold {best['threshold']}", flush=True)
-    return {"threshold": best["threshold"], "topk": topk, "calibration": res}
+        for r in rows:
+            log(f"  thr {r['threshold']:8.4g}: {r['mpix_s']:9.1f} Mpix/s  gaps {r['gaps_db']} dB  fallback {r['fallback']:.3f}", flush=True)
+        log(f"  exhaustive {full:.1f} Mpix/s -> threshold {best['threshold']}", flush=True)
+    return {"threshold": best["threshold"], "topk": topk, "textures": [list(t) for t in textures], "exhaustive_mpix_s": full,
+            "rows": rows}
 
 
 def save(lib, state, meta: dict, path: str = MODEL_PATH) -> None:
@@ -207,8 +222,7 @@ if __name__ == "__main__":
     import argparse
 
     ap = argparse.ArgumentParser(description="train the BC7 proposal network on the GPU")
-    ap.add_argument("--textures", type=int, default=128)
-    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--mix", default=",".join(f"{s}x{c}" for s, c in TRAIN_MIX), help="training textures: side x count, ...")
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -220,7 +234,8 @@ if __name__ == "__main__":
     a = ap.parse_args()
     lib = binding.load()
     t0 = time.time()
-    xb, yb, db = training_set(lib, a.textures, a.size)
+    mix = tuple(tuple(int(v) for v in m.split("x")) for m in a.mix.split(","))
+    xb, yb, db = training_set(lib, mix)
     t_label = time.time() - t0
     print(f"training set: {xb.shape[0]} blocks labelled in {t_label:.1f}s", flush=True)
     t1 = time.time()
@@ -231,9 +246,10 @@ if __name__ == "__main__":
     n_blocks = int(xb.shape[0])
     del xb, yb, db
     cal = calibrate(lib, st, a.topk)
-    save(lib, st, {"blocks": n_blocks, "textures": a.textures, "size": a.size, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+    save(lib, st, {"blocks": n_blocks, "mix": [list(m) for m in mix], "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
                    "temperature": a.temperature, "risk_weight": a.risk_weight, "features": a.features,
                    "label_seconds": round(t_label, 1), "train_seconds": round(t_train, 1), "history": hist,
-                   "train_seeds": f"{TRAIN_SEED0}..{TRAIN_SEED0 + a.textures - 1}", "calibration_seed": CALIB_SEED,
-                   "threshold": cal["threshold"], "topk": a.topk, "calibration": cal["calibration"]}, a.out)
+                   "train_seeds": f"{TRAIN_SEED0}..{TRAIN_SEED0 + sum(c for _, c in mix) - 1}",
+                   "threshold": cal["threshold"], "topk": a.topk,
+                   "calibration": {k: cal[k] for k in ("textures", "exhaustive_mpix_s", "rows")}}, a.out)
     print(json.dumps({"last_epoch": hist[-1], "threshold": cal["threshold"]}), flush=True)
