@@ -12,6 +12,9 @@ Also reported, measured inside this run:
                 duration from HIP events on its launch stream; the mip chain's HBM GB/s is given beside it
   cpu_baseline  the reference encoder (oracle/_ref, built from zap_tex.h) on this host's cores, banded into
                 8-block rows like the reference viewer, over the same 13-level chain
+  batch         the C4 workload: 512 seeded textures (sides 64..4096 incl. non-powers of two; 30% BC1, 15% BC3,
+                15% BC5 normal maps, 40% BC7) with full mip chains, one graph launch (zaptex_batch_run) per step,
+                launch shapes from the autotuner (zap_amd/tuning/mi355x.json)
   learned       the learned-proposal encode (zap_amd/models/bc7_proposal.npy, top-4 exact + exhaustive fallback at
                 the model's calibrated threshold) against the exhaustive all-mode search on the same chain:
                 Mpix/s of both, chain PSNR of both (tex_bench.c's RGBA definition) and the gap
@@ -92,6 +95,39 @@ def learned_leg(lib, c2, steps: int = 5) -> dict:
     return out
 
 
+def batch_leg(lib, steps: int = 5) -> dict:
+    """The 512-texture batch with full chains, one graph launch per step, autotuned launch shapes."""
+    import torch
+
+    from zap_amd import autotune
+    from zap_amd import batch as B
+    autotune.reset(lib)
+    shapes = autotune.apply(lib, "c4")
+    t0 = time.perf_counter()
+    specs, bases = B.c4_textures(512, 4)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    bt = B.Batch(lib, [(bases[i], w, h, fmt, srgb, None) for i, (w, h, fmt, srgb, kind) in enumerate(specs)])
+    for _ in range(2):
+        bt.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        bt.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    fmts = {f: sum(1 for s in specs if s[2] == f) for f in ("bc1", "bc3", "bc5", "bc7")}
+    out = {"textures": len(specs), "formats": fmts, "pixels": bt.pixels, "blocks": bt.blocks, "ms": round(ms, 3),
+           "mpix_s": round(bt.pixels / (ms / 1e3) / 1e6, 1), "graph_launches_per_step": 1, "launch_shapes": shapes,
+           "generate_s": round(gen_s, 2)}
+    bt.close()
+    autotune.reset(lib)
+    log(f"batch: {len(specs)} textures, {bt.pixels / 1e6:.0f} Mpx incl. chains, {ms:.2f} ms -> {out['mpix_s']} Mpix/s")
+    return out
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -110,6 +146,8 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-texture batch leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-entry (PCIe-inclusive) leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the profiler's HBM byte counters)")
@@ -135,6 +173,8 @@ def main() -> int:
 
     lib = zap_amd.load()
     torch.cuda.set_device(0)
+    from zap_amd import autotune
+    shapes_c2 = autotune.apply(lib, "c2")
     golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
     ops = json.load(open(os.path.join(ROOT, "tests", "golden", "ops.json")))["c2_bc7_chain"]
 
@@ -209,14 +249,18 @@ def main() -> int:
 
     # end to end through the drop-in's host entry (zap_bc_encode: host buffers in, PCIe both ways), per level
     levels_np = [c2] + [l.cpu().numpy() for l in levels]
-    lib.bc_encode(levels_np[0], "bc7")
-    t0 = time.perf_counter()
-    for lvl in levels_np:
-        lib.bc_encode(lvl, "bc7")
-    e2e_mpix = npx / (time.perf_counter() - t0) / 1e6
-    log(f"host entry (H2D + encode + D2H per level): {e2e_mpix:.1f} Mpix/s")
+    e2e_mpix = None
+    if not args.no_host:
+        lib.bc_encode(levels_np[0], "bc7")
+        t0 = time.perf_counter()
+        for lvl in levels_np:
+            lib.bc_encode(lvl, "bc7")
+        e2e_mpix = round(npx / (time.perf_counter() - t0) / 1e6, 1)
+        log(f"host entry (H2D + encode + D2H per level): {e2e_mpix} Mpix/s")
 
+    autotune.reset(lib)
     learned_res = None if args.no_learned else learned_leg(lib, c2)
+    batch_res = None if args.no_batch else batch_leg(lib)
     cpu = None if args.no_cpu else cpu_baseline(levels_np, args.cpu_threads)
     line = {
         "metric": "BC7 Mpix/s on 1 MI355X: plain vs learned-proposal at \u22640.05 dB PSNR gap",
@@ -238,9 +282,11 @@ def main() -> int:
                               "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "bytes_per_chain": MIP_CHAIN_BYTES,
                               "ms": round(stages["mips"], 5)}},
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
-        "host_entry_mpix_s": round(e2e_mpix, 1),
+        "host_entry_mpix_s": e2e_mpix,
+        "launch_shapes": shapes_c2,
         "kernels": kern,
         "learned": learned_res,
+        "batch": batch_res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

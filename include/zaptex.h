@@ -129,6 +129,37 @@ size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int to
 int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, void *ws, size_t ws_bytes,
                            void *stream);
 
+/* ---------------- batch: many textures with their mip chains in one graph launch (no reference counterpart:
+ * the reference encodes one image per call and its callers loop)
+ * A texture: level 0 in device memory; `levels` levels are encoded (1..zap_mip_levels(w, h)), levels 1.. generated
+ * with zap_mip_next's box filter (`srgb` as there) into the `mips` arena (zaptex_batch_mip_bytes; level l at the
+ * sum of the 256-byte-aligned tight sizes of levels 1..l-1), and every level's blocks go to the `out` arena
+ * (zaptex_batch_out_bytes; level l at the sum of the 256-byte-aligned block sizes of levels 0..l-1).  Arenas
+ * 256-byte aligned; formats BC1, BC3, BC4, BC5, BC7.  Bytes equal zap_mip_next + zap_bc_encode per level. */
+typedef struct {
+    const void *base;
+    size_t stride;
+    int w, h, fmt, srgb, levels;
+    void *mips;
+    void *out;
+} zaptex_texture;
+typedef struct zaptex_batch zaptex_batch;
+size_t zaptex_batch_mip_bytes(int w, int h, int levels);
+size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels);
+size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n);
+/* Builds the plan: device job tables (uploaded once into the caller's workspace) and one captured graph:
+ * the mip levels of every texture (one launch per level across textures), then the BC1/3/4/5 and BC7 encoders
+ * over all blocks.  The launch shapes in force (zaptex_set_launch) are baked in.  The texture buffers and the
+ * workspace must outlive the plan. */
+int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out);
+/* one launch of the graph on `stream`; runs of one plan must not overlap (they share the workspace) */
+int zaptex_batch_run(zaptex_batch *b, void *stream);
+void zaptex_batch_destroy(zaptex_batch *b);
+/* launch shape of a grid-stride kernel ("zt_bc14_encode", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1",
+ * "zt_mip_batch"): workgroup size 64/128/256 and the cap on workgroups (the autotuner sets these) */
+int zaptex_set_launch(const char *kernel, int wg, int max_groups);
+int zaptex_get_launch(const char *kernel, int *wg, int *max_groups);
+
 /* zap_mip_next on device memory */
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream);
 /* zap_mip_next_f on device memory (strides in floats) */

@@ -1,0 +1,125 @@
+"""Batched encode (zaptex_batch_*): every texture's generated mip levels and every level's blocks equal the
+reference path (the oracle's zap_mip_next chain + zap_bc_encode per level) for odd, non-power-of-two and 1x1
+sizes, all five formats, linear and sRGB filters, partial chains and a strided base; graph replays give the same
+bytes every time; a slice of the C4 batch equals the per-texture device entry points; other launch shapes
+(as the autotuner may choose) change nothing."""
+import numpy as np
+import pytest
+
+from zap_amd import batch, binding
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [  # (w, h, fmt, srgb, levels, kind)
+    (1, 1, "bc7", 0, None, "alpha"), (3, 5, "bc1", 1, None, "colour"), (7, 2, "bc4", 0, None, "alpha"),
+    (64, 64, "bc5", 0, None, "normal"), (300, 200, "bc7", 1, None, "alpha"), (129, 67, "bc3", 1, None, "alpha"),
+    (256, 4, "bc1", 0, None, "colour"), (5, 260, "bc7", 0, None, "alpha"), (1000, 750, "bc1", 1, 3, "colour"),
+    (96, 96, "bc7", 1, 2, "alpha"), (33, 17, "bc5", 0, None, "normal"), (130, 66, "bc4", 1, None, "alpha"),
+]
+
+
+def _make(zt, specs, seed0=900, stride_pad=0):
+    import torch
+    bases, cpu = [], []
+    for k, (w, h, fmt, srgb, levels, kind) in enumerate(specs):
+        img = batch.texture_cpu(w, h, kind, seed0 + k)
+        cpu.append(img)
+        if stride_pad and k % 3 == 1:   # a strided view: rows padded by stride_pad pixels
+            full = torch.zeros((h, w + stride_pad, 4), dtype=torch.uint8, device="cuda")
+            full[:, :w] = torch.from_numpy(img).cuda()
+            bases.append(full[:, :w])
+        else:
+            bases.append(torch.from_numpy(img).cuda())
+    b = batch.Batch(zt, [(bases[k], w, h, fmt, srgb, levels) for k, (w, h, fmt, srgb, levels, kind) in enumerate(specs)])
+    return b, cpu
+
+
+def _snapshot(b):
+    return [([b.level(i, lv).cpu().numpy().copy() for lv in range(1, it["levels"])],
+             [b.blocks_of(i, lv).cpu().numpy().copy() for lv in range(it["levels"])]) for i, it in enumerate(b.items)]
+
+
+def test_batch_matches_oracle(zt, oracle):
+    import torch
+    b, cpu = _make(zt, SMALL, stride_pad=3)
+    b.run()
+    torch.cuda.synchronize()
+    for i, (w, h, fmt, srgb, levels, kind) in enumerate(SMALL):
+        levels = b.items[i]["levels"]
+        chain = oracle.mip_chain(cpu[i], srgb)[:levels]
+        for lv in range(1, levels):
+            assert np.array_equal(b.level(i, lv).cpu().numpy(), chain[lv]), (i, lv)
+        for lv in range(levels):
+            assert np.array_equal(b.blocks_of(i, lv).cpu().numpy(), oracle.bc_encode(chain[lv], fmt)), (i, lv, fmt)
+    b.close()
+
+
+def test_batch_graph_replays(zt):
+    import torch
+    b, _ = _make(zt, SMALL[:8])
+    b.run()
+    torch.cuda.synchronize()
+    first = _snapshot(b)
+    for rep in range(3):
+        for it in b.items:           # eager work between replays: clear every output
+            it["out"].zero_()
+            it["mips"].zero_()
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        again = _snapshot(b)
+        for (m0, o0), (m1, o1) in zip(first, again):
+            assert all(np.array_equal(x, y) for x, y in zip(m0, m1)) and all(np.array_equal(x, y) for x, y in zip(o0, o1)), rep
+    b.close()
+
+
+def test_c4_slice_equals_single_calls(zt):
+    import torch
+    specs = batch.c4_specs(512, 4)[:24]
+    bases = [batch.texture_gpu(w, h, kind, 4 * 100000 + i) for i, (w, h, fmt, srgb, kind) in enumerate(specs)]
+    b = batch.Batch(zt, [(bases[i], w, h, fmt, srgb, None) for i, (w, h, fmt, srgb, kind) in enumerate(specs)])
+    b.run()
+    torch.cuda.synchronize()
+    for i, (w, h, fmt, srgb, kind) in enumerate(specs):
+        sizes = binding.mip_sizes(w, h)
+        levels = [torch.empty((hh, ww, 4), dtype=torch.uint8, device="cuda") for (ww, hh) in sizes[1:]]
+        if levels:
+            zt.mip_chain_dev(bases[i], w, h, w * 4, levels, srgb=srgb)
+        srcs = [bases[i]] + levels
+        for l0 in range(0, len(sizes), 16):
+            idx = range(l0, min(len(sizes), l0 + 16))
+            outs = {l: torch.empty(binding.bc_size(*sizes[l], fmt), dtype=torch.uint8, device="cuda") for l in idx}
+            imgs = zt.images((srcs[l], outs[l], sizes[l][0], sizes[l][1], sizes[l][0] * 4, fmt) for l in idx)
+            ws = torch.empty(max(1, zt.workspace_size(imgs)), dtype=torch.uint8, device="cuda")
+            zt.encode_dev(imgs, ws)
+            torch.cuda.synchronize()
+            for l in idx:
+                if l:
+                    assert torch.equal(b.level(i, l), srcs[l]), (i, l)
+                assert torch.equal(b.blocks_of(i, l), outs[l]), (i, l, fmt, (w, h))
+    b.close()
+
+
+def test_launch_shapes_do_not_change_bytes(zt):
+    import torch
+    b, _ = _make(zt, SMALL[:10], seed0=700)
+    b.run()
+    torch.cuda.synchronize()
+    ref = _snapshot(b)
+    b.close()
+    names = ["zt_bc14_encode", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1", "zt_mip_batch"]
+    saved = {k: zt.get_launch(k) for k in names}
+    try:
+        for wg, cap in ((64, 3), (128, 37)):
+            for k in names:
+                zt.set_launch(k, wg, cap)
+            b2, _ = _make(zt, SMALL[:10], seed0=700)
+            b2.run()
+            torch.cuda.synchronize()
+            got = _snapshot(b2)
+            b2.close()
+            for (m0, o0), (m1, o1) in zip(ref, got):
+                assert all(np.array_equal(x, y) for x, y in zip(m0, m1)) and all(np.array_equal(x, y) for x, y in zip(o0, o1)), (wg, cap)
+    finally:
+        for k, (wg, cap) in saved.items():
+            zt.set_launch(k, wg, cap)

@@ -61,6 +61,11 @@ class ZapImage(C.Structure):
                 ("w", C.c_int), ("h", C.c_int), ("fmt", C.c_int)]
 
 
+class ZapTexture(C.Structure):
+    _fields_ = [("base", C.c_void_p), ("stride", C.c_size_t), ("w", C.c_int), ("h", C.c_int), ("fmt", C.c_int), ("srgb", C.c_int),
+                ("levels", C.c_int), ("mips", C.c_void_p), ("out", C.c_void_p)]
+
+
 class MlpTargets(C.Structure):
     _fields_ = [("labels", C.c_void_p), ("deltas", C.c_void_p), ("temperature", C.c_float), ("risk_k", C.c_int),
                 ("risk_weight", C.c_float)]
@@ -123,6 +128,20 @@ class Library:
         L.zaptex_mlp_workspace_size.argtypes = [i]; L.zaptex_mlp_workspace_size.restype = C.c_size_t
         L.zaptex_bc7_learned_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
         L.zaptex_bc7_learned_workspace_size.restype = C.c_size_t
+        for name, args in {"zaptex_batch_mip_bytes": [i, i, i], "zaptex_batch_out_bytes": [i, i, i, i],
+                           "zaptex_batch_workspace_size": [C.POINTER(ZapTexture), i]}.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = C.c_size_t
+        L.zaptex_batch_create.argtypes = [C.POINTER(ZapTexture), i, vp, sz, C.POINTER(vp)]
+        L.zaptex_batch_create.restype = C.c_int
+        L.zaptex_batch_run.argtypes = [vp, vp]
+        L.zaptex_batch_run.restype = C.c_int
+        L.zaptex_batch_destroy.argtypes = [vp]
+        L.zaptex_batch_destroy.restype = None
+        L.zaptex_set_launch.argtypes = [C.c_char_p, i, i]
+        L.zaptex_set_launch.restype = C.c_int
+        L.zaptex_get_launch.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.zaptex_get_launch.restype = C.c_int
         L.zaptex_bc7_full_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
         L.zaptex_bc7_full_workspace_size.restype = C.c_size_t
         L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
@@ -322,6 +341,45 @@ class Library:
     def bc7_learned_dev(self, imgs, state, topk: int, threshold: float, ws, stream=None) -> None:
         self._check(self.lib.zaptex_bc7_learned_dev(imgs, len(imgs), state.data_ptr(), topk, threshold, ws.data_ptr(), ws.numel(),
                                                     self._stream(stream)), "zaptex_bc7_learned_dev")
+
+    # ------------------------------------------------------------------ batch (device buffers as torch tensors)
+    def batch_mip_bytes(self, w: int, h: int, levels: int) -> int:
+        return int(self.lib.zaptex_batch_mip_bytes(w, h, levels))
+
+    def batch_out_bytes(self, w: int, h: int, fmt, levels: int) -> int:
+        return int(self.lib.zaptex_batch_out_bytes(w, h, fmt_id(fmt), levels))
+
+    @staticmethod
+    def textures(specs) -> C.Array:
+        """specs: (base tensor, stride, w, h, fmt, srgb, levels, mips tensor or None, out tensor)"""
+        specs = list(specs)
+        arr = (ZapTexture * len(specs))()
+        for k, (base, stride, w, h, fmt, srgb, levels, mips, out) in enumerate(specs):
+            arr[k] = ZapTexture(base.data_ptr(), stride, w, h, fmt_id(fmt), int(srgb), levels, mips.data_ptr() if mips is not None else None,
+                                out.data_ptr())
+        return arr
+
+    def batch_workspace_size(self, texs) -> int:
+        return int(self.lib.zaptex_batch_workspace_size(texs, len(texs)))
+
+    def batch_create(self, texs, ws) -> int:
+        h = C.c_void_p()
+        self._check(self.lib.zaptex_batch_create(texs, len(texs), ws.data_ptr(), ws.numel(), C.byref(h)), "zaptex_batch_create")
+        return h.value
+
+    def batch_run(self, handle: int, stream=None) -> None:
+        self._check(self.lib.zaptex_batch_run(handle, self._stream(stream)), "zaptex_batch_run")
+
+    def batch_destroy(self, handle: int) -> None:
+        self.lib.zaptex_batch_destroy(handle)
+
+    def set_launch(self, kernel: str, wg: int, max_groups: int) -> None:
+        self._check(self.lib.zaptex_set_launch(kernel.encode(), wg, max_groups), "zaptex_set_launch")
+
+    def get_launch(self, kernel: str) -> tuple[int, int]:
+        wg, mg = C.c_int(), C.c_int()
+        self._check(self.lib.zaptex_get_launch(kernel.encode(), C.byref(wg), C.byref(mg)), "zaptex_get_launch")
+        return wg.value, mg.value
 
     # ------------------------------------------------------------------ kernel timing
     def profile(self, on: bool) -> None:

@@ -21,8 +21,22 @@ thread_local char g_err[512];
 std::mutex g_mu;     // serialises host entry points (shared staging buffers)
 int g_m1k = 4;
 
-// optional per-kernel timing: HIP events recorded on the launch stream around every kernel
+// launch shapes of the grid-stride kernels: workgroup size (64, 128 or 256) and the cap on workgroups;
+// zaptex_set_launch changes them (the autotuner's results), a batch captures those in force when it is built
+struct LaunchCfg { const char *name; int wg, max_groups; };
+LaunchCfg g_cfg[] = { { "zt_bc14_encode", 256, 8192 }, { "zt_bc7_stage6", 256, 8192 }, { "zt_bc7_stage5", 256, 2048 },
+                      { "zt_bc7_s1", 256, 1024 }, { "zt_mip_batch", 256, 8192 } };
+enum { CFG_BC14, CFG_S6, CFG_S5, CFG_S1, CFG_MIPB, CFG_N };
+// workgroups for `items` work items of `per_group` each under config c (at least 1)
+int groups_for(int c, uint64_t items, uint64_t per_group) {
+    uint64_t g = (items + per_group - 1) / per_group;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)g_cfg[c].max_groups));
+}
+
+// optional per-kernel timing: HIP events recorded on the launch stream around every kernel (not while a
+// batch graph is being captured)
 bool g_profile = false;
+bool g_capturing = false;
 std::mutex g_prof_mu;
 struct ProfPending { const char *name; hipEvent_t a, b; };
 struct ProfTotal { const char *name; double ms; long n; };
@@ -45,14 +59,15 @@ int fail(int code, const char *fmt, ...) {
 // launch `fn` on stream s; with profiling on, bracket it with events
 template <typename F> int launch(const char *name, hipStream_t s, F &&fn) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (g_profile) {
+    const bool prof = g_profile && !g_capturing;
+    if (prof) {
         HIP_TRY(hipEventCreate(&a)); HIP_TRY(hipEventCreate(&b));
         HIP_TRY(hipEventRecord(a, s));
     }
     fn();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ZAPTEX_E_DEVICE, "launch of %s: %s", name, hipGetErrorString(e));
-    if (g_profile) {
+    if (prof) {
         HIP_TRY(hipEventRecord(b, s));
         std::lock_guard<std::mutex> lk(g_prof_mu);
         g_prof_pending.push_back({ name, a, b });
@@ -164,6 +179,40 @@ int grid_for(uint64_t items, int per) {
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 8192));
 }
 
+// BC1/BC3/BC4/BC5 over one job table
+int enqueue_bc14(const zt_jobs &J, uint32_t tot, hipStream_t s) {
+    const int wg = g_cfg[CFG_BC14].wg;
+    return launch("zt_bc14_encode", s, [&] { hipLaunchKernelGGL(zt_bc14_encode, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot); });
+}
+
+// the reference BC7 search (mode 6, then 5, then 1) over one job table; ws >= ws_bytes_for(tot)
+int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (ws_bytes < ws_bytes_for(tot) || !ws) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, ws_bytes_for(tot));
+    zt_bc7_lists L = lists_in(ws, tot);
+    int rc;
+    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, L.counters, 64); }))) return rc;
+    const int w6 = g_cfg[CFG_S6].wg, w5 = g_cfg[CFG_S5].wg, w1 = g_cfg[CFG_S1].wg;
+    if ((rc = launch("zt_bc7_stage6", s, [&] {
+            hipLaunchKernelGGL(zt_bc7_stage6, dim3(groups_for(CFG_S6, tot, w6)), dim3(w6), 0, s, J, tot, L); })))
+        return rc;
+    if ((rc = launch("zt_bc7_stage5", s, [&] {
+            hipLaunchKernelGGL(zt_bc7_stage5, dim3(groups_for(CFG_S5, (uint64_t)tot * 4, w5)), dim3(w5), 0, s, J, L); })))
+        return rc;
+    if (g_m1k <= 8) {
+        const int gsz = groups_for(CFG_S1, (uint64_t)tot / 16 + 1, w1 / 64);
+        if ((rc = launch("zt_bc7_s1_rank", s, [&] { hipLaunchKernelGGL(zt_bc7_s1_rank, dim3(gsz), dim3(w1), 0, s, L, g_m1k, L.cand1); })))
+            return rc;
+        if ((rc = launch("zt_bc7_s1_fit", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_s1_fit, dim3(gsz), dim3(w1), 0, s, L, g_m1k, (const uint8_t *)L.cand1, L.sub_err, L.sub_state); })))
+            return rc;
+        return launch("zt_bc7_s1_pick", s, [&] {
+            hipLaunchKernelGGL(zt_bc7_s1_pick, dim3(gsz), dim3(w1), 0, s, J, L, g_m1k, (const uint8_t *)L.cand1, (const int *)L.sub_err,
+                               (const uint4 *)L.sub_state); });
+    }
+    return launch("zt_bc7_stage1", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot, 4)), dim3(256), 0, s, J, L, g_m1k); });
+}
+
 // enqueue the encoders for up to ZT_INLINE_JOBS images (already validated)
 int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStream_t s) {
     zt_jobs fam[3];  // 0: BC1-5, 1: BC7, 2: BC6H, ASTC handled per kernel below
@@ -184,41 +233,14 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
         *t += nblocks(im);
     }
     int rc;
-    if (tot[0] && (rc = launch("zt_bc14_encode", s, [&] {
-            hipLaunchKernelGGL(zt_bc14_encode, dim3(grid_for(tot[0], 256)), dim3(256), 0, s, fam[0], tot[0]); })))
-        return rc;
+    if (tot[0] && (rc = enqueue_bc14(fam[0], tot[0], s))) return rc;
     if (tot_b6 && (rc = launch("zt_bc6h_encode", s, [&] {
             hipLaunchKernelGGL(zt_bc6h_encode, dim3(grid_for(tot_b6, 256)), dim3(256), 0, s, b6, tot_b6); })))
         return rc;
     if (tot_astc && (rc = launch("zt_astc_encode", s, [&] {
             hipLaunchKernelGGL(zt_astc_encode, dim3(grid_for(tot_astc, 256)), dim3(256), 0, s, astc, tot_astc); })))
         return rc;
-    if (tot[1]) {
-        if (ws_bytes < ws_bytes_for(tot[1]) || !ws) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, ws_bytes_for(tot[1]));
-        zt_bc7_lists L = lists_in(ws, tot[1]);
-        if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, L.counters, 64); })))
-            return rc;
-        if ((rc = launch("zt_bc7_stage6", s, [&] {
-                hipLaunchKernelGGL(zt_bc7_stage6, dim3(grid_for(tot[1], 256)), dim3(256), 0, s, fam[1], tot[1], L); })))
-            return rc;
-        if ((rc = launch("zt_bc7_stage5", s, [&] {
-                hipLaunchKernelGGL(zt_bc7_stage5, dim3(std::min(2048, grid_for((uint64_t)tot[1] * 4, 256))), dim3(256), 0, s, fam[1], L); })))
-            return rc;
-        if (g_m1k <= 8) {
-            const int gsz = std::max(1, std::min(1024, grid_for((uint64_t)tot[1] / 16 + 1, 4)));
-            if ((rc = launch("zt_bc7_s1_rank", s, [&] { hipLaunchKernelGGL(zt_bc7_s1_rank, dim3(gsz), dim3(256), 0, s, L, g_m1k, L.cand1); })))
-                return rc;
-            if ((rc = launch("zt_bc7_s1_fit", s, [&] {
-                    hipLaunchKernelGGL(zt_bc7_s1_fit, dim3(gsz), dim3(256), 0, s, L, g_m1k, (const uint8_t *)L.cand1, L.sub_err, L.sub_state); })))
-                return rc;
-            if ((rc = launch("zt_bc7_s1_pick", s, [&] {
-                    hipLaunchKernelGGL(zt_bc7_s1_pick, dim3(gsz), dim3(256), 0, s, fam[1], L, g_m1k, (const uint8_t *)L.cand1, (const int *)L.sub_err,
-                                       (const uint4 *)L.sub_state); })))
-                return rc;
-        } else if ((rc = launch("zt_bc7_stage1", s, [&] {
-                       hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot[1], 4)), dim3(256), 0, s, fam[1], L, g_m1k); })))
-            return rc;
-    }
+    if (tot[1]) return enqueue_bc7(fam[1], tot[1], ws, ws_bytes, s);
     return ZAPTEX_OK;
 }
 
@@ -866,6 +888,223 @@ int zaptex_bc_decode_dev(const void *bc, int w, int h, int fmt, void *rgba, size
     hipLaunchKernelGGL(zt_bc_decode, dim3((bw * bh + 255) / 256), dim3(256), 0, (hipStream_t)stream, (const uint8_t *)bc, w, h, fmt, (uint8_t *)rgba, (uint64_t)stride);
     HIP_TRY(hipGetLastError());
     return ZAPTEX_OK;
+}
+
+// ===================================================================================== batch
+}  // extern "C"
+namespace {
+// byte offsets of levels 1..levels-1 in a texture's mip arena (tight rows, each level 256-B aligned) and of
+// levels 0..levels-1 in its block arena
+void batch_layout(int w, int h, int fmt, int levels, std::vector<size_t> *mip_off, std::vector<size_t> *out_off, size_t *mip_total,
+                  size_t *out_total) {
+    size_t mo = 0, oo = 0;
+    int cw = w, ch = h;
+    for (int l = 0; l < levels; l++) {
+        if (out_off) out_off->push_back(oo);
+        oo += align_up(out_bytes(cw, ch, fmt), 256);
+        if (l + 1 < levels) {
+            cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
+            if (mip_off) mip_off->push_back(mo);
+            mo += align_up((size_t)cw * ch * 4, 256);
+        }
+    }
+    if (mip_total) *mip_total = mo;
+    if (out_total) *out_total = oo;
+}
+int max_levels(int w, int h) { int n = 1; while (w > 1 || h > 1) { w = w > 1 ? w / 2 : 1; h = h > 1 ? h / 2 : 1; n++; } return n; }
+int check_texture(const zaptex_texture &t) {
+    if (t.fmt < ZAPTEX_BC1 || t.fmt > ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "batch formats are BC1, BC3, BC4, BC5 and BC7 (got %d)", t.fmt);
+    if (t.w < 1 || t.h < 1 || t.w > (1 << 16) || t.h > (1 << 16)) return fail(ZAPTEX_E_ARG, "bad size %dx%d", t.w, t.h);
+    if (t.levels < 1 || t.levels > max_levels(t.w, t.h)) return fail(ZAPTEX_E_ARG, "levels %d outside 1..%d", t.levels, max_levels(t.w, t.h));
+    if (!t.base || !t.out || (t.levels > 1 && !t.mips)) return fail(ZAPTEX_E_ARG, "null texture pointer");
+    if (t.stride < (size_t)t.w * 4 || (((uintptr_t)t.base) | t.stride) % 4) return fail(ZAPTEX_E_ARG, "base rows must be 4-byte aligned, stride >= 4 w");
+    if (((uintptr_t)t.out | (uintptr_t)t.mips) % 256) return fail(ZAPTEX_E_ARG, "mips and out arenas must be 256-byte aligned");
+    return ZAPTEX_OK;
+}
+struct BatchSizes { size_t mipjobs, jobs14, jobs7; uint64_t blocks14, blocks7; int nmip, n14, n7; };
+BatchSizes batch_sizes(const zaptex_texture *t, int n) {
+    BatchSizes z{};
+    for (int i = 0; i < n; i++) {
+        z.nmip += t[i].levels - 1;
+        int cw = t[i].w, ch = t[i].h;
+        for (int l = 0; l < t[i].levels; l++) {
+            const uint64_t nb = (uint64_t)((cw + 3) / 4) * ((ch + 3) / 4);
+            if (t[i].fmt == ZAPTEX_BC7) { z.n7++; z.blocks7 += nb; } else { z.n14++; z.blocks14 += nb; }
+            cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
+        }
+    }
+    z.mipjobs = align_up((size_t)z.nmip * sizeof(zt_mipjob), 256);
+    z.jobs14 = align_up((size_t)z.n14 * sizeof(zt_job), 256);
+    z.jobs7 = align_up((size_t)z.n7 * sizeof(zt_job), 256);
+    return z;
+}
+}  // namespace
+
+struct zaptex_batch {
+    int device = 0;
+    std::vector<int> level_jobs;          // mip jobs per destination level 1.. (device tables back to back)
+    std::vector<uint64_t> level_px;       // destination pixels per level
+    std::vector<int> level_srgb;          // any sRGB job in the level
+    zt_mipjob *mipjobs = nullptr;
+    zt_jobs j14{}, j7{};
+    uint32_t tot14 = 0, tot7 = 0;
+    void *bc7ws = nullptr;
+    size_t bc7ws_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+size_t zaptex_batch_mip_bytes(int w, int h, int levels) {
+    size_t m = 0;
+    if (w < 1 || h < 1 || levels < 1) return 0;
+    batch_layout(w, h, ZAPTEX_BC1, levels, nullptr, nullptr, &m, nullptr);
+    return m;
+}
+size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels) {
+    size_t o = 0;
+    if (w < 1 || h < 1 || levels < 1) return 0;
+    batch_layout(w, h, fmt, levels, nullptr, nullptr, nullptr, &o);
+    return o;
+}
+size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
+    if (!t || n < 1) return 0;
+    BatchSizes z = batch_sizes(t, n);
+    return 256 + z.mipjobs + z.jobs14 + z.jobs7 + align_up(ws_bytes_for(z.blocks7), 256);
+}
+
+int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out) {
+    if (!t || n < 1 || !out) return fail(ZAPTEX_E_ARG, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++)
+        if (int rc = check_texture(t[i])) return rc;
+    BatchSizes z = batch_sizes(t, n);
+    if (z.blocks7 >= (1ull << 31) || z.blocks14 >= (1ull << 31)) return fail(ZAPTEX_E_ARG, "too many blocks in one batch");
+    if (!ws || ws_bytes < zaptex_batch_workspace_size(t, n)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, zaptex_batch_workspace_size(t, n));
+    DeviceState *st;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (int rc = ensure_ready(st)) return rc;
+    }
+    zaptex_batch *b = new zaptex_batch();
+    HIP_TRY(hipGetDevice(&b->device));
+    uint8_t *p = (uint8_t *)align_up((uintptr_t)ws, 256);
+    b->mipjobs = (zt_mipjob *)p; p += z.mipjobs;
+    zt_job *d14 = (zt_job *)p; p += z.jobs14;
+    zt_job *d7 = (zt_job *)p; p += z.jobs7;
+    b->bc7ws = p;
+    b->bc7ws_bytes = ws_bytes_for(z.blocks7);
+    // host tables: mip jobs grouped by destination level, encode jobs per family in texture/level order
+    int maxl = 0;
+    for (int i = 0; i < n; i++) maxl = std::max(maxl, t[i].levels);
+    std::vector<std::vector<zt_mipjob>> mj(std::max(0, maxl - 1));
+    std::vector<zt_job> h14, h7;
+    for (int i = 0; i < n; i++) {
+        std::vector<size_t> mo, oo;
+        batch_layout(t[i].w, t[i].h, t[i].fmt, t[i].levels, &mo, &oo, nullptr, nullptr);
+        int cw = t[i].w, ch = t[i].h;
+        const uint8_t *src = (const uint8_t *)t[i].base;
+        uint64_t stride = t[i].stride;
+        for (int l = 0; l < t[i].levels; l++) {
+            zt_job j{};
+            j.src = src; j.dst = (uint8_t *)t[i].out + oo[l]; j.stride = stride; j.w = cw; j.h = ch; j.bw = (cw + 3) / 4; j.fmt = t[i].fmt;
+            const uint32_t nb = (uint32_t)(((cw + 3) / 4) * ((ch + 3) / 4));
+            if (t[i].fmt == ZAPTEX_BC7) { j.first = b->tot7; b->tot7 += nb; h7.push_back(j); }
+            else { j.first = b->tot14; b->tot14 += nb; h14.push_back(j); }
+            if (l + 1 < t[i].levels) {
+                zt_mipjob m{};
+                m.src = src; m.dst = (uint8_t *)t[i].mips + mo[l]; m.stride = stride; m.w = cw; m.h = ch; m.srgb = t[i].srgb ? 1 : 0;
+                mj[l].push_back(m);
+                src = m.dst;
+                cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
+                stride = (uint64_t)cw * 4;
+            }
+        }
+    }
+    std::vector<zt_mipjob> flat;
+    for (auto &lv : mj) {
+        uint64_t px = 0;
+        int srgb = 0;
+        for (auto &m : lv) {
+            m.first = px;
+            px += (uint64_t)(m.w > 1 ? m.w / 2 : 1) * (uint64_t)(m.h > 1 ? m.h / 2 : 1);
+            srgb |= m.srgb;
+        }
+        b->level_jobs.push_back((int)lv.size());
+        b->level_px.push_back(px);
+        b->level_srgb.push_back(srgb);
+        flat.insert(flat.end(), lv.begin(), lv.end());
+    }
+    hipStream_t s = st->stream;
+    if (!flat.empty()) HIP_TRY(hipMemcpyAsync(b->mipjobs, flat.data(), flat.size() * sizeof(zt_mipjob), hipMemcpyHostToDevice, s));
+    if (!h14.empty()) HIP_TRY(hipMemcpyAsync(d14, h14.data(), h14.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
+    if (!h7.empty()) HIP_TRY(hipMemcpyAsync(d7, h7.data(), h7.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    b->j14.ext = d14; b->j14.n = (int)h14.size();
+    b->j7.ext = d7; b->j7.n = (int)h7.size();
+    // capture: mip levels in order, then the encoders (BC1-5, then BC7), as one graph
+    auto body = [&](hipStream_t cs) -> int {
+        int rc;
+        const zt_mipjob *lvjobs = b->mipjobs;
+        for (size_t l = 0; l < b->level_jobs.size(); l++) {
+            const int nj = b->level_jobs[l];
+            const uint64_t px = b->level_px[l];
+            const int srgb = b->level_srgb[l], wg = g_cfg[CFG_MIPB].wg;
+            if (nj && (rc = launch("zt_mip_batch", cs, [&] {
+                           hipLaunchKernelGGL(zt_mip_batch, dim3(groups_for(CFG_MIPB, px, wg)), dim3(wg), 0, cs, lvjobs, nj, px, srgb); })))
+                return rc;
+            lvjobs += nj;
+        }
+        if (b->tot14 && (rc = enqueue_bc14(b->j14, b->tot14, cs))) return rc;
+        if (b->tot7 && (rc = enqueue_bc7(b->j7, b->tot7, b->bc7ws, b->bc7ws_bytes, cs))) return rc;
+        return ZAPTEX_OK;
+    };
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
+        int rc = body(s);
+        g_capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); delete b; return rc; }
+        if (e != hipSuccess) { delete b; return fail(ZAPTEX_E_DEVICE, "graph capture: %s", hipGetErrorString(e)); }
+        b->graph = g;
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) { (void)hipGraphDestroy(g); delete b; return fail(ZAPTEX_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+    }
+    *out = b;
+    return ZAPTEX_OK;
+}
+
+int zaptex_batch_run(zaptex_batch *b, void *stream) {
+    if (!b || !b->exec) return fail(ZAPTEX_E_ARG, "no batch");
+    int dev;
+    if (int rc = current_device(dev)) return rc;
+    if (dev != b->device) return fail(ZAPTEX_E_ARG, "batch was built on device %d, current device is %d", b->device, dev);
+    HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return ZAPTEX_OK;
+}
+
+void zaptex_batch_destroy(zaptex_batch *b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    delete b;
+}
+
+int zaptex_set_launch(const char *kernel, int wg, int max_groups) {
+    if (!kernel || (wg != 64 && wg != 128 && wg != 256) || max_groups < 1)
+        return fail(ZAPTEX_E_ARG, "workgroup size must be 64, 128 or 256 and the group cap >= 1");
+    for (LaunchCfg &c : g_cfg)
+        if (!strcmp(c.name, kernel)) { c.wg = wg; c.max_groups = max_groups; return ZAPTEX_OK; }
+    return fail(ZAPTEX_E_ARG, "no tunable kernel named %s", kernel);
+}
+int zaptex_get_launch(const char *kernel, int *wg, int *max_groups) {
+    for (LaunchCfg &c : g_cfg)
+        if (kernel && !strcmp(c.name, kernel)) { if (wg) *wg = c.wg; if (max_groups) *max_groups = c.max_groups; return ZAPTEX_OK; }
+    return fail(ZAPTEX_E_ARG, "no tunable kernel named %s", kernel ? kernel : "(null)");
 }
 
 int zaptex_set_option(int key, int value) {

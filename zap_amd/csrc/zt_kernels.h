@@ -31,6 +31,15 @@ struct zt_gemm_args {
 };
 
 // destination levels of a fused mip launch (packed rows, 4 bytes per pixel)
+// one texture's level in a batched mip launch: source level (w x h, `stride` bytes per row) -> destination rows
+// packed at (w/2) * 4 bytes; `first` = index of its first destination pixel among all jobs of the launch
+struct zt_mipjob {
+    const uint8_t *src;
+    uint8_t *dst;
+    uint64_t stride;
+    uint64_t first;
+    int32_t w, h, srgb, pad;
+};
 struct zt_mip_levels {
     uint8_t *p[16];
     int w[16], h[16];
@@ -90,6 +99,7 @@ __global__ void zt_bc_shuffle(const uint8_t *in, uint8_t *out, uint64_t nb, int 
 __global__ void zt_mip_level(const uint8_t *src, int w, int h, uint64_t stride, int srgb, uint8_t *dst, uint64_t dst_stride);
 __global__ void zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
 __global__ void zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
+__global__ void zt_mip_batch(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb);
 __global__ void zt_mip_level_f(const float *src, int w, int h, uint64_t stride, float *dst, uint64_t dst_stride);
 }
 

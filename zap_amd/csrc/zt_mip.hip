@@ -179,3 +179,36 @@ extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *sr
         cw = nw; ch = nh;
     }
 }
+
+// ---------------------------------------------------------------- batched level
+// One level of many textures in one launch: lane per destination pixel over all jobs (grid-stride); a job is one
+// texture's level l from its level l-1, destination rows packed (width * 4 bytes).  Jobs sorted by `first`.
+namespace {
+ZT_DEV int find_mip_job(const zt_mipjob *J, int n, uint64_t p) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (J[mid].first <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256) zt_mip_batch(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb) {
+    __shared__ float lin[256], thr[256];
+    if (any_srgb)
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
+    __syncthreads();
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += step) {
+        const zt_mipjob J = jobs[find_mip_job(jobs, njobs, p)];
+        const int nw = J.w > 1 ? J.w / 2 : 1;
+        const uint64_t local = p - J.first;
+        const int x = (int)(local % (uint64_t)nw), y = (int)(local / (uint64_t)nw);
+        const int x0 = src_x0(x, J.w), x1 = src_x1(x, J.w), y0 = y * 2 < J.h ? y * 2 : J.h - 1, y1 = y * 2 + 1 < J.h ? y * 2 + 1 : J.h - 1;
+        const uint8_t *r0 = J.src + (uint64_t)y0 * J.stride, *r1 = J.src + (uint64_t)y1 * J.stride;
+        const uint32_t a = *(const uint32_t *)(r0 + x0 * 4), b = *(const uint32_t *)(r0 + x1 * 4);
+        const uint32_t c = *(const uint32_t *)(r1 + x0 * 4), d = *(const uint32_t *)(r1 + x1 * 4);
+        *(uint32_t *)(J.dst + local * 4) = box4(a, b, c, d, J.srgb, lin, thr);
+    }
+}
