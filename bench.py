@@ -141,13 +141,14 @@ def _cpu_model() -> str:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-texture batch leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-entry (PCIe-inclusive) leg")
+    ap.add_argument("--batch-only", action="store_true", help="only build and time the 512-texture batch (for profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the profiler's HBM byte counters)")
@@ -174,6 +175,9 @@ def main() -> int:
     lib = zap_amd.load()
     torch.cuda.set_device(0)
     from zap_amd import autotune
+    if args.batch_only:
+        print(json.dumps(batch_leg(lib, args.steps)), flush=True)
+        return 0
     shapes_c2 = autotune.apply(lib, "c2")
     golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
     ops = json.load(open(os.path.join(ROOT, "tests", "golden", "ops.json")))["c2_bc7_chain"]

@@ -34,7 +34,8 @@ enum { ZAPTEX_BC1 = 0, ZAPTEX_BC3 = 1, ZAPTEX_BC4 = 2, ZAPTEX_BC5 = 3, ZAPTEX_BC
 /* options */
 enum {
     ZAPTEX_OPT_BC7_M1K = 1,  /* mode-1 partitions fully fitted after ranking, 1..64 (zap_tex.h:750-752, default 4) */
-    ZAPTEX_OPT_PROFILE = 2   /* 1: time every kernel with HIP events on its launch stream (zaptex_profile_*) */
+    ZAPTEX_OPT_PROFILE = 2,  /* 1: time every kernel with HIP events on its launch stream (zaptex_profile_*) */
+    ZAPTEX_OPT_CLIMB_REFILL = 3  /* BC1 hill climb: idle lanes (1..64) a wave gathers before it refills them */
 };
 
 /* ---------------- host entry points (drop-in bodies of zap_tex.h) */
@@ -74,7 +75,8 @@ typedef struct {
 /* workspace bytes zaptex_encode_dev needs for these images */
 size_t zaptex_workspace_size(const zaptex_image *imgs, int n);
 /* Encode n (1..16) images, any mix of formats, on `stream`.  The workspace is device memory of at least
- * zaptex_workspace_size bytes, owned by the caller; calls sharing a workspace must not overlap. */
+ * zaptex_workspace_size bytes (256-byte aligned), owned by the caller; calls sharing a workspace must not
+ * overlap. */
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream);
 /* Exhaustive all-mode BC7 search (no reference counterpart: the reference tries modes 6, 5 and 1 only).
  * 285 candidates per block (mode*64 + slot); never worse than zaptex_encode_dev's reference search.
@@ -155,8 +157,8 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
 /* one launch of the graph on `stream`; runs of one plan must not overlap (they share the workspace) */
 int zaptex_batch_run(zaptex_batch *b, void *stream);
 void zaptex_batch_destroy(zaptex_batch *b);
-/* launch shape of a grid-stride kernel ("zt_bc14_encode", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1",
- * "zt_mip_batch"): workgroup size 64/128/256 and the cap on workgroups (the autotuner sets these) */
+/* launch shape of a grid-stride kernel ("zt_bc14_encode", "zt_bc1_climb", "zt_bc7_stage6", "zt_bc7_stage5",
+ * "zt_bc7_s1", "zt_mip_batch"): workgroup size 64/128/256 and the cap on workgroups (the autotuner sets these) */
 int zaptex_set_launch(const char *kernel, int wg, int max_groups);
 int zaptex_get_launch(const char *kernel, int *wg, int *max_groups);
 

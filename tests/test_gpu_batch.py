@@ -107,7 +107,7 @@ def test_launch_shapes_do_not_change_bytes(zt):
     torch.cuda.synchronize()
     ref = _snapshot(b)
     b.close()
-    names = ["zt_bc14_encode", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1", "zt_mip_batch"]
+    names = ["zt_bc14_encode", "zt_bc1_climb", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1", "zt_mip_batch"]
     saved = {k: zt.get_launch(k) for k in names}
     try:
         for wg, cap in ((64, 3), (128, 37)):

@@ -77,7 +77,7 @@ def test_c2_bc7_4096_chain_on_device(zt, golden):
     # the same chain as BC1 below level 2 in one mixed call
     outs1 = [torch.empty(binding.bc_size(w, h, "bc1"), dtype=torch.uint8, device="cuda") for (w, h) in sizes[2:]]
     imgs1 = zt.images((lin_levels[i + 2], outs1[i], w, h, w * 4, "bc1") for i, (w, h) in enumerate(sizes[2:]))
-    zt.encode_dev(imgs1, None)
+    zt.encode_dev(imgs1, torch.empty(zt.workspace_size(imgs1), dtype=torch.uint8, device="cuda"))
     torch.cuda.synchronize()
     assert [sha(o.cpu().numpy()) for o in outs1] == golden["outputs"]["c2_bc1_chain"]
 

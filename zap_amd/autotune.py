@@ -18,7 +18,7 @@ from . import binding
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_PATH = os.path.join(HERE, "tuning", "mi355x.json")
-KERNELS = {"c4": ["zt_bc7_stage6", "zt_bc14_encode", "zt_bc7_stage5", "zt_mip_batch", "zt_bc7_s1"],
+KERNELS = {"c4": ["zt_bc1_climb", "zt_bc7_stage6", "zt_bc14_encode", "zt_bc7_stage5", "zt_mip_batch", "zt_bc7_s1"],
            "c2": ["zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1"]}
 WGS = (64, 128, 256)
 CAPS = (256, 512, 1024, 2048, 4096, 8192, 16384, 1 << 20)
@@ -80,7 +80,7 @@ def apply(lib, workload: str, path: str = TUNING_PATH) -> dict | None:
 
 
 DEFAULTS = {"zt_bc14_encode": (256, 8192), "zt_bc7_stage6": (256, 8192), "zt_bc7_stage5": (256, 2048), "zt_bc7_s1": (256, 1024),
-            "zt_mip_batch": (256, 8192)}
+            "zt_mip_batch": (256, 8192), "zt_bc1_climb": (256, 8192)}
 
 
 def reset(lib) -> None:

@@ -2,7 +2,9 @@
 //
 // BC1 (zap_tex.h:97-168): PCA by power iteration, two starts (inset or not) with <= 3 least-squares
 // refinements, then a 565 hill climb (<= 8 passes of 12 one-step moves).  Every trial palette argmin is an
-// exact integer max over 4 keys built with v_dot4_u32_u8.  BC4 (zap_tex.h:198-213): 9 inset endpoint pairs,
+// exact integer max over 4 keys built with v_dot4_u32_u8.  The hill climb's length varies per block (a
+// wave would wait for its slowest lane, about twice the mean), so it runs in its own kernel where every lane
+// that finishes takes the next block of its wave's range: each step is one trial per lane.  BC4 (zap_tex.h:198-213): 9 inset endpoint pairs,
 // 8-entry palette argmin, integer only.  BC3 = BC4(alpha) @0 + BC1 @8; BC5 = BC4(R) @0 + BC4(G) @8
 // (zap_tex.h:1490-1501 at rdo = 0).
 #include "zt_common.h"
@@ -20,9 +22,19 @@ ZT_DEV uint32_t to565(float r, float g, float b) {  // zap_tex.h:45-49
 }
 
 // SSE and indices of a canonical (c0 >= c1) 4-colour endpoint pair; c0 == c1 is solid with all indices 0
-// (zap_tex.h:52-94 with force4 = 1).  idx: 2 bits per pixel.
-ZT_DEV int bc1_trial(const uint32_t px[16], uint32_t c0, uint32_t c1, uint32_t &idx) {
-    uint32_t a = rgb565_expand(c0), b = rgb565_expand(c1);
+// (zap_tex.h:52-94 with force4 = 1).  idx: 2 bits per pixel.  pm = the block's pixels with alpha cleared,
+// spp = sum of |pm|^2 (both per block).  key_j = 8 p.pal_j - 4 |pal_j|^2 + (3 - j) = 4 S_j + (3 - j) with
+// S_j = 2 p.pal_j - |pal_j|^2 = |p|^2 - SSE_j: the largest key is the smallest SSE, lowest index on ties;
+// key >> 2 = S_j exactly, and the low 2 bits are 3 - j.
+ZT_DEV int bc1_trial(const uint32_t pm[16], int spp, uint32_t c0, uint32_t c1, uint32_t &idx) {
+    const uint32_t a = rgb565_expand(c0), b = rgb565_expand(c1);
+    if (c0 == c1) {
+        uint32_t pa = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) pa = __builtin_amdgcn_udot4(pm[i], a, pa, false);
+        idx = 0;
+        return spp - 2 * (int)pa + 16 * (int)dot4(a, a);
+    }
     uint32_t p2 = 0, p3 = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -30,40 +42,36 @@ ZT_DEV int bc1_trial(const uint32_t px[16], uint32_t c0, uint32_t c1, uint32_t &
         p2 |= (uint32_t)((2 * x + y + 1) / 3) << (8 * k);
         p3 |= (uint32_t)((x + 2 * y + 1) / 3) << (8 * k);
     }
-    if (c0 == c1) {
-        int err = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            uint32_t p = px[i] & 0xFFFFFFu;
-            err += (int)dot4(p, p) - 2 * (int)dot4(p, a) + (int)dot4(a, a);
-        }
-        idx = 0;
-        return err;
-    }
-    const uint32_t pal[4] = { a, b, p2, p3 };
-    int K[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) K[j] = 15 - j - 16 * (int)dot4(pal[j], pal[j]);
-    int err = 0;
-    uint32_t bits = 0;
+    const int K0 = 3 - 4 * (int)dot4(a, a), K1 = 2 - 4 * (int)dot4(b, b), K2 = 1 - 4 * (int)dot4(p2, p2), K3 = -4 * (int)dot4(p3, p3);
+    int sum = 0;
+    uint32_t inv = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        uint32_t p = px[i] & 0xFFFFFFu;
-        int best = (int)(dot4(p, pal[0]) << 5) + K[0];
-#pragma unroll
-        for (int j = 1; j < 4; j++) best = max(best, (int)(dot4(p, pal[j]) << 5) + K[j]);
-        bits |= (uint32_t)(15 - (best & 15)) << (2 * i);
-        err += (int)dot4(p, p) - (best >> 4);
+        const int k0 = (int)(dot4(pm[i], a) << 3) + K0, k1 = (int)(dot4(pm[i], b) << 3) + K1;
+        const int k2 = (int)(dot4(pm[i], p2) << 3) + K2, k3 = (int)(dot4(pm[i], p3) << 3) + K3;
+        const int best = max(max(k0, k1), max(k2, k3));
+        inv |= (uint32_t)(best & 3) << (2 * i);
+        sum += best >> 2;
     }
-    idx = bits;
-    return err;
+    idx = ~inv;
+    return spp - sum;
 }
-ZT_DEV int bc1_attempt(const uint32_t px[16], uint32_t &c0, uint32_t &c1, uint32_t &idx) {  // zap_tex.h:90-94
+ZT_DEV int bc1_attempt(const uint32_t pm[16], int spp, uint32_t &c0, uint32_t &c1, uint32_t &idx) {  // zap_tex.h:90-94
     if (c0 < c1) { uint32_t t = c0; c0 = c1; c1 = t; }
-    return bc1_trial(px, c0, c1, idx);
+    return bc1_trial(pm, spp, c0, c1, idx);
+}
+// alpha-cleared pixels and their sum of squares
+ZT_DEV int bc1_prepare(const uint32_t px[16], uint32_t pm[16]) {
+    uint32_t spp = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) { pm[i] = px[i] & 0xFFFFFFu; spp = __builtin_amdgcn_udot4(pm[i], pm[i], spp, false); }
+    return (int)spp;
 }
 
-ZT_DEV int bc1_block(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32_t &oidx) {
+// everything before the hill climb: PCA, two starts, least squares; returns the best SSE
+ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32_t &oidx) {
+    uint32_t pm[16];
+    const int spp = bc1_prepare(px, pm);
     float mean[3] = { 0.0f, 0.0f, 0.0f }, cv[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
     for (int i = 0; i < 16; i++)
@@ -99,7 +107,7 @@ ZT_DEV int bc1_block(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32
         uint32_t c0 = to565(mean[0] + ax[0] * u, mean[1] + ax[1] * u, mean[2] + ax[2] * u);
         uint32_t c1 = to565(mean[0] + ax[0] * l, mean[1] + ax[1] * l, mean[2] + ax[2] * l);
         uint32_t idx = 0;
-        int e = bc1_attempt(px, c0, c1, idx);
+        int e = bc1_attempt(pm, spp, c0, c1, idx);
         for (int it = 0; it < 3 && e > 0; it++) {  // least squares on the chosen indices
             const float W1 = 1.0f, W2 = 0.0f, W3 = 2.0f / 3.0f, W4 = 1.0f / 3.0f;
             float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[3] = { 0.0f, 0.0f, 0.0f }, sb[3] = { 0.0f, 0.0f, 0.0f };
@@ -117,98 +125,188 @@ ZT_DEV int bc1_block(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32
 #pragma unroll
             for (int c = 0; c < 3; c++) { A[c] = (sa[c] * bb - sb[c] * ab) / det; B[c] = (sb[c] * aa - sa[c] * ab) / det; }
             uint32_t n0 = to565(A[0], A[1], A[2]), n1 = to565(B[0], B[1], B[2]), ni = 0;
-            int ne = bc1_attempt(px, n0, n1, ni);
+            int ne = bc1_attempt(pm, spp, n0, n1, ni);
             if (ne >= e) break;
             e = ne; c0 = n0; c1 = n1; idx = ni;
         }
         if (e < best) { best = e; b0 = c0; b1 = c1; bidx = idx; }
     }
-    // hill climb: +-1 on each 565 channel of each endpoint while it helps (order: ep, channel, -1 then +1)
-    for (int pass = 0, improved = 1; improved && pass < 8 && best > 0; pass++) {
-        improved = 0;
-        for (int mv = 0; mv < 12; mv++) {
-            int ep = mv / 6, ch = (mv >> 1) % 3, d = (mv & 1) ? 1 : -1;
-            int sh = ch == 0 ? 11 : ch == 1 ? 5 : 0, mx = ch == 1 ? 63 : 31;
-            uint32_t c = ep ? b1 : b0;
-            int v = (int)((c >> sh) & (uint32_t)mx);
-            if (v + d < 0 || v + d > mx) continue;
-            c = (c & ~((uint32_t)mx << sh)) | ((uint32_t)(v + d) << sh);
-            uint32_t n0 = ep ? b0 : c, n1 = ep ? c : b1, ni = 0;
-            int ne = bc1_attempt(px, n0, n1, ni);
-            if (ne < best) { best = ne; b0 = n0; b1 = n1; bidx = ni; improved = 1; }
-        }
-    }
     oc0 = b0; oc1 = b1; oidx = bidx;
     return best;
 }
 
-// BC4 on channel `ch` of the block; returns the 8-byte block (a0, a1, 48 index bits)
-ZT_DEV uint64_t bc4_block(const uint32_t px[16], int ch) {
-    int v[16], lo = 255, hi = 0;
+// hill-climb state of one block: +-1 on each 565 channel of each endpoint (order: endpoint, channel, -1 then +1),
+// passes of 12 moves while a pass improved, at most 8 (zap_tex.h:152-166)
+struct Climb {
+    uint32_t b0, b1, bidx;
+    int best, pass, mv, improved;
+};
+// the next valid move's endpoint pair (cheap; no trial); false when the climb is over.  Stopping as soon as the
+// SSE is 0 gives the reference's result: no later move can improve on 0.
+ZT_DEV bool climb_next(Climb &s, uint32_t &n0, uint32_t &n1) {
+    for (int guard = 0; guard < 16; guard++) {  // >= 6 of any pass's 12 moves are valid, so this always ends early
+        if (s.best == 0) return false;
+        if (s.mv == 12) {
+            if (!s.improved || s.pass == 7) return false;
+            s.pass++; s.mv = 0; s.improved = 0;
+        }
+        const int mv = s.mv, ep = mv / 6, ch = (mv >> 1) % 3, d = (mv & 1) ? 1 : -1;
+        const int sh = ch == 0 ? 11 : ch == 1 ? 5 : 0, mx = ch == 1 ? 63 : 31;
+        uint32_t c = ep ? s.b1 : s.b0;
+        const int v = (int)((c >> sh) & (uint32_t)mx);
+        s.mv++;
+        if (v + d < 0 || v + d > mx) continue;
+        c = (c & ~((uint32_t)mx << sh)) | ((uint32_t)(v + d) << sh);
+        n0 = ep ? s.b0 : c; n1 = ep ? c : s.b1;
+        return true;
+    }
+    return false;
+}
+// fold a trial (endpoints as bc1_attempt left them, canonical order) into the state
+ZT_DEV void climb_apply(Climb &s, uint32_t n0, uint32_t n1, int ne, uint32_t ni) {
+    if (ne < s.best) { s.best = ne; s.b0 = n0; s.b1 = n1; s.bidx = ni; s.improved = 1; }
+}
+ZT_DEV uint64_t bc1_pack(uint32_t c0, uint32_t c1, uint32_t idx) { return (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)idx << 32; }
+
+// BC4 palette of an endpoint pair (zap_tex.h:172-176): 8 interpolants when a0 > a1, else 6 plus 0 and 255
+ZT_DEV void bc4_palette(int a0, int a1, int pal[8]) {
+    pal[0] = a0; pal[1] = a1;
+    if (a0 > a1) {
 #pragma unroll
-    for (int i = 0; i < 16; i++) { v[i] = chan(px[i], ch); lo = min(lo, v[i]); hi = max(hi, v[i]); }
+        for (int i = 2; i < 8; i++) pal[i] = ((8 - i) * a0 + (i - 1) * a1 + 3) / 7;
+    } else {
+#pragma unroll
+        for (int i = 2; i < 6; i++) pal[i] = ((6 - i) * a0 + (i - 1) * a1 + 2) / 5;
+        pal[6] = 0; pal[7] = 255;
+    }
+}
+
+// BC4 on channel `ch` of the block; returns the 8-byte block (a0, a1, 48 index bits).  The 9 inset pairs
+// (zap_tex.h:198-213) are compared by SSE alone, which does not depend on how ties between palette entries are
+// broken: key_j = v * 2 pal_j - pal_j^2 (one 24-bit multiply-add) and SSE = sum (v^2 - max_j key_j).  Only the
+// chosen pair's indices are then computed, with the reference's lowest-index rule.
+ZT_DEV uint64_t bc4_block(const uint32_t px[16], int ch) {
+    int v[16], lo = 255, hi = 0, svv = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) { v[i] = chan(px[i], ch); lo = min(lo, v[i]); hi = max(hi, v[i]); svv += v[i] * v[i]; }
     int best = 1 << 30, b0 = hi, b1 = lo;
-    uint64_t bi = 0;
-    for (int d0 = 0; d0 <= 2; d0++)
-        for (int d1 = 0; d1 <= 2; d1++) {
+#pragma unroll 1
+    for (int pr = 0; pr < 9; pr++) {
+            const int d0 = pr / 3, d1 = pr % 3;
             int a0 = hi - d0, a1 = lo + d1;
             if (a0 <= a1) { a0 = hi; a1 = lo; }
-            int pal[8];
-            pal[0] = a0; pal[1] = a1;
-            if (a0 > a1) {
-#pragma unroll
-                for (int i = 2; i < 8; i++) pal[i] = ((8 - i) * a0 + (i - 1) * a1 + 3) / 7;
-            } else {
-#pragma unroll
-                for (int i = 2; i < 6; i++) pal[i] = ((6 - i) * a0 + (i - 1) * a1 + 2) / 5;
-                pal[6] = 0; pal[7] = 255;
-            }
-            int e = 0;
-            uint64_t idx = 0;
+            int e;
             if (a0 == a1) {
+                e = 0;
 #pragma unroll
                 for (int i = 0; i < 16; i++) e += (v[i] - a0) * (v[i] - a0);
             } else {
-                int K[8];
+                int pal[8], P2[8], K[8];
+                bc4_palette(a0, a1, pal);
 #pragma unroll
-                for (int j = 0; j < 8; j++) K[j] = 15 - j - 16 * pal[j] * pal[j];
+                for (int j = 0; j < 8; j++) { P2[j] = 2 * pal[j]; K[j] = -pal[j] * pal[j]; }
+                int s = 0;
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
-                    int bk = ((v[i] * pal[0]) << 5) + K[0];
+                    int m = __mul24(v[i], P2[0]) + K[0];
 #pragma unroll
-                    for (int j = 1; j < 8; j++) bk = max(bk, ((v[i] * pal[j]) << 5) + K[j]);
-                    idx |= (uint64_t)(15 - (bk & 15)) << (3 * i);
-                    e += v[i] * v[i] - (bk >> 4);
+                    for (int j = 1; j < 8; j++) m = max(m, __mul24(v[i], P2[j]) + K[j]);
+                    s += m;
                 }
+                e = svv - s;
             }
-            if (e < best) { best = e; b0 = a0; b1 = a1; bi = idx; }
+            if (e < best) { best = e; b0 = a0; b1 = a1; }
+    }
+    uint64_t bi = 0;
+    if (b0 != b1) {
+        int pal[8], P32[8], K[8];
+        bc4_palette(b0, b1, pal);
+#pragma unroll
+        for (int j = 0; j < 8; j++) { P32[j] = 32 * pal[j]; K[j] = 15 - j - 16 * pal[j] * pal[j]; }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            int bk = __mul24(v[i], P32[0]) + K[0];
+#pragma unroll
+            for (int j = 1; j < 8; j++) bk = max(bk, __mul24(v[i], P32[j]) + K[j]);
+            bi |= (uint64_t)(15 - (bk & 15)) << (3 * i);
         }
+    }
     return (uint64_t)b0 | (uint64_t)b1 << 8 | bi << 16;
 }
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, uint32_t total) {
+// BC4 / BC5 blocks and the BC3 alpha half complete here; BC1 blocks and the BC3 colour half stop before the hill
+// climb and leave their state in climb[g] = (c0 | c1 << 16, SSE, indices, job) for zt_bc1_climb; the other
+// blocks leave (0, 0, 0, job).
+extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb) {
     const uint32_t step = gridDim.x * blockDim.x;
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
-        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        const int ji = zt_find_job(jobs, g);
+        const zt_job J = job_at(jobs, ji);
         uint32_t b = g - J.first;
         int bx = (int)(b % (uint32_t)J.bw), by = (int)(b / (uint32_t)J.bw);
         uint32_t px[16];
         gather_block(J.src, J.w, J.h, J.stride, bx, by, px);
-        if (J.fmt == 0) {
+        if (J.fmt == 0 || J.fmt == 1) {
+            if (J.fmt == 1) store8(J.dst + (uint64_t)b * 16, bc4_block(px, 3));
             uint32_t c0, c1, ix;
-            bc1_block(px, c0, c1, ix);
-            store8(J.dst + (uint64_t)b * 8, (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)ix << 32);
-        } else if (J.fmt == 2) {
-            store8(J.dst + (uint64_t)b * 8, bc4_block(px, 0));
-        } else if (J.fmt == 1) {
-            uint64_t a = bc4_block(px, 3);
-            uint32_t c0, c1, ix;
-            bc1_block(px, c0, c1, ix);
-            store16(J.dst + (uint64_t)b * 16, a, (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)ix << 32);
+            const int e = bc1_start(px, c0, c1, ix);
+            climb[g] = make_uint4(c0 | c1 << 16, (uint32_t)e, ix, (uint32_t)ji);
         } else {
-            store16(J.dst + (uint64_t)b * 16, bc4_block(px, 0), bc4_block(px, 1));
+            climb[g] = make_uint4(0, 0, 0, (uint32_t)ji);  // every entry names its job: zt_bc1_climb reads it first
+            if (J.fmt == 2) store8(J.dst + (uint64_t)b * 8, bc4_block(px, 0));
+            else store16(J.dst + (uint64_t)b * 16, bc4_block(px, 0), bc4_block(px, 1));
+        }
+    }
+}
+
+// The BC1 hill climb with lane refill.  Each wave owns ranges of ZT_CLIMB_RANGE blocks (wave-strided); a lane that
+// has no block (or has finished one) takes the next block of the range, numbered by a ballot prefix count, so
+// every step runs one trial on (nearly) every lane.  Non-BC1 blocks are skipped at refill.
+// Refills wait until at least refill_min lanes are idle (or none is busy): a refill costs every lane of the wave
+// a job lookup and a 16-pixel gather, so doing it for one lane per step would cost more than the idle lanes.
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint64_t r0 = (uint64_t)wave * ZT_CLIMB_RANGE; r0 < total; r0 += (uint64_t)nwaves * ZT_CLIMB_RANGE) {
+        const uint32_t end = (uint32_t)min<uint64_t>(total, r0 + ZT_CLIMB_RANGE);
+        uint32_t next = (uint32_t)r0;  // same on every lane
+        bool active = false;
+        uint32_t px[16];  // alpha cleared (bc1_prepare)
+        int spp = 0;
+        Climb s = { 0, 0, 0, 0, 0, 0, 0 };
+        uint8_t *dst = nullptr;
+        for (;;) {
+            const uint64_t need = __ballot(!active);
+            const int idle = __popcll(need);
+            const bool refill = idle >= refill_min || idle == 64;
+            const uint32_t mine = next + (uint32_t)__popcll(need & below);
+            if (refill) next += (uint32_t)idle;
+            if (refill && !active && mine < end) {
+                const uint4 st = climb[mine];
+                const zt_job J = job_at(jobs, (int)st.w);
+                if (J.fmt == 0 || J.fmt == 1) {
+                    const uint32_t b = mine - J.first;
+                    dst = J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8);
+                    s.b0 = st.x & 0xFFFFu; s.b1 = st.x >> 16; s.best = (int)st.y; s.bidx = st.z; s.pass = 0; s.mv = 0; s.improved = 0;
+                    gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+                    spp = bc1_prepare(px, px);
+                    active = true;
+                }
+            }
+            if (__ballot(active) == 0 && next >= end) break;
+            uint32_t n0 = 0, n1 = 0;
+            const bool has = active && climb_next(s, n0, n1);
+            if (active && !has) { store8(dst, bc1_pack(s.b0, s.b1, s.bidx)); active = false; }
+            if (has) {
+                // the full trial: an error-only trial plus indices for improvements would run both on nearly
+                // every step, since some lane of the wave improves on most steps
+                uint32_t ni = 0;
+                const int ne = bc1_attempt(px, spp, n0, n1, ni);
+                climb_apply(s, n0, n1, ne, ni);
+            }
         }
     }
 }

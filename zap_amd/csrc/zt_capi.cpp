@@ -20,13 +20,14 @@ namespace {
 thread_local char g_err[512];
 std::mutex g_mu;     // serialises host entry points (shared staging buffers)
 int g_m1k = 4;
+int g_climb_refill = 16;  // zt_bc1_climb refills once this many lanes of a wave are idle
 
 // launch shapes of the grid-stride kernels: workgroup size (64, 128 or 256) and the cap on workgroups;
 // zaptex_set_launch changes them (the autotuner's results), a batch captures those in force when it is built
 struct LaunchCfg { const char *name; int wg, max_groups; };
 LaunchCfg g_cfg[] = { { "zt_bc14_encode", 256, 8192 }, { "zt_bc7_stage6", 256, 8192 }, { "zt_bc7_stage5", 256, 2048 },
-                      { "zt_bc7_s1", 256, 1024 }, { "zt_mip_batch", 256, 8192 } };
-enum { CFG_BC14, CFG_S6, CFG_S5, CFG_S1, CFG_MIPB, CFG_N };
+                      { "zt_bc7_s1", 256, 1024 }, { "zt_mip_batch", 256, 8192 }, { "zt_bc1_climb", 256, 8192 } };
+enum { CFG_BC14, CFG_S6, CFG_S5, CFG_S1, CFG_MIPB, CFG_CLIMB, CFG_N };
 // workgroups for `items` work items of `per_group` each under config c (at least 1)
 int groups_for(int c, uint64_t items, uint64_t per_group) {
     uint64_t g = (items + per_group - 1) / per_group;
@@ -160,6 +161,9 @@ int check_image(const zaptex_image &im) {
 
 // workspace layout over the BC7 blocks: [counters 256 B][meta5 16][px5 64][meta1 16][cand1 8][sub_err 64][sub_state 256] B/blk
 size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? 256 + bc7_blocks * 424 : 0; }
+// an encode workspace: the BC7 lists, then 16 B of BC1 hill-climb state per BC1/BC3/BC4/BC5 block
+size_t ws_climb_offset(uint64_t bc7_blocks) { return (ws_bytes_for(bc7_blocks) + 255) / 256 * 256; }
+size_t ws_total(uint64_t bc7_blocks, uint64_t bc14_blocks) { return ws_climb_offset(bc7_blocks) + bc14_blocks * 16; }
 
 zt_bc7_lists lists_in(void *ws, uint64_t n) {
     uint8_t *p = (uint8_t *)ws;
@@ -179,10 +183,15 @@ int grid_for(uint64_t items, int per) {
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 8192));
 }
 
-// BC1/BC3/BC4/BC5 over one job table
-int enqueue_bc14(const zt_jobs &J, uint32_t tot, hipStream_t s) {
-    const int wg = g_cfg[CFG_BC14].wg;
-    return launch("zt_bc14_encode", s, [&] { hipLaunchKernelGGL(zt_bc14_encode, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot); });
+// BC1/BC3/BC4/BC5 over one job table: everything but the BC1 hill climb, then the hill climb (climb: 16 B/block)
+int enqueue_bc14(const zt_jobs &J, uint32_t tot, uint4 *climb, hipStream_t s) {
+    const int wg = g_cfg[CFG_BC14].wg, wc = g_cfg[CFG_CLIMB].wg;
+    if (int rc = launch("zt_bc14_encode", s, [&] {
+            hipLaunchKernelGGL(zt_bc14_encode, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, climb); }))
+        return rc;
+    const uint64_t waves = (tot + ZT_CLIMB_RANGE - 1) / ZT_CLIMB_RANGE;
+    return launch("zt_bc1_climb", s, [&] {
+        hipLaunchKernelGGL(zt_bc1_climb, dim3(groups_for(CFG_CLIMB, waves, wc / 64)), dim3(wc), 0, s, J, tot, (const uint4 *)climb, g_climb_refill); });
 }
 
 // the reference BC7 search (mode 6, then 5, then 1) over one job table; ws >= ws_bytes_for(tot)
@@ -232,8 +241,10 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
         j.first = *t; j.w = im.w; j.h = im.h; j.bw = (im.w + 3) / 4; j.fmt = im.fmt; j.pad = 0;
         *t += nblocks(im);
     }
+    if (ws_bytes < ws_total(tot[1], tot[0]) || (!ws && ws_total(tot[1], tot[0])))
+        return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, ws_total(tot[1], tot[0]));
     int rc;
-    if (tot[0] && (rc = enqueue_bc14(fam[0], tot[0], s))) return rc;
+    if (tot[0] && (rc = enqueue_bc14(fam[0], tot[0], (uint4 *)((uint8_t *)ws + ws_climb_offset(tot[1])), s))) return rc;
     if (tot_b6 && (rc = launch("zt_bc6h_encode", s, [&] {
             hipLaunchKernelGGL(zt_bc6h_encode, dim3(grid_for(tot_b6, 256)), dim3(256), 0, s, b6, tot_b6); })))
         return rc;
@@ -299,10 +310,10 @@ int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, i
     size_t row = (size_t)w * px_bytes, inb = row * (size_t)h;
     zaptex_image im{ nullptr, nullptr, row, w, h, fmt };
     size_t ob = image_out_bytes(im);
-    uint64_t n7 = fmt == ZAPTEX_BC7 ? nblocks(im) : 0;
+    uint64_t n7 = fmt == ZAPTEX_BC7 ? nblocks(im) : 0, n14 = fmt <= ZAPTEX_BC5 ? nblocks(im) : 0;
     if (int rc = ensure_buf(st->in, st->in_cap, inb)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, ob)) return rc;
-    if (n7) if (int rc = ensure_buf(st->ws, st->ws_cap, ws_bytes_for(n7))) return rc;
+    if (n7 || n14) if (int rc = ensure_buf(st->ws, st->ws_cap, ws_total(n7, n14))) return rc;
     im.src = st->in; im.dst = st->out;
     HIP_TRY(hipMemcpy2DAsync(st->in, row, src, stride, row, (size_t)h, hipMemcpyHostToDevice, st->stream));
     if (int rc = enqueue(&im, 1, st->ws, st->ws_cap, st->stream)) return rc;
@@ -443,12 +454,16 @@ int zaptex_bc_split(const void *bc, size_t n, int fmt, void *out) { return shuff
 int zaptex_bc_merge(const void *planes, size_t n, int fmt, void *out) { return shuffle(planes, n, fmt, out, 0); }
 
 size_t zaptex_workspace_size(const zaptex_image *imgs, int n) {
-    uint64_t n7 = 0;
-    for (int i = 0; i < n; i++) if (imgs[i].fmt == ZAPTEX_BC7) n7 += nblocks(imgs[i]);
-    return ws_bytes_for(n7);
+    uint64_t n7 = 0, n14 = 0;
+    for (int i = 0; i < n; i++) {
+        if (imgs[i].fmt == ZAPTEX_BC7) n7 += nblocks(imgs[i]);
+        else if (imgs[i].fmt <= ZAPTEX_BC5) n14 += nblocks(imgs[i]);
+    }
+    return ws_total(n7, n14);
 }
 
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream) {
+    if (((uintptr_t)workspace) % 256) return fail(ZAPTEX_E_ARG, "workspace must be 256-byte aligned");
     if (!imgs || n < 1 || n > ZT_INLINE_JOBS) return fail(ZAPTEX_E_ARG, "image count %d outside 1..%d", n, ZT_INLINE_JOBS);
     uint64_t total = 0;
     for (int i = 0; i < n; i++) {
@@ -950,6 +965,7 @@ struct zaptex_batch {
     uint32_t tot14 = 0, tot7 = 0;
     void *bc7ws = nullptr;
     size_t bc7ws_bytes = 0;
+    uint4 *climb = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -971,7 +987,7 @@ size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels) {
 size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
     if (!t || n < 1) return 0;
     BatchSizes z = batch_sizes(t, n);
-    return 256 + z.mipjobs + z.jobs14 + z.jobs7 + align_up(ws_bytes_for(z.blocks7), 256);
+    return 256 + z.mipjobs + z.jobs14 + z.jobs7 + ws_total(z.blocks7, z.blocks14);
 }
 
 int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out) {
@@ -995,6 +1011,7 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     zt_job *d7 = (zt_job *)p; p += z.jobs7;
     b->bc7ws = p;
     b->bc7ws_bytes = ws_bytes_for(z.blocks7);
+    b->climb = (uint4 *)(p + ws_climb_offset(z.blocks7));
     // host tables: mip jobs grouped by destination level, encode jobs per family in texture/level order
     int maxl = 0;
     for (int i = 0; i < n; i++) maxl = std::max(maxl, t[i].levels);
@@ -1056,7 +1073,7 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
                 return rc;
             lvjobs += nj;
         }
-        if (b->tot14 && (rc = enqueue_bc14(b->j14, b->tot14, cs))) return rc;
+        if (b->tot14 && (rc = enqueue_bc14(b->j14, b->tot14, b->climb, cs))) return rc;
         if (b->tot7 && (rc = enqueue_bc7(b->j7, b->tot7, b->bc7ws, b->bc7ws_bytes, cs))) return rc;
         return ZAPTEX_OK;
     };
@@ -1117,9 +1134,16 @@ int zaptex_set_option(int key, int value) {
         g_m1k = value;
         return ZAPTEX_OK;
     }
+    if (key == ZAPTEX_OPT_CLIMB_REFILL) {
+        if (value < 1 || value > 64) return fail(ZAPTEX_E_ARG, "refill threshold must be 1..64");
+        g_climb_refill = value;
+        return ZAPTEX_OK;
+    }
     return fail(ZAPTEX_E_ARG, "unknown option %d", key);
 }
-int zaptex_get_option(int key) { return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : -1; }
+int zaptex_get_option(int key) {
+    return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : key == ZAPTEX_OPT_CLIMB_REFILL ? g_climb_refill : -1;
+}
 
 int zaptex_profile_collect(void) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
