@@ -79,7 +79,7 @@ template <int K> ZT_DEV int fit_weights(const uint32_t px[16], const int *uv, ui
 #pragma unroll
             for (int c = 0; c < 4; c++) v |= (uint32_t)lerp(e0[c], e1[c], wt) << (8 * c);
             pal[k][j] = v & CM;
-            Kk[k][j] = IM - j - (1 << IB) * (int)dot4(pal[k][j], pal[k][j]);
+            Kk[k][j] = no_fold(IM - j - (1 << IB) * (int)dot4(pal[k][j], pal[k][j]));
         }
     }
     int err = 0;

@@ -42,7 +42,8 @@ ZT_DEV int bc1_trial(const uint32_t pm[16], int spp, uint32_t c0, uint32_t c1, u
         p2 |= (uint32_t)((2 * x + y + 1) / 3) << (8 * k);
         p3 |= (uint32_t)((x + 2 * y + 1) / 3) << (8 * k);
     }
-    const int K0 = 3 - 4 * (int)dot4(a, a), K1 = 2 - 4 * (int)dot4(b, b), K2 = 1 - 4 * (int)dot4(p2, p2), K3 = -4 * (int)dot4(p3, p3);
+    const int K0 = no_fold(3 - 4 * (int)dot4(a, a)), K1 = no_fold(2 - 4 * (int)dot4(b, b)), K2 = no_fold(1 - 4 * (int)dot4(p2, p2)),
+              K3 = no_fold(-4 * (int)dot4(p3, p3));
     int sum = 0;
     uint32_t inv = 0;
 #pragma unroll

@@ -34,16 +34,18 @@ template <int NC, int C0> ZT_DEV uint32_t sub_px(uint32_t p) {
 // ---- p-bit quantiser for one channel (zap_tex.h:426-438); N = bits including the p-bit
 template <int N> ZT_DEV int quant_pbit1(float v, int p, int &q) {
     const int mx = (1 << (N - 1)) - 1;
-    int g = (int)((v / 255.0f * (float)((1 << N) - 1) - (float)p) / 2.0f + 0.5f), bq = 0, be = 1 << 30;
+    const int g = (int)((v / 255.0f * (float)((1 << N) - 1) - (float)p) / 2.0f + 0.5f);
+    // the three candidates g-1, g, g+1 in order; strict < keeps the first of equal errors, i.e. the smallest of
+    // the packed keys (err << 2 | candidate)
+    uint32_t best = 0xFFFFFFFFu;
 #pragma unroll
-    for (int dt = -1; dt <= 1; dt++) {
-        int t = g + dt, tq = t < 0 ? 0 : t > mx ? mx : t, x = expand_bits(tq << 1 | p, N);
-        float d = (float)x - v;
-        int e = (int)(d * d);
-        if (e < be) { be = e; bq = tq; }
+    for (int dt = 0; dt < 3; dt++) {
+        const int tq = clampi(g + dt - 1, 0, mx), x = expand_bits(tq << 1 | p, N);
+        const float d = (float)x - v;
+        best = min(best, (uint32_t)(int)(d * d) << 2 | (uint32_t)dt);
     }
-    q = bq;
-    return be;
+    q = clampi(g + (int)(best & 3u) - 1, 0, mx);
+    return (int)(best >> 2);
 }
 
 // ---- argmin over a palette of NP packed entries: returns SSE, writes the index (ties -> lowest index).
@@ -57,7 +59,7 @@ template <int NP> ZT_DEV int best_entry(uint32_t p, const uint32_t pal[NP], cons
 }
 template <int NP> ZT_DEV void palette_keys(const uint32_t pal[NP], int K[NP]) {
 #pragma unroll
-    for (int j = 0; j < NP; j++) K[j] = 15 - j - 16 * (int)dot4(pal[j], pal[j]);
+    for (int j = 0; j < NP; j++) K[j] = no_fold(15 - j - 16 * (int)dot4(pal[j], pal[j]));
 }
 
 // ---- one subset: quantised endpoints + indices (zap_tex.h:466-486).  idx: 4 bits per pixel at 4*i.
@@ -95,7 +97,7 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], c
         const int w = wgt<IB>(j);
         uint32_t v = 0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
+        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
         pal[j] = v;
     }
     palette_keys<NP>(pal, K);
@@ -198,7 +200,7 @@ template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], const int e0
         const int w = wgt<2>(j);
         uint32_t v = 0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
+        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
         pal[j] = v;
     }
     palette_keys<4>(pal, K);

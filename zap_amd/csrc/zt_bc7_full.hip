@@ -33,7 +33,7 @@ ZT_DEV int fit_subset_plain(const uint32_t px[16], uint32_t mask, const float lo
         const int w = wgt<IB>(j);
         uint32_t v = 0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
+        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
         pal[j] = v;
     }
     palette_keys<NP>(pal, K);
@@ -186,7 +186,7 @@ template <int NC, int C0, int IB> ZT_DEV int fit_nbit(const uint32_t r[16], cons
         const int w = wgt<IB>(j);
         uint32_t v = 0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)(((64 - w) * e0[c] + w * e1[c] + 32) >> 6) << (8 * c);
+        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
         pal[j] = v;
     }
     palette_keys<NP>(pal, K);

@@ -936,11 +936,96 @@ int check_texture(const zaptex_texture &t) {
     if (((uintptr_t)t.out | (uintptr_t)t.mips) % 256) return fail(ZAPTEX_E_ARG, "mips and out arenas must be 256-byte aligned");
     return ZAPTEX_OK;
 }
-struct BatchSizes { size_t mipjobs, jobs14, jobs7; uint64_t blocks14, blocks7; int nmip, n14, n7; };
-BatchSizes batch_sizes(const zaptex_texture *t, int n) {
+// The mip chains of a batch as launches: each texture's chain is cut into steps exactly as a single chain is
+// (a fused tile pass of up to 7 levels while the source exceeds 128 on a side, single levels while a side is 1
+// and the other exceeds 128, then one workgroup for a source of at most 128 x 128); step s of every texture goes
+// into the launches of round s (one per step kind), so each texture's steps run in order.
+struct MipLaunch { int kind; size_t first; int count; uint64_t px; int srgb; };  // kind 0 tiles, 1 single level, 2 tail
+struct MipPlan {
+    std::vector<zt_mipstep> steps;   // tile and tail launches' steps, launch by launch
+    std::vector<uint2> items;        // tile items (step within its launch, tile)
+    std::vector<zt_mipjob> jobs;     // single-level jobs, launch by launch
+    std::vector<MipLaunch> launches; // first = offset into items (kind 0), jobs (kind 1) or steps (kind 2)
+    std::vector<size_t> tile_steps;  // kind-0 launches: offset of their steps
+};
+MipPlan plan_mips(const zaptex_texture *t, int n) {
+    struct TexStep { int kind, l, k; };
+    std::vector<std::vector<TexStep>> ts(n);
+    std::vector<std::vector<int>> LW(n), LH(n);
+    std::vector<std::vector<size_t>> MO(n);
+    size_t rounds = 0;
+    for (int i = 0; i < n; i++) {
+        const int L = t[i].levels;
+        int cw = t[i].w, ch = t[i].h;
+        for (int l = 0; l < L; l++) { LW[i].push_back(cw); LH[i].push_back(ch); cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1; }
+        batch_layout(t[i].w, t[i].h, t[i].fmt, L, &MO[i], nullptr, nullptr, nullptr);
+        for (int l = 0; l < L - 1;) {
+            const int w = LW[i][l], h = LH[i][l];
+            TexStep st;
+            if ((w > 128 || h > 128) && w >= 2 && h >= 2) {
+                int k = 0;
+                while (k < 7 && l + k < L - 1 && LW[i][l + k] >= 2 && LH[i][l + k] >= 2) k++;
+                st = { 0, l, k };
+            } else if (w <= 128 && h <= 128) {
+                st = { 2, l, L - 1 - l };
+            } else {
+                st = { 1, l, 1 };
+            }
+            ts[i].push_back(st);
+            l += st.k;
+        }
+        rounds = std::max(rounds, ts[i].size());
+    }
+    auto level_ptr = [&](int i, int l) -> uint8_t * {
+        return l == 0 ? (uint8_t *)t[i].base : (uint8_t *)t[i].mips + MO[i][l - 1];
+    };
+    auto make_step = [&](int i, const TexStep &st) {
+        zt_mipstep m;
+        memset(&m, 0, sizeof m);
+        m.src = level_ptr(i, st.l);
+        m.stride = st.l == 0 ? t[i].stride : (uint64_t)LW[i][st.l] * 4;
+        m.w = LW[i][st.l]; m.h = LH[i][st.l]; m.srgb = t[i].srgb ? 1 : 0; m.nlev = st.k;
+        for (int j = 0; j < st.k; j++) { m.L.p[j] = level_ptr(i, st.l + 1 + j); m.L.w[j] = LW[i][st.l + 1 + j]; m.L.h[j] = LH[i][st.l + 1 + j]; }
+        return m;
+    };
+    MipPlan P;
+    for (size_t r = 0; r < rounds; r++)
+        for (int kind = 0; kind < 3; kind++) {
+            MipLaunch ml{ kind, 0, 0, 0, 0 };
+            ml.first = kind == 0 ? P.items.size() : kind == 1 ? P.jobs.size() : P.steps.size();
+            const size_t step0 = P.steps.size();
+            for (int i = 0; i < n; i++) {
+                if (ts[i].size() <= r || ts[i][r].kind != kind) continue;
+                const TexStep &st = ts[i][r];
+                if (kind == 1) {
+                    zt_mipjob j{};
+                    j.src = level_ptr(i, st.l); j.dst = level_ptr(i, st.l + 1);
+                    j.stride = st.l == 0 ? t[i].stride : (uint64_t)LW[i][st.l] * 4;
+                    j.first = ml.px; j.w = LW[i][st.l]; j.h = LH[i][st.l]; j.srgb = t[i].srgb ? 1 : 0;
+                    ml.px += (uint64_t)LW[i][st.l + 1] * LH[i][st.l + 1];
+                    ml.srgb |= j.srgb;
+                    P.jobs.push_back(j);
+                } else {
+                    const uint32_t si = (uint32_t)(P.steps.size() - step0);
+                    P.steps.push_back(make_step(i, st));
+                    if (kind == 0) {
+                        const uint32_t tiles = (uint32_t)(((LW[i][st.l + 1] + 63) / 64) * ((LH[i][st.l + 1] + 63) / 64));
+                        for (uint32_t k = 0; k < tiles; k++) P.items.push_back(make_uint2(si, k));
+                    }
+                }
+                ml.count++;
+            }
+            if (!ml.count) continue;
+            if (kind == 0) { P.tile_steps.push_back(step0); ml.count = (int)(P.items.size() - ml.first); }
+            P.launches.push_back(ml);
+        }
+    return P;
+}
+
+struct BatchSizes { size_t steps, items, mipjobs, jobs14, jobs7; uint64_t blocks14, blocks7; int n14, n7; };
+BatchSizes batch_sizes(const zaptex_texture *t, int n, const MipPlan &P) {
     BatchSizes z{};
     for (int i = 0; i < n; i++) {
-        z.nmip += t[i].levels - 1;
         int cw = t[i].w, ch = t[i].h;
         for (int l = 0; l < t[i].levels; l++) {
             const uint64_t nb = (uint64_t)((cw + 3) / 4) * ((ch + 3) / 4);
@@ -948,7 +1033,9 @@ BatchSizes batch_sizes(const zaptex_texture *t, int n) {
             cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
         }
     }
-    z.mipjobs = align_up((size_t)z.nmip * sizeof(zt_mipjob), 256);
+    z.steps = align_up(P.steps.size() * sizeof(zt_mipstep), 256);
+    z.items = align_up(P.items.size() * sizeof(uint2), 256);
+    z.mipjobs = align_up(P.jobs.size() * sizeof(zt_mipjob), 256);
     z.jobs14 = align_up((size_t)z.n14 * sizeof(zt_job), 256);
     z.jobs7 = align_up((size_t)z.n7 * sizeof(zt_job), 256);
     return z;
@@ -957,9 +1044,10 @@ BatchSizes batch_sizes(const zaptex_texture *t, int n) {
 
 struct zaptex_batch {
     int device = 0;
-    std::vector<int> level_jobs;          // mip jobs per destination level 1.. (device tables back to back)
-    std::vector<uint64_t> level_px;       // destination pixels per level
-    std::vector<int> level_srgb;          // any sRGB job in the level
+    std::vector<MipLaunch> mip_launches;
+    std::vector<size_t> tile_steps;
+    zt_mipstep *steps = nullptr;
+    uint2 *items = nullptr;
     zt_mipjob *mipjobs = nullptr;
     zt_jobs j14{}, j7{};
     uint32_t tot14 = 0, tot7 = 0;
@@ -986,8 +1074,11 @@ size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels) {
 }
 size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
     if (!t || n < 1) return 0;
-    BatchSizes z = batch_sizes(t, n);
-    return 256 + z.mipjobs + z.jobs14 + z.jobs7 + ws_total(z.blocks7, z.blocks14);
+    for (int i = 0; i < n; i++)
+        if (t[i].w < 1 || t[i].h < 1 || t[i].levels < 1 || t[i].levels > max_levels(t[i].w, t[i].h)) return 0;
+    MipPlan P = plan_mips(t, n);
+    BatchSizes z = batch_sizes(t, n, P);
+    return 256 + z.steps + z.items + z.mipjobs + z.jobs14 + z.jobs7 + ws_total(z.blocks7, z.blocks14);
 }
 
 int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out) {
@@ -995,7 +1086,8 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     *out = nullptr;
     for (int i = 0; i < n; i++)
         if (int rc = check_texture(t[i])) return rc;
-    BatchSizes z = batch_sizes(t, n);
+    MipPlan P = plan_mips(t, n);
+    BatchSizes z = batch_sizes(t, n, P);
     if (z.blocks7 >= (1ull << 31) || z.blocks14 >= (1ull << 31)) return fail(ZAPTEX_E_ARG, "too many blocks in one batch");
     if (!ws || ws_bytes < zaptex_batch_workspace_size(t, n)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, zaptex_batch_workspace_size(t, n));
     DeviceState *st;
@@ -1006,16 +1098,17 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     zaptex_batch *b = new zaptex_batch();
     HIP_TRY(hipGetDevice(&b->device));
     uint8_t *p = (uint8_t *)align_up((uintptr_t)ws, 256);
+    b->steps = (zt_mipstep *)p; p += z.steps;
+    b->items = (uint2 *)p; p += z.items;
     b->mipjobs = (zt_mipjob *)p; p += z.mipjobs;
     zt_job *d14 = (zt_job *)p; p += z.jobs14;
     zt_job *d7 = (zt_job *)p; p += z.jobs7;
     b->bc7ws = p;
     b->bc7ws_bytes = ws_bytes_for(z.blocks7);
     b->climb = (uint4 *)(p + ws_climb_offset(z.blocks7));
-    // host tables: mip jobs grouped by destination level, encode jobs per family in texture/level order
-    int maxl = 0;
-    for (int i = 0; i < n; i++) maxl = std::max(maxl, t[i].levels);
-    std::vector<std::vector<zt_mipjob>> mj(std::max(0, maxl - 1));
+    b->mip_launches = P.launches;
+    b->tile_steps = P.tile_steps;
+    // encode jobs per family in texture / level order (level l >= 1 read from the mip arena)
     std::vector<zt_job> h14, h7;
     for (int i = 0; i < n; i++) {
         std::vector<size_t> mo, oo;
@@ -1030,31 +1123,16 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
             if (t[i].fmt == ZAPTEX_BC7) { j.first = b->tot7; b->tot7 += nb; h7.push_back(j); }
             else { j.first = b->tot14; b->tot14 += nb; h14.push_back(j); }
             if (l + 1 < t[i].levels) {
-                zt_mipjob m{};
-                m.src = src; m.dst = (uint8_t *)t[i].mips + mo[l]; m.stride = stride; m.w = cw; m.h = ch; m.srgb = t[i].srgb ? 1 : 0;
-                mj[l].push_back(m);
-                src = m.dst;
+                src = (const uint8_t *)t[i].mips + mo[l];
                 cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
                 stride = (uint64_t)cw * 4;
             }
         }
     }
-    std::vector<zt_mipjob> flat;
-    for (auto &lv : mj) {
-        uint64_t px = 0;
-        int srgb = 0;
-        for (auto &m : lv) {
-            m.first = px;
-            px += (uint64_t)(m.w > 1 ? m.w / 2 : 1) * (uint64_t)(m.h > 1 ? m.h / 2 : 1);
-            srgb |= m.srgb;
-        }
-        b->level_jobs.push_back((int)lv.size());
-        b->level_px.push_back(px);
-        b->level_srgb.push_back(srgb);
-        flat.insert(flat.end(), lv.begin(), lv.end());
-    }
     hipStream_t s = st->stream;
-    if (!flat.empty()) HIP_TRY(hipMemcpyAsync(b->mipjobs, flat.data(), flat.size() * sizeof(zt_mipjob), hipMemcpyHostToDevice, s));
+    if (!P.steps.empty()) HIP_TRY(hipMemcpyAsync(b->steps, P.steps.data(), P.steps.size() * sizeof(zt_mipstep), hipMemcpyHostToDevice, s));
+    if (!P.items.empty()) HIP_TRY(hipMemcpyAsync(b->items, P.items.data(), P.items.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
+    if (!P.jobs.empty()) HIP_TRY(hipMemcpyAsync(b->mipjobs, P.jobs.data(), P.jobs.size() * sizeof(zt_mipjob), hipMemcpyHostToDevice, s));
     if (!h14.empty()) HIP_TRY(hipMemcpyAsync(d14, h14.data(), h14.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
     if (!h7.empty()) HIP_TRY(hipMemcpyAsync(d7, h7.data(), h7.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1063,15 +1141,24 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     // capture: mip levels in order, then the encoders (BC1-5, then BC7), as one graph
     auto body = [&](hipStream_t cs) -> int {
         int rc;
-        const zt_mipjob *lvjobs = b->mipjobs;
-        for (size_t l = 0; l < b->level_jobs.size(); l++) {
-            const int nj = b->level_jobs[l];
-            const uint64_t px = b->level_px[l];
-            const int srgb = b->level_srgb[l], wg = g_cfg[CFG_MIPB].wg;
-            if (nj && (rc = launch("zt_mip_batch", cs, [&] {
-                           hipLaunchKernelGGL(zt_mip_batch, dim3(groups_for(CFG_MIPB, px, wg)), dim3(wg), 0, cs, lvjobs, nj, px, srgb); })))
-                return rc;
-            lvjobs += nj;
+        size_t tl = 0;  // kind-0 launches seen
+        for (const MipLaunch &ml : b->mip_launches) {
+            if (ml.kind == 0) {
+                const zt_mipstep *steps = b->steps + b->tile_steps[tl++];
+                const uint2 *items = b->items + ml.first;
+                if ((rc = launch("zt_mip_tiles_batch", cs, [&] { hipLaunchKernelGGL(zt_mip_tiles_batch, dim3(ml.count), dim3(256), 0, cs, steps, items); })))
+                    return rc;
+            } else if (ml.kind == 1) {
+                const zt_mipjob *jobs = b->mipjobs + ml.first;
+                const int wg = g_cfg[CFG_MIPB].wg;
+                if ((rc = launch("zt_mip_batch", cs, [&] {
+                         hipLaunchKernelGGL(zt_mip_batch, dim3(groups_for(CFG_MIPB, ml.px, wg)), dim3(wg), 0, cs, jobs, ml.count, ml.px, ml.srgb); })))
+                    return rc;
+            } else {
+                const zt_mipstep *steps = b->steps + ml.first;
+                if ((rc = launch("zt_mip_tail_batch", cs, [&] { hipLaunchKernelGGL(zt_mip_tail_batch, dim3(ml.count), dim3(1024), 0, cs, steps); })))
+                    return rc;
+            }
         }
         if (b->tot14 && (rc = enqueue_bc14(b->j14, b->tot14, b->climb, cs))) return rc;
         if (b->tot7 && (rc = enqueue_bc7(b->j7, b->tot7, b->bc7ws, b->bc7ws_bytes, cs))) return rc;

@@ -54,6 +54,18 @@ ZT_DEV int chan(uint32_t p, int c) { return (int)((p >> (8 * c)) & 255u); }
 ZT_DEV float chanf(uint32_t p, int c) { return (float)chan(p, c); }
 ZT_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 ZT_DEV uint32_t dot4(uint32_t a, uint32_t b) { return __builtin_amdgcn_udot4(a, b, 0u, false); }
+// v as an opaque register value: the optimiser cannot fold it into surrounding arithmetic.  Palette keys are
+// (dot4 << s) + K; left visible, K's constant part is re-associated into a separate OR and SUB, three
+// instructions per key instead of one v_lshl_add_u32.
+ZT_DEV int no_fold(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+#else
+    return v;  // the host pass (and CPU builds of these sources) only need the value
+#endif
+}
 
 // 16 pixels of block (bx, by), edge-clamped like zap_tex.h:1482-1488, packed RGBA8 (R in the low byte)
 ZT_DEV void gather_block(const uint8_t *src, int w, int h, uint64_t stride, int bx, int by, uint32_t px[16]) {

@@ -44,6 +44,13 @@ struct zt_mip_levels {
     uint8_t *p[16];
     int w[16], h[16];
 };
+// one texture's fused mip step in a batched launch: nlev levels from source level (src, w x h, stride bytes/row)
+struct zt_mipstep {
+    const uint8_t *src;
+    uint64_t stride;
+    int32_t w, h, srgb, nlev;
+    zt_mip_levels L;
+};
 
 extern "C" {
 __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
@@ -103,6 +110,8 @@ __global__ void zt_mip_level(const uint8_t *src, int w, int h, uint64_t stride, 
 __global__ void zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
 __global__ void zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
 __global__ void zt_mip_batch(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb);
+__global__ void zt_mip_tiles_batch(const zt_mipstep *steps, const uint2 *items);
+__global__ void zt_mip_tail_batch(const zt_mipstep *steps);
 __global__ void zt_mip_level_f(const float *src, int w, int h, uint64_t stride, float *dst, uint64_t dst_stride);
 }
 

@@ -97,16 +97,16 @@ ZT_DEV uint32_t box4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int srgb, c
 }
 }  // namespace
 
-// grid: tiles of 64 x 64 level-1 pixels; block 256.  lv[i] / lw[i], lh[i]: level i+1 buffers (packed rows)
-extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
-                                                                 zt_mip_levels L) {
+// One 64 x 64 tile (tx, ty) of level 1 and the matching tiles of levels 2..nlev.  L.p[i] / L.w[i], L.h[i]:
+// level i+1 buffers (packed rows).  Called by every thread of a workgroup (it synchronises).
+namespace {
+ZT_DEV void tiles_body(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, const zt_mip_levels &L, int tx, int ty) {
     __shared__ float lin[256], thr[256];
     __shared__ uint32_t t0[64 * 64], t1[32 * 32];
     const int tid = threadIdx.x, nt = blockDim.x;
     if (srgb)
         for (int i = tid; i < 256; i += nt) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
     __syncthreads();
-    const int tx = blockIdx.x, ty = blockIdx.y;
     // level 1: 64 x 64 pixels of the tile, 16 per thread, 2 x 8-byte loads per pixel pair row
     {
         const int w1 = L.w[0], h1 = L.h[0];
@@ -145,10 +145,17 @@ extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles(const uint8_t *sr
         __syncthreads();
     }
 }
+}  // namespace
+
+// grid: tiles of 64 x 64 level-1 pixels; block 256
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
+                                                                 zt_mip_levels L) {
+    tiles_body(src, w, h, stride, srgb, nlev, L, blockIdx.x, blockIdx.y);
+}
 
 // one workgroup: levels from a source of at most 128 x 128, with the reference's clamping when a side is 1
-extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
-                                                                 zt_mip_levels L) {
+namespace {
+ZT_DEV void tail_body(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, const zt_mip_levels &L) {
     __shared__ float lin[256], thr[256];
     __shared__ uint32_t a[64 * 64], b[32 * 64];
     const int tid = threadIdx.x;
@@ -178,6 +185,26 @@ extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *sr
         __syncthreads();
         cw = nw; ch = nh;
     }
+}
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
+                                                                 zt_mip_levels L) {
+    tail_body(src, w, h, stride, srgb, nlev, L);
+}
+
+// ---------------------------------------------------------------- batched fused steps
+// Many textures' fused steps in one launch: zt_mip_tiles_batch runs one tile item per workgroup (items name a
+// step and a tile), zt_mip_tail_batch one step per workgroup.
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles_batch(const zt_mipstep *steps, const uint2 *items) {
+    const uint2 it = items[blockIdx.x];  // (step, tile index in row-major tile order of the step's level 1)
+    const zt_mipstep &S = steps[it.x];
+    const int tiles_x = (S.L.w[0] + 63) / 64;
+    tiles_body(S.src, S.w, S.h, S.stride, S.srgb, S.nlev, S.L, (int)(it.y % (uint32_t)tiles_x), (int)(it.y / (uint32_t)tiles_x));
+}
+extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail_batch(const zt_mipstep *steps) {
+    const zt_mipstep &S = steps[blockIdx.x];
+    tail_body(S.src, S.w, S.h, S.stride, S.srgb, S.nlev, S.L);
 }
 
 // ---------------------------------------------------------------- batched level
