@@ -40,8 +40,8 @@ MIP_CHAIN_BYTES = 89_478_484                 # read the 4096^2 base once + write
 # zt_bc7_stage6 on this workload, rocprofv3 --pmc per dispatch (profiles/zt_bc7_stage6_c2_chain.md):
 # FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the counter reads for a plain 256 MiB device copy
 # (FETCH 131,087 KiB -> x2.0, WRITE 262,144 KiB -> x1.0); SQ_INSTS_VALU in wave64 instructions.
-STAGE6_COUNTERS = {"fetch_kib": 43_997.0, "fetch_scale": 2.0, "write_kib": 47_723.2, "write_scale": 1.0,
-                   "sq_insts_valu": 202_885_748}
+STAGE6_COUNTERS = {"fetch_kib": 43_992.3, "fetch_scale": 2.0, "write_kib": 47_812.5, "write_scale": 1.0,
+                   "sq_insts_valu": 180_583_620}
 
 
 def log(*a):

@@ -150,9 +150,10 @@ size_t zaptex_batch_mip_bytes(int w, int h, int levels);
 size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels);
 size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n);
 /* Builds the plan: device job tables (uploaded once into the caller's workspace) and one captured graph:
- * the mip levels of every texture (one launch per level across textures), then the BC1/3/4/5 and BC7 encoders
- * over all blocks.  The launch shapes in force (zaptex_set_launch) are baked in.  The texture buffers and the
- * workspace must outlive the plan. */
+ * the mip chains of every texture (each chain cut into fused tile passes, single levels and a <= 128x128 tail;
+ * step s of all textures shares the launches of round s), then the BC1/3/4/5 encoder, the BC1 hill climb and the
+ * BC7 stages over all blocks.  The launch shapes in force (zaptex_set_launch) are baked in.  The texture buffers
+ * and the workspace must outlive the plan. */
 int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out);
 /* one launch of the graph on `stream`; runs of one plan must not overlap (they share the workspace) */
 int zaptex_batch_run(zaptex_batch *b, void *stream);
