@@ -198,7 +198,9 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, 
         const uint32_t m1 = kP2[part], mask = sb ? m1 : (~m1 & 0xFFFFu);
         int q[2][4], pb[2];
         uint64_t idx;
-        const int err = encode_subset<3, 7, 3, true, false>(px, mask, __popc(mask), q, pb, idx);
+        ChanWords T;
+        chan_words(px, T);
+        const int err = encode_subset<3, 7, 3, true, false>(px, T, mask, __popc(mask), q, pb, idx);
         const uint64_t o = (uint64_t)e * 16 + k * 2 + sb;
         sub_err[o] = err;
         // the 6 endpoint values (6 bits each: endpoint e, channel c at bit 18e + 6c) and the shared p-bit at bit 36

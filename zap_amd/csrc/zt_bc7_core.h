@@ -62,6 +62,72 @@ template <int NP> ZT_DEV void palette_keys(const uint32_t pal[NP], int K[NP]) {
     for (int j = 0; j < NP; j++) K[j] = no_fold(15 - j - 16 * (int)dot4(pal[j], pal[j]));
 }
 
+// ---- exact least-squares sums (zap_tex.h:497-501, :627-632).  A BC7 weight is w = k/64 with k an integer in
+// 0..64 and pixels are 8-bit integers, so every product the reference accumulates in float is exact and every
+// partial sum is a multiple of 2^-12 (aa, ab, bb; at most 16 * 4096 units) or of 2^-6 (ax, bx; at most
+// 16 * 64 * 255 units) below 2^24 units: the float sums are exact whatever their order, and equal
+//   aa = sum (64-k)^2 / 4096,  ab = sum (64-k) k / 4096,  bb = sum k^2 / 4096,
+//   ax_c = sum (64-k) p_c / 64 = (64 P_c - S_c) / 64,  bx_c = S_c / 64,  with P_c = sum p_c and S_c = sum k p_c.
+// The integer sums take four pixels per v_dot4_u32_u8: channel words (byte t = channel c of pixel 4g+t) against
+// weight words (byte t = k of pixel 4g+t; 0 for pixels outside the subset).
+struct ChanWords { uint32_t w[4][4]; };   // w[g][c]
+ZT_DEV void chan_words(const uint32_t px[16], ChanWords &T) {
+#pragma unroll
+    for (int g = 0; g < 4; g++) {   // 4x4 byte transpose of pixels 4g..4g+3
+        const uint32_t a = px[4 * g], b = px[4 * g + 1], c = px[4 * g + 2], d = px[4 * g + 3];
+        const uint32_t rg01 = perm(b, a, 0x05010400u), ba01 = perm(b, a, 0x07030602u);
+        const uint32_t rg23 = perm(d, c, 0x05010400u), ba23 = perm(d, c, 0x07030602u);
+        T.w[g][0] = perm(rg23, rg01, 0x05040100u); T.w[g][1] = perm(rg23, rg01, 0x07060302u);
+        T.w[g][2] = perm(ba23, ba01, 0x05040100u); T.w[g][3] = perm(ba23, ba01, 0x07060302u);
+    }
+}
+// P_c over the subset `mask` (all 16 pixels when FULL)
+template <int NC, int C0, bool FULL> ZT_DEV void chan_sums(const ChanWords &T, uint32_t mask, int P[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) P[c] = 0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        // one byte per pixel: bit t of the group's mask nibble moves to bit 8t (the shifted copies never overlap)
+        const uint32_t sel = FULL ? 0x01010101u : (__umul24((mask >> (4 * g)) & 15u, 0x204081u) & 0x01010101u);
+#pragma unroll
+        for (int c = 0; c < NC; c++) P[c] = (int)dot4acc(T.w[g][C0 + c], sel, (uint32_t)P[c]);
+    }
+}
+// index bytes of pixels 4g..4g+3 from 4-bit fields (pixel k at bit 4k) or from 2-bit fields (pixel k at bit 2k)
+ZT_DEV uint32_t idx_bytes4(uint64_t idx, int g) {
+    const uint32_t u = perm((uint32_t)(idx >> 32), (uint32_t)idx, 0x0C000C00u | (2u * g + 1u) << 16 | 2u * g);
+    return (u & 0x000F000Fu) | ((u << 4) & 0x0F000F00u);
+}
+ZT_DEV uint32_t idx_bytes2(uint32_t idx, int g) {
+    const uint32_t b = (idx >> (8 * g)) & 255u, u = b | b << 12;
+    return (u | u << 6) & 0x03030303u;
+}
+// k = round(64 j / (2^IB - 1)) of each index byte j: 4j + (j+2)/4 (IB 4), 9j + j/4 (IB 3), 21j + j/2 (IB 2)
+template <int IB> ZT_DEV uint32_t weight_bytes(uint32_t J) {
+    if (IB == 4) return (J << 2) + (((J + 0x02020202u) >> 2) & 0x3F3F3F3Fu);
+    if (IB == 3) return (J << 3) + J + ((J >> 2) & 0x01010101u);
+    return (J << 4) + (J << 2) + J + ((J >> 1) & 0x01010101u);
+}
+template <int NC, int C0>
+ZT_DEV void ls_sums(const ChanWords &T, const uint32_t KW[4], const int P[NC], int n, float &aa, float &ab, float &bb,
+                    float sa[NC], float sb[NC]) {
+    uint32_t k1 = 0, k2 = 0, s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) s[c] = 0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        k1 = dot4acc(KW[g], 0x01010101u, k1);
+        k2 = dot4acc(KW[g], KW[g], k2);
+#pragma unroll
+        for (int c = 0; c < NC; c++) s[c] = dot4acc(T.w[g][C0 + c], KW[g], s[c]);
+    }
+    aa = (float)(4096 * n - 128 * (int)k1 + (int)k2) * (1.0f / 4096.0f);
+    ab = (float)(64 * (int)k1 - (int)k2) * (1.0f / 4096.0f);
+    bb = (float)k2 * (1.0f / 4096.0f);
+#pragma unroll
+    for (int c = 0; c < NC; c++) { sa[c] = (float)(64 * P[c] - (int)s[c]) * (1.0f / 64.0f); sb[c] = (float)s[c] * (1.0f / 64.0f); }
+}
+
 // ---- one subset: quantised endpoints + indices (zap_tex.h:466-486).  idx: 4 bits per pixel at 4*i.
 template <int NC, int N, int IB, bool SHARED>
 ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], const float hi[4], int q[2][4], int pb[2],
@@ -116,22 +182,19 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], c
 
 // ---- PCA start, fit, at most 2 least-squares refinements (zap_tex.h:489-515)
 template <int NC, int N, int IB, bool SHARED, bool FULL>
-ZT_DEV int encode_subset(const uint32_t px[16], uint32_t mask, int cnt, int q[2][4], int pb[2], uint64_t &idx) {
+ZT_DEV int encode_subset(const uint32_t px[16], const ChanWords &T, uint32_t mask, int cnt, int q[2][4], int pb[2], uint64_t &idx) {
+    int P[NC];
+    chan_sums<NC, 0, FULL>(T, mask, P);
     float lo[4], hi[4];
-    pca<NC, FULL>(px, mask, cnt, lo, hi);
+    pca<NC, FULL>(px, mask, cnt, lo, hi, P);
     int err = fit_subset<NC, N, IB, SHARED>(px, mask, lo, hi, q, pb, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
-        float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
+        // pixels outside the subset have index 0, hence weight 0
+        uint32_t KW[4];
 #pragma unroll
-        for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            if (!((mask >> k) & 1)) continue;
-            float w = (float)wgt<IB>((int)((idx >> (4 * k)) & 15)) / 64.0f, v = 1 - w;
-            aa += v * v; ab += v * w; bb += w * w;
-#pragma unroll
-            for (int c = 0; c < NC; c++) { sa[c] += v * chanf(px[k], c); sb[c] += w * chanf(px[k], c); }
-        }
+        for (int g = 0; g < 4; g++) KW[g] = weight_bytes<IB>(idx_bytes4(idx, g));
+        float aa, ab, bb, sa[NC], sb[NC];
+        ls_sums<NC, 0>(T, KW, P, cnt, aa, ab, bb, sa, sb);
         float det = aa * bb - ab * ab;
         if (det < 1e-6f && det > -1e-6f) break;
         float nlo[4], nhi[4];
@@ -157,7 +220,9 @@ ZT_DEV int encode_subset(const uint32_t px[16], uint32_t mask, int cnt, int q[2]
 ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
     int q[2][4], pb[2];
     uint64_t idx;
-    int err = encode_subset<4, 8, 4, false, true>(px, 0xFFFFu, 16, q, pb, idx);
+    ChanWords T;
+    chan_words(px, T);
+    int err = encode_subset<4, 8, 4, false, true>(px, T, 0xFFFFu, 16, q, pb, idx);
     if (idx & 8) {  // anchor MSB set: swap endpoints and p-bits, invert indices
 #pragma unroll
         for (int c = 0; c < 4; c++) { int t = q[0][c]; q[0][c] = q[1][c]; q[1][c] = t; }
@@ -215,17 +280,12 @@ template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], const int e0
     idx = bits;
     return err;
 }
-template <int NC, int C0> ZT_DEV bool lsq_2bit(const uint32_t r[16], uint32_t idx, float lo[], float hi[]) {
-    float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
+template <int NC, int C0> ZT_DEV bool lsq_2bit(const ChanWords &T, const int P[4], uint32_t idx, float lo[], float hi[]) {
+    uint32_t KW[4];
 #pragma unroll
-    for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        float w = (float)wgt<2>((int)((idx >> (2 * i)) & 3)) / 64.0f, v = 1 - w;
-        aa += v * v; ab += v * w; bb += w * w;
-#pragma unroll
-        for (int c = 0; c < NC; c++) { sa[c] += v * chanf(r[i], C0 + c); sb[c] += w * chanf(r[i], C0 + c); }
-    }
+    for (int g = 0; g < 4; g++) KW[g] = weight_bytes<2>(idx_bytes2(idx, g));
+    float aa, ab, bb, sa[NC], sb[NC];
+    ls_sums<NC, C0>(T, KW, P + C0, 16, aa, ab, bb, sa, sb);
     float det = aa * bb - ab * ab;
     if (det < 1e-6f && det > -1e-6f) return false;
 #pragma unroll
@@ -235,13 +295,15 @@ template <int NC, int C0> ZT_DEV bool lsq_2bit(const uint32_t r[16], uint32_t id
     }
     return true;
 }
-template <int NC, int C0> ZT_DEV int part_mode5(const uint32_t r[16], int n, const float lo[], const float hi[], int q[2][3], uint32_t &idx) {
+template <int NC, int C0>
+ZT_DEV int part_mode5(const uint32_t r[16], const ChanWords &T, const int P[4], int n, const float lo[], const float hi[], int q[2][3],
+                      uint32_t &idx) {
     int e0[NC], e1[NC];
     quant_plain<NC>(lo, n, q[0], e0); quant_plain<NC>(hi, n, q[1], e1);
     int err = fit_2bit<NC, C0>(r, e0, e1, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
         float nlo[NC], nhi[NC];
-        if (!lsq_2bit<NC, C0>(r, idx, nlo, nhi)) break;
+        if (!lsq_2bit<NC, C0>(T, P, idx, nlo, nhi)) break;
         int tq[2][3], te0[NC], te1[NC];
         quant_plain<NC>(nlo, n, tq[0], te0); quant_plain<NC>(nhi, n, tq[1], te1);
         uint32_t ti;
@@ -265,16 +327,20 @@ ZT_DEV int mode5_rotation(const uint32_t px[16], int rot, int cq[2][3], int aq[2
     uint32_t r[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) r[i] = rotate_px(px[i], rot);
+    ChanWords T;
+    chan_words(r, T);
+    int P[4];
+    chan_sums<4, 0, true>(T, 0xFFFFu, P);
     float lo[4], hi[4], alo[1] = { 255.0f }, ahi[1] = { 0.0f };
-    pca<3, true>(r, 0xFFFFu, 16, lo, hi);
-    int err = part_mode5<3, 0>(r, 7, lo, hi, cq, ci);
+    pca<3, true>(r, 0xFFFFu, 16, lo, hi, P);
+    int err = part_mode5<3, 0>(r, T, P, 7, lo, hi, cq, ci);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         float a = (float)(r[i] >> 24);
         if (a < alo[0]) alo[0] = a;
         if (a > ahi[0]) ahi[0] = a;
     }
-    err += part_mode5<1, 3>(r, 8, alo, ahi, aq, ai);
+    err += part_mode5<1, 3>(r, T, P, 8, alo, ahi, aq, ai);
     return err;
 }
 ZT_DEV void pack_mode5(int rot, int cq[2][3], int aq[2][3], uint32_t ci, uint32_t ai, uint64_t &lo, uint64_t &hi) {
@@ -348,8 +414,10 @@ ZT_DEV int mode1(const uint32_t px[16], int part, bool pack, uint64_t &lo, uint6
     const uint32_t m1 = kP2[part], m0 = ~m1 & 0xFFFFu;
     int q[2][2][4], pb[2][2];
     uint64_t idx[2];
-    int err = encode_subset<3, 7, 3, true, false>(px, m0, __popc(m0), q[0], pb[0], idx[0]);
-    err += encode_subset<3, 7, 3, true, false>(px, m1, __popc(m1), q[1], pb[1], idx[1]);
+    ChanWords T;
+    chan_words(px, T);
+    int err = encode_subset<3, 7, 3, true, false>(px, T, m0, __popc(m0), q[0], pb[0], idx[0]);
+    err += encode_subset<3, 7, 3, true, false>(px, T, m1, __popc(m1), q[1], pb[1], idx[1]);
     if (!pack) return err;
     const int anc = kA2[part];
     uint64_t full = idx[0] | idx[1];  // disjoint pixel sets, 4 bits per pixel

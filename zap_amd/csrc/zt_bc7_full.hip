@@ -50,22 +50,18 @@ ZT_DEV int fit_subset_plain(const uint32_t px[16], uint32_t mask, const float lo
     return err;
 }
 template <int NC, int N, int IB>
-ZT_DEV int encode_subset_plain(const uint32_t px[16], uint32_t mask, int cnt, int q[2][4], uint64_t &idx) {
+ZT_DEV int encode_subset_plain(const uint32_t px[16], const ChanWords &T, uint32_t mask, int cnt, int q[2][4], uint64_t &idx) {
+    int P[NC];
+    chan_sums<NC, 0, false>(T, mask, P);
     float lo[4], hi[4];
     pca<NC, false>(px, mask, cnt, lo, hi);
     int err = fit_subset_plain<NC, N, IB>(px, mask, lo, hi, q, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
-        float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
+        uint32_t KW[4];   // pixels outside the subset have index 0, hence weight 0
 #pragma unroll
-        for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            if (!((mask >> k) & 1)) continue;
-            float w = (float)wgt<IB>((int)((idx >> (4 * k)) & 15)) / 64.0f, v = 1 - w;
-            aa += v * v; ab += v * w; bb += w * w;
-#pragma unroll
-            for (int c = 0; c < NC; c++) { sa[c] += v * chanf(px[k], c); sb[c] += w * chanf(px[k], c); }
-        }
+        for (int g = 0; g < 4; g++) KW[g] = weight_bytes<IB>(idx_bytes4(idx, g));
+        float aa, ab, bb, sa[NC], sb[NC];
+        ls_sums<NC, 0>(T, KW, P, cnt, aa, ab, bb, sa, sb);
         float det = aa * bb - ab * ab;
         if (det < 1e-6f && det > -1e-6f) break;
         float nlo[4], nhi[4];
@@ -114,14 +110,16 @@ template <int M> ZT_DEV int cand_subsets(const uint32_t px[16], int part, uint64
     uint64_t full = 0;
     uint32_t masks[T::NS];
     int err = 0;
+    ChanWords CW;
+    chan_words(px, CW);
 #pragma unroll
     for (int s = 0; s < T::NS; s++) {
         masks[s] = subset_mask(T::NS, part, s);
         const int cnt = __popc(masks[s]);
         uint64_t idx;
         pb[s][0] = pb[s][1] = 0;
-        if (HASP) err += encode_subset<NC, N, T::IB, (bool)T::SPB, false>(px, masks[s], cnt, q[s], pb[s], idx);
-        else err += encode_subset_plain<NC, N, T::IB>(px, masks[s], cnt, q[s], idx);
+        if (HASP) err += encode_subset<NC, N, T::IB, (bool)T::SPB, (T::NS == 1)>(px, CW, masks[s], cnt, q[s], pb[s], idx);
+        else err += encode_subset_plain<NC, N, T::IB>(px, CW, masks[s], cnt, q[s], idx);
         full |= idx;
     }
     if (NC == 3) {
@@ -201,17 +199,12 @@ template <int NC, int C0, int IB> ZT_DEV int fit_nbit(const uint32_t r[16], cons
     idx = bits;
     return err;
 }
-template <int NC, int C0, int IB> ZT_DEV bool lsq_nbit(const uint32_t r[16], uint64_t idx, float lo[], float hi[]) {
-    float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[NC], sb[NC];
+template <int NC, int C0, int IB> ZT_DEV bool lsq_nbit(const ChanWords &T, const int P[4], uint64_t idx, float lo[], float hi[]) {
+    uint32_t KW[4];
 #pragma unroll
-    for (int c = 0; c < NC; c++) { sa[c] = 0.0f; sb[c] = 0.0f; }
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        float w = (float)wgt<IB>((int)((idx >> (4 * i)) & 15)) / 64.0f, v = 1 - w;
-        aa += v * v; ab += v * w; bb += w * w;
-#pragma unroll
-        for (int c = 0; c < NC; c++) { sa[c] += v * chanf(r[i], C0 + c); sb[c] += w * chanf(r[i], C0 + c); }
-    }
+    for (int g = 0; g < 4; g++) KW[g] = weight_bytes<IB>(idx_bytes4(idx, g));
+    float aa, ab, bb, sa[NC], sb[NC];
+    ls_sums<NC, C0>(T, KW, P + C0, 16, aa, ab, bb, sa, sb);
     float det = aa * bb - ab * ab;
     if (det < 1e-6f && det > -1e-6f) return false;
 #pragma unroll
@@ -221,13 +214,15 @@ template <int NC, int C0, int IB> ZT_DEV bool lsq_nbit(const uint32_t r[16], uin
     }
     return true;
 }
-template <int NC, int C0, int IB> ZT_DEV int part_nbit(const uint32_t r[16], int n, const float lo[], const float hi[], int q[2][3], uint64_t &idx) {
+template <int NC, int C0, int IB>
+ZT_DEV int part_nbit(const uint32_t r[16], const ChanWords &T, const int P[4], int n, const float lo[], const float hi[], int q[2][3],
+                     uint64_t &idx) {
     int e0[NC], e1[NC];
     quant_plain<NC>(lo, n, q[0], e0); quant_plain<NC>(hi, n, q[1], e1);
     int err = fit_nbit<NC, C0, IB>(r, e0, e1, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
         float nlo[NC], nhi[NC];
-        if (!lsq_nbit<NC, C0, IB>(r, idx, nlo, nhi)) break;
+        if (!lsq_nbit<NC, C0, IB>(T, P, idx, nlo, nhi)) break;
         int tq[2][3], te0[NC], te1[NC];
         quant_plain<NC>(nlo, n, tq[0], te0); quant_plain<NC>(nhi, n, tq[1], te1);
         uint64_t ti;
@@ -249,15 +244,19 @@ template <int M, int ISB> ZT_DEV int cand_rotation(const uint32_t px[16], int ro
     float lo[4], hi[4], alo[1] = { 255.0f }, ahi[1] = { 0.0f };
     int cq[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } }, aq[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
     uint64_t ci, ai;
-    pca<3, true>(r, 0xFFFFu, 16, lo, hi);
-    int err = part_nbit<3, 0, IBC>(r, CB, lo, hi, cq, ci);
+    ChanWords CW;
+    chan_words(r, CW);
+    int P[4];
+    chan_sums<4, 0, true>(CW, 0xFFFFu, P);
+    pca<3, true>(r, 0xFFFFu, 16, lo, hi, P);
+    int err = part_nbit<3, 0, IBC>(r, CW, P, CB, lo, hi, cq, ci);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         float a = (float)(r[i] >> 24);
         if (a < alo[0]) alo[0] = a;
         if (a > ahi[0]) ahi[0] = a;
     }
-    err += part_nbit<1, 3, IBA>(r, AB, alo, ahi, aq, ai);
+    err += part_nbit<1, 3, IBA>(r, CW, P, AB, alo, ahi, aq, ai);
     if (ci & (1u << (IBC - 1))) {
 #pragma unroll
         for (int c = 0; c < 3; c++) { int t = cq[0][c]; cq[0][c] = cq[1][c]; cq[1][c] = t; }

@@ -54,6 +54,9 @@ ZT_DEV int chan(uint32_t p, int c) { return (int)((p >> (8 * c)) & 255u); }
 ZT_DEV float chanf(uint32_t p, int c) { return (float)chan(p, c); }
 ZT_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 ZT_DEV uint32_t dot4(uint32_t a, uint32_t b) { return __builtin_amdgcn_udot4(a, b, 0u, false); }
+ZT_DEV uint32_t dot4acc(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_udot4(a, b, acc, false); }
+// v_perm_b32: byte i of the result = byte sel.byte[i] of the 8 bytes {hi:lo} (0-3 from lo, 4-7 from hi; 0x0C = 0)
+ZT_DEV uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amdgcn_perm(hi, lo, sel); }
 // v as an opaque register value: the optimiser cannot fold it into surrounding arithmetic.  Palette keys are
 // (dot4 << s) + K; left visible, K's constant part is re-associated into a separate OR and SUB, three
 // instructions per key instead of one v_lshl_add_u32.

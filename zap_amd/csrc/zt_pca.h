@@ -4,8 +4,10 @@
 #include "zt_common.h"
 
 namespace {
+// `sums` (optional, FULL only): the integer channel sums.  Each term p/16 and each partial sum of the reference's
+// mean is then exact (multiples of 1/16 below 256), so the mean is exactly sum * (1/16).
 template <int NC, bool FULL>
-ZT_DEV void pca(const uint32_t px[16], uint32_t mask, int cnt, float lo[4], float hi[4]) {
+ZT_DEV void pca(const uint32_t px[16], uint32_t mask, int cnt, float lo[4], float hi[4], const int *sums = nullptr) {
     float mean[NC], cv[NC][NC], ax[NC];
     const float fc = (float)cnt;
 #pragma unroll
@@ -14,11 +16,16 @@ ZT_DEV void pca(const uint32_t px[16], uint32_t mask, int cnt, float lo[4], floa
     for (int a = 0; a < NC; a++)
 #pragma unroll
         for (int b = 0; b < NC; b++) cv[a][b] = 0.0f;
+    if (FULL && sums) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (!FULL && !((mask >> k) & 1)) continue;
+        for (int c = 0; c < NC; c++) mean[c] = (float)sums[c] * 0.0625f;
+    } else {
 #pragma unroll
-        for (int c = 0; c < NC; c++) mean[c] += FULL ? chanf(px[k], c) / 16.0f : chanf(px[k], c) / fc;
+        for (int k = 0; k < 16; k++) {
+            if (!FULL && !((mask >> k) & 1)) continue;
+#pragma unroll
+            for (int c = 0; c < NC; c++) mean[c] += FULL ? chanf(px[k], c) / 16.0f : chanf(px[k], c) / fc;
+        }
     }
 #pragma unroll
     for (int k = 0; k < 16; k++) {
