@@ -31,22 +31,42 @@ template <int NC, int C0> ZT_DEV uint32_t sub_px(uint32_t p) {
     return v & (uint32_t)((1ull << (8 * NC)) - 1ull);
 }
 
-// ---- p-bit quantiser for one channel (zap_tex.h:426-438); N = bits including the p-bit
-template <int N> ZT_DEV int quant_pbit1(float v, int p, int &q) {
-    const int mx = (1 << (N - 1)) - 1;
-    const int g = (int)((v / 255.0f * (float)((1 << N) - 1) - (float)p) / 2.0f + 0.5f);
-    // the three candidates g-1, g, g+1 in order; strict < keeps the first of equal errors, i.e. the smallest of
-    // the packed keys (err << 2 | candidate)
-    uint32_t best = 0xFFFFFFFFu;
+// RN(v / 255) from one product and one FMA correction (the correction of a product by the correctly rounded
+// reciprocal gives the correctly rounded quotient): equal to the division for every finite float except v = -0,
+// which gives +0 (every caller adds 0.5 or subtracts a p-bit next, which erases the sign of a zero).  Checked
+// exhaustively against v / 255.0f by tests/quant_exhaustive.c.
+ZT_DEV float div255(float v) {
+    const float R = 1.0f / 255.0f, q0 = v * R;
+    return __builtin_fmaf(__builtin_fmaf(-q0, 255.0f, v), R, q0);
+}
+
+// ---- p-bit quantiser for one channel (zap_tex.h:426-438); N = bits including the p-bit, t = v / 255 * (2^N - 1)
+// (shared by both p-bits).  The reference tries g-1, g, g+1, each clamped to [0, mx], and keeps the first of the
+// smallest errors.  Here g is clamped to [1, mx-1] first, so the three candidates need no clamp: where that moves
+// the set (g < 1 or g > mx-1), the candidate it adds lies beyond the ones the reference tries and its error is
+// strictly larger, so the choice is the same.  A candidate's key is err << 2 | dt; the smallest key is the
+// reference's choice.  Checked exhaustively against the reference's arithmetic for every float |v| <= 2048 and
+// N = 5..8 (tests/quant_exhaustive.c); endpoint values here stay within [-1020, 1275] (PCA: the mean plus a
+// projection of at most 4 * 255 along an axis whose largest component is 1; least squares: clamped to [0, 255]).
+template <int N> ZT_DEV int quant_pbit1(float v, float t, int p, int &q) {
+    const int mx = (1 << (N - 1)) - 1, g = clampi((int)((t - (float)p) * 0.5f + 0.5f), 1, mx - 1), vm = g << 1 | p;
+    uint32_t k[3];
+    if (N == 8) {   // x = tq << 1 | p: the neighbours are 2 apart, exactly, in float too
+        const float xm = (float)vm, xs[3] = { xm - 2.0f, xm, xm + 2.0f };
 #pragma unroll
-    for (int dt = 0; dt < 3; dt++) {
-        const int tq = clampi(g + dt - 1, 0, mx), x = expand_bits(tq << 1 | p, N);
-        const float d = (float)x - v;
-        best = min(best, (uint32_t)(int)(d * d) << 2 | (uint32_t)dt);
+        for (int dt = 0; dt < 3; dt++) { const float d = xs[dt] - v; k[dt] = (uint32_t)(int)(d * d) << 2 | (uint32_t)dt; }
+    } else {
+#pragma unroll
+        for (int dt = 0; dt < 3; dt++) {
+            const float d = (float)expand_bits(vm + 2 * (dt - 1), N) - v;
+            k[dt] = (uint32_t)(int)(d * d) << 2 | (uint32_t)dt;
+        }
     }
-    q = clampi(g + (int)(best & 3u) - 1, 0, mx);
+    const uint32_t best = min(min(k[0], k[1]), k[2]);
+    q = g + (int)(best & 3u) - 1;
     return (int)(best >> 2);
 }
+template <int N> ZT_DEV float quant_scale(float v) { return div255(v) * (float)((1 << N) - 1); }
 
 // ---- argmin over a palette of NP packed entries: returns SSE, writes the index (ties -> lowest index).
 // key_j = 16*(2 p.pal_j - |pal_j|^2) + (15 - j): maximising it minimises |p - pal_j|^2 exactly.
@@ -137,7 +157,10 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], c
     for (int p = 0; p < 2; p++) {
         el[p] = 0; eh[p] = 0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) { el[p] += quant_pbit1<N>(lo[c], p, ql[p][c]); eh[p] += quant_pbit1<N>(hi[c], p, qh[p][c]); }
+        for (int c = 0; c < NC; c++) {
+            el[p] += quant_pbit1<N>(lo[c], quant_scale<N>(lo[c]), p, ql[p][c]);
+            eh[p] += quant_pbit1<N>(hi[c], quant_scale<N>(hi[c]), p, qh[p][c]);
+        }
     }
     int best = 1 << 30, b0 = 0, b1 = 0;
 #pragma unroll
@@ -242,19 +265,24 @@ ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
 }
 
 // ---- mode 5 pieces (zap_tex.h:597-657); NC = 3 (colour, 7 bits) or 1 (alpha, 8 bits)
+// plain n-bit quantiser (zap_tex.h:613-624): candidates and keys as in quant_pbit1 (checked exhaustively for
+// every float |v| <= 2048 and n = 4..8)
 template <int NC> ZT_DEV void quant_plain(const float *v, int n, int q[], int e[]) {
     const int mx = (1 << n) - 1;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        int g = (int)(v[c] * (float)mx / 255.0f + 0.5f), bq = 0, bd = 1 << 30;
+        const int g = clampi((int)(div255(v[c] * (float)mx) + 0.5f), 1, mx - 1);
+        uint32_t k[3];
+        if (n == 8) {
+            const float xm = (float)g, xs[3] = { xm - 1.0f, xm, xm + 1.0f };
 #pragma unroll
-        for (int dt = -1; dt <= 1; dt++) {
-            int t = g + dt, tq = t < 0 ? 0 : t > mx ? mx : t, x = expand_bits(tq, n);
-            float df = (float)x - v[c];
-            int d = (int)(df < 0 ? v[c] - (float)x : df);
-            if (d < bd) { bd = d; bq = tq; }
+            for (int dt = 0; dt < 3; dt++) k[dt] = (uint32_t)(int)fabsf(xs[dt] - v[c]) << 2 | (uint32_t)dt;
+        } else {
+#pragma unroll
+            for (int dt = 0; dt < 3; dt++) k[dt] = (uint32_t)(int)fabsf((float)expand_bits(g + dt - 1, n) - v[c]) << 2 | (uint32_t)dt;
         }
-        q[c] = bq; e[c] = expand_bits(bq, n);
+        const uint32_t best = min(min(k[0], k[1]), k[2]);
+        q[c] = g + (int)(best & 3u) - 1; e[c] = expand_bits(q[c], n);
     }
 }
 template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], const int e0[], const int e1[], uint32_t &idx) {
