@@ -18,6 +18,8 @@ Also reported, measured inside this run:
   learned       the learned-proposal encode (zap_amd/models/bc7_proposal.npy, top-4 exact + exhaustive fallback at
                 the model's calibrated threshold) against the exhaustive all-mode search on the same chain:
                 Mpix/s of both, chain PSNR of both (tex_bench.c's RGBA definition) and the gap
+  configs       BC1 of the 1024^2 image, BC6H of the 2048^2 HDR field, ASTC 4x4 of the 2048^2 texture: device
+                Mpix/s, golden-hash check, kernel times, and the reference encoder's rate on the host cores
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -48,8 +50,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(levels_np, threads: int, repeats: int = 5) -> dict:
-    """Reference zap_tex.h encoder (oracle/_ref; the oracle restatement if that is absent) on host cores."""
+def cpu_baseline(levels_np, threads: int, repeats: int = 5, configs: dict | None = None) -> dict:
+    """Reference zap_tex.h encoder (oracle/_ref; the oracle restatement if that is absent) on host cores; with
+    `configs` ({name: image} of the other workloads) also each one's rate, banded the same way, median of 3."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import loader
@@ -72,10 +75,26 @@ def cpu_baseline(levels_np, threads: int, repeats: int = 5) -> dict:
     t1 = time.perf_counter()
     lib.bc_encode(one, "bc7")
     t1 = time.perf_counter() - t1
-    return {"value": round(npx / t / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": kind,
-            "sample": f"BC7 of the full 13-level 4096^2 chain ({npx} px), 8-block-row bands on {threads} threads, median of {repeats}",
-            "single_thread_mpix_s": round(one.shape[0] * one.shape[1] / t1 / 1e6, 3),
-            "cpu": _cpu_model(), "nproc": os.cpu_count()}
+    res = {"value": round(npx / t / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": kind,
+           "sample": f"BC7 of the full 13-level 4096^2 chain ({npx} px), 8-block-row bands on {threads} threads, median of {repeats}",
+           "single_thread_mpix_s": round(one.shape[0] * one.shape[1] / t1 / 1e6, 3),
+           "cpu": _cpu_model(), "nproc": os.cpu_count()}
+    if configs:
+        import numpy as np
+        fmts = {name: fmt for name, fmt, _, _ in CONFIG_WORK}
+        res["configs_mpix_s"] = {}
+        for name, img in configs.items():
+            fmt = fmts[name]
+            enc = {"bc6h": lib.bc6h_encode, "astc": lib.astc_encode}.get(fmt, lambda b, f=fmt: lib.bc_encode(b, f))
+            bands = [np.ascontiguousarray(img[y0:y0 + 32]) for y0 in range(0, img.shape[0], 32)]
+            ts = []
+            with ThreadPoolExecutor(threads) as ex:
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    list(ex.map(enc, bands))
+                    ts.append(time.perf_counter() - t0)
+            res["configs_mpix_s"][name] = round(img.shape[0] * img.shape[1] / statistics.median(ts) / 1e6, 2)
+    return res
 
 
 def learned_leg(lib, c2, steps: int = 5) -> dict:
@@ -128,6 +147,65 @@ def batch_leg(lib, steps: int = 5) -> dict:
     return out
 
 
+CONFIG_WORK = [  # (name, format, golden key, kernels)
+    ("c1_bc1_1024", "bc1", "c1_1024_bc1", ["zt_bc14_encode", "zt_bc1_climb"]),
+    ("c3_bc6h_2048", "bc6h", "c3_bc6h_2048", ["zt_bc6h_encode"]),
+    ("c3_astc_2048", "astc", "c3_astc_2048", ["zt_astc_encode"]),
+]
+
+
+def config_images() -> dict:
+    from zap_amd import synth
+    return {"c1_bc1_1024": synth.c1_bc1(1024, 1), "c3_bc6h_2048": synth.c3_hdr(2048, 5), "c3_astc_2048": synth.c3_astc(2048, 3)}
+
+
+def configs_leg(lib, golden: dict, images: dict, steps: int) -> dict:
+    """The other single-texture workloads, on the device entry point with inputs resident in HBM: BC1 of the
+    1024^2 gradient+noise+rectangles image, BC6H of the 2048^2 HDR field, ASTC 4x4 of the 2048^2 noise texture.
+    Each is checked against its golden hash; per-kernel durations come from the library's HIP events."""
+    import numpy as np
+    import torch
+
+    from zap_amd import binding
+    out = {}
+    for name, fmt, gkey, kernels in CONFIG_WORK:
+        img = images[name]
+        h, w = img.shape[:2]
+        src = torch.from_numpy(np.ascontiguousarray(img)).cuda()
+        dst = torch.empty(binding.bc_size(w, h, fmt), dtype=torch.uint8, device="cuda")
+        imgs = lib.images([(src, dst, w, h, w * img.itemsize * 4, fmt)])
+        ws = torch.empty(max(1, lib.workspace_size(imgs)), dtype=torch.uint8, device="cuda")
+        lib.encode_dev(imgs, ws)
+        torch.cuda.synchronize()
+        exact = hashlib.sha256(dst.cpu().numpy().tobytes()).hexdigest() == golden["outputs"][gkey]
+        for _ in range(3):
+            lib.encode_dev(imgs, ws)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            lib.encode_dev(imgs, ws)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        lib.profile(True)
+        lib.profile_reset()
+        for _ in range(5):
+            lib.encode_dev(imgs, ws)
+        torch.cuda.synchronize()
+        kern = {}
+        for k in kernels:
+            t, n = lib.profile_get(k)
+            if n:
+                kern[k] = round(t / n, 5)
+        lib.profile(False)
+        out[name] = {"pixels": w * h, "blocks": ((w + 3) // 4) * ((h + 3) // 4), "ms": round(ms, 4),
+                     "mpix_s": round(w * h / (ms / 1e3) / 1e6, 1), "byte_exact_vs_reference": exact, "kernel_ms": kern}
+        log(f"{name}: {out[name]['mpix_s']} Mpix/s ({ms:.3f} ms), exact {exact}, kernels {kern}")
+        del src, dst, imgs, ws
+    return out
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -147,6 +225,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-texture batch leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the BC1 1024^2 / BC6H 2048^2 / ASTC 2048^2 legs")
     ap.add_argument("--no-host", action="store_true", help="skip the host-entry (PCIe-inclusive) leg")
     ap.add_argument("--batch-only", action="store_true", help="only build and time the 512-texture batch (for profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -265,7 +344,12 @@ def main() -> int:
     autotune.reset(lib)
     learned_res = None if args.no_learned else learned_leg(lib, c2)
     batch_res = None if args.no_batch else batch_leg(lib)
-    cpu = None if args.no_cpu else cpu_baseline(levels_np, args.cpu_threads)
+    cfg_imgs = None if args.no_configs else config_images()
+    configs_res = None if args.no_configs else configs_leg(lib, golden, cfg_imgs, 20)
+    cpu = None if args.no_cpu else cpu_baseline(levels_np, args.cpu_threads, configs=cfg_imgs)
+    if cpu and configs_res:
+        for name, v in cpu.get("configs_mpix_s", {}).items():
+            configs_res[name]["cpu_reference_mpix_s"] = v
     line = {
         "metric": "BC7 Mpix/s on 1 MI355X: plain vs learned-proposal at \u22640.05 dB PSNR gap",
         "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -291,6 +375,7 @@ def main() -> int:
         "kernels": kern,
         "learned": learned_res,
         "batch": batch_res,
+        "configs": configs_res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -262,17 +262,19 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, u
     }
 }
 
-// The BC1 hill climb with lane refill.  Each wave owns ranges of ZT_CLIMB_RANGE blocks (wave-strided); a lane that
-// has no block (or has finished one) takes the next block of the range, numbered by a ballot prefix count, so
-// every step runs one trial on (nearly) every lane.  Non-BC1 blocks are skipped at refill.
+// The BC1 hill climb with lane refill.  Each wave owns ranges of `range` blocks (wave-strided; a multiple of 64, at
+// most ZT_CLIMB_RANGE, chosen by the host so that small jobs still spread over enough waves); a lane that has no
+// block (or has finished one) takes the next block of the range, numbered by a ballot prefix count, so every step
+// runs one trial on (nearly) every lane.  Non-BC1 blocks are skipped at refill.
 // Refills wait until at least refill_min lanes are idle (or none is busy): a refill costs every lane of the wave
 // a job lookup and a 16-pixel gather, so doing it for one lane per step would cost more than the idle lanes.
-extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min) {
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min,
+                                                                 uint32_t range) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint64_t r0 = (uint64_t)wave * ZT_CLIMB_RANGE; r0 < total; r0 += (uint64_t)nwaves * ZT_CLIMB_RANGE) {
-        const uint32_t end = (uint32_t)min<uint64_t>(total, r0 + ZT_CLIMB_RANGE);
+    for (uint64_t r0 = (uint64_t)wave * range; r0 < total; r0 += (uint64_t)nwaves * range) {
+        const uint32_t end = (uint32_t)min<uint64_t>(total, r0 + range);
         uint32_t next = (uint32_t)r0;  // same on every lane
         bool active = false;
         uint32_t px[16];  // alpha cleared (bc1_prepare)

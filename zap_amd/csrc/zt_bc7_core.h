@@ -68,18 +68,75 @@ template <int N> ZT_DEV int quant_pbit1(float v, float t, int p, int &q) {
 }
 template <int N> ZT_DEV float quant_scale(float v) { return div255(v) * (float)((1 << N) - 1); }
 
-// ---- argmin over a palette of NP packed entries: returns SSE, writes the index (ties -> lowest index).
-// key_j = 16*(2 p.pal_j - |pal_j|^2) + (15 - j): maximising it minimises |p - pal_j|^2 exactly.
-template <int NP> ZT_DEV int best_entry(uint32_t p, const uint32_t pal[NP], const int K[NP], int &idx) {
+// ---- argmin over a palette of NP packed entries (ties -> lowest index).  key_j = 32 p.pal_j - 16 |pal_j|^2 +
+// (15 - j) = 16 (|p|^2 - |p - pal_j|^2) + (15 - j), one v_lshl_add per entry (|key| < 2^24): the largest key is
+// the reference's choice, (key & 15) ^ 15 = j and key >> 4 = |p|^2 - SSE.  Over a subset, sum (key >> 4) =
+// (sum key - sum (key & 15)) / 16 exactly, so a fit's SSE is spp minus that, with spp = sum |p|^2 of the subset.
+template <int NP> ZT_DEV int best_key(uint32_t p, const uint32_t pal[NP], const int K[NP]) {
     int best = (int)(dot4(p, pal[0]) << 5) + K[0];
 #pragma unroll
     for (int j = 1; j < NP; j++) best = max(best, (int)(dot4(p, pal[j]) << 5) + K[j]);
-    idx = 15 - (best & 15);
-    return (int)dot4(p, p) - (best >> 4);
+    return best;
 }
 template <int NP> ZT_DEV void palette_keys(const uint32_t pal[NP], int K[NP]) {
 #pragma unroll
     for (int j = 0; j < NP; j++) K[j] = no_fold(15 - j - 16 * (int)dot4(pal[j], pal[j]));
+}
+// 0xF in the nibble of every pixel of `mask`
+ZT_DEV uint64_t subset_nibbles(uint32_t mask) {
+    uint64_t m = mask;
+    m = (m | m << 24) & 0x000000FF000000FFull;
+    m = (m | m << 12) & 0x000F000F000F000Full;
+    m = (m | m << 6) & 0x0303030303030303ull;
+    m = (m | m << 3) & 0x1111111111111111ull;
+    return m * 15u;
+}
+// sum of the 16 4-bit fields of b / of the 16 2-bit fields of w
+ZT_DEV int nibble_sum(uint64_t b) {
+    const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32), m = 0x0F0F0F0Fu, one = 0x01010101u;
+    return (int)dot4acc((hi >> 4) & m, one, dot4acc(hi & m, one, dot4acc((lo >> 4) & m, one, dot4(lo & m, one))));
+}
+ZT_DEV int pair_sum(uint32_t w) {
+    const uint32_t x = (w & 0x33333333u) + ((w >> 2) & 0x33333333u);   // nibbles of at most 6
+    return (int)dot4acc((x >> 4) & 0x0F0F0F0Fu, 0x01010101u, dot4(x & 0x0F0F0F0Fu, 0x01010101u));
+}
+// sum of |p|^2 over the subset (channels [C0, C0 + NC))
+template <int NC, int C0> ZT_DEV int subset_sq(const uint32_t px[16], uint32_t mask) {
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (!((mask >> k) & 1)) continue;
+        const uint32_t p = sub_px<NC, C0>(px[k]);
+        s += (int)dot4(p, p);
+    }
+    return s;
+}
+
+// ---- palette of an endpoint pair: entry j = ((64 - w_j) e0 + w_j e1 + 32) >> 6 per channel (zap_tex.h:477-478,
+// :539-541, :599), NC channels packed from byte 0.  Two channels share a 32-bit word in 16-bit lanes; with both
+// weights scaled by 4 a lane's sum, 4 * ((64 - w) e0 + w e1 + 32) <= 65408, cannot carry into the next lane and
+// holds the entry in its high byte, so one v_perm gathers an entry's four bytes.
+template <int NC, int IB> ZT_DEV void build_palette(const int e0[], const int e1[], uint32_t pal[1 << IB]) {
+    constexpr int NP = 1 << IB;
+    if (NC == 1) {
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const int w = wgt<IB>(j);
+            pal[j] = (__umul24(64 - w, e0[0]) + __umul24(w, e1[0]) + 32) >> 6;
+        }
+        return;
+    }
+    const uint32_t a_rb = (uint32_t)e0[0] | (uint32_t)e0[NC > 2 ? 2 : 0] << 16 * (NC > 2);
+    const uint32_t a_ga = (uint32_t)e0[1] | (NC > 3 ? (uint32_t)e0[NC - 1] << 16 : 0u);
+    const uint32_t b_rb = (uint32_t)e1[0] | (uint32_t)e1[NC > 2 ? 2 : 0] << 16 * (NC > 2);
+    const uint32_t b_ga = (uint32_t)e1[1] | (NC > 3 ? (uint32_t)e1[NC - 1] << 16 : 0u);
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        const uint32_t w = (uint32_t)wgt<IB>(j);
+        const uint32_t rb = __umul24(b_rb, 4u * w) + __umul24(a_rb, 4u * (64u - w)) + 0x00800080u;
+        const uint32_t ga = __umul24(b_ga, 4u * w) + __umul24(a_ga, 4u * (64u - w)) + 0x00800080u;
+        pal[j] = perm(ga, rb, 0x07030501u);   // with NC = 3 the alpha lane holds 0x80 >> 8 = 0
+    }
 }
 
 // ---- exact least-squares sums (zap_tex.h:497-501, :627-632).  A BC7 weight is w = k/64 with k an integer in
@@ -150,8 +207,8 @@ ZT_DEV void ls_sums(const ChanWords &T, const uint32_t KW[4], const int P[NC], i
 
 // ---- one subset: quantised endpoints + indices (zap_tex.h:466-486).  idx: 4 bits per pixel at 4*i.
 template <int NC, int N, int IB, bool SHARED>
-ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], const float hi[4], int q[2][4], int pb[2],
-                      uint64_t &idx) {
+ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t snib, const float lo[4], const float hi[4], int q[2][4],
+                      int pb[2], uint64_t &idx) {
     int ql[2][NC], qh[2][NC], el[2], eh[2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
@@ -181,26 +238,19 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, const float lo[4], c
     constexpr int NP = 1 << IB;
     uint32_t pal[NP];
     int K[NP];
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-        const int w = wgt<IB>(j);
-        uint32_t v = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
-        pal[j] = v;
-    }
+    build_palette<NC, IB>(e0, e1, pal);
     palette_keys<NP>(pal, K);
-    int err = 0;
+    int sk = 0;
     uint64_t bits = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         if (!((mask >> k) & 1)) continue;
-        int j;
-        err += best_entry<NP>(sub_px<NC, 0>(px[k]), pal, K, j);
-        bits |= (uint64_t)j << (4 * k);
+        const int key = best_key<NP>(sub_px<NC, 0>(px[k]), pal, K);
+        sk += key;
+        bits |= (uint64_t)(key & 15) << (4 * k);
     }
-    idx = bits;
-    return err;
+    idx = bits ^ snib;
+    return spp - ((sk - nibble_sum(bits)) >> 4);
 }
 
 // ---- PCA start, fit, at most 2 least-squares refinements (zap_tex.h:489-515)
@@ -210,7 +260,9 @@ ZT_DEV int encode_subset(const uint32_t px[16], const ChanWords &T, uint32_t mas
     chan_sums<NC, 0, FULL>(T, mask, P);
     float lo[4], hi[4];
     pca<NC, FULL>(px, mask, cnt, lo, hi, P);
-    int err = fit_subset<NC, N, IB, SHARED>(px, mask, lo, hi, q, pb, idx);
+    const int spp = subset_sq<NC, 0>(px, mask);
+    const uint64_t snib = FULL ? ~0ull : subset_nibbles(mask);
+    int err = fit_subset<NC, N, IB, SHARED>(px, mask, spp, snib, lo, hi, q, pb, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
         // pixels outside the subset have index 0, hence weight 0
         uint32_t KW[4];
@@ -230,7 +282,7 @@ ZT_DEV int encode_subset(const uint32_t px[16], const ChanWords &T, uint32_t mas
         }
         int nq[2][4], np[2];
         uint64_t ni;
-        int ne = fit_subset<NC, N, IB, SHARED>(px, mask, nlo, nhi, nq, np, ni);
+        int ne = fit_subset<NC, N, IB, SHARED>(px, mask, spp, snib, nlo, nhi, nq, np, ni);
         if (ne >= err) break;
         err = ne; pb[0] = np[0]; pb[1] = np[1]; idx = ni;
 #pragma unroll
@@ -285,28 +337,21 @@ template <int NC> ZT_DEV void quant_plain(const float *v, int n, int q[], int e[
         q[c] = g + (int)(best & 3u) - 1; e[c] = expand_bits(q[c], n);
     }
 }
-template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], const int e0[], const int e1[], uint32_t &idx) {
+template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], int spp, const int e0[], const int e1[], uint32_t &idx) {
     uint32_t pal[4];
     int K[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int w = wgt<2>(j);
-        uint32_t v = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
-        pal[j] = v;
-    }
+    build_palette<NC, 2>(e0, e1, pal);
     palette_keys<4>(pal, K);
-    int err = 0;
+    int sk = 0;
     uint32_t bits = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        int j;
-        err += best_entry<4>(sub_px<NC, C0>(r[i]), pal, K, j);
-        bits |= (uint32_t)j << (2 * i);
+        const int key = best_key<4>(sub_px<NC, C0>(r[i]), pal, K);   // low nibble 15 - j, j < 4
+        sk += key;
+        bits |= (uint32_t)(key & 3) << (2 * i);
     }
-    idx = bits;
-    return err;
+    idx = ~bits;                                                      // (key & 3) ^ 3 = j
+    return spp - ((sk - 12 * 16 - pair_sum(bits)) >> 4);              // key & 15 = 12 + (key & 3)
 }
 template <int NC, int C0> ZT_DEV bool lsq_2bit(const ChanWords &T, const int P[4], uint32_t idx, float lo[], float hi[]) {
     uint32_t KW[4];
@@ -328,14 +373,15 @@ ZT_DEV int part_mode5(const uint32_t r[16], const ChanWords &T, const int P[4], 
                       uint32_t &idx) {
     int e0[NC], e1[NC];
     quant_plain<NC>(lo, n, q[0], e0); quant_plain<NC>(hi, n, q[1], e1);
-    int err = fit_2bit<NC, C0>(r, e0, e1, idx);
+    const int spp = subset_sq<NC, C0>(r, 0xFFFFu);
+    int err = fit_2bit<NC, C0>(r, spp, e0, e1, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
         float nlo[NC], nhi[NC];
         if (!lsq_2bit<NC, C0>(T, P, idx, nlo, nhi)) break;
         int tq[2][3], te0[NC], te1[NC];
         quant_plain<NC>(nlo, n, tq[0], te0); quant_plain<NC>(nhi, n, tq[1], te1);
         uint32_t ti;
-        int e = fit_2bit<NC, C0>(r, te0, te1, ti);
+        int e = fit_2bit<NC, C0>(r, spp, te0, te1, ti);
         if (e >= err) break;
         err = e; idx = ti;
 #pragma unroll

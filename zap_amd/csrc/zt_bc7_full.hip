@@ -22,32 +22,26 @@ template <> struct B7<7> { static constexpr int NS = 2, PB = 6, CB = 5, AB = 5, 
 
 // subset without p-bits: plain n-bit endpoints, palette argmin (idx: 4 bits per pixel)
 template <int NC, int N, int IB>
-ZT_DEV int fit_subset_plain(const uint32_t px[16], uint32_t mask, const float lo[4], const float hi[4], int q[2][4], uint64_t &idx) {
+ZT_DEV int fit_subset_plain(const uint32_t px[16], uint32_t mask, int spp, uint64_t snib, const float lo[4], const float hi[4], int q[2][4],
+                            uint64_t &idx) {
     int e0[NC], e1[NC];
     quant_plain<NC>(lo, N, q[0], e0); quant_plain<NC>(hi, N, q[1], e1);
     constexpr int NP = 1 << IB;
     uint32_t pal[NP];
     int K[NP];
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-        const int w = wgt<IB>(j);
-        uint32_t v = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
-        pal[j] = v;
-    }
+    build_palette<NC, IB>(e0, e1, pal);
     palette_keys<NP>(pal, K);
-    int err = 0;
+    int sk = 0;
     uint64_t bits = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         if (!((mask >> k) & 1)) continue;
-        int j;
-        err += best_entry<NP>(sub_px<NC, 0>(px[k]), pal, K, j);
-        bits |= (uint64_t)j << (4 * k);
+        const int key = best_key<NP>(sub_px<NC, 0>(px[k]), pal, K);
+        sk += key;
+        bits |= (uint64_t)(key & 15) << (4 * k);
     }
-    idx = bits;
-    return err;
+    idx = bits ^ snib;
+    return spp - ((sk - nibble_sum(bits)) >> 4);
 }
 template <int NC, int N, int IB>
 ZT_DEV int encode_subset_plain(const uint32_t px[16], const ChanWords &T, uint32_t mask, int cnt, int q[2][4], uint64_t &idx) {
@@ -55,7 +49,9 @@ ZT_DEV int encode_subset_plain(const uint32_t px[16], const ChanWords &T, uint32
     chan_sums<NC, 0, false>(T, mask, P);
     float lo[4], hi[4];
     pca<NC, false>(px, mask, cnt, lo, hi);
-    int err = fit_subset_plain<NC, N, IB>(px, mask, lo, hi, q, idx);
+    const int spp = subset_sq<NC, 0>(px, mask);
+    const uint64_t snib = subset_nibbles(mask);
+    int err = fit_subset_plain<NC, N, IB>(px, mask, spp, snib, lo, hi, q, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
         uint32_t KW[4];   // pixels outside the subset have index 0, hence weight 0
 #pragma unroll
@@ -74,7 +70,7 @@ ZT_DEV int encode_subset_plain(const uint32_t px[16], const ChanWords &T, uint32
         }
         int nq[2][4];
         uint64_t ni;
-        int ne = fit_subset_plain<NC, N, IB>(px, mask, nlo, nhi, nq, ni);
+        int ne = fit_subset_plain<NC, N, IB>(px, mask, spp, snib, nlo, nhi, nq, ni);
         if (ne >= err) break;
         err = ne; idx = ni;
 #pragma unroll
@@ -175,29 +171,22 @@ template <int M> ZT_DEV int cand_subsets(const uint32_t px[16], int part, uint64
 }
 
 // n-bit endpoints, IB-bit indices on channels [C0, C0 + NC) of the rotated block (generalises the mode-5 part)
-template <int NC, int C0, int IB> ZT_DEV int fit_nbit(const uint32_t r[16], const int e0[], const int e1[], uint64_t &idx) {
+template <int NC, int C0, int IB> ZT_DEV int fit_nbit(const uint32_t r[16], int spp, const int e0[], const int e1[], uint64_t &idx) {
     constexpr int NP = 1 << IB;
     uint32_t pal[NP];
     int K[NP];
-#pragma unroll
-    for (int j = 0; j < NP; j++) {
-        const int w = wgt<IB>(j);
-        uint32_t v = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) v |= (uint32_t)((__umul24(64 - w, e0[c]) + __umul24(w, e1[c]) + 32) >> 6) << (8 * c);
-        pal[j] = v;
-    }
+    build_palette<NC, IB>(e0, e1, pal);
     palette_keys<NP>(pal, K);
-    int err = 0;
+    int sk = 0;
     uint64_t bits = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        int j;
-        err += best_entry<NP>(sub_px<NC, C0>(r[i]), pal, K, j);
-        bits |= (uint64_t)j << (4 * i);
+        const int key = best_key<NP>(sub_px<NC, C0>(r[i]), pal, K);
+        sk += key;
+        bits |= (uint64_t)(key & 15) << (4 * i);
     }
-    idx = bits;
-    return err;
+    idx = ~bits;
+    return spp - ((sk - nibble_sum(bits)) >> 4);
 }
 template <int NC, int C0, int IB> ZT_DEV bool lsq_nbit(const ChanWords &T, const int P[4], uint64_t idx, float lo[], float hi[]) {
     uint32_t KW[4];
@@ -219,14 +208,15 @@ ZT_DEV int part_nbit(const uint32_t r[16], const ChanWords &T, const int P[4], i
                      uint64_t &idx) {
     int e0[NC], e1[NC];
     quant_plain<NC>(lo, n, q[0], e0); quant_plain<NC>(hi, n, q[1], e1);
-    int err = fit_nbit<NC, C0, IB>(r, e0, e1, idx);
+    const int spp = subset_sq<NC, C0>(r, 0xFFFFu);
+    int err = fit_nbit<NC, C0, IB>(r, spp, e0, e1, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
         float nlo[NC], nhi[NC];
         if (!lsq_nbit<NC, C0, IB>(T, P, idx, nlo, nhi)) break;
         int tq[2][3], te0[NC], te1[NC];
         quant_plain<NC>(nlo, n, tq[0], te0); quant_plain<NC>(nhi, n, tq[1], te1);
         uint64_t ti;
-        int e = fit_nbit<NC, C0, IB>(r, te0, te1, ti);
+        int e = fit_nbit<NC, C0, IB>(r, spp, te0, te1, ti);
         if (e >= err) break;
         err = e; idx = ti;
 #pragma unroll

@@ -93,10 +93,10 @@ ZT_PROP_DECL(zt_bc7_prop_m0) ZT_PROP_DECL(zt_bc7_prop_m1) ZT_PROP_DECL(zt_bc7_pr
 ZT_PROP_DECL(zt_bc7_prop_m4) ZT_PROP_DECL(zt_bc7_prop_m5) ZT_PROP_DECL(zt_bc7_prop_m6) ZT_PROP_DECL(zt_bc7_prop_m7)
 #undef ZT_PROP_DECL
 
-// blocks per wave range of zt_bc1_climb (a wave's lanes refill from its range)
+// largest block range per wave of zt_bc1_climb (a wave's lanes refill from its range)
 #define ZT_CLIMB_RANGE 1024u
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb);
-__global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min);
+__global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min, uint32_t range);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
 __global__ void zt_astc_encode(zt_jobs jobs, uint32_t total);
 
