@@ -38,8 +38,12 @@ template <int CEM> ZT_DEV void endpoints(const int *v, int e0[4], int e1[4]) {
     }
 }
 
+// the configurations' 8-bit -> ISE maps and ISE -> 8-bit values, staged into LDS by each workgroup (every
+// endpoint quantisation looks them up per lane)
+struct AstcLds { uint8_t q[8][256]; uint8_t unq[8][256]; };
+
 // float endpoints -> ISE values and their unquantised values (zap_tex.h:1183-1194)
-template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int *ise, int *uv) {
+template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int *ise, int *uv, const AstcLds &T) {
     constexpr int NV = AC<K>::CEM == 8 ? 6 : 8;
     float s0 = lo[0] + lo[1] + lo[2], s1 = hi[0] + hi[1] + hi[2];
     const bool keep = s1 >= s0;
@@ -47,8 +51,8 @@ template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int 
     for (int k = 0; k < NV / 2; k++) {
         float a = keep ? lo[k] : hi[k], b = keep ? hi[k] : lo[k];
         int x0 = clampi(cvt_i32(a + 0.5f), 0, 255), x1 = clampi(cvt_i32(b + 0.5f), 0, 255);
-        ise[2 * k] = g_astc.q[K][x0]; ise[2 * k + 1] = g_astc.q[K][x1];
-        uv[2 * k] = g_astc.unq[K][ise[2 * k]]; uv[2 * k + 1] = g_astc.unq[K][ise[2 * k + 1]];
+        ise[2 * k] = T.q[K][x0]; ise[2 * k + 1] = T.q[K][x1];
+        uv[2 * k] = T.unq[K][ise[2 * k]]; uv[2 * k + 1] = T.unq[K][ise[2 * k + 1]];
     }
     if (uv[1] + uv[3] + uv[5] < uv[0] + uv[2] + uv[4]) {
 #pragma unroll
@@ -59,50 +63,58 @@ template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int 
     }
 }
 
-// weights for fixed endpoints (zap_tex.h:1197-1210); pm: bit i = texel i in partition 1.
-// key_j = 2^(IB+1) dot(p, pal_j) - 2^IB |pal_j|^2 + (2^IB - 1 - j): the max is the min SSE, lowest j on ties;
-// IB index bits hold up to 32 entries (5-bit weights), keys stay below 2^25.
-template <int K> ZT_DEV int fit_weights(const uint32_t px[16], const int *uv, uint32_t pm, int w[16]) {
-    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, L = 1 << WB, NC = CEM == 12 ? 4 : 3, NV = CEM == 8 ? 6 : 8;
+// weights for fixed endpoints (zap_tex.h:1197-1210); pm: bit i = texel i in partition 1; w: texel i's weight in
+// byte i % 4 of w[i / 4].  key_j = 2^(IB+1) dot(p, pal_j) - 2^IB |pal_j|^2 + (2^IB - 1 - j): the max is the min
+// SSE, lowest j on ties; IB index bits hold up to 32 entries (5-bit weights), keys stay below 2^25.  The loop runs
+// over palette entries outside and texels inside, so only the 16 running maxima stay live.  The palette's unused
+// alpha byte is 0, so texel alpha drops out of dot(p, pal_j); spp = sum |p|^2 over the used channels.
+template <int K> ZT_DEV int fit_weights(const uint32_t px[16], int spp, const int *uv, uint32_t pm, uint32_t w[4]) {
+    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, L = 1 << WB, NV = CEM == 8 ? 6 : 8;
     constexpr int IB = WB > 4 ? WB : 4, IM = (1 << IB) - 1;
-    constexpr uint32_t CM = NC == 4 ? 0xFFFFFFFFu : 0x00FFFFFFu;
-    uint32_t pal[P][L];
-    int Kk[P][L];
+    constexpr uint32_t CM = CEM == 12 ? 0xFFFFFFFFu : 0x00FFFFFFu;
+    int e0[P][4], e1[P][4];
 #pragma unroll
-    for (int k = 0; k < P; k++) {
-        int e0[4], e1[4];
-        endpoints<CEM>(uv + k * NV, e0, e1);
+    for (int k = 0; k < P; k++) endpoints<CEM>(uv + k * NV, e0[k], e1[k]);
+    int best[16];
 #pragma unroll
-        for (int j = 0; j < L; j++) {
-            const int wt = zt_weight_unquant(j, WB);
+    for (int j = 0; j < L; j++) {
+        const int wt = zt_weight_unquant(j, WB);
+        uint32_t pj[P];
+        int kj[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) {
             uint32_t v = 0;
 #pragma unroll
-            for (int c = 0; c < 4; c++) v |= (uint32_t)lerp(e0[c], e1[c], wt) << (8 * c);
-            pal[k][j] = v & CM;
-            Kk[k][j] = no_fold(IM - j - (1 << IB) * (int)dot4(pal[k][j], pal[k][j]));
+            for (int c = 0; c < 4; c++) v |= (uint32_t)lerp(e0[k][c], e1[k][c], wt) << (8 * c);
+            pj[k] = v & CM;
+            kj[k] = no_fold(IM - j - (1 << IB) * (int)dot4(pj[k], pj[k]));
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const bool second = P == 2 && ((pm >> i) & 1);
+            const int key = (int)(dot4(px[i], second ? pj[P - 1] : pj[0]) << (IB + 1)) + (second ? kj[P - 1] : kj[0]);
+            best[i] = j ? max(best[i], key) : key;
         }
     }
-    int err = 0;
+    int sk = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const uint32_t p = px[i] & CM;
-        const bool second = P == 2 && ((pm >> i) & 1);
-        int best = -0x7FFFFFFF;
-#pragma unroll
-        for (int j = 0; j < L; j++) {
-            uint32_t pj = (P == 2 && second) ? pal[P - 1][j] : pal[0][j];
-            int kj = (P == 2 && second) ? Kk[P - 1][j] : Kk[0][j];
-            best = max(best, (int)(dot4(p, pj) << (IB + 1)) + kj);
-        }
-        w[i] = IM - (best & IM);
-        err += (int)dot4(p, p) - (best >> IB);
+        sk += best[i] >> IB;
+        w[i >> 2] |= (uint32_t)(IM - (best[i] & IM)) << (8 * (i & 3));
     }
-    return err;
+    return spp - sk;
 }
+ZT_DEV int wbyte(const uint32_t w[4], int i) { return (int)((w[i >> 2] >> (8 * (i & 3))) & 255u); }
 
 // one trial configuration (zap_tex.h:1212-1244)
-template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12], int w[16]) {
+template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12], uint32_t w[4], const AstcLds &T) {
     constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NC = CEM == 12 ? 4 : 3, NV = CEM == 8 ? 6 : 8;
+    constexpr uint32_t CM = NC == 4 ? 0xFFFFFFFFu : 0x00FFFFFFu;
+    int spp = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) spp += (int)dot4(px[i] & CM, px[i] & CM);
     int uv[12];
 #pragma unroll
     for (int k = 0; k < P; k++) {
@@ -114,11 +126,12 @@ template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12
             if (!cnt) { m = 1; cnt = 1; }
             pca<NC, false>(px, m, cnt, lo, hi);
         }
-        quantise<K>(lo, hi, ise + k * NV, uv + k * NV);
+        quantise<K>(lo, hi, ise + k * NV, uv + k * NV, T);
     }
-    int err = fit_weights<K>(px, uv, pm, w);
+    int err = fit_weights<K>(px, spp, uv, pm, w);
     for (int it = 0; it < 3 && err > 0; it++) {
-        int ti[12], tuv[12], tw[16];
+        int ti[12], tuv[12];
+        uint32_t tw[4];
 #pragma unroll
         for (int v = 0; v < NV * P; v++) { ti[v] = ise[v]; tuv[v] = uv[v]; }
 #pragma unroll
@@ -130,7 +143,7 @@ template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 if (!((m >> i) & 1)) continue;
-                float f = (float)zt_weight_unquant(w[i], WB) / 64.0f, g = 1 - f;
+                float f = (float)zt_weight_unquant(wbyte(w, i), WB) / 64.0f, g = 1 - f;
                 aa += g * g; ab += g * f; bb += f * f;
 #pragma unroll
                 for (int c = 0; c < NC; c++) { xa[c] += g * chanf(px[i], c); xb[c] += f * chanf(px[i], c); }
@@ -143,15 +156,15 @@ template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12
                 if (c < NC) { nlo[c] = (xa[c] * bb - xb[c] * ab) / det; nhi[c] = (xb[c] * aa - xa[c] * ab) / det; }
                 else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
             }
-            quantise<K>(nlo, nhi, ti + k * NV, tuv + k * NV);
+            quantise<K>(nlo, nhi, ti + k * NV, tuv + k * NV, T);
         }
-        int e = fit_weights<K>(px, tuv, pm, tw);
+        int e = fit_weights<K>(px, spp, tuv, pm, tw);
         if (e >= err) break;
         err = e;
 #pragma unroll
         for (int v = 0; v < NV * P; v++) { ise[v] = ti[v]; uv[v] = tuv[v]; }
 #pragma unroll
-        for (int i = 0; i < 16; i++) w[i] = tw[i];
+        for (int q = 0; q < 4; q++) w[q] = tw[q];
     }
     return err;
 }
@@ -179,7 +192,7 @@ template <int K, int N> ZT_DEV void ise_put(Bits128 &b, int pos, const int *v) {
     }
 }
 
-template <int K> ZT_DEV void pack(int seed, const int *ise, const int w[16], uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:1246-1260
+template <int K> ZT_DEV void pack(int seed, const int *ise, const uint32_t w[4], uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:1246-1260
     constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NV = CEM == 8 ? 6 : 8;
     constexpr int R = WB == 2 ? 4 : WB == 3 ? 7 : WB == 4 ? 4 : 7, H = WB >= 4 ? 1 : 0;
     constexpr uint32_t mode = (R >> 1 & 1) | (R >> 2 & 1) << 1 | (R & 1) << 4 | 2 << 5 | H << 9;
@@ -190,7 +203,7 @@ template <int K> ZT_DEV void pack(int seed, const int *ise, const int w[16], uin
     ise_put<K, NV * P>(b, pos, ise);
     Bits128 wb{ 0, 0 };
 #pragma unroll
-    for (int i = 0; i < 16; i++) wb.put(i * WB, (uint64_t)w[i], WB);
+    for (int i = 0; i < 16; i++) wb.put(i * WB, (uint64_t)wbyte(w, i), WB);
     // weight bit i lands at block bit 127 - i
     olo = b.lo | __brevll(wb.hi);
     ohi = b.hi | __brevll(wb.lo);
@@ -235,36 +248,32 @@ ZT_DEV uint32_t two_means(const uint32_t px[16]) {  // zap_tex.h:1263-1283
     return m;
 }
 
-template <int K> ZT_DEV void try_cfg(const uint32_t px[16], uint32_t pm, int seed, int &best, int &bc, int &bseed, int bi[12], int bw[16]) {
-    int ise[12], w[16];
-    int e = trial<K>(px, pm, ise, w);
+// one configuration; an improvement is packed at once, so only the best block's 16 bytes stay live
+template <int K> ZT_DEV void try_cfg(const uint32_t px[16], uint32_t pm, int seed, int &best, uint64_t &blo, uint64_t &bhi, const AstcLds &T) {
+    int ise[12];
+    uint32_t w[4];
+    int e = trial<K>(px, pm, ise, w, T);
     if (e < best) {
-        best = e; bc = K; bseed = seed;
-#pragma unroll
-        for (int v = 0; v < 12; v++) bi[v] = ise[v];
-#pragma unroll
-        for (int i = 0; i < 16; i++) bw[i] = w[i];
+        best = e;
+        pack<K>(seed, ise, w, blo, bhi);
     }
 }
 
-ZT_DEV void encode_block(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {  // zap_tex.h:1318-1340
+ZT_DEV void encode_block(const uint32_t px[16], const AstcLds &T, uint64_t &lo, uint64_t &hi) {  // zap_tex.h:1318-1340
     uint32_t amin = 255;
 #pragma unroll
     for (int i = 0; i < 16; i++) amin = min(amin, px[i] >> 24);
     const bool opaque = amin == 255;
-    int best = 1 << 30, bc = 0, bseed = 0, bi[12], bw[16];
-#pragma unroll
-    for (int v = 0; v < 12; v++) bi[v] = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) bw[i] = 0;
+    int best = 1 << 30;
+    lo = 0; hi = 0;
     if (opaque) {
-        try_cfg<0>(px, 0, 0, best, bc, bseed, bi, bw);
-        try_cfg<1>(px, 0, 0, best, bc, bseed, bi, bw);
-        try_cfg<2>(px, 0, 0, best, bc, bseed, bi, bw);
+        try_cfg<0>(px, 0, 0, best, lo, hi, T);
+        try_cfg<1>(px, 0, 0, best, lo, hi, T);
+        try_cfg<2>(px, 0, 0, best, lo, hi, T);
     } else {
-        try_cfg<3>(px, 0, 0, best, bc, bseed, bi, bw);
-        try_cfg<4>(px, 0, 0, best, bc, bseed, bi, bw);
-        try_cfg<5>(px, 0, 0, best, bc, bseed, bi, bw);
+        try_cfg<3>(px, 0, 0, best, lo, hi, T);
+        try_cfg<4>(px, 0, 0, best, lo, hi, T);
+        try_cfg<5>(px, 0, 0, best, lo, hi, T);
     }
     if (opaque && best > 16 * 3 * 4) {
         uint32_t km = two_means(px);
@@ -284,19 +293,9 @@ ZT_DEV void encode_block(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {  /
             if (cand[j] < 0) break;
             uint32_t pm = g_astc.pmask[cand[j]];
             int seed = g_astc.pseed[cand[j]];
-            try_cfg<6>(px, pm, seed, best, bc, bseed, bi, bw);
-            try_cfg<7>(px, pm, seed, best, bc, bseed, bi, bw);
+            try_cfg<6>(px, pm, seed, best, lo, hi, T);
+            try_cfg<7>(px, pm, seed, best, lo, hi, T);
         }
-    }
-    switch (bc) {
-    case 0: pack<0>(bseed, bi, bw, lo, hi); break;
-    case 1: pack<1>(bseed, bi, bw, lo, hi); break;
-    case 2: pack<2>(bseed, bi, bw, lo, hi); break;
-    case 3: pack<3>(bseed, bi, bw, lo, hi); break;
-    case 4: pack<4>(bseed, bi, bw, lo, hi); break;
-    case 5: pack<5>(bseed, bi, bw, lo, hi); break;
-    case 6: pack<6>(bseed, bi, bw, lo, hi); break;
-    default: pack<7>(bseed, bi, bw, lo, hi); break;
     }
 }
 
@@ -381,7 +380,15 @@ hipError_t zt_astc_upload_tables(const void *setup, size_t bytes) {
     return hipMemcpyToSymbol(HIP_SYMBOL(g_astc), setup, bytes);
 }
 
-extern "C" __global__ void __launch_bounds__(256) zt_astc_encode(zt_jobs jobs, uint32_t total) {
+// at least 2 waves per SIMD: the fully unrolled trials want more than 256 registers, and the few values spilled to
+// reach 2 waves cost far less than running one wave per SIMD
+extern "C" __global__ void __launch_bounds__(256, 2) zt_astc_encode(zt_jobs jobs, uint32_t total) {
+    __shared__ AstcLds T;
+    for (int i = threadIdx.x; i < 8 * 256 / 4; i += blockDim.x) {
+        ((uint32_t *)T.q)[i] = ((const uint32_t *)g_astc.q)[i];
+        ((uint32_t *)T.unq)[i] = ((const uint32_t *)g_astc.unq)[i];
+    }
+    __syncthreads();
     const uint32_t step = gridDim.x * blockDim.x;
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
         const zt_job J = job_at(jobs, zt_find_job(jobs, g));
@@ -390,7 +397,7 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_encode(zt_jobs jobs, u
         uint32_t px[16];
         gather_block(J.src, J.w, J.h, J.stride, bx, by, px);
         uint64_t lo, hi;
-        encode_block(px, lo, hi);
+        encode_block(px, T, lo, hi);
         store16(J.dst + (uint64_t)b * 16, lo, hi);
     }
 }
