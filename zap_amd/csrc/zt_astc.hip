@@ -1,10 +1,13 @@
-// zt_astc.hip — ASTC 4x4 LDR encoder and decoder for gfx950, one lane per block (zap_tex.h:977-1356).
+// zt_astc.hip — ASTC 4x4 LDR encoder and decoder for gfx950 (zap_tex.h:977-1356).
 //
 // Per block: three 1-partition trial configurations (CEM 8 for opaque blocks, CEM 12 otherwise), each a PCA
 // start, quantisation through the configuration's 8-bit -> ISE map, a weight argmin and <= 3 least-squares
 // refinements; opaque blocks with SSE > 192 then try the 4 stored 2-partition patterns nearest to a 2-means
 // split.  Configurations are compile-time (their colour ranges are fixed by the spec rule, checked at set-up),
-// so every per-texel loop unrolls into registers.
+// so every per-texel loop unrolls into registers.  The encoder runs in stages (zt_kernels.h): one kernel per
+// configuration over a list of the blocks that try it (one lane per block or per (block, pattern)), so each
+// kernel holds one trial's registers and no wave mixes opaque and translucent blocks; picks replay the
+// reference's comparison order.
 #include "zt_astc_setup.h"
 #include "zt_common.h"
 #include "zt_kernels.h"
@@ -129,6 +132,7 @@ template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12
         quantise<K>(lo, hi, ise + k * NV, uv + k * NV, T);
     }
     int err = fit_weights<K>(px, spp, uv, pm, w);
+#pragma unroll 1
     for (int it = 0; it < 3 && err > 0; it++) {
         int ti[12], tuv[12];
         uint32_t tw[4];
@@ -248,55 +252,23 @@ ZT_DEV uint32_t two_means(const uint32_t px[16]) {  // zap_tex.h:1263-1283
     return m;
 }
 
-// one configuration; an improvement is packed at once, so only the best block's 16 bytes stay live
-template <int K> ZT_DEV void try_cfg(const uint32_t px[16], uint32_t pm, int seed, int &best, uint64_t &blo, uint64_t &bhi, const AstcLds &T) {
-    int ise[12];
-    uint32_t w[4];
-    int e = trial<K>(px, pm, ise, w, T);
-    if (e < best) {
-        best = e;
-        pack<K>(seed, ise, w, blo, bhi);
-    }
-}
-
-ZT_DEV void encode_block(const uint32_t px[16], const AstcLds &T, uint64_t &lo, uint64_t &hi) {  // zap_tex.h:1318-1340
-    uint32_t amin = 255;
-#pragma unroll
-    for (int i = 0; i < 16; i++) amin = min(amin, px[i] >> 24);
-    const bool opaque = amin == 255;
-    int best = 1 << 30;
-    lo = 0; hi = 0;
-    if (opaque) {
-        try_cfg<0>(px, 0, 0, best, lo, hi, T);
-        try_cfg<1>(px, 0, 0, best, lo, hi, T);
-        try_cfg<2>(px, 0, 0, best, lo, hi, T);
-    } else {
-        try_cfg<3>(px, 0, 0, best, lo, hi, T);
-        try_cfg<4>(px, 0, 0, best, lo, hi, T);
-        try_cfg<5>(px, 0, 0, best, lo, hi, T);
-    }
-    if (opaque && best > 16 * 3 * 4) {
-        uint32_t km = two_means(px);
-        int cand[4] = { -1, -1, -1, -1 }, cd[4] = { 99, 99, 99, 99 };
-        const int np = g_astc.np;
-        for (int pi = 0; pi < np && cd[3]; pi++) {  // stops once 4 exact matches are found
-            int dd = __popc((uint32_t)g_astc.pmask[pi] ^ km), di = 16 - dd, dist = dd < di ? dd : di;
-            if (dist < cd[3]) {  // insertion into the sorted 4-list (strict <: earlier patterns stay first)
-                if (dist < cd[0]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = cd[0]; cand[1] = cand[0]; cd[0] = dist; cand[0] = pi; }
-                else if (dist < cd[1]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = dist; cand[1] = pi; }
-                else if (dist < cd[2]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = dist; cand[2] = pi; }
-                else { cd[3] = dist; cand[3] = pi; }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (cand[j] < 0) break;
-            uint32_t pm = g_astc.pmask[cand[j]];
-            int seed = g_astc.pseed[cand[j]];
-            try_cfg<6>(px, pm, seed, best, lo, hi, T);
-            try_cfg<7>(px, pm, seed, best, lo, hi, T);
+// the 4 stored 2-partition patterns nearest (Hamming, either labelling) to the block's 2-means split
+// (zap_tex.h:1328-1336): mask | seed << 16 each, all ones where fewer than 4 patterns exist
+ZT_DEV void nearest_patterns(const uint32_t px[16], uint32_t out[4]) {
+    const uint32_t km = two_means(px);
+    int cand[4] = { -1, -1, -1, -1 }, cd[4] = { 99, 99, 99, 99 };
+    const int np = g_astc.np;
+    for (int pi = 0; pi < np && cd[3]; pi++) {  // stops once 4 exact matches are found
+        int dd = __popc((uint32_t)g_astc.pmask[pi] ^ km), di = 16 - dd, dist = dd < di ? dd : di;
+        if (dist < cd[3]) {  // insertion into the sorted 4-list (strict <: earlier patterns stay first)
+            if (dist < cd[0]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = cd[0]; cand[1] = cand[0]; cd[0] = dist; cand[0] = pi; }
+            else if (dist < cd[1]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = dist; cand[1] = pi; }
+            else if (dist < cd[2]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = dist; cand[2] = pi; }
+            else { cd[3] = dist; cand[3] = pi; }
         }
     }
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[j] = cand[j] < 0 ? 0xFFFFFFFFu : (uint32_t)g_astc.pmask[cand[j]] | (uint32_t)g_astc.pseed[cand[j]] << 16;
 }
 
 // ---- decoder (zap_tex.h:1135-1168): the encoder's subset plus void-extent blocks; anything else is magenta
@@ -380,25 +352,171 @@ hipError_t zt_astc_upload_tables(const void *setup, size_t bytes) {
     return hipMemcpyToSymbol(HIP_SYMBOL(g_astc), setup, bytes);
 }
 
-// at least 2 waves per SIMD: the fully unrolled trials want more than 256 registers, and the few values spilled to
-// reach 2 waves cost far less than running one wave per SIMD
-extern "C" __global__ void __launch_bounds__(256, 2) zt_astc_encode(zt_jobs jobs, uint32_t total) {
-    __shared__ AstcLds T;
+// ---- the staged encoder (declared with its stages in zt_kernels.h)
+namespace {
+ZT_DEV void stage_tables(AstcLds &T) {
     for (int i = threadIdx.x; i < 8 * 256 / 4; i += blockDim.x) {
         ((uint32_t *)T.q)[i] = ((const uint32_t *)g_astc.q)[i];
         ((uint32_t *)T.unq)[i] = ((const uint32_t *)g_astc.unq)[i];
     }
     __syncthreads();
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
-        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
-        uint32_t b = g - J.first;
-        int bx = (int)(b % (uint32_t)J.bw), by = (int)(b / (uint32_t)J.bw);
+}
+ZT_DEV void load_px(const zt_astc_ws &W, uint32_t g, uint32_t px[16]) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) { const uint4 v = W.px[(uint64_t)g * 4 + r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+}
+// append g to `list` where `pred` holds: one atomic per wave, slots in lane order
+ZT_DEV void wave_append(bool pred, uint32_t g, unsigned *count, uint32_t *list) {
+    const uint64_t m = __ballot(pred);
+    if (!m) return;
+    const int lane = __lane_id(), leader = __ffsll((unsigned long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) list[base + __popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull))] = g;
+}
+template <int K> ZT_DEV void single_body(const zt_astc_ws &W, const AstcLds &T) {
+    const uint32_t n = W.counters[K < 3 ? 0 : 1], step = gridDim.x * blockDim.x;
+    const uint32_t *list = K < 3 ? W.lo : W.lt;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
+        const uint32_t g = list[e];
         uint32_t px[16];
-        gather_block(J.src, J.w, J.h, J.stride, bx, by, px);
+        load_px(W, g, px);
+        int ise[12];
+        uint32_t w[4];
         uint64_t lo, hi;
-        encode_block(px, T, lo, hi);
-        store16(J.dst + (uint64_t)b * 16, lo, hi);
+        const int err = trial<K>(px, 0, ise, w, T);
+        pack<K>(0, ise, w, lo, hi);
+        W.err1[(uint64_t)g * 3 + K % 3] = err;
+        W.blk1[(uint64_t)g * 3 + K % 3] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    }
+}
+template <int K> ZT_DEV void pair_body(const zt_astc_ws &W, const AstcLds &T) {
+    const uint32_t n = W.counters[2] * 4u, step = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint32_t c = W.cand[i], o = (i >> 2) * 8 + (i & 3) * 2 + (K - 6);
+        if (c == 0xFFFFFFFFu) { W.err2[o] = 0x7FFFFFFF; continue; }
+        uint32_t px[16];
+        load_px(W, W.lp[i >> 2], px);
+        int ise[12];
+        uint32_t w[4];
+        uint64_t lo, hi;
+        const int err = trial<K>(px, c & 0xFFFFu, ise, w, T);
+        pack<K>((int)(c >> 16), ise, w, lo, hi);
+        W.err2[o] = err;
+        W.blk2[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    }
+}
+// the 1-partition winner: first smallest error in configuration order
+ZT_DEV int first_best(const zt_astc_ws &W, uint32_t g, int &slot) {
+    int best = W.err1[(uint64_t)g * 3];
+    slot = 0;
+#pragma unroll
+    for (int k = 1; k < 3; k++) { const int e = W.err1[(uint64_t)g * 3 + k]; if (e < best) { best = e; slot = k; } }
+    return best;
+}
+ZT_DEV void store_block(const zt_jobs &jobs, uint32_t g, uint4 v) {
+    const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+    store16(J.dst + (uint64_t)(g - J.first) * 16, (uint64_t)v.x | (uint64_t)v.y << 32, (uint64_t)v.z | (uint64_t)v.w << 32);
+}
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256) zt_astc_classify(zt_jobs jobs, uint32_t total, zt_astc_ws W) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < total; g0 += step) {   // whole waves stay in the loop
+        const uint32_t g = g0 + threadIdx.x;
+        bool opaque = false;
+        const bool valid = g < total;
+        if (valid) {
+            const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+            const uint32_t b = g - J.first;
+            uint32_t px[16];
+            gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+            uint32_t amin = 255;
+#pragma unroll
+            for (int i = 0; i < 16; i++) amin = min(amin, px[i] >> 24);
+            opaque = amin == 255;
+#pragma unroll
+            for (int r = 0; r < 4; r++) W.px[(uint64_t)g * 4 + r] = make_uint4(px[4 * r], px[4 * r + 1], px[4 * r + 2], px[4 * r + 3]);
+        }
+        wave_append(valid && opaque, g, W.counters + 0, W.lo);
+        wave_append(valid && !opaque, g, W.counters + 1, W.lt);
+    }
+}
+
+// one kernel per configuration, so that each gets the registers of its own trial only; at least 4 waves per SIMD
+// (a handful of values spill to reach them)
+#define ZT_ASTC_SINGLE(K)                                                                     \
+    extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_single##K(zt_astc_ws W) {     \
+        __shared__ AstcLds T;                                                                \
+        stage_tables(T);                                                                     \
+        single_body<K>(W, T);                                                                \
+    }
+ZT_ASTC_SINGLE(0)
+ZT_ASTC_SINGLE(1)
+ZT_ASTC_SINGLE(2)
+ZT_ASTC_SINGLE(3)
+ZT_ASTC_SINGLE(4)
+ZT_ASTC_SINGLE(5)
+
+extern "C" __global__ void __launch_bounds__(256) zt_astc_pick1(zt_jobs jobs, uint32_t total, zt_astc_ws W) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < total; g0 += step) {
+        const uint32_t g = g0 + threadIdx.x;
+        bool search = false;
+        if (g < total) {
+            int slot;
+            const int best = first_best(W, g, slot);
+            uint32_t amin = 255;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint4 v = W.px[(uint64_t)g * 4 + r];
+                amin = min(min(amin, v.x >> 24), min(v.y >> 24, min(v.z >> 24, v.w >> 24)));
+            }
+            search = amin == 255 && best > 16 * 3 * 4;   // opaque and not good enough yet (zap_tex.h:1327)
+            if (!search) store_block(jobs, g, W.blk1[(uint64_t)g * 3 + slot]);
+        }
+        wave_append(search, g, W.counters + 2, W.lp);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) zt_astc_search(zt_astc_ws W) {
+    const uint32_t n = W.counters[2], step = gridDim.x * blockDim.x;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
+        uint32_t px[16];
+        load_px(W, W.lp[e], px);
+        uint32_t c[4];
+        nearest_patterns(px, c);
+#pragma unroll
+        for (int j = 0; j < 4; j++) W.cand[(uint64_t)e * 4 + j] = c[j];
+    }
+}
+
+#define ZT_ASTC_PAIR(K)                                                                       \
+    extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_pair##K(zt_astc_ws W) {       \
+        __shared__ AstcLds T;                                                                \
+        stage_tables(T);                                                                     \
+        pair_body<K>(W, T);                                                                  \
+    }
+ZT_ASTC_PAIR(6)
+ZT_ASTC_PAIR(7)
+
+extern "C" __global__ void __launch_bounds__(256) zt_astc_pick2(zt_jobs jobs, zt_astc_ws W) {
+    const uint32_t n = W.counters[2], step = gridDim.x * blockDim.x;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
+        const uint32_t g = W.lp[e];
+        int slot;
+        int best = first_best(W, g, slot);
+        uint4 v = W.blk1[(uint64_t)g * 3 + slot];
+        for (int j = 0; j < 4; j++) {   // the reference's order: pattern j with configuration 6, then 7
+            if (W.cand[(uint64_t)e * 4 + j] == 0xFFFFFFFFu) break;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int err = W.err2[(uint64_t)e * 8 + j * 2 + k];
+                if (err < best) { best = err; v = W.blk2[(uint64_t)e * 8 + j * 2 + k]; }
+            }
+        }
+        store_block(jobs, g, v);
     }
 }
 

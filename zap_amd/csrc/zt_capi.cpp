@@ -164,6 +164,27 @@ size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? 256 + bc7_blocks 
 // an encode workspace: the BC7 lists, then 16 B of BC1 hill-climb state per BC1/BC3/BC4/BC5 block
 size_t ws_climb_offset(uint64_t bc7_blocks) { return (ws_bytes_for(bc7_blocks) + 255) / 256 * 256; }
 size_t ws_total(uint64_t bc7_blocks, uint64_t bc14_blocks) { return ws_climb_offset(bc7_blocks) + bc14_blocks * 16; }
+// then the ASTC stages' regions (zt_astc_ws), 312 B per ASTC block plus 256 B of counters
+size_t ws_astc_offset(uint64_t bc7_blocks, uint64_t bc14_blocks) { return (ws_total(bc7_blocks, bc14_blocks) + 255) / 256 * 256; }
+size_t ws_astc_bytes(uint64_t n) { return n ? 256 + n * 312 : 0; }
+size_t ws_total(uint64_t bc7_blocks, uint64_t bc14_blocks, uint64_t astc_blocks) {
+    return astc_blocks ? ws_astc_offset(bc7_blocks, bc14_blocks) + ws_astc_bytes(astc_blocks) : ws_total(bc7_blocks, bc14_blocks);
+}
+zt_astc_ws astc_ws_in(void *ws, uint64_t n) {
+    uint8_t *p = (uint8_t *)ws;
+    zt_astc_ws W;
+    W.counters = (unsigned *)p; p += 256;
+    W.px = (uint4 *)p; p += n * 64;
+    W.blk1 = (uint4 *)p; p += n * 48;
+    W.blk2 = (uint4 *)p; p += n * 128;
+    W.err1 = (int *)p; p += n * 12;
+    W.err2 = (int *)p; p += n * 32;
+    W.cand = (uint32_t *)p; p += n * 16;
+    W.lo = (uint32_t *)p; p += n * 4;
+    W.lt = (uint32_t *)p; p += n * 4;
+    W.lp = (uint32_t *)p;
+    return W;
+}
 
 zt_bc7_lists lists_in(void *ws, uint64_t n) {
     uint8_t *p = (uint8_t *)ws;
@@ -225,6 +246,23 @@ int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipSt
         hipLaunchKernelGGL(zt_bc7_stage1, dim3(grid_for((uint64_t)tot, 4)), dim3(256), 0, s, J, L, g_m1k); });
 }
 
+// ASTC 4x4 in stages: classify, the 1-partition configurations, pick, pattern search, 2-partition trials, pick
+int enqueue_astc(const zt_jobs &J, uint32_t tot, const zt_astc_ws &W, hipStream_t s) {
+    const int g = grid_for(tot, 256);
+    int rc;
+    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, W.counters, 64); }))) return rc;
+    if ((rc = launch("zt_astc_classify", s, [&] { hipLaunchKernelGGL(zt_astc_classify, dim3(g), dim3(256), 0, s, J, tot, W); }))) return rc;
+    void (*single[6])(zt_astc_ws) = { zt_astc_single0, zt_astc_single1, zt_astc_single2, zt_astc_single3, zt_astc_single4, zt_astc_single5 };
+    for (int k = 0; k < 6; k++)
+        if ((rc = launch("zt_astc_single", s, [&] { hipLaunchKernelGGL(single[k], dim3(g), dim3(256), 0, s, W); }))) return rc;
+    if ((rc = launch("zt_astc_pick1", s, [&] { hipLaunchKernelGGL(zt_astc_pick1, dim3(g), dim3(256), 0, s, J, tot, W); }))) return rc;
+    if ((rc = launch("zt_astc_search", s, [&] { hipLaunchKernelGGL(zt_astc_search, dim3(g), dim3(256), 0, s, W); }))) return rc;
+    const int g4 = grid_for((uint64_t)tot * 4, 256);
+    if ((rc = launch("zt_astc_pair", s, [&] { hipLaunchKernelGGL(zt_astc_pair6, dim3(g4), dim3(256), 0, s, W); }))) return rc;
+    if ((rc = launch("zt_astc_pair", s, [&] { hipLaunchKernelGGL(zt_astc_pair7, dim3(g4), dim3(256), 0, s, W); }))) return rc;
+    return launch("zt_astc_pick2", s, [&] { hipLaunchKernelGGL(zt_astc_pick2, dim3(g), dim3(256), 0, s, J, W); });
+}
+
 // enqueue the encoders for up to ZT_INLINE_JOBS images (already validated)
 int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStream_t s) {
     zt_jobs fam[3];  // 0: BC1-5, 1: BC7, 2: BC6H, ASTC handled per kernel below
@@ -244,16 +282,14 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
         j.first = *t; j.w = im.w; j.h = im.h; j.bw = (im.w + 3) / 4; j.fmt = im.fmt; j.pad = 0;
         *t += nblocks(im);
     }
-    if (ws_bytes < ws_total(tot[1], tot[0]) || (!ws && ws_total(tot[1], tot[0])))
-        return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, ws_total(tot[1], tot[0]));
+    const size_t need = ws_total(tot[1], tot[0], tot_astc);
+    if (ws_bytes < need || (!ws && need)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, need);
     int rc;
     if (tot[0] && (rc = enqueue_bc14(fam[0], tot[0], (uint4 *)((uint8_t *)ws + ws_climb_offset(tot[1])), s))) return rc;
     if (tot_b6 && (rc = launch("zt_bc6h_encode", s, [&] {
             hipLaunchKernelGGL(zt_bc6h_encode, dim3(grid_for(tot_b6, 256)), dim3(256), 0, s, b6, tot_b6); })))
         return rc;
-    if (tot_astc && (rc = launch("zt_astc_encode", s, [&] {
-            hipLaunchKernelGGL(zt_astc_encode, dim3(grid_for(tot_astc, 256)), dim3(256), 0, s, astc, tot_astc); })))
-        return rc;
+    if (tot_astc && (rc = enqueue_astc(astc, tot_astc, astc_ws_in((uint8_t *)ws + ws_astc_offset(tot[1], tot[0]), tot_astc), s))) return rc;
     if (tot[1]) return enqueue_bc7(fam[1], tot[1], ws, ws_bytes, s);
     return ZAPTEX_OK;
 }
@@ -313,10 +349,10 @@ int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, i
     size_t row = (size_t)w * px_bytes, inb = row * (size_t)h;
     zaptex_image im{ nullptr, nullptr, row, w, h, fmt };
     size_t ob = image_out_bytes(im);
-    uint64_t n7 = fmt == ZAPTEX_BC7 ? nblocks(im) : 0, n14 = fmt <= ZAPTEX_BC5 ? nblocks(im) : 0;
+    uint64_t n7 = fmt == ZAPTEX_BC7 ? nblocks(im) : 0, n14 = fmt <= ZAPTEX_BC5 ? nblocks(im) : 0, na = fmt == ZAPTEX_ASTC ? nblocks(im) : 0;
     if (int rc = ensure_buf(st->in, st->in_cap, inb)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, ob)) return rc;
-    if (n7 || n14) if (int rc = ensure_buf(st->ws, st->ws_cap, ws_total(n7, n14))) return rc;
+    if (n7 || n14 || na) if (int rc = ensure_buf(st->ws, st->ws_cap, ws_total(n7, n14, na))) return rc;
     im.src = st->in; im.dst = st->out;
     HIP_TRY(hipMemcpy2DAsync(st->in, row, src, stride, row, (size_t)h, hipMemcpyHostToDevice, st->stream));
     if (int rc = enqueue(&im, 1, st->ws, st->ws_cap, st->stream)) return rc;
@@ -457,12 +493,13 @@ int zaptex_bc_split(const void *bc, size_t n, int fmt, void *out) { return shuff
 int zaptex_bc_merge(const void *planes, size_t n, int fmt, void *out) { return shuffle(planes, n, fmt, out, 0); }
 
 size_t zaptex_workspace_size(const zaptex_image *imgs, int n) {
-    uint64_t n7 = 0, n14 = 0;
+    uint64_t n7 = 0, n14 = 0, na = 0;
     for (int i = 0; i < n; i++) {
         if (imgs[i].fmt == ZAPTEX_BC7) n7 += nblocks(imgs[i]);
+        else if (imgs[i].fmt == ZAPTEX_ASTC) na += nblocks(imgs[i]);
         else if (imgs[i].fmt <= ZAPTEX_BC5) n14 += nblocks(imgs[i]);
     }
-    return ws_total(n7, n14);
+    return ws_total(n7, n14, na);
 }
 
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream) {

@@ -98,7 +98,37 @@ ZT_PROP_DECL(zt_bc7_prop_m4) ZT_PROP_DECL(zt_bc7_prop_m5) ZT_PROP_DECL(zt_bc7_pr
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb);
 __global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min, uint32_t range);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
-__global__ void zt_astc_encode(zt_jobs jobs, uint32_t total);
+// ASTC 4x4 encode in stages (workspace regions, each sized for `total` blocks):
+//   classify   gather each block once (px), list opaque and translucent blocks
+//   single     one launch per 1-partition configuration K (0..2 over the opaque list, 3..5 over the translucent
+//              one): error and packed block per (block, K % 3)
+//   pick1      the reference's first smallest error; blocks that go on to the 2-partition search are listed,
+//              the others are stored
+//   search     2-means split and the 4 stored patterns nearest to it, per listed block
+//   pair       one launch per 2-partition configuration (6, 7) over (listed block, pattern) items
+//   pick2      the first smallest error over the 1-partition winner and the 8 pattern trials, stored
+struct zt_astc_ws {
+    unsigned *counters;     // [0] opaque, [1] translucent, [2] listed for the 2-partition search
+    uint32_t *lo, *lt, *lp; // block lists
+    uint4 *px;              // [block * 4]: 16 pixels
+    int *err1;              // [block * 3]
+    uint4 *blk1;            // [block * 3]
+    uint32_t *cand;         // [listed * 4]: pattern mask | seed << 16 (all ones: none)
+    int *err2;              // [listed * 8]: pattern j, configuration 6 + k at 2j + k
+    uint4 *blk2;            // [listed * 8]
+};
+__global__ void zt_astc_classify(zt_jobs jobs, uint32_t total, zt_astc_ws W);
+__global__ void zt_astc_single0(zt_astc_ws W);
+__global__ void zt_astc_single1(zt_astc_ws W);
+__global__ void zt_astc_single2(zt_astc_ws W);
+__global__ void zt_astc_single3(zt_astc_ws W);
+__global__ void zt_astc_single4(zt_astc_ws W);
+__global__ void zt_astc_single5(zt_astc_ws W);
+__global__ void zt_astc_pick1(zt_jobs jobs, uint32_t total, zt_astc_ws W);
+__global__ void zt_astc_search(zt_astc_ws W);
+__global__ void zt_astc_pair6(zt_astc_ws W);
+__global__ void zt_astc_pair7(zt_astc_ws W);
+__global__ void zt_astc_pick2(zt_jobs jobs, zt_astc_ws W);
 
 __global__ void zt_bc_decode(const uint8_t *bc, int w, int h, int fmt, uint8_t *rgba, uint64_t stride);
 __global__ void zt_bc6h_decode(const uint8_t *bc, int w, int h, uint16_t *rgba, uint64_t stride);
