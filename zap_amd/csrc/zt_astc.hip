@@ -365,21 +365,20 @@ ZT_DEV void load_px(const zt_astc_ws &W, uint32_t g, uint32_t px[16]) {
 #pragma unroll
     for (int r = 0; r < 4; r++) { const uint4 v = W.px[(uint64_t)g * 4 + r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
 }
-// append g to `list` where `pred` holds: one atomic per wave, slots in lane order
-ZT_DEV void wave_append(bool pred, uint32_t g, unsigned *count, uint32_t *list) {
-    const uint64_t m = __ballot(pred);
-    if (!m) return;
-    const int lane = __lane_id(), leader = __ffsll((unsigned long long)m) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
-    if (pred) list[base + __popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull))] = g;
+// the sharded lists' counters: list k's shards start at counters + k * ZT_SHARDS * ZT_SHARD_STRIDE
+ZT_DEV unsigned *list_counts(const zt_astc_ws &W, int k) { return W.counters + k * ZT_SHARDS * ZT_SHARD_STRIDE; }
+// append g to sharded list k where `pred` holds (item0: the wave's first item)
+ZT_DEV void list_append(const zt_astc_ws &W, int k, uint32_t *list, bool pred, uint32_t g, uint32_t item0) {
+    unsigned *c = shard_counter(list_counts(W, k), item0);
+    const int slot = wave_append(pred, c);
+    if (slot >= 0) list[(uint32_t)((c - list_counts(W, k)) / ZT_SHARD_STRIDE) * W.cap + (uint32_t)slot] = g;
 }
 template <int K> ZT_DEV void single_body(const zt_astc_ws &W, const AstcLds &T) {
-    const uint32_t n = W.counters[K < 3 ? 0 : 1], step = gridDim.x * blockDim.x;
+    const unsigned *counts = list_counts(W, K < 3 ? 0 : 1);
+    const uint32_t n = shard_total(counts), step = gridDim.x * blockDim.x;
     const uint32_t *list = K < 3 ? W.lo : W.lt;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
-        const uint32_t g = list[e];
+        const uint32_t g = list[shard_slot(counts, W.cap, e)];
         uint32_t px[16];
         load_px(W, g, px);
         int ise[12];
@@ -392,12 +391,12 @@ template <int K> ZT_DEV void single_body(const zt_astc_ws &W, const AstcLds &T) 
     }
 }
 template <int K> ZT_DEV void pair_body(const zt_astc_ws &W, const AstcLds &T) {
-    const uint32_t n = W.counters[2] * 4u, step = gridDim.x * blockDim.x;
+    const uint32_t n = shard_total(list_counts(W, 2)) * 4u, step = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
         const uint32_t c = W.cand[i], o = (i >> 2) * 8 + (i & 3) * 2 + (K - 6);
         if (c == 0xFFFFFFFFu) { W.err2[o] = 0x7FFFFFFF; continue; }
         uint32_t px[16];
-        load_px(W, W.lp[i >> 2], px);
+        load_px(W, W.lp[shard_slot(list_counts(W, 2), W.cap, i >> 2)], px);
         int ise[12];
         uint32_t w[4];
         uint64_t lo, hi;
@@ -439,8 +438,9 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_classify(zt_jobs jobs,
 #pragma unroll
             for (int r = 0; r < 4; r++) W.px[(uint64_t)g * 4 + r] = make_uint4(px[4 * r], px[4 * r + 1], px[4 * r + 2], px[4 * r + 3]);
         }
-        wave_append(valid && opaque, g, W.counters + 0, W.lo);
-        wave_append(valid && !opaque, g, W.counters + 1, W.lt);
+        const uint32_t item0 = g0 + (threadIdx.x & ~63u);
+        list_append(W, 0, W.lo, valid && opaque, g, item0);
+        list_append(W, 1, W.lt, valid && !opaque, g, item0);
     }
 }
 
@@ -476,15 +476,15 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_pick1(zt_jobs jobs, ui
             search = amin == 255 && best > 16 * 3 * 4;   // opaque and not good enough yet (zap_tex.h:1327)
             if (!search) store_block(jobs, g, W.blk1[(uint64_t)g * 3 + slot]);
         }
-        wave_append(search, g, W.counters + 2, W.lp);
+        list_append(W, 2, W.lp, search, g, g0 + (threadIdx.x & ~63u));
     }
 }
 
 extern "C" __global__ void __launch_bounds__(256) zt_astc_search(zt_astc_ws W) {
-    const uint32_t n = W.counters[2], step = gridDim.x * blockDim.x;
+    const uint32_t n = shard_total(list_counts(W, 2)), step = gridDim.x * blockDim.x;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
         uint32_t px[16];
-        load_px(W, W.lp[e], px);
+        load_px(W, W.lp[shard_slot(list_counts(W, 2), W.cap, e)], px);
         uint32_t c[4];
         nearest_patterns(px, c);
 #pragma unroll
@@ -502,9 +502,9 @@ ZT_ASTC_PAIR(6)
 ZT_ASTC_PAIR(7)
 
 extern "C" __global__ void __launch_bounds__(256) zt_astc_pick2(zt_jobs jobs, zt_astc_ws W) {
-    const uint32_t n = W.counters[2], step = gridDim.x * blockDim.x;
+    const uint32_t n = shard_total(list_counts(W, 2)), step = gridDim.x * blockDim.x;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
-        const uint32_t g = W.lp[e];
+        const uint32_t g = W.lp[shard_slot(list_counts(W, 2), W.cap, e)];
         int slot;
         int best = first_best(W, g, slot);
         uint4 v = W.blk1[(uint64_t)g * 3 + slot];

@@ -164,24 +164,29 @@ size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? 256 + bc7_blocks 
 // an encode workspace: the BC7 lists, then 16 B of BC1 hill-climb state per BC1/BC3/BC4/BC5 block
 size_t ws_climb_offset(uint64_t bc7_blocks) { return (ws_bytes_for(bc7_blocks) + 255) / 256 * 256; }
 size_t ws_total(uint64_t bc7_blocks, uint64_t bc14_blocks) { return ws_climb_offset(bc7_blocks) + bc14_blocks * 16; }
-// then the ASTC stages' regions (zt_astc_ws), 312 B per ASTC block plus 256 B of counters
+// then the ASTC stages' regions (zt_astc_ws): the sharded counters of three lists (ZT_SHARDS lines of 256 B each),
+// 300 B per ASTC block, and the three lists at 4 B per block rounded up to 512 blocks
+constexpr size_t kAstcCounters = 3 * ZT_SHARDS * ZT_SHARD_STRIDE;   // uint32 counters
 size_t ws_astc_offset(uint64_t bc7_blocks, uint64_t bc14_blocks) { return (ws_total(bc7_blocks, bc14_blocks) + 255) / 256 * 256; }
-size_t ws_astc_bytes(uint64_t n) { return n ? 256 + n * 312 : 0; }
+uint64_t astc_list_rows(uint64_t n) { return (n + 511) / 512 * 512; }
+size_t ws_astc_bytes(uint64_t n) { return n ? kAstcCounters * 4 + n * 300 + astc_list_rows(n) * 12 : 0; }
 size_t ws_total(uint64_t bc7_blocks, uint64_t bc14_blocks, uint64_t astc_blocks) {
     return astc_blocks ? ws_astc_offset(bc7_blocks, bc14_blocks) + ws_astc_bytes(astc_blocks) : ws_total(bc7_blocks, bc14_blocks);
 }
 zt_astc_ws astc_ws_in(void *ws, uint64_t n) {
     uint8_t *p = (uint8_t *)ws;
     zt_astc_ws W;
-    W.counters = (unsigned *)p; p += 256;
+    W.counters = (unsigned *)p; p += kAstcCounters * 4;
     W.px = (uint4 *)p; p += n * 64;
     W.blk1 = (uint4 *)p; p += n * 48;
     W.blk2 = (uint4 *)p; p += n * 128;
     W.err1 = (int *)p; p += n * 12;
     W.err2 = (int *)p; p += n * 32;
     W.cand = (uint32_t *)p; p += n * 16;
-    W.lo = (uint32_t *)p; p += n * 4;
-    W.lt = (uint32_t *)p; p += n * 4;
+    const uint64_t r = astc_list_rows(n);
+    W.cap = (uint32_t)(r / ZT_SHARDS);
+    W.lo = (uint32_t *)p; p += r * 4;
+    W.lt = (uint32_t *)p; p += r * 4;
     W.lp = (uint32_t *)p;
     return W;
 }
@@ -250,7 +255,9 @@ int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipSt
 int enqueue_astc(const zt_jobs &J, uint32_t tot, const zt_astc_ws &W, hipStream_t s) {
     const int g = grid_for(tot, 256);
     int rc;
-    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, W.counters, 64); }))) return rc;
+    if ((rc = launch("zt_zero_u32", s, [&] {
+            hipLaunchKernelGGL(zt_zero_u32, dim3((kAstcCounters + 255) / 256), dim3(256), 0, s, W.counters, (int)kAstcCounters); })))
+        return rc;
     if ((rc = launch("zt_astc_classify", s, [&] { hipLaunchKernelGGL(zt_astc_classify, dim3(g), dim3(256), 0, s, J, tot, W); }))) return rc;
     void (*single[6])(zt_astc_ws) = { zt_astc_single0, zt_astc_single1, zt_astc_single2, zt_astc_single3, zt_astc_single4, zt_astc_single5 };
     for (int k = 0; k < 6; k++)

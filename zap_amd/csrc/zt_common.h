@@ -120,6 +120,30 @@ ZT_DEV void store8(uint8_t *dst, uint64_t v) {
     *(uint2 *)dst = t;
 }
 
+// Lists that many waves append to keep one counter per shard, each counter on its own 256-byte line (atomics on
+// one line complete one after another, about 10 ns apart, so a single counter paces a short kernel): the wave
+// holding items [64k, 64k + 64) appends to shard k mod ZT_SHARDS.  Shard s owns slots [s * cap, s * cap + count_s);
+// with one append per lane a shard needs cap = 64 * ceil(items / 512).
+#define ZT_SHARDS 8
+#define ZT_SHARD_STRIDE 64   // counters (uint32) between consecutive shards
+ZT_DEV unsigned *shard_counter(unsigned *counts, uint32_t item) { return counts + ((item >> 6) & (ZT_SHARDS - 1)) * ZT_SHARD_STRIDE; }
+ZT_DEV uint32_t shard_total(const unsigned *counts) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int s = 0; s < ZT_SHARDS; s++) n += counts[s * ZT_SHARD_STRIDE];
+    return n;
+}
+// slot of flat entry e (shards in order)
+ZT_DEV uint32_t shard_slot(const unsigned *counts, uint32_t cap, uint32_t e) {
+#pragma unroll
+    for (int s = 0; s < ZT_SHARDS; s++) {
+        const uint32_t c = counts[s * ZT_SHARD_STRIDE];
+        if (e < c) return (uint32_t)s * cap + e;
+        e -= c;
+    }
+    return 0xFFFFFFFFu;
+}
+
 // wave-level stream compaction: returns this lane's slot in the list (or -1), one atomic per wave
 ZT_DEV int wave_append(bool pred, unsigned *counter) {
     uint64_t mask = __ballot(pred);

@@ -108,8 +108,9 @@ __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
 //   pair       one launch per 2-partition configuration (6, 7) over (listed block, pattern) items
 //   pick2      the first smallest error over the 1-partition winner and the 8 pattern trials, stored
 struct zt_astc_ws {
-    unsigned *counters;     // [0] opaque, [1] translucent, [2] listed for the 2-partition search
-    uint32_t *lo, *lt, *lp; // block lists
+    unsigned *counters;     // sharded list lengths (ZT_SHARDS x ZT_SHARD_STRIDE each): opaque, translucent, listed
+    uint32_t cap;           // slots per shard of each list
+    uint32_t *lo, *lt, *lp; // block lists (opaque, translucent, listed for the 2-partition search)
     uint4 *px;              // [block * 4]: 16 pixels
     int *err1;              // [block * 3]
     uint4 *blk1;            // [block * 3]
