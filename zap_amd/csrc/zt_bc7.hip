@@ -39,8 +39,10 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, ui
             opaque = amin == 255;
             pred = err > 64;
         }
-        int slot = wave_append(pred, &L.counters[0]);
+        unsigned *c = shard_counter(L.counters, base + (threadIdx.x & ~63u));
+        int slot = wave_append(pred, c);
         if (slot >= 0) {
+            slot += (int)((uint32_t)((c - L.counters) / ZT_SHARD_STRIDE) * L.cap5);
             L.meta5[slot] = make_uint4(g, (uint32_t)err, (uint32_t)j, (uint32_t)opaque);
             uint4 *d = L.px5 + (uint64_t)slot * 4;
             d[0] = make_uint4(px[0], px[1], px[2], px[3]); d[1] = make_uint4(px[4], px[5], px[6], px[7]);
@@ -51,7 +53,7 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, ui
 
 // Stage 5: lane = (list-5 entry, rotation). Keeps mode 5 when it beats mode 6; opaque blocks go to list 1.
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L) {
-    const uint32_t n5 = L.counters[0], items = n5 * 4, step = gridDim.x * blockDim.x;
+    const uint32_t n5 = shard_total(L.counters), items = n5 * 4, step = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < items; base += step) {
         uint32_t item = base + threadIdx.x, e = item >> 2;
         int rot = (int)(item & 3);
@@ -62,6 +64,7 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage5(zt_jobs jobs, zt
         int err = 0x3FFFFFF;
         uint4 meta = make_uint4(0, 0, 0, 0);
         if (valid) {
+            e = shard_slot(L.counters, L.cap5, e);
             meta = L.meta5[e];
             const uint4 *s = L.px5 + (uint64_t)e * 4;
 #pragma unroll
@@ -81,18 +84,21 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage5(zt_jobs jobs, zt
             const zt_job J = job_at(jobs, (int)meta.z);
             store16(J.dst + (uint64_t)(meta.x - J.first) * 16, lo, hi);
         }
-        int slot = wave_append(winner && meta.w, &L.counters[1]);
-        if (slot >= 0) L.meta1[slot] = make_uint4(e, (uint32_t)(improved ? e5 : (int)meta.y), 0, 0);
+        unsigned *c = shard_counter(L.counters + ZT_BC7_LIST1, base + (threadIdx.x & ~63u));   // <= 16 appends per wave
+        int slot = wave_append(winner && meta.w, c);
+        if (slot >= 0)
+            L.meta1[(uint32_t)((c - L.counters - ZT_BC7_LIST1) / ZT_SHARD_STRIDE) * L.cap1 + (uint32_t)slot] =
+                make_uint4(e, (uint32_t)(improved ? e5 : (int)meta.y), 0, 0);
     }
 }
 
 // Stage 1: one wave per list-1 entry, lane = partition.
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k) {
-    const uint32_t n1 = L.counters[1];
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t nw = gridDim.x * (blockDim.x >> 6);
     for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < n1; e += nw) {
-        uint4 m = L.meta1[e];
+        uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         uint4 meta = L.meta5[m.x];
         int cur = (int)m.y;
         uint32_t px[16];
@@ -153,11 +159,11 @@ extern "C" __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 //   pick  one lane per entry: first candidate in rank order with the smallest SSE below the current one, packed
 // Capacity: entry e owns cand[8e..8e+7] and items [16e, 16e + 16).
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand) {
-    const uint32_t n1 = L.counters[1];
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t nw = gridDim.x * (blockDim.x >> 6);
     for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < n1; e += nw) {
-        const uint4 m = L.meta1[e];
+        const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         uint32_t px[16];
         const uint4 *s = L.px5 + (uint64_t)m.x * 4;
 #pragma unroll
@@ -186,10 +192,10 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L,
 }
 
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state) {
-    const uint32_t n1 = L.counters[1], items = n1 * (uint32_t)m1k * 2, step = gridDim.x * blockDim.x;
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), items = n1 * (uint32_t)m1k * 2, step = gridDim.x * blockDim.x;
     for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < items; it += step) {
         const uint32_t e = it / (2u * m1k), k = (it >> 1) % (uint32_t)m1k, sb = it & 1;
-        const uint4 m = L.meta1[e];
+        const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         uint32_t px[16];
         const uint4 *s = L.px5 + (uint64_t)m.x * 4;
 #pragma unroll
@@ -212,9 +218,9 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, 
 
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err,
                                                                    const uint4 *sub_state) {
-    const uint32_t n1 = L.counters[1], step = gridDim.x * blockDim.x;
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), step = gridDim.x * blockDim.x;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n1; e += step) {
-        const uint4 m = L.meta1[e];
+        const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         int best = (int)m.y, bk = -1;
         for (int k = 0; k < m1k; k++) {
             const int err = sub_err[(uint64_t)e * 16 + 2 * k] + sub_err[(uint64_t)e * 16 + 2 * k + 1];

@@ -159,8 +159,11 @@ int check_image(const zaptex_image &im) {
     return ZAPTEX_OK;
 }
 
-// workspace layout over the BC7 blocks: [counters 256 B][meta5 16][px5 64][meta1 16][cand1 8][sub_err 64][sub_state 256] B/blk
-size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? 256 + bc7_blocks * 424 : 0; }
+// workspace layout over the BC7 blocks: [sharded counters][meta5 16][px5 64][meta1 16][cand1 8][sub_err 64][sub_state 256] B
+// per block, for the block count rounded up to 512: list 5 takes one append per stage-6 lane (64 * ceil(n / 512)
+// slots per shard), list 1 at most 16 per stage-5 wave of 64 (entry, rotation) lanes (16 * ceil(n / 128) per shard)
+uint64_t bc7_rows(uint64_t n) { return (n + 511) / 512 * 512; }
+size_t ws_bytes_for(uint64_t bc7_blocks) { return bc7_blocks ? ZT_BC7_COUNTERS * 4 + bc7_rows(bc7_blocks) * 424 : 0; }
 // an encode workspace: the BC7 lists, then 16 B of BC1 hill-climb state per BC1/BC3/BC4/BC5 block
 size_t ws_climb_offset(uint64_t bc7_blocks) { return (ws_bytes_for(bc7_blocks) + 255) / 256 * 256; }
 size_t ws_total(uint64_t bc7_blocks, uint64_t bc14_blocks) { return ws_climb_offset(bc7_blocks) + bc14_blocks * 16; }
@@ -191,16 +194,19 @@ zt_astc_ws astc_ws_in(void *ws, uint64_t n) {
     return W;
 }
 
-zt_bc7_lists lists_in(void *ws, uint64_t n) {
-    uint8_t *p = (uint8_t *)ws;
+zt_bc7_lists lists_in(void *ws, uint64_t blocks) {
+    uint8_t *p = (uint8_t *)ws + ZT_BC7_COUNTERS * 4;
+    const uint64_t n = bc7_rows(blocks);
     zt_bc7_lists L;
-    L.counters = (unsigned *)p;
-    L.meta5 = (uint4 *)(p + 256);
-    L.px5 = (uint4 *)(p + 256 + n * 16);
-    L.meta1 = (uint4 *)(p + 256 + n * 80);
-    L.cand1 = p + 256 + n * 96;
-    L.sub_err = (int *)(p + 256 + n * 104);
-    L.sub_state = (uint4 *)(p + 256 + n * 168);
+    L.cap5 = (uint32_t)((blocks + 511) / 512 * 64);
+    L.cap1 = (uint32_t)((blocks + 127) / 128 * 16);
+    L.counters = (unsigned *)ws;
+    L.meta5 = (uint4 *)p;
+    L.px5 = (uint4 *)(p + n * 16);
+    L.meta1 = (uint4 *)(p + n * 80);
+    L.cand1 = p + n * 96;
+    L.sub_err = (int *)(p + n * 104);
+    L.sub_state = (uint4 *)(p + n * 168);
     return L;
 }
 
@@ -228,7 +234,9 @@ int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipSt
     if (ws_bytes < ws_bytes_for(tot) || !ws) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes", ws_bytes, ws_bytes_for(tot));
     zt_bc7_lists L = lists_in(ws, tot);
     int rc;
-    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, L.counters, 64); }))) return rc;
+    if ((rc = launch("zt_zero_u32", s, [&] {
+            hipLaunchKernelGGL(zt_zero_u32, dim3(ZT_BC7_COUNTERS / 256), dim3(256), 0, s, L.counters, ZT_BC7_COUNTERS); })))
+        return rc;
     const int w6 = g_cfg[CFG_S6].wg, w5 = g_cfg[CFG_S5].wg, w1 = g_cfg[CFG_S1].wg;
     if ((rc = launch("zt_bc7_stage6", s, [&] {
             hipLaunchKernelGGL(zt_bc7_stage6, dim3(groups_for(CFG_S6, tot, w6)), dim3(w6), 0, s, J, tot, L); })))

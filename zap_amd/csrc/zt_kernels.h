@@ -7,12 +7,15 @@ struct zt_job;
 struct zt_jobs;
 
 // BC7 compaction lists (device memory, capacity = total blocks of the launch)
+#define ZT_BC7_LIST1 (ZT_SHARDS * ZT_SHARD_STRIDE)
+#define ZT_BC7_COUNTERS (2 * ZT_SHARDS * ZT_SHARD_STRIDE)
 struct zt_bc7_lists {
-    unsigned *counters;  // [0] list-5 length, [1] list-1 length
-    uint4 *meta5;        // (global block, mode-6 SSE, job, opaque)
-    uint4 *px5;          // 4 x uint4 = the block's 16 RGBA8 pixels
-    uint4 *meta1;        // (list-5 entry, current best SSE, 0, 0)
-    uint8_t *cand1;      // 8 ranked partitions per list-1 entry
+    unsigned *counters;  // sharded lengths (ZT_SHARDS x ZT_SHARD_STRIDE): list 5 from 0, list 1 from ZT_BC7_LIST1
+    uint32_t cap5, cap1; // slots per shard
+    uint4 *meta5;        // by list-5 slot: (global block, mode-6 SSE, job, opaque)
+    uint4 *px5;          // by list-5 slot: 4 x uint4 = the block's 16 RGBA8 pixels
+    uint4 *meta1;        // by list-1 slot: (list-5 slot, current best SSE, 0, 0)
+    uint8_t *cand1;      // 8 ranked partitions per list-1 entry (flat entry index, like sub_err and sub_state)
     int *sub_err;        // 16 subset fits per list-1 entry
     uint4 *sub_state;
 };
