@@ -42,8 +42,8 @@ MIP_CHAIN_BYTES = 89_478_484                 # read the 4096^2 base once + write
 # zt_bc7_stage6 on this workload, rocprofv3 --pmc per dispatch (profiles/zt_bc7_stage6_c2_chain.md):
 # FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the counter reads for a plain 256 MiB device copy
 # (FETCH 131,087 KiB -> x2.0, WRITE 262,144 KiB -> x1.0); SQ_INSTS_VALU in wave64 instructions.
-STAGE6_COUNTERS = {"fetch_kib": 43_946.6, "fetch_scale": 2.0, "write_kib": 47_925.4, "write_scale": 1.0,
-                   "sq_insts_valu": 135_875_310}
+STAGE6_COUNTERS = {"fetch_kib": 43_973.4, "fetch_scale": 2.0, "write_kib": 47_247.0, "write_scale": 1.0,
+                   "sq_insts_valu": 136_058_480}
 
 
 def log(*a):
