@@ -105,6 +105,11 @@ int zaptex_mlp_set_params(void *state, const float *host_params, void *stream);
 int zaptex_mlp_set_features(void *state, int mode, void *stream);
 /* logits [batch x 512] for batch blocks of 16 RGBA8 pixels (64 bytes each) */
 int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *logits, void *ws, size_t ws_bytes, void *stream);
+/* The proposal itself, fused (features, three layers and top-k in one kernel, no workspace): cand [n x topk] gets
+ * each block's topk candidate slots of largest logit among the 285 the search uses (largest first, lowest slot on
+ * ties), risk [n] = exp(logit 385) - 1 (0 below 0, 2^20 from 13.9 up).  The logits are bit-identical to
+ * zaptex_mlp_forward's.  topk 1..16. */
+int zaptex_mlp_propose(const void *state, const void *blocks, int n, int topk, uint32_t *cand, float *risk, void *stream);
 /* Training targets (device arrays, one row per block):
  *   labels       uint32 slot per block: one-hot cross-entropy targets (used when temperature <= 0);
  *   deltas       zaptex_bc7_full_dev's 512 uint16 per block (may be NULL without temperature or risk_k);

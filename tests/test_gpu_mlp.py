@@ -38,6 +38,37 @@ def test_forward_matches_oracle(zt, feat):
     assert err < 2e-2, err
 
 
+@pytest.mark.parametrize("feat,topk,risk_shift,B", [(0, 4, 0.0, 1000), (4, 1, 30.0, 37), (3, 16, -30.0, 517), (2, 7, 0.0, 5)])
+def test_propose_matches_forward_topk(zt, feat, topk, risk_shift, B):
+    """The fused proposal kernel equals the three-GEMM forward followed by a top-k over the used slots: same
+    candidates (largest logit first, lowest slot on ties) and risk = expm1 of the risk logit, clamped."""
+    import torch
+    params, blocks, labels, st, tb, tl = _setup(zt, B, 11 + topk, feat)
+    params[mlp_ref.OFF["b3"][0] + mlp_ref.RISK] += risk_shift   # push the risk logit into either clamp
+    st = zt.mlp_state(params=params, features=feat)
+    ws = zt.mlp_workspace(B)
+    z = torch.empty((B, 512), dtype=torch.float32, device="cuda")
+    zt.mlp_forward(st, tb, B, z, ws)
+    cand = torch.full((B, topk), -1, dtype=torch.int32, device="cuda")
+    risk = torch.full((B,), -1.0, dtype=torch.float32, device="cuda")
+    zt.mlp_propose(st, tb, B, topk, cand, risk)
+    torch.cuda.synchronize()
+    zn = z.cpu().numpy()
+    slots = np.nonzero(mlp_ref.VALID)[0]
+    for b in range(B):
+        v = zn[b, slots]
+        order = np.lexsort((slots, -v))[:topk]
+        assert cand[b].cpu().numpy().tolist() == slots[order].tolist(), b
+    r = zn[:, mlp_ref.RISK].astype(np.float64)
+    exp = np.where(r <= 0, 0.0, np.where(r >= 13.9, 1048576.0, np.expm1(r)))
+    got = risk.cpu().numpy().astype(np.float64)
+    assert np.allclose(got, exp, rtol=1e-5, atol=2e-6), np.abs(got - exp).max()
+    if risk_shift > 0:
+        assert (got == 1048576.0).all()
+    if risk_shift < 0:
+        assert (got == 0.0).all()
+
+
 def test_gradients_match_oracle(zt):
     import torch
     B = 512

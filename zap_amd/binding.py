@@ -109,7 +109,7 @@ class Library:
             "zaptex_bc7_full_dev": [C.POINTER(ZapImage), i, vp, vp, vp, sz, vp],
             "zaptex_mlp_init": [vp, C.c_uint32, vp], "zaptex_mlp_set_params": [vp, vp, vp],
             "zaptex_mlp_set_features": [vp, i, vp],
-            "zaptex_mlp_forward": [vp, vp, i, vp, vp, sz, vp],
+            "zaptex_mlp_forward": [vp, vp, i, vp, vp, sz, vp], "zaptex_mlp_propose": [vp, vp, i, i, vp, vp, vp],
             "zaptex_mlp_gradients": [vp, vp, C.POINTER(MlpTargets), i, vp, sz, vp, vp],
             "zaptex_mlp_train_step": [vp, vp, C.POINTER(MlpTargets), i, f, i, vp, sz, vp, vp],
             "zaptex_bc7_gather_dev": [C.POINTER(ZapImage), i, vp, vp],
@@ -311,6 +311,11 @@ class Library:
     def mlp_forward(self, state, blocks, batch: int, logits, ws) -> None:
         self._check(self.lib.zaptex_mlp_forward(state.data_ptr(), blocks.data_ptr(), batch, logits.data_ptr(), ws.data_ptr(),
                                                 ws.numel(), self._stream()), "zaptex_mlp_forward")
+
+    def mlp_propose(self, state, blocks, n: int, topk: int, cand, risk) -> None:
+        """top-k candidate slots [n x topk] (uint32 as int32) and predicted risk [n] of n 64-byte blocks, fused"""
+        self._check(self.lib.zaptex_mlp_propose(state.data_ptr(), blocks.data_ptr(), n, topk, cand.data_ptr(), risk.data_ptr(),
+                                                self._stream()), "zaptex_mlp_propose")
 
     @staticmethod
     def _targets(labels, deltas, temperature: float, risk_k: int, risk_weight: float) -> MlpTargets:

@@ -781,6 +781,25 @@ int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *
     return ZAPTEX_OK;
 }
 
+namespace {
+// the network's top-K proposals and risk for n dense blocks, one persistent launch (a workgroup per CU)
+int enqueue_propose(const MlpState &st, const uint4 *blocks, uint32_t n, int topk, uint32_t *cand, float *risk, hipStream_t s) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    const uint64_t groups = ((uint64_t)n + 63) / 64;   // 4 waves x 16 blocks per workgroup pass
+    zt_propose_args P{ blocks, n, topk, st.w, st.p + oB1, st.p + oB2, st.p + oB3, st.cfg, cand, risk };
+    return launch("zt_mlp_propose", s, [&] {
+        hipLaunchKernelGGL(zt_mlp_propose, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(groups, (uint64_t)cus))), dim3(256), 0, s, P); });
+}
+}  // namespace
+
+int zaptex_mlp_propose(const void *state, const void *blocks, int n, int topk, uint32_t *cand, float *risk, void *stream) {
+    if (!state || !blocks || !cand || !risk || n < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
+    if (topk < 1 || topk > 16) return fail(ZAPTEX_E_ARG, "topk must be 1..16");
+    if (((uintptr_t)blocks) % 16) return fail(ZAPTEX_E_ARG, "blocks must be 16-byte aligned");
+    return enqueue_propose(mlp_view((void *)state), (const uint4 *)blocks, (uint32_t)n, topk, cand, risk, (hipStream_t)stream);
+}
+
 int zaptex_mlp_gradients(void *state, const void *blocks, const zaptex_mlp_targets *targets, int batch, void *ws, size_t ws_bytes, float *stats,
                          void *stream) {
     if (!state || !blocks || !targets || batch < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
@@ -828,11 +847,8 @@ int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *s
 
 // ---- learned-proposal BC7 encode
 namespace {
-constexpr int kChunk = 1 << 17;  // network rows per forward pass
-struct LearnedWs { uint4 *blocks; uint2 *items; uint32_t *cand, *fallback, *key; float *risk; unsigned *counters; void *mlp; };
-size_t learned_ws_bytes(uint64_t total, int K) {
-    return 256 + 256 + total * (64 + 8 * K + 4 * K + 4 + 4 + 4) + 1024 + mlp_ws_bytes((int)std::min<uint64_t>(total, kChunk));
-}
+struct LearnedWs { uint4 *blocks; uint2 *items; uint32_t *cand, *fallback, *key; float *risk; unsigned *counters; };
+size_t learned_ws_bytes(uint64_t total, int K) { return 256 + 256 + total * (64 + 8 * K + 4 * K + 4 + 4 + 4) + 1024; }
 LearnedWs learned_ws(void *ws, uint64_t total, int K) {
     uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
     LearnedWs v;
@@ -842,8 +858,7 @@ LearnedWs learned_ws(void *ws, uint64_t total, int K) {
     v.cand = (uint32_t *)b; b += total * 4 * K;
     v.key = (uint32_t *)b; b += total * 4;
     v.risk = (float *)b; b += total * 4;
-    v.fallback = (uint32_t *)b; b += total * 4;
-    v.mlp = (void *)align_up((uintptr_t)b, 256);
+    v.fallback = (uint32_t *)b;
     return v;
 }
 }  // namespace
@@ -876,14 +891,7 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
         return rc;
     if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, w.blocks); })))
         return rc;
-    for (uint32_t c0 = 0; c0 < total; c0 += kChunk) {
-        const int B = (int)std::min<uint32_t>(kChunk, total - c0);
-        MlpWs mw = mlp_ws(w.mlp, B);
-        if ((rc = mlp_forward(st, w.blocks + (uint64_t)c0 * 4, B, mw, s))) return rc;
-        if ((rc = launch("zt_mlp_topk", s, [&] {
-                hipLaunchKernelGGL(zt_mlp_topk, dim3((B + 3) / 4), dim3(256), 0, s, mw.Z, B, topk, w.cand + (uint64_t)c0 * topk, w.risk + c0); })))
-            return rc;
-    }
+    if ((rc = enqueue_propose(st, w.blocks, total, topk, w.cand, w.risk, s))) return rc;
     for (uint32_t pass = 0; pass < 2; pass++) {
         if ((rc = launch("zt_bc7_bucket", s, [&] {
                 hipLaunchKernelGGL(zt_bc7_bucket, dim3(grid_for(total, 256)), dim3(256), 0, s, (const uint32_t *)w.cand, (const float *)w.risk, total, topk,

@@ -20,6 +20,17 @@ struct zt_bc7_lists {
     uint4 *sub_state;
 };
 
+// proposal network inference (zt_mlp.hip): dense blocks -> top-K candidate slots and the predicted risk per block
+struct zt_propose_args {
+    const uint4 *blocks;   // 64 B per block
+    uint32_t n;
+    int topk;
+    const uint16_t *w;     // bf16 weight copies [out x in]: W1 at 0, W2 at 64 * 128, W3 at 64 * 128 + 128 * 128
+    const float *b1, *b2, *b3;
+    const int *cfg;        // [0] feature normalisation
+    uint32_t *cand;        // [n * topk]
+    float *risk;           // [n]
+};
 // bf16 GEMM (zt_mlp.hip): C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]
 struct zt_gemm_args {
     const uint16_t *A, *B;
@@ -84,6 +95,7 @@ __global__ void zt_mlp_adam(float *p, const float *gr, float *m1, float *m2, int
                             float eps, uint16_t *wbf, const int *bf_index);
 __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t *wbf, const int *bf_index, const float *scale);
 __global__ void zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *risk);
+__global__ void zt_mlp_propose(zt_propose_args P);
 __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index);
 // learned-proposal BC7 (zt_bc7_full.hip)
 __global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);

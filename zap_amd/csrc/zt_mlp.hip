@@ -107,14 +107,8 @@ extern "C" __global__ void __launch_bounds__(256) zt_gemm_bf16(zt_gemm_args g) {
 //   2: (v - m_c) / max(1, R), R the block's largest channel range (max - min)
 //   3: (v - m_c) / sqrt(64 max(1, R))   (contrast-normalised, keeps a monotone cue of the block's contrast)
 //   4: colour as 3 with R over R, G, B only; alpha as (a - 255) / 255 (0 when opaque: modes 0-3 decode alpha 255)
-extern "C" __global__ void zt_mlp_features(const uint8_t *blocks, int n, const int *cfg, uint16_t *X) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const int mode = cfg[0];
-    uint32_t px[16];
-    const uint4 *src = (const uint4 *)(blocks + b * 64);
-#pragma unroll
-    for (int r = 0; r < 4; r++) { uint4 v = src[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+namespace {
+ZT_DEV void block_features(const uint32_t px[16], int mode, uint16_t *x, int x_step) {
     float mean[4], inv = 1.0f / 255.0f;
     int rng = 0;
 #pragma unroll
@@ -131,11 +125,21 @@ extern "C" __global__ void zt_mlp_features(const uint8_t *blocks, int n, const i
     else if (mode >= 3) inv = 1.0f / sqrtf(64.0f * r);
     float ainv = inv;
     if (mode == 4) { mean[3] = 255.0f; ainv = 1.0f / 255.0f; }
-    uint16_t *x = X + b * 64;
 #pragma unroll
     for (int k = 0; k < 16; k++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) x[4 * k + c] = to_bf16(((float)chan(px[k], c) - mean[c]) * (c == 3 ? ainv : inv));
+        for (int c = 0; c < 4; c++) x[(4 * k + c) * x_step] = to_bf16(((float)chan(px[k], c) - mean[c]) * (c == 3 ? ainv : inv));
+}
+}  // namespace
+
+extern "C" __global__ void zt_mlp_features(const uint8_t *blocks, int n, const int *cfg, uint16_t *X) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    uint32_t px[16];
+    const uint4 *src = (const uint4 *)(blocks + b * 64);
+#pragma unroll
+    for (int r = 0; r < 4; r++) { uint4 v = src[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+    block_features(px, cfg[0], X + b * 64, 1);
 }
 
 // softmax cross-entropy over the used candidate slots, one wave per row; dZ = (p - q) / B in bf16 for targets q;
@@ -307,4 +311,132 @@ extern "C" __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, 
     if (i >= n) return;
     const int j = bf_index[i];
     if (j >= 0) wbf[j] = to_bf16(p[i]);
+}
+
+// ---- inference: features, the three layers and the top-K proposals in one persistent kernel ----------------------
+// The weights stay in LDS for the kernel's life: W1, W2 and, of W3, only the rows the proposals read (the 285
+// candidate slots in ascending order, then the risk slot; 2 zero rows pad to 18 MFMA tiles).  Each wave takes
+// 16 blocks at a time: features into its LDS rows, then each layer as 16x16x16 bf16 MFMAs over the same K order
+// and with the same epilogue as zt_gemm_bf16, so logits, candidates and risk are bit-identical to the three GEMMs
+// plus zt_mlp_topk; the logits never leave registers.
+#define ZT_PR_S1 68   // LDS row strides in bf16 (padded against bank conflicts): W1
+#define ZT_PR_S 132   // W2, W3 and the activation rows
+#define ZT_PR_U 288   // W3 rows kept
+namespace {
+// candidate slot (mode * 64 + slot) of kept W3 row u: the used slots in ascending order, then the risk slot
+ZT_DEV int prop_slot(int u) {
+    return u < 16 ? u : u < 80 ? 64 + (u - 16) : u < 144 ? 128 + (u - 80) : u < 208 ? 192 + (u - 144) : u < 216 ? 256 + (u - 208)
+         : u < 220 ? 320 + (u - 216) : u == 220 ? 384 : u < 285 ? 448 + (u - 221) : u == 285 ? ZT_RISK_SLOT : -1;
+}
+ZT_DEV bf16x4_t lds4(const uint16_t *p) { const uint2 t = *(const uint2 *)p; return bf16x4_t{ (short)(t.x & 0xFFFF), (short)(t.x >> 16), (short)(t.y & 0xFFFF), (short)(t.y >> 16) }; }
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256, 1) zt_mlp_propose(zt_propose_args P) {
+    __shared__ uint16_t W1s[128 * ZT_PR_S1];
+    __shared__ uint16_t W2s[128 * ZT_PR_S];
+    __shared__ uint16_t W3s[ZT_PR_U * ZT_PR_S];
+    __shared__ float B1s[128], B2s[128], B3s[ZT_PR_U];
+    __shared__ uint16_t Act[4][16 * ZT_PR_S];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 128 * 16; i += 256) *(uint2 *)(W1s + (i >> 4) * ZT_PR_S1 + 4 * (i & 15)) = *(const uint2 *)(P.w + (i >> 4) * 64 + 4 * (i & 15));
+    for (int i = tid; i < 128 * 32; i += 256) *(uint2 *)(W2s + (i >> 5) * ZT_PR_S + 4 * (i & 31)) = *(const uint2 *)(P.w + 64 * 128 + (i >> 5) * 128 + 4 * (i & 31));
+    for (int i = tid; i < ZT_PR_U * 32; i += 256) {
+        const int u = i >> 5, id = prop_slot(u);
+        *(uint2 *)(W3s + u * ZT_PR_S + 4 * (i & 31)) = id < 0 ? make_uint2(0, 0) : *(const uint2 *)(P.w + 64 * 128 + 128 * 128 + id * 128 + 4 * (i & 31));
+    }
+    for (int i = tid; i < 128; i += 256) { B1s[i] = P.b1[i]; B2s[i] = P.b2[i]; }
+    for (int u = tid; u < ZT_PR_U; u += 256) { const int id = prop_slot(u); B3s[u] = id < 0 ? 0.0f : P.b3[id]; }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6, r = lane & 15, q = lane >> 4, mode = P.cfg[0], K = P.topk;
+    uint16_t *A = Act[wv];
+    const uint32_t ntile = (P.n + 15) / 16;
+    for (uint32_t t0 = blockIdx.x * 4; t0 < ntile; t0 += gridDim.x * 4) {   // same trip count on every wave
+        const uint32_t row0 = (t0 + wv) * 16;
+        if (lane < 16) {
+            uint32_t px[16];
+            const uint32_t row = row0 + lane;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint4 v = row < P.n ? P.blocks[(uint64_t)row * 4 + k] : make_uint4(0, 0, 0, 0);
+                px[4 * k] = v.x; px[4 * k + 1] = v.y; px[4 * k + 2] = v.z; px[4 * k + 3] = v.w;
+            }
+            block_features(px, mode, A + lane * ZT_PR_S, 1);
+        }
+        __syncthreads();
+        // layer 1: 64 -> 128, ReLU
+        f32x4_t h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            h[j] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                h[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds4(A + r * ZT_PR_S + 16 * c + 4 * q), lds4(W1s + (16 * j + r) * ZT_PR_S1 + 16 * c + 4 * q), h[j], 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                float x = h[j][v] + B1s[16 * j + r];
+                if (x < 0.0f) x = 0.0f;
+                A[(4 * q + v) * ZT_PR_S + 16 * j + r] = to_bf16(x);
+            }
+        __syncthreads();
+        // layer 2: 128 -> 128, ReLU
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            h[j] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                h[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds4(A + r * ZT_PR_S + 16 * c + 4 * q), lds4(W2s + (16 * j + r) * ZT_PR_S + 16 * c + 4 * q), h[j], 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                float x = h[j][v] + B2s[16 * j + r];
+                if (x < 0.0f) x = 0.0f;
+                A[(4 * q + v) * ZT_PR_S + 16 * j + r] = to_bf16(x);
+            }
+        __syncthreads();
+        // layer 3 on the kept rows: logits of rows 4q + v, columns 16 u + r
+        f32x4_t z[ZT_PR_U / 16];
+#pragma unroll
+        for (int u = 0; u < ZT_PR_U / 16; u++) {
+            z[u] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                z[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds4(A + r * ZT_PR_S + 16 * c + 4 * q), lds4(W3s + (16 * u + r) * ZT_PR_S + 16 * c + 4 * q), z[u], 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < ZT_PR_U / 16; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) z[u][v] += B3s[16 * u + r];
+        // top-K of each row over the 285 candidate columns: largest logit first, lowest slot on ties (zt_mlp_topk)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const uint32_t row = row0 + 4 * q + v;
+            for (int k = 0; k < K; k++) {
+                float mx = -3.0e38f;
+                int am = 1 << 20;
+#pragma unroll
+                for (int u = 0; u < ZT_PR_U / 16; u++) if (16 * u + r < 285 && z[u][v] > mx) { mx = z[u][v]; am = 16 * u + r; }
+#pragma unroll
+                for (int off = 1; off <= 8; off <<= 1) {
+                    const float om = __shfl_xor(mx, off);
+                    const int oa = __shfl_xor(am, off);
+                    if (om > mx || (om == mx && oa < am)) { mx = om; am = oa; }
+                }
+                if (r == 0 && row < P.n) P.cand[(uint64_t)row * K + k] = am < 285 ? (uint32_t)prop_slot(am) : (uint32_t)am;
+#pragma unroll
+                for (int u = 0; u < ZT_PR_U / 16; u++) if (16 * u + r == am) z[u][v] = -3.0e38f;
+            }
+            if (r == 285 - 16 * 17 && row < P.n) {   // the risk column (row 285 of the kept W3) sits in tile 17
+                const float x = z[17][v];
+                P.risk[row] = x <= 0.0f ? 0.0f : x >= 13.9f ? 1048576.0f : __expf(x) - 1.0f;
+            }
+        }
+        __syncthreads();
+    }
 }
