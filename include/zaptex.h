@@ -110,6 +110,9 @@ int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *
  * ties), risk [n] = exp(logit 385) - 1 (0 below 0, 2^20 from 13.9 up).  The logits are bit-identical to
  * zaptex_mlp_forward's.  topk 1..16. */
 int zaptex_mlp_propose(const void *state, const void *blocks, int n, int topk, uint32_t *cand, float *risk, void *stream);
+/* test hook: the fused kernel's logits of the kept output rows (the 285 candidate slots ascending, the risk slot,
+ * two zero rows) [n x 288] */
+int zaptex_mlp_propose_logits(const void *state, const void *blocks, int n, float *logits, void *stream);
 /* Training targets (device arrays, one row per block):
  *   labels       uint32 slot per block: one-hot cross-entropy targets (used when temperature <= 0);
  *   deltas       zaptex_bc7_full_dev's 512 uint16 per block (may be NULL without temperature or risk_k);

@@ -53,8 +53,16 @@ def test_propose_matches_forward_topk(zt, feat, topk, risk_shift, B):
     risk = torch.full((B,), -1.0, dtype=torch.float32, device="cuda")
     zt.mlp_propose(st, tb, B, topk, cand, risk)
     torch.cuda.synchronize()
+    lg = torch.full((B, 288), float("nan"), dtype=torch.float32, device="cuda")
+    zt.mlp_propose_logits(st, tb, B, lg)
+    torch.cuda.synchronize()
     zn = z.cpu().numpy()
     slots = np.nonzero(mlp_ref.VALID)[0]
+    # the fused kernel's logits: bit-identical to the GEMM path on the kept rows (candidate slots, then the risk slot)
+    kept = zn[:, np.concatenate([slots, [mlp_ref.RISK]])]
+    lgn = lg.cpu().numpy()
+    assert np.array_equal(lgn[:, :286].view(np.uint32), kept.view(np.uint32))
+    assert (lgn[:, 286:] == 0).all()
     for b in range(B):
         v = zn[b, slots]
         order = np.lexsort((slots, -v))[:topk]

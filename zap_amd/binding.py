@@ -110,6 +110,7 @@ class Library:
             "zaptex_mlp_init": [vp, C.c_uint32, vp], "zaptex_mlp_set_params": [vp, vp, vp],
             "zaptex_mlp_set_features": [vp, i, vp],
             "zaptex_mlp_forward": [vp, vp, i, vp, vp, sz, vp], "zaptex_mlp_propose": [vp, vp, i, i, vp, vp, vp],
+            "zaptex_mlp_propose_logits": [vp, vp, i, vp, vp],
             "zaptex_mlp_gradients": [vp, vp, C.POINTER(MlpTargets), i, vp, sz, vp, vp],
             "zaptex_mlp_train_step": [vp, vp, C.POINTER(MlpTargets), i, f, i, vp, sz, vp, vp],
             "zaptex_bc7_gather_dev": [C.POINTER(ZapImage), i, vp, vp],
@@ -316,6 +317,11 @@ class Library:
         """top-k candidate slots [n x topk] (uint32 as int32) and predicted risk [n] of n 64-byte blocks, fused"""
         self._check(self.lib.zaptex_mlp_propose(state.data_ptr(), blocks.data_ptr(), n, topk, cand.data_ptr(), risk.data_ptr(),
                                                 self._stream()), "zaptex_mlp_propose")
+
+    def mlp_propose_logits(self, state, blocks, n: int, logits) -> None:
+        """the fused kernel's logits of the kept rows [n x 288] (test hook)"""
+        self._check(self.lib.zaptex_mlp_propose_logits(state.data_ptr(), blocks.data_ptr(), n, logits.data_ptr(), self._stream()),
+                    "zaptex_mlp_propose_logits")
 
     @staticmethod
     def _targets(labels, deltas, temperature: float, risk_k: int, risk_weight: float) -> MlpTargets:

@@ -782,14 +782,16 @@ int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *
 }
 
 namespace {
-// the network's top-K proposals and risk for n dense blocks, one persistent launch (a workgroup per CU)
-int enqueue_propose(const MlpState &st, const uint4 *blocks, uint32_t n, int topk, uint32_t *cand, float *risk, hipStream_t s) {
+// the network's top-K proposals and risk (and, for tests, the kept logits) for n dense blocks, one persistent launch
+// of 8 waves per CU; a wave takes 32 blocks per pass
+int enqueue_propose(const MlpState &st, const uint4 *blocks, uint32_t n, int topk, uint32_t *cand, float *risk, float *logits, hipStream_t s) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-    const uint64_t groups = ((uint64_t)n + 63) / 64;   // 4 waves x 16 blocks per workgroup pass
-    zt_propose_args P{ blocks, n, topk, st.w, st.p + oB1, st.p + oB2, st.p + oB3, st.cfg, cand, risk };
-    return launch("zt_mlp_propose", s, [&] {
-        hipLaunchKernelGGL(zt_mlp_propose, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(groups, (uint64_t)cus))), dim3(256), 0, s, P); });
+    const uint64_t groups = ((uint64_t)n + 255) / 256;   // 8 waves x 32 blocks per workgroup pass
+    const dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(groups, (uint64_t)cus)));
+    zt_propose_args P{ blocks, n, topk, st.w, st.p + oB1, st.p + oB2, st.p + oB3, st.cfg, cand, risk, logits };
+    if (topk <= 4) return launch("zt_mlp_propose4", s, [&] { hipLaunchKernelGGL(zt_mlp_propose4, grid, dim3(512), 0, s, P); });
+    return launch("zt_mlp_propose16", s, [&] { hipLaunchKernelGGL(zt_mlp_propose16, grid, dim3(512), 0, s, P); });
 }
 }  // namespace
 
@@ -797,7 +799,13 @@ int zaptex_mlp_propose(const void *state, const void *blocks, int n, int topk, u
     if (!state || !blocks || !cand || !risk || n < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
     if (topk < 1 || topk > 16) return fail(ZAPTEX_E_ARG, "topk must be 1..16");
     if (((uintptr_t)blocks) % 16) return fail(ZAPTEX_E_ARG, "blocks must be 16-byte aligned");
-    return enqueue_propose(mlp_view((void *)state), (const uint4 *)blocks, (uint32_t)n, topk, cand, risk, (hipStream_t)stream);
+    return enqueue_propose(mlp_view((void *)state), (const uint4 *)blocks, (uint32_t)n, topk, cand, risk, nullptr, (hipStream_t)stream);
+}
+
+int zaptex_mlp_propose_logits(const void *state, const void *blocks, int n, float *logits, void *stream) {
+    if (!state || !blocks || !logits || n < 1) return fail(ZAPTEX_E_ARG, "bad arguments");
+    if (((uintptr_t)blocks) % 16 || ((uintptr_t)logits) % 16) return fail(ZAPTEX_E_ARG, "blocks and logits must be 16-byte aligned");
+    return enqueue_propose(mlp_view((void *)state), (const uint4 *)blocks, (uint32_t)n, 1, nullptr, nullptr, logits, (hipStream_t)stream);
 }
 
 int zaptex_mlp_gradients(void *state, const void *blocks, const zaptex_mlp_targets *targets, int batch, void *ws, size_t ws_bytes, float *stats,
@@ -891,7 +899,7 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
         return rc;
     if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, w.blocks); })))
         return rc;
-    if ((rc = enqueue_propose(st, w.blocks, total, topk, w.cand, w.risk, s))) return rc;
+    if ((rc = enqueue_propose(st, w.blocks, total, topk, w.cand, w.risk, nullptr, s))) return rc;
     for (uint32_t pass = 0; pass < 2; pass++) {
         if ((rc = launch("zt_bc7_bucket", s, [&] {
                 hipLaunchKernelGGL(zt_bc7_bucket, dim3(grid_for(total, 256)), dim3(256), 0, s, (const uint32_t *)w.cand, (const float *)w.risk, total, topk,

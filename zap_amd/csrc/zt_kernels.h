@@ -28,8 +28,9 @@ struct zt_propose_args {
     const uint16_t *w;     // bf16 weight copies [out x in]: W1 at 0, W2 at 64 * 128, W3 at 64 * 128 + 128 * 128
     const float *b1, *b2, *b3;
     const int *cfg;        // [0] feature normalisation
-    uint32_t *cand;        // [n * topk]
-    float *risk;           // [n]
+    uint32_t *cand;        // [n * topk], or null
+    float *risk;           // [n], or null
+    float *logits;         // [n * 288] logits of the kept W3 rows (the candidate slots ascending, then the risk slot), or null
 };
 // bf16 GEMM (zt_mlp.hip): C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]
 struct zt_gemm_args {
@@ -94,8 +95,8 @@ __global__ void zt_mlp_colsum(const uint16_t *src, int M, int N, float *out);
 __global__ void zt_mlp_adam(float *p, const float *gr, float *m1, float *m2, int n, float lr, float b1, float b2, float c1, float c2,
                             float eps, uint16_t *wbf, const int *bf_index);
 __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t *wbf, const int *bf_index, const float *scale);
-__global__ void zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *risk);
-__global__ void zt_mlp_propose(zt_propose_args P);
+__global__ void zt_mlp_propose4(zt_propose_args P);    // topk <= 4
+__global__ void zt_mlp_propose16(zt_propose_args P);   // topk <= 16
 __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index);
 // learned-proposal BC7 (zt_bc7_full.hip)
 __global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);

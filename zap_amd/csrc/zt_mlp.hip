@@ -108,27 +108,39 @@ extern "C" __global__ void __launch_bounds__(256) zt_gemm_bf16(zt_gemm_args g) {
 //   3: (v - m_c) / sqrt(64 max(1, R))   (contrast-normalised, keeps a monotone cue of the block's contrast)
 //   4: colour as 3 with R over R, G, B only; alpha as (a - 255) / 255 (0 when opaque: modes 0-3 decode alpha 255)
 namespace {
-ZT_DEV void block_features(const uint32_t px[16], int mode, uint16_t *x, int x_step) {
-    float mean[4], inv = 1.0f / 255.0f;
+struct FeatScale { float mean[4], inv, ainv; };
+// the block's normalisation from its channel sums, minima and maxima over the 16 pixels
+ZT_DEV FeatScale feature_scale(const int sum[4], const int lo[4], const int hi[4], int mode) {
+    FeatScale f;
+    f.inv = 1.0f / 255.0f;
     int rng = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        int sum = 0, lo = 255, hi = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) { const int v = chan(px[k], c); sum += v; lo = min(lo, v); hi = max(hi, v); }
-        mean[c] = mode ? (float)sum / 16.0f : 0.0f;
-        if (mode != 4 || c < 3) rng = max(rng, hi - lo);
+        f.mean[c] = mode ? (float)sum[c] / 16.0f : 0.0f;
+        if (mode != 4 || c < 3) rng = max(rng, hi[c] - lo[c]);
     }
     const float r = (float)max(1, rng);
-    if (mode == 1) inv = 1.0f / 64.0f;
-    else if (mode == 2) inv = 1.0f / r;
-    else if (mode >= 3) inv = 1.0f / sqrtf(64.0f * r);
-    float ainv = inv;
-    if (mode == 4) { mean[3] = 255.0f; ainv = 1.0f / 255.0f; }
+    if (mode == 1) f.inv = 1.0f / 64.0f;
+    else if (mode == 2) f.inv = 1.0f / r;
+    else if (mode >= 3) f.inv = 1.0f / sqrtf(64.0f * r);
+    f.ainv = f.inv;
+    if (mode == 4) { f.mean[3] = 255.0f; f.ainv = 1.0f / 255.0f; }
+    return f;
+}
+ZT_DEV uint16_t feature(const FeatScale &f, uint32_t p, int c) { return to_bf16(((float)chan(p, c) - f.mean[c]) * (c == 3 ? f.ainv : f.inv)); }
+ZT_DEV void block_features(const uint32_t px[16], int mode, uint16_t *x, int x_step) {
+    int sum[4], lo[4], hi[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        sum[c] = 0; lo[c] = 255; hi[c] = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) { const int v = chan(px[k], c); sum[c] += v; lo[c] = min(lo[c], v); hi[c] = max(hi[c], v); }
+    }
+    const FeatScale f = feature_scale(sum, lo, hi, mode);
 #pragma unroll
     for (int k = 0; k < 16; k++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) x[(4 * k + c) * x_step] = to_bf16(((float)chan(px[k], c) - mean[c]) * (c == 3 ? ainv : inv));
+        for (int c = 0; c < 4; c++) x[(4 * k + c) * x_step] = feature(f, px[k], c);
 }
 }  // namespace
 
@@ -273,38 +285,6 @@ extern "C" __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t 
     if (j >= 0) wbf[j] = to_bf16(x);
 }
 
-// top-K used candidate slots per row (ties to the lower slot) and the top-1 softmax probability
-// top-K used slots per row (largest logit first, lowest slot on ties) and the row's predicted SSE cost of searching
-// only them, expm1 of the risk slot (clamped to [0, 2^20])
-extern "C" __global__ void __launch_bounds__(256) zt_mlp_topk(const float *Z, int B, int K, uint32_t *cand, float *risk) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= B) return;
-    const float *z = Z + (int64_t)row * 512;
-    float v[8];
-#pragma unroll
-    for (int t = 0; t < 8; t++) v[t] = cand_valid(lane + 64 * t) ? z[lane + 64 * t] : -3.0e38f;
-    for (int k = 0; k < K; k++) {
-        float mx = -3.0e38f;
-        int am = 1 << 20;
-#pragma unroll
-        for (int t = 0; t < 8; t++) if (v[t] > mx) { mx = v[t]; am = lane + 64 * t; }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            float om = __shfl_xor(mx, off);
-            int oa = __shfl_xor(am, off);
-            if (om > mx || (om == mx && oa < am)) { mx = om; am = oa; }
-        }
-        if (lane == 0) cand[(int64_t)row * K + k] = (uint32_t)am;
-#pragma unroll
-        for (int t = 0; t < 8; t++) if (lane + 64 * t == am) v[t] = -3.0e38f;
-    }
-    if (lane == 0) {
-        const float r = z[ZT_RISK_SLOT];
-        risk[row] = r <= 0.0f ? 0.0f : r >= 13.9f ? 1048576.0f : __expf(r) - 1.0f;
-    }
-}
-
 // re-round the bf16 weight copies from the fp32 master weights (after loading parameters)
 extern "C" __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -313,130 +293,210 @@ extern "C" __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, 
     if (j >= 0) wbf[j] = to_bf16(p[i]);
 }
 
-// ---- inference: features, the three layers and the top-K proposals in one persistent kernel ----------------------
-// The weights stay in LDS for the kernel's life: W1, W2 and, of W3, only the rows the proposals read (the 285
-// candidate slots in ascending order, then the risk slot; 2 zero rows pad to 18 MFMA tiles).  Each wave takes
-// 16 blocks at a time: features into its LDS rows, then each layer as 16x16x16 bf16 MFMAs over the same K order
-// and with the same epilogue as zt_gemm_bf16, so logits, candidates and risk are bit-identical to the three GEMMs
-// plus zt_mlp_topk; the logits never leave registers.
-#define ZT_PR_S1 68   // LDS row strides in bf16 (padded against bank conflicts): W1
-#define ZT_PR_S 132   // W2, W3 and the activation rows
+// ---- inference: features, the three layers and the top-k proposals in one persistent kernel ----------------------
+// Each layer runs transposed, H^T = W X^T: the weights are the MFMA's A operand (from LDS), the activations its B
+// operand.  In the 16x16x16 layout the accumulator of output tile j (lane (r, q) holds neurons 16 j + 4 q + i of
+// block r) is then, after bias, ReLU and bf16 rounding, exactly the B operand of the next layer's K step j, so
+// activations never leave registers.  The products, their grouping into 16-term MFMA steps and the K order are
+// those of zt_gemm_bf16 (only the roles of A and B swap), and the epilogue is the same, so the logits are
+// bit-identical to the three GEMMs.
+// The weights stay in LDS for the kernel's life in fragment order (each wave reads 64 consecutive 16-byte operand
+// pairs: no bank conflicts): W1, W2 and, of W3, only the rows the proposals read (the 285 candidate slots in
+// ascending order, then the risk slot; 2 zero rows pad to 18 MFMA tiles).  A wave takes 32 blocks per pass, two
+// 16-block column tiles sharing every weight fragment.  Layer-3 logits stream into a sorted top-KT list per lane
+// (lane (r, q) sees kept rows 16 t + 4 q + i in ascending order, so a strict > keeps the lowest slot first among
+// equal logits); the four lanes of a block then merge their lists K times by (largest logit, lowest slot) -- the
+// order of a top-k over all 285 slots with ties to the lower slot.
 #define ZT_PR_U 288   // W3 rows kept
+#define ZT_PR_NC 2    // 16-block column tiles per wave pass
 namespace {
 // candidate slot (mode * 64 + slot) of kept W3 row u: the used slots in ascending order, then the risk slot
 ZT_DEV int prop_slot(int u) {
     return u < 16 ? u : u < 80 ? 64 + (u - 16) : u < 144 ? 128 + (u - 80) : u < 208 ? 192 + (u - 144) : u < 216 ? 256 + (u - 208)
          : u < 220 ? 320 + (u - 216) : u == 220 ? 384 : u < 285 ? 448 + (u - 221) : u == 285 ? ZT_RISK_SLOT : -1;
 }
-ZT_DEV bf16x4_t lds4(const uint16_t *p) { const uint2 t = *(const uint2 *)p; return bf16x4_t{ (short)(t.x & 0xFFFF), (short)(t.x >> 16), (short)(t.y & 0xFFFF), (short)(t.y >> 16) }; }
-}  // namespace
-
-extern "C" __global__ void __launch_bounds__(256, 1) zt_mlp_propose(zt_propose_args P) {
-    __shared__ uint16_t W1s[128 * ZT_PR_S1];
-    __shared__ uint16_t W2s[128 * ZT_PR_S];
-    __shared__ uint16_t W3s[ZT_PR_U * ZT_PR_S];
-    __shared__ float B1s[128], B2s[128], B3s[ZT_PR_U];
-    __shared__ uint16_t Act[4][16 * ZT_PR_S];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < 128 * 16; i += 256) *(uint2 *)(W1s + (i >> 4) * ZT_PR_S1 + 4 * (i & 15)) = *(const uint2 *)(P.w + (i >> 4) * 64 + 4 * (i & 15));
-    for (int i = tid; i < 128 * 32; i += 256) *(uint2 *)(W2s + (i >> 5) * ZT_PR_S + 4 * (i & 31)) = *(const uint2 *)(P.w + 64 * 128 + (i >> 5) * 128 + 4 * (i & 31));
-    for (int i = tid; i < ZT_PR_U * 32; i += 256) {
-        const int u = i >> 5, id = prop_slot(u);
-        *(uint2 *)(W3s + u * ZT_PR_S + 4 * (i & 31)) = id < 0 ? make_uint2(0, 0) : *(const uint2 *)(P.w + 64 * 128 + 128 * 128 + id * 128 + 4 * (i & 31));
-    }
-    for (int i = tid; i < 128; i += 256) { B1s[i] = P.b1[i]; B2s[i] = P.b2[i]; }
-    for (int u = tid; u < ZT_PR_U; u += 256) { const int id = prop_slot(u); B3s[u] = id < 0 ? 0.0f : P.b3[id]; }
-    __syncthreads();
-    const int lane = tid & 63, wv = tid >> 6, r = lane & 15, q = lane >> 4, mode = P.cfg[0], K = P.topk;
-    uint16_t *A = Act[wv];
-    const uint32_t ntile = (P.n + 15) / 16;
-    for (uint32_t t0 = blockIdx.x * 4; t0 < ntile; t0 += gridDim.x * 4) {   // same trip count on every wave
-        const uint32_t row0 = (t0 + wv) * 16;
-        if (lane < 16) {
-            uint32_t px[16];
-            const uint32_t row = row0 + lane;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint4 v = row < P.n ? P.blocks[(uint64_t)row * 4 + k] : make_uint4(0, 0, 0, 0);
-                px[4 * k] = v.x; px[4 * k + 1] = v.y; px[4 * k + 2] = v.z; px[4 * k + 3] = v.w;
-            }
-            block_features(px, mode, A + lane * ZT_PR_S, 1);
+ZT_DEV bf16x4_t bf4(uint32_t lo, uint32_t hi) { return bf16x4_t{ (short)(lo & 0xFFFF), (short)(lo >> 16), (short)(hi & 0xFFFF), (short)(hi >> 16) }; }
+// weights [rows x IN] (bf16, row-major) -> LDS fragments: for row tile j, K-step pair p and lane l, 16 bytes =
+// A operands of K steps 2p and 2p + 1 (row 16 j + l % 16, columns 16 (2p + s) + 4 (l / 16) + 0..3)
+template <int IN>
+ZT_DEV void stage_fragments(uint4 *dst, const uint16_t *w, int rows_kept, int tid, int nthreads, bool w3) {
+    constexpr int CP = IN / 32;
+    for (int e = tid; e < rows_kept / 16 * CP * 64; e += nthreads) {
+        const int l = e & 63, p = (e >> 6) % CP, j = (e >> 6) / CP;
+        const int u = 16 * j + (l & 15), row = w3 ? prop_slot(u) : u, col = 32 * p + 4 * (l >> 4);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (row >= 0) {
+            const uint2 a = *(const uint2 *)(w + (int64_t)row * IN + col), b = *(const uint2 *)(w + (int64_t)row * IN + col + 16);
+            v = make_uint4(a.x, a.y, b.x, b.y);
         }
-        __syncthreads();
-        // layer 1: 64 -> 128, ReLU
-        f32x4_t h[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            h[j] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                h[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds4(A + r * ZT_PR_S + 16 * c + 4 * q), lds4(W1s + (16 * j + r) * ZT_PR_S1 + 16 * c + 4 * q), h[j], 0, 0, 0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                float x = h[j][v] + B1s[16 * j + r];
-                if (x < 0.0f) x = 0.0f;
-                A[(4 * q + v) * ZT_PR_S + 16 * j + r] = to_bf16(x);
-            }
-        __syncthreads();
-        // layer 2: 128 -> 128, ReLU
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            h[j] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-            for (int c = 0; c < 8; c++)
-                h[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds4(A + r * ZT_PR_S + 16 * c + 4 * q), lds4(W2s + (16 * j + r) * ZT_PR_S + 16 * c + 4 * q), h[j], 0, 0, 0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                float x = h[j][v] + B2s[16 * j + r];
-                if (x < 0.0f) x = 0.0f;
-                A[(4 * q + v) * ZT_PR_S + 16 * j + r] = to_bf16(x);
-            }
-        __syncthreads();
-        // layer 3 on the kept rows: logits of rows 4q + v, columns 16 u + r
-        f32x4_t z[ZT_PR_U / 16];
-#pragma unroll
-        for (int u = 0; u < ZT_PR_U / 16; u++) {
-            z[u] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-            for (int c = 0; c < 8; c++)
-                z[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds4(A + r * ZT_PR_S + 16 * c + 4 * q), lds4(W3s + (16 * u + r) * ZT_PR_S + 16 * c + 4 * q), z[u], 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < ZT_PR_U / 16; u++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) z[u][v] += B3s[16 * u + r];
-        // top-K of each row over the 285 candidate columns: largest logit first, lowest slot on ties (zt_mlp_topk)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const uint32_t row = row0 + 4 * q + v;
-            for (int k = 0; k < K; k++) {
-                float mx = -3.0e38f;
-                int am = 1 << 20;
-#pragma unroll
-                for (int u = 0; u < ZT_PR_U / 16; u++) if (16 * u + r < 285 && z[u][v] > mx) { mx = z[u][v]; am = 16 * u + r; }
-#pragma unroll
-                for (int off = 1; off <= 8; off <<= 1) {
-                    const float om = __shfl_xor(mx, off);
-                    const int oa = __shfl_xor(am, off);
-                    if (om > mx || (om == mx && oa < am)) { mx = om; am = oa; }
-                }
-                if (r == 0 && row < P.n) P.cand[(uint64_t)row * K + k] = am < 285 ? (uint32_t)prop_slot(am) : (uint32_t)am;
-#pragma unroll
-                for (int u = 0; u < ZT_PR_U / 16; u++) if (16 * u + r == am) z[u][v] = -3.0e38f;
-            }
-            if (r == 285 - 16 * 17 && row < P.n) {   // the risk column (row 285 of the kept W3) sits in tile 17
-                const float x = z[17][v];
-                P.risk[row] = x <= 0.0f ? 0.0f : x >= 13.9f ? 1048576.0f : __expf(x) - 1.0f;
-            }
-        }
-        __syncthreads();
+        dst[e] = v;
     }
 }
+// bias, optional ReLU and bf16 rounding of an accumulator tile (zt_gemm_bf16's epilogue) -> next layer's B operand
+ZT_DEV bf16x4_t epilogue(f32x4_t d, const float *bias) {
+    const float4 b = *(const float4 *)bias;
+    float x0 = d[0] + b.x, x1 = d[1] + b.y, x2 = d[2] + b.z, x3 = d[3] + b.w;
+    if (x0 < 0.0f) x0 = 0.0f;
+    if (x1 < 0.0f) x1 = 0.0f;
+    if (x2 < 0.0f) x2 = 0.0f;
+    if (x3 < 0.0f) x3 = 0.0f;
+    return bf16x4_t{ (short)to_bf16(x0), (short)to_bf16(x1), (short)to_bf16(x2), (short)to_bf16(x3) };
+}
+// one layer, NJ output row tiles over NK K steps: out[ct][j] = epilogue(sum_k W[j][k] in[ct][k])
+template <int NJ, int NK>
+ZT_DEV void layer(const uint4 *Wf, const float *bias, const bf16x4_t (&in)[ZT_PR_NC][NK], bf16x4_t (&out)[ZT_PR_NC][NJ], int lane, int q) {
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        f32x4_t acc[ZT_PR_NC];
+#pragma unroll
+        for (int ct = 0; ct < ZT_PR_NC; ct++) acc[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int p = 0; p < NK / 2; p++) {
+            const uint4 a = Wf[(j * (NK / 2) + p) * 64 + lane];
+#pragma unroll
+            for (int ct = 0; ct < ZT_PR_NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), in[ct][2 * p], acc[ct], 0, 0, 0);
+#pragma unroll
+            for (int ct = 0; ct < ZT_PR_NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), in[ct][2 * p + 1], acc[ct], 0, 0, 0);
+        }
+#pragma unroll
+        for (int ct = 0; ct < ZT_PR_NC; ct++) out[ct][j] = epilogue(acc[ct], bias + 16 * j + 4 * q);
+    }
+}
+// sorted (largest first) top-KT of a stream of (logit, row) in ascending row order; ties keep the earlier row first
+template <int KT>
+struct TopList {
+    float t[KT];
+    int u[KT];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int i = 0; i < KT; i++) { t[i] = -INFINITY; u[i] = 0x7FFFFFFF; }
+    }
+    __device__ __forceinline__ void insert(float x, int ux) {
+        bool c[KT];
+#pragma unroll
+        for (int i = 0; i < KT; i++) c[i] = x > t[i];
+#pragma unroll
+        for (int i = KT - 1; i > 0; i--) {
+            u[i] = c[i - 1] ? u[i - 1] : c[i] ? ux : u[i];
+            t[i] = c[i - 1] ? t[i - 1] : c[i] ? x : t[i];
+        }
+        u[0] = c[0] ? ux : u[0];
+        t[0] = c[0] ? x : t[0];
+    }
+    __device__ __forceinline__ void pop_if(bool pop) {
+#pragma unroll
+        for (int i = 0; i + 1 < KT; i++) { t[i] = pop ? t[i + 1] : t[i]; u[i] = pop ? u[i + 1] : u[i]; }
+        t[KT - 1] = pop ? -INFINITY : t[KT - 1];
+        u[KT - 1] = pop ? 0x7FFFFFFF : u[KT - 1];
+    }
+};
+
+template <int KT>
+ZT_DEV void propose_body(const zt_propose_args &P) {
+    __shared__ uint4 W1f[128 / 16 * 2 * 64];       // 16 KiB
+    __shared__ uint4 W2f[128 / 16 * 4 * 64];       // 32 KiB
+    __shared__ uint4 W3f[ZT_PR_U / 16 * 4 * 64];   // 72 KiB
+    __shared__ __align__(16) float B1s[128], B2s[128], B3s[ZT_PR_U];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    stage_fragments<64>(W1f, P.w, 128, tid, nth, false);
+    stage_fragments<128>(W2f, P.w + 64 * 128, 128, tid, nth, false);
+    stage_fragments<128>(W3f, P.w + 64 * 128 + 128 * 128, ZT_PR_U, tid, nth, true);
+    for (int i = tid; i < 128; i += nth) { B1s[i] = P.b1[i]; B2s[i] = P.b2[i]; }
+    for (int u = tid; u < ZT_PR_U; u += nth) { const int id = prop_slot(u); B3s[u] = id < 0 ? 0.0f : P.b3[id]; }
+    __syncthreads();
+    const int lane = tid & 63, r = lane & 15, q = lane >> 4, mode = P.cfg[0], K = P.topk;
+    const uint32_t waves = gridDim.x * (nth >> 6), groups = (P.n + 16 * ZT_PR_NC - 1) / (16 * ZT_PR_NC);
+    for (uint32_t g = blockIdx.x * (nth >> 6) + (tid >> 6); g < groups; g += waves) {
+        // features: lane (r, q) holds pixels q, 4 + q, 8 + q, 12 + q of block r = layer 1's B operand of K steps 0..3
+        bf16x4_t x[ZT_PR_NC][4];
+#pragma unroll
+        for (int ct = 0; ct < ZT_PR_NC; ct++) {
+            const uint32_t row = (g * ZT_PR_NC + ct) * 16 + r;
+            uint32_t px[4];
+            const uint32_t *src = (const uint32_t *)P.blocks + (uint64_t)row * 16 + q;
+#pragma unroll
+            for (int k = 0; k < 4; k++) px[k] = row < P.n ? src[4 * k] : 0u;
+            int sum[4], lo[4], hi[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                sum[c] = 0; lo[c] = 255; hi[c] = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) { const int v = chan(px[k], c); sum[c] += v; lo[c] = min(lo[c], v); hi[c] = max(hi[c], v); }
+#pragma unroll
+                for (int off = 16; off <= 32; off <<= 1) {
+                    sum[c] += __shfl_xor(sum[c], off);
+                    lo[c] = min(lo[c], __shfl_xor(lo[c], off));
+                    hi[c] = max(hi[c], __shfl_xor(hi[c], off));
+                }
+            }
+            const FeatScale f = feature_scale(sum, lo, hi, mode);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                x[ct][k] = bf16x4_t{ (short)feature(f, px[k], 0), (short)feature(f, px[k], 1), (short)feature(f, px[k], 2), (short)feature(f, px[k], 3) };
+        }
+        bf16x4_t h1[ZT_PR_NC][8], h2[ZT_PR_NC][8];
+        layer<8, 4>(W1f, B1s, x, h1, lane, q);
+        layer<8, 8>(W2f, B2s, h1, h2, lane, q);
+        // layer 3, one row tile at a time, straight into the top lists
+        TopList<KT> T[ZT_PR_NC];
+        float riskv[ZT_PR_NC];
+#pragma unroll
+        for (int ct = 0; ct < ZT_PR_NC; ct++) { T[ct].init(); riskv[ct] = 0.0f; }
+#pragma unroll
+        for (int t = 0; t < ZT_PR_U / 16; t++) {
+            f32x4_t z[ZT_PR_NC];
+#pragma unroll
+            for (int ct = 0; ct < ZT_PR_NC; ct++) z[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const uint4 a = W3f[(t * 4 + p) * 64 + lane];
+#pragma unroll
+                for (int ct = 0; ct < ZT_PR_NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), h2[ct][2 * p], z[ct], 0, 0, 0);
+#pragma unroll
+                for (int ct = 0; ct < ZT_PR_NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), h2[ct][2 * p + 1], z[ct], 0, 0, 0);
+            }
+            const float4 b = *(const float4 *)(B3s + 16 * t + 4 * q);
+#pragma unroll
+            for (int ct = 0; ct < ZT_PR_NC; ct++) {
+                const float v[4] = { z[ct][0] + b.x, z[ct][1] + b.y, z[ct][2] + b.z, z[ct][3] + b.w };
+                const uint32_t row = (g * ZT_PR_NC + ct) * 16 + r;
+                if (P.logits && row < P.n) *(float4 *)(P.logits + (uint64_t)row * ZT_PR_U + 16 * t + 4 * q) = make_float4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int u = 16 * t + 4 * q + i;
+                    if (t == ZT_PR_U / 16 - 1) {   // the last tile holds rows 272..287: slots up to 284, the risk row 285
+                        if (u == 285) riskv[ct] = v[i];
+                        T[ct].insert(u < 285 ? v[i] : -INFINITY, u);
+                    } else {
+                        T[ct].insert(v[i], u);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < ZT_PR_NC; ct++) {
+            const uint32_t row = (g * ZT_PR_NC + ct) * 16 + r;
+            for (int k = 0; k < K; k++) {
+                float hv = T[ct].t[0];
+                int hu = T[ct].u[0];
+#pragma unroll
+                for (int off = 16; off <= 32; off <<= 1) {
+                    const float ov = __shfl_xor(hv, off);
+                    const int ou = __shfl_xor(hu, off);
+                    if (ov > hv || (ov == hv && ou < hu)) { hv = ov; hu = ou; }
+                }
+                if (P.cand && q == 0 && row < P.n) P.cand[(uint64_t)row * K + k] = (uint32_t)prop_slot(hu);
+                T[ct].pop_if(hu == T[ct].u[0]);
+            }
+            if (P.risk && q == 3 && row < P.n) {
+                const float x0 = riskv[ct];
+                P.risk[row] = x0 <= 0.0f ? 0.0f : x0 >= 13.9f ? 1048576.0f : __expf(x0) - 1.0f;
+            }
+        }
+    }
+}
+}  // namespace
+
+// K <= 4 and K <= 16
+extern "C" __global__ void __launch_bounds__(512) zt_mlp_propose4(zt_propose_args P) { propose_body<4>(P); }
+extern "C" __global__ void __launch_bounds__(512) zt_mlp_propose16(zt_propose_args P) { propose_body<16>(P); }
