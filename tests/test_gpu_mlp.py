@@ -189,7 +189,7 @@ def test_learned_encode_contract(zt, oracle):
     imgs_full = zt.images([(src, full, 64, 64, 256, "bc7")])
     zt.bc7_full_dev(imgs_full)
     st = zt.mlp_state(seed=9)
-    for thr in (-1.0, 1e30):      # every block falls back; no block falls back
+    for thr in (-1.0, 1e30, 1.0):      # every block falls back; no block falls back; some do
         dst = torch.zeros_like(full)
         imgs = zt.images([(src, dst, 64, 64, 256, "bc7")])
         ws = zt.bc7_learned_workspace(imgs, 4)
@@ -198,12 +198,13 @@ def test_learned_encode_contract(zt, oracle):
         outs.append(dst.cpu().numpy())
     f = full.cpu().numpy()
     assert np.array_equal(outs[0], f)                      # everything fell back to the exhaustive search
-    learned = outs[1]
-    sse_l, sse_f = _decode_sse(oracle, learned, img), _decode_sse(oracle, f, img)
-    assert (sse_l >= sse_f).all()
-    for b in range(0, 256, 5):
-        by, bx = divmod(b, 16)
-        blk = img[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4].reshape(16, 4)
-        c = _cand_of(learned[16 * b:16 * b + 16])
-        e, exp = oracle.bc7_candidate(blk, c)
-        assert np.array_equal(exp, learned[16 * b:16 * b + 16]) and e == sse_l[b], (b, c)
+    sse_f = _decode_sse(oracle, f, img)
+    for learned in outs[1:]:
+        sse_l = _decode_sse(oracle, learned, img)
+        assert (sse_l >= sse_f).all()
+        for b in range(0, 256, 5):
+            by, bx = divmod(b, 16)
+            blk = img[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4].reshape(16, 4)
+            c = _cand_of(learned[16 * b:16 * b + 16])
+            e, exp = oracle.bc7_candidate(blk, c)
+            assert np.array_equal(exp, learned[16 * b:16 * b + 16]) and e == sse_l[b], (b, c)

@@ -354,12 +354,14 @@ ZT_EXH(zt_bc7_exh_m6, 6, 0)
 ZT_EXH(zt_bc7_exh_m7, 7, 0)
 #undef ZT_EXH
 
-// The winner of every (listed) block: its candidate re-encoded (one candidate per block, so the all-mode switch
-// costs little), the 16 bytes stored, and the candidate slot written to best_cand (optional).
+// The winner of every block (or, with `list`, of the count[0] listed blocks): its candidate re-encoded (one
+// candidate per block, so the all-mode switch costs little), the 16 bytes stored, and the candidate slot written to
+// best_cand (optional).
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key,
-                                                                  uint32_t *best_cand) {
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+                                                                  uint32_t *best_cand, const uint32_t *list, const unsigned *count) {
+    const uint32_t step = gridDim.x * blockDim.x, n = list ? count[0] : total;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const uint32_t g = list ? list[i] : i;
         const uint32_t k = key[g];
         if (k == 0xFFFFFFFFu) continue;
         uint32_t px[16];
@@ -402,12 +404,13 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_gather(zt_jobs jobs, ui
 }
 
 // pass 0: count proposals per mode (counters[0..7]); pass 1: scatter them into mode-contiguous runs starting
-// at counters[16 + mode] (prefix sums filled in between by zt_bc7_bucket_scan), and high-risk blocks into the
-// fallback list (counters[8]; they get no proposals).  Each workgroup tallies in LDS and reserves its ranges with
+// at counters[16 + mode] (prefix sums filled in between by zt_bc7_bucket_scan), record each proposal's item index
+// in slot[block * K + k] (all 0xFFFFFFFF for a fallback block), and put high-risk blocks into the fallback list
+// (counters[8]; they get no proposals).  Each workgroup tallies in LDS and reserves its ranges with
 // one global atomic per mode, so the global counters see a few atomics per workgroup, not one per proposal.
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K,
                                                                   float threshold, unsigned *counters, uint2 *items, uint32_t pass,
-                                                                  uint32_t *fallback) {
+                                                                  uint32_t *fallback, uint32_t *slot) {
     __shared__ unsigned cnt[9], base[9];
     const uint32_t step = gridDim.x * blockDim.x;
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < total; b0 += step) {
@@ -430,8 +433,10 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *
         }
         __syncthreads();
         if (pass == 1) {
-            for (int k = 0; k < K; k++)
+            for (int k = 0; k < K; k++) {
                 if (c[k] != 0xFFFFFFFFu) items[base[c[k] >> 6] + local[k]] = make_uint2(g, c[k]);
+                if (valid) slot[(uint64_t)g * K + k] = c[k] != 0xFFFFFFFFu ? base[c[k] >> 6] + local[k] : 0xFFFFFFFFu;
+            }
             if (fb) fallback[base[8] + lfb] = g;
         }
         __syncthreads();
@@ -444,9 +449,10 @@ extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
     }
 }
 
-// exact evaluation of the proposals of mode M (lane per proposal), folded into the blocks' keys
+// exact evaluation of the proposals of mode M (lane per proposal), folded into the blocks' keys; the encoded block
+// is kept per item (enc) for zt_bc7_finish_learned
 template <int M>
-ZT_DEV void prop_body(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key) {
+ZT_DEV void prop_body(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key, uint4 *enc) {
     const uint32_t begin = counters[24 + M], n = counters[M], step = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
         const uint2 it = items[begin + i];
@@ -460,13 +466,14 @@ ZT_DEV void prop_body(const uint4 *blocks, const uint2 *items, const unsigned *c
         } else if (slot < kSlots<M, 0>) {
             err = eval_mode<M, 0>(px, slot, lo, hi);
         }
+        if (err >= 0) enc[begin + i] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
         atomicMin(key + it.x, cand_key(err, (int)it.y));
     }
 }
 #define ZT_PROP(M)                                                                                                   \
     extern "C" __global__ void __launch_bounds__(256) zt_bc7_prop_m##M(const uint4 *blocks, const uint2 *items,               \
-                                                                        const unsigned *counters, uint32_t *key) {           \
-        prop_body<M>(blocks, items, counters, key);                                                                            \
+                                                                        const unsigned *counters, uint32_t *key, uint4 *enc) {\
+        prop_body<M>(blocks, items, counters, key, enc);                                                                       \
     }
 ZT_PROP(0)
 ZT_PROP(1)
@@ -477,3 +484,21 @@ ZT_PROP(5)
 ZT_PROP(6)
 ZT_PROP(7)
 #undef ZT_PROP
+
+// every proposal-searched block's winner: the stored encoding of its winning proposal (fallback blocks, slot
+// 0xFFFFFFFF, are encoded by zt_bc7_finish over the fallback list)
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_finish_learned(zt_jobs jobs, uint32_t total, int K, const uint32_t *cand,
+                                                                          const uint32_t *slot, const uint4 *enc, const uint32_t *key) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const uint32_t win = key[g] & 511u;
+        uint32_t it = 0xFFFFFFFFu;
+        for (int k = 0; k < K; k++) {
+            const uint32_t sl = slot[(uint64_t)g * K + k];
+            if (sl != 0xFFFFFFFFu && cand[(uint64_t)g * K + k] == win) it = sl;
+        }
+        if (it == 0xFFFFFFFFu) continue;
+        const zt_job J = job_at(jobs, zt_find_job(jobs, g));
+        *(uint4 *)(J.dst + (uint64_t)(g - J.first) * 16) = enc[it];
+    }
+}

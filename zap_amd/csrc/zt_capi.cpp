@@ -601,7 +601,8 @@ int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, ui
         return rc;
     if ((rc = enqueue_exhaustive(blocks, total, nullptr, nullptr, key, errs, s))) return rc;
     if ((rc = launch("zt_bc7_finish", s, [&] {
-            hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total, 256)), dim3(256), 0, s, J, (const uint4 *)blocks, total, (const uint32_t *)key, best_cand); })))
+            hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total, 256)), dim3(256), 0, s, J, (const uint4 *)blocks, total, (const uint32_t *)key, best_cand,
+                               (const uint32_t *)nullptr, (const unsigned *)nullptr); })))
         return rc;
     if (!deltas) return ZAPTEX_OK;
     return launch("zt_bc7_deltas", s, [&] {
@@ -855,15 +856,19 @@ int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *s
 
 // ---- learned-proposal BC7 encode
 namespace {
-struct LearnedWs { uint4 *blocks; uint2 *items; uint32_t *cand, *fallback, *key; float *risk; unsigned *counters; };
-size_t learned_ws_bytes(uint64_t total, int K) { return 256 + 256 + total * (64 + 8 * K + 4 * K + 4 + 4 + 4) + 1024; }
+// per block: its pixels (64 B), K proposals (item 8 B, encoding 16 B, candidate 4 B, item index 4 B), key, risk and
+// a fallback-list entry
+struct LearnedWs { uint4 *blocks, *enc; uint2 *items; uint32_t *cand, *slot, *fallback, *key; float *risk; unsigned *counters; };
+size_t learned_ws_bytes(uint64_t total, int K) { return 256 + 256 + total * (64 + 32 * K + 4 + 4 + 4) + 1024; }
 LearnedWs learned_ws(void *ws, uint64_t total, int K) {
     uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
     LearnedWs v;
     v.counters = (unsigned *)b; b += 256;
     v.blocks = (uint4 *)b; b += total * 64;
+    v.enc = (uint4 *)b; b += total * 16 * K;
     v.items = (uint2 *)b; b += total * 8 * K;
     v.cand = (uint32_t *)b; b += total * 4 * K;
+    v.slot = (uint32_t *)b; b += total * 4 * K;
     v.key = (uint32_t *)b; b += total * 4;
     v.risk = (float *)b; b += total * 4;
     v.fallback = (uint32_t *)b;
@@ -903,24 +908,29 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
     for (uint32_t pass = 0; pass < 2; pass++) {
         if ((rc = launch("zt_bc7_bucket", s, [&] {
                 hipLaunchKernelGGL(zt_bc7_bucket, dim3(grid_for(total, 256)), dim3(256), 0, s, (const uint32_t *)w.cand, (const float *)w.risk, total, topk,
-                                   threshold, w.counters, w.items, pass, w.fallback); })))
+                                   threshold, w.counters, w.items, pass, w.fallback, w.slot); })))
             return rc;
         if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters); })))
             return rc;
     }
-    typedef void (*Prop)(const uint4 *, const uint2 *, const unsigned *, uint32_t *);
+    typedef void (*Prop)(const uint4 *, const uint2 *, const unsigned *, uint32_t *, uint4 *);
     const Prop props[8] = { zt_bc7_prop_m0, zt_bc7_prop_m1, zt_bc7_prop_m2, zt_bc7_prop_m3, zt_bc7_prop_m4, zt_bc7_prop_m5, zt_bc7_prop_m6, zt_bc7_prop_m7 };
     static const char *prop_names[8] = { "zt_bc7_prop_m0", "zt_bc7_prop_m1", "zt_bc7_prop_m2", "zt_bc7_prop_m3",
                                          "zt_bc7_prop_m4", "zt_bc7_prop_m5", "zt_bc7_prop_m6", "zt_bc7_prop_m7" };
     for (int m = 0; m < 8; m++)
         if ((rc = launch(prop_names[m], s, [&] {
                 hipLaunchKernelGGL(props[m], dim3(grid_for((uint64_t)total * topk, 256)), dim3(256), 0, s, (const uint4 *)w.blocks, (const uint2 *)w.items,
-                                   (const unsigned *)w.counters, w.key); })))
+                                   (const unsigned *)w.counters, w.key, w.enc); })))
             return rc;
     if ((rc = enqueue_exhaustive(w.blocks, total, w.fallback, w.counters + 8, w.key, nullptr, s))) return rc;
-    return launch("zt_bc7_finish", s, [&] {
-        hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total, 256)), dim3(256), 0, s, J, (const uint4 *)w.blocks, total, (const uint32_t *)w.key,
-                           (uint32_t *)nullptr); });
+    // fallback blocks re-encode their winner; the others copy their winning proposal's stored encoding
+    if ((rc = launch("zt_bc7_finish", s, [&] {
+            hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total / 16 + 1, 256)), dim3(256), 0, s, J, (const uint4 *)w.blocks, total,
+                               (const uint32_t *)w.key, (uint32_t *)nullptr, (const uint32_t *)w.fallback, (const unsigned *)(w.counters + 8)); })))
+        return rc;
+    return launch("zt_bc7_finish_learned", s, [&] {
+        hipLaunchKernelGGL(zt_bc7_finish_learned, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, topk, (const uint32_t *)w.cand,
+                           (const uint32_t *)w.slot, (const uint4 *)w.enc, (const uint32_t *)w.key); });
 }
 
 int zaptex_mip_next_dev(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, void *stream) {

@@ -82,7 +82,8 @@ __global__ void zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint
 ZT_EXH_DECL(zt_bc7_exh_m0) ZT_EXH_DECL(zt_bc7_exh_m1) ZT_EXH_DECL(zt_bc7_exh_m2) ZT_EXH_DECL(zt_bc7_exh_m3)
 ZT_EXH_DECL(zt_bc7_exh_m4i0) ZT_EXH_DECL(zt_bc7_exh_m4i1) ZT_EXH_DECL(zt_bc7_exh_m5) ZT_EXH_DECL(zt_bc7_exh_m6) ZT_EXH_DECL(zt_bc7_exh_m7)
 #undef ZT_EXH_DECL
-__global__ void zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key, uint32_t *best_cand);
+__global__ void zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key, uint32_t *best_cand, const uint32_t *list,
+                              const unsigned *count);
 __global__ void zt_bc7_deltas(const uint32_t *errs, const uint32_t *key, uint32_t total, uint16_t *deltas);
 
 __global__ void zt_gemm_bf16(zt_gemm_args g);
@@ -101,13 +102,15 @@ __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *
 // learned-proposal BC7 (zt_bc7_full.hip)
 __global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);
 __global__ void zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K, float threshold, unsigned *counters,
-                              uint2 *items, uint32_t pass, uint32_t *fallback);
+                              uint2 *items, uint32_t pass, uint32_t *fallback, uint32_t *slot);
 __global__ void zt_bc7_bucket_scan(unsigned *counters);
 __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
-#define ZT_PROP_DECL(NAME) __global__ void NAME(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key);
+#define ZT_PROP_DECL(NAME) __global__ void NAME(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key, uint4 *enc);
 ZT_PROP_DECL(zt_bc7_prop_m0) ZT_PROP_DECL(zt_bc7_prop_m1) ZT_PROP_DECL(zt_bc7_prop_m2) ZT_PROP_DECL(zt_bc7_prop_m3)
 ZT_PROP_DECL(zt_bc7_prop_m4) ZT_PROP_DECL(zt_bc7_prop_m5) ZT_PROP_DECL(zt_bc7_prop_m6) ZT_PROP_DECL(zt_bc7_prop_m7)
 #undef ZT_PROP_DECL
+__global__ void zt_bc7_finish_learned(zt_jobs jobs, uint32_t total, int K, const uint32_t *cand, const uint32_t *slot, const uint4 *enc,
+                                      const uint32_t *key);
 
 // largest block range per wave of zt_bc1_climb (a wave's lanes refill from its range)
 #define ZT_CLIMB_RANGE 1024u
