@@ -102,9 +102,10 @@ def learned_leg(lib, c2, steps: int = 5) -> dict:
     from zap_amd import learned
     state, meta = learned.load(lib)
     chain = learned.Chain(lib, c2)
-    r = learned.evaluate(lib, chain, state, meta["topk"], [meta["threshold"]], steps)
+    fk = int(meta.get("fallback_k", 0))
+    r = learned.evaluate(lib, chain, state, meta["topk"], [meta["threshold"]], steps, fallback_k=fk)
     lr, full, plain = r["learned"][0], r["full"], r["plain"]
-    out = {"threshold": meta["threshold"], "topk": meta["topk"], "features": meta["features"], "train_blocks": meta["blocks"],
+    out = {"threshold": meta["threshold"], "topk": meta["topk"], "fallback_k": fk, "features": meta["features"], "train_blocks": meta["blocks"],
            "mpix_s": round(lr["mpix_s"], 1), "exhaustive_mpix_s": round(full["mpix_s"], 1),
            "speedup_vs_exhaustive": round(lr["mpix_s"] / full["mpix_s"], 3), "psnr_db": round(lr["psnr"], 4),
            "exhaustive_psnr_db": round(full["psnr"], 4), "gap_db": round(lr["gap_db"], 4), "fallback": round(lr["fallback"], 4),

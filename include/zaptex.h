@@ -135,9 +135,12 @@ int zaptex_mlp_train_step(void *state, const void *blocks, const zaptex_mlp_targ
                           size_t ws_bytes, float *stats, void *stream);
 /* every block of the images as 64 contiguous bytes (training inputs) */
 int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *stream);
-size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int topk);
-int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, void *ws, size_t ws_bytes,
-                           void *stream);
+/* The learned encode.  Blocks whose predicted risk exceeds `threshold` fall back: with fallback_k = 0 to the
+ * exhaustive search, with fallback_k = 1..285 to the network's top fallback_k slots (largest logits, ties to the
+ * lowest slot) encoded exactly; fallback_k = 285 evaluates every candidate and equals the exhaustive search. */
+size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int topk, int fallback_k);
+int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, int fallback_k, void *ws,
+                           size_t ws_bytes, void *stream);
 
 /* ---------------- batch: many textures with their mip chains in one graph launch (no reference counterpart:
  * the reference encodes one image per call and its callers loop)

@@ -23,7 +23,7 @@ def chain(zt):
 
 def test_learned_within_gap_at_calibrated_threshold(zt, model, chain):
     state, meta = model
-    r = learned.evaluate(zt, chain, state, meta["topk"], [meta["threshold"]], steps=1)
+    r = learned.evaluate(zt, chain, state, meta["topk"], [meta["threshold"]], steps=1, fallback_k=int(meta.get("fallback_k", 0)))
     lr = r["learned"][0]
     assert lr["gap_db"] <= 0.05, lr
     assert lr["fallback"] <= 0.35, lr                      # >= 65% of blocks: 4 candidates instead of 285

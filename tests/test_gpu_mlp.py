@@ -189,18 +189,24 @@ def test_learned_encode_contract(zt, oracle):
     imgs_full = zt.images([(src, full, 64, 64, 256, "bc7")])
     zt.bc7_full_dev(imgs_full)
     st = zt.mlp_state(seed=9)
-    for thr in (-1.0, 1e30, 1.0):      # every block falls back; no block falls back; some do
+    # (threshold, fallback_k): every block falls back to the exhaustive search; none falls back; some do; every
+    # block falls back to the network's top 285 (all candidates: the exhaustive result again), top 16, top 16 for some
+    runs = [(-1.0, 0), (1e30, 0), (1.0, 0), (-1.0, 285), (-1.0, 16), (1.0, 16)]
+    for thr, fk in runs:
         dst = torch.zeros_like(full)
         imgs = zt.images([(src, dst, 64, 64, 256, "bc7")])
-        ws = zt.bc7_learned_workspace(imgs, 4)
-        zt.bc7_learned_dev(imgs, st, 4, thr, ws)
+        ws = zt.bc7_learned_workspace(imgs, 4, fk)
+        zt.bc7_learned_dev(imgs, st, 4, thr, ws, fallback_k=fk)
         torch.cuda.synchronize()
         outs.append(dst.cpu().numpy())
     f = full.cpu().numpy()
     assert np.array_equal(outs[0], f)                      # everything fell back to the exhaustive search
+    assert np.array_equal(outs[3], f)                      # the wide fallback over all 285 candidates is that search
     sse_f = _decode_sse(oracle, f, img)
-    for learned in outs[1:]:
-        sse_l = _decode_sse(oracle, learned, img)
+    sse = [_decode_sse(oracle, o, img) for o in outs]
+    assert (sse[4] <= sse[1]).all()                        # top 16 contain the top 4
+    assert (sse[5] <= sse[1]).all()
+    for learned, sse_l in zip(outs[1:], sse[1:]):
         assert (sse_l >= sse_f).all()
         for b in range(0, 256, 5):
             by, bx = divmod(b, 16)

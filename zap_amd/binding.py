@@ -114,7 +114,7 @@ class Library:
             "zaptex_mlp_gradients": [vp, vp, C.POINTER(MlpTargets), i, vp, sz, vp, vp],
             "zaptex_mlp_train_step": [vp, vp, C.POINTER(MlpTargets), i, f, i, vp, sz, vp, vp],
             "zaptex_bc7_gather_dev": [C.POINTER(ZapImage), i, vp, vp],
-            "zaptex_bc7_learned_dev": [C.POINTER(ZapImage), i, vp, i, f, vp, sz, vp],
+            "zaptex_bc7_learned_dev": [C.POINTER(ZapImage), i, vp, i, f, i, vp, sz, vp],
             "zaptex_set_option": [i, i], "zaptex_get_option": [i],
         }
         for name, args in sig.items():
@@ -127,7 +127,7 @@ class Library:
         L.zaptex_profile_reset.restype = None
         L.zaptex_mlp_state_size.restype = C.c_size_t
         L.zaptex_mlp_workspace_size.argtypes = [i]; L.zaptex_mlp_workspace_size.restype = C.c_size_t
-        L.zaptex_bc7_learned_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
+        L.zaptex_bc7_learned_workspace_size.argtypes = [C.POINTER(ZapImage), i, i, i]
         L.zaptex_bc7_learned_workspace_size.restype = C.c_size_t
         for name, args in {"zaptex_batch_mip_bytes": [i, i, i], "zaptex_batch_out_bytes": [i, i, i, i],
                            "zaptex_batch_workspace_size": [C.POINTER(ZapTexture), i]}.items():
@@ -344,14 +344,15 @@ class Library:
     def bc7_gather_dev(self, imgs, blocks) -> None:
         self._check(self.lib.zaptex_bc7_gather_dev(imgs, len(imgs), blocks.data_ptr(), self._stream()), "zaptex_bc7_gather_dev")
 
-    def bc7_learned_workspace(self, imgs, topk: int):
+    def bc7_learned_workspace(self, imgs, topk: int, fallback_k: int = 0):
         import torch
-        n = int(self.lib.zaptex_bc7_learned_workspace_size(imgs, len(imgs), topk))
+        n = int(self.lib.zaptex_bc7_learned_workspace_size(imgs, len(imgs), topk, fallback_k))
         return torch.empty(n, dtype=torch.uint8, device="cuda")
 
-    def bc7_learned_dev(self, imgs, state, topk: int, threshold: float, ws, stream=None) -> None:
-        self._check(self.lib.zaptex_bc7_learned_dev(imgs, len(imgs), state.data_ptr(), topk, threshold, ws.data_ptr(), ws.numel(),
-                                                    self._stream(stream)), "zaptex_bc7_learned_dev")
+    def bc7_learned_dev(self, imgs, state, topk: int, threshold: float, ws, fallback_k: int = 0, stream=None) -> None:
+        """fallback_k = 0: blocks over the risk threshold get the exhaustive search; 1..285: the network's top fallback_k"""
+        self._check(self.lib.zaptex_bc7_learned_dev(imgs, len(imgs), state.data_ptr(), topk, threshold, fallback_k, ws.data_ptr(),
+                                                    ws.numel(), self._stream(stream)), "zaptex_bc7_learned_dev")
 
     # ------------------------------------------------------------------ batch (device buffers as torch tensors)
     def batch_mip_bytes(self, w: int, h: int, levels: int) -> int:

@@ -442,6 +442,29 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *
         __syncthreads();
     }
 }
+// the wide proposals of the fallback list, bucketed like zt_bc7_bucket (pass 0 counts per mode, pass 1 scatters into
+// the runs zt_bc7_bucket_scan lays out): item e < count[0] * K is candidate cand[e] of block list[e / K]
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket_wide(const uint32_t *cand, const uint32_t *list, const unsigned *count, int K,
+                                                                       unsigned *counters, uint2 *items, uint32_t pass) {
+    __shared__ unsigned cnt[8], base[8];
+    const uint64_t n = (uint64_t)count[0] * (uint64_t)K, step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += step) {
+        if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t e = b0 + threadIdx.x;
+        const bool valid = e < n;
+        uint32_t c = 0, local = 0;
+        if (valid) { c = cand[e]; local = atomicAdd(&cnt[c >> 6], 1u); }
+        __syncthreads();
+        if (threadIdx.x < 8 && cnt[threadIdx.x]) {
+            if (pass == 0) atomicAdd(&counters[threadIdx.x], cnt[threadIdx.x]);
+            else base[threadIdx.x] = atomicAdd(&counters[16 + threadIdx.x], cnt[threadIdx.x]);
+        }
+        __syncthreads();
+        if (pass == 1 && valid) items[base[c >> 6] + local] = make_uint2(list[e / (uint64_t)K], c);
+        __syncthreads();
+    }
+}
 extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         unsigned s = 0;
@@ -450,7 +473,7 @@ extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
 }
 
 // exact evaluation of the proposals of mode M (lane per proposal), folded into the blocks' keys; the encoded block
-// is kept per item (enc) for zt_bc7_finish_learned
+// is kept per item (enc, optional) for zt_bc7_finish_learned
 template <int M>
 ZT_DEV void prop_body(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key, uint4 *enc) {
     const uint32_t begin = counters[24 + M], n = counters[M], step = gridDim.x * blockDim.x;
@@ -466,7 +489,7 @@ ZT_DEV void prop_body(const uint4 *blocks, const uint2 *items, const unsigned *c
         } else if (slot < kSlots<M, 0>) {
             err = eval_mode<M, 0>(px, slot, lo, hi);
         }
-        if (err >= 0) enc[begin + i] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+        if (enc && err >= 0) enc[begin + i] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
         atomicMin(key + it.x, cand_key(err, (int)it.y));
     }
 }

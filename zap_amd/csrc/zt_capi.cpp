@@ -857,49 +857,74 @@ int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *s
 // ---- learned-proposal BC7 encode
 namespace {
 // per block: its pixels (64 B), K proposals (item 8 B, encoding 16 B, candidate 4 B, item index 4 B), key, risk and
-// a fallback-list entry
-struct LearnedWs { uint4 *blocks, *enc; uint2 *items; uint32_t *cand, *slot, *fallback, *key; float *risk; unsigned *counters; };
-size_t learned_ws_bytes(uint64_t total, int K) { return 256 + 256 + total * (64 + 32 * K + 4 + 4 + 4) + 1024; }
-LearnedWs learned_ws(void *ws, uint64_t total, int K) {
+// a fallback-list entry; with a wide fallback (K2 > 0) also K2 wide candidates (4 B) and items (8 B)
+struct LearnedWs {
+    uint4 *blocks, *enc;
+    uint2 *items, *items2;
+    uint32_t *cand, *slot, *fallback, *key, *cand2;
+    float *risk;
+    unsigned *counters, *counters2;
+};
+size_t learned_ws_bytes(uint64_t total, int K, int K2) {
+    return 256 + 512 + total * (64 + 32 * (uint64_t)K + 4 + 4 + 4) + total * 12 * (uint64_t)K2 + 1024;
+}
+LearnedWs learned_ws(void *ws, uint64_t total, int K, int K2) {
     uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
     LearnedWs v;
     v.counters = (unsigned *)b; b += 256;
+    v.counters2 = (unsigned *)b; b += 256;
     v.blocks = (uint4 *)b; b += total * 64;
     v.enc = (uint4 *)b; b += total * 16 * K;
     v.items = (uint2 *)b; b += total * 8 * K;
+    v.items2 = (uint2 *)b; b += total * 8 * (uint64_t)K2;
     v.cand = (uint32_t *)b; b += total * 4 * K;
     v.slot = (uint32_t *)b; b += total * 4 * K;
+    v.cand2 = (uint32_t *)b; b += total * 4 * (uint64_t)K2;
     v.key = (uint32_t *)b; b += total * 4;
     v.risk = (float *)b; b += total * 4;
     v.fallback = (uint32_t *)b;
     return v;
 }
+typedef void (*PropFn)(const uint4 *, const uint2 *, const unsigned *, uint32_t *, uint4 *);
+// one launch per mode over the bucketed items (counters: per-mode counts at [0..7], run starts at [24..31])
+int enqueue_props(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key, uint4 *enc, uint64_t max_items,
+                  hipStream_t s) {
+    const PropFn props[8] = { zt_bc7_prop_m0, zt_bc7_prop_m1, zt_bc7_prop_m2, zt_bc7_prop_m3, zt_bc7_prop_m4, zt_bc7_prop_m5, zt_bc7_prop_m6, zt_bc7_prop_m7 };
+    static const char *names[8] = { "zt_bc7_prop_m0", "zt_bc7_prop_m1", "zt_bc7_prop_m2", "zt_bc7_prop_m3",
+                                    "zt_bc7_prop_m4", "zt_bc7_prop_m5", "zt_bc7_prop_m6", "zt_bc7_prop_m7" };
+    const dim3 grid((unsigned)std::min<uint64_t>((uint64_t)grid_for(max_items, 256), 65536));
+    for (int m = 0; m < 8; m++)
+        if (int rc = launch(names[m], s, [&] { hipLaunchKernelGGL(props[m], grid, dim3(256), 0, s, blocks, items, counters, key, enc); })) return rc;
+    return ZAPTEX_OK;
+}
 }  // namespace
 
-size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int topk) {
+size_t zaptex_bc7_learned_workspace_size(const zaptex_image *imgs, int n, int topk, int fallback_k) {
     uint64_t t = 0;
     for (int i = 0; i < n; i++) t += nblocks(imgs[i]);
-    return learned_ws_bytes(t, topk);
+    return learned_ws_bytes(t, topk, fallback_k > 0 ? fallback_k : 0);
 }
 
-int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, void *ws, size_t ws_bytes,
-                           void *stream) {
+int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, int topk, float threshold, int fallback_k, void *ws,
+                           size_t ws_bytes, void *stream) {
     if (!state) return fail(ZAPTEX_E_ARG, "null network state");
     if (topk < 1 || topk > 16) return fail(ZAPTEX_E_ARG, "topk must be 1..16");
+    if (fallback_k < 0 || fallback_k > 285) return fail(ZAPTEX_E_ARG, "fallback_k must be 0 (exhaustive) or 1..285");
     zt_jobs J;
     uint32_t total;
     if (int rc = bc7_jobs(imgs, n, J, total, "the learned proposal")) return rc;
-    if (!ws || ws_bytes < learned_ws_bytes(total, topk)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, learned_ws_bytes(total, topk));
+    const size_t need = learned_ws_bytes(total, topk, fallback_k);
+    if (!ws || ws_bytes < need) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, need);
     DeviceState *dst;
     {
         std::lock_guard<std::mutex> lk(g_mu);
         if (int rc = ensure_ready(dst)) return rc;
     }
     hipStream_t s = (hipStream_t)stream;
-    LearnedWs w = learned_ws(ws, total, topk);
+    LearnedWs w = learned_ws(ws, total, topk, fallback_k);
     MlpState st = mlp_view((void *)state);
     int rc;
-    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(64), 0, s, w.counters, 64); }))) return rc;
+    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(128), 0, s, w.counters, 128); }))) return rc;
     if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(grid_for(total, 256)), dim3(256), 0, s, w.key, (uint64_t)total, 0xFFFFFFFFu); })))
         return rc;
     if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, w.blocks); })))
@@ -913,16 +938,28 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
         if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters); })))
             return rc;
     }
-    typedef void (*Prop)(const uint4 *, const uint2 *, const unsigned *, uint32_t *, uint4 *);
-    const Prop props[8] = { zt_bc7_prop_m0, zt_bc7_prop_m1, zt_bc7_prop_m2, zt_bc7_prop_m3, zt_bc7_prop_m4, zt_bc7_prop_m5, zt_bc7_prop_m6, zt_bc7_prop_m7 };
-    static const char *prop_names[8] = { "zt_bc7_prop_m0", "zt_bc7_prop_m1", "zt_bc7_prop_m2", "zt_bc7_prop_m3",
-                                         "zt_bc7_prop_m4", "zt_bc7_prop_m5", "zt_bc7_prop_m6", "zt_bc7_prop_m7" };
-    for (int m = 0; m < 8; m++)
-        if ((rc = launch(prop_names[m], s, [&] {
-                hipLaunchKernelGGL(props[m], dim3(grid_for((uint64_t)total * topk, 256)), dim3(256), 0, s, (const uint4 *)w.blocks, (const uint2 *)w.items,
-                                   (const unsigned *)w.counters, w.key, w.enc); })))
-            return rc;
-    if ((rc = enqueue_exhaustive(w.blocks, total, w.fallback, w.counters + 8, w.key, nullptr, s))) return rc;
+    if ((rc = enqueue_props(w.blocks, w.items, w.counters, w.key, w.enc, (uint64_t)total * topk, s))) return rc;
+    if (fallback_k == 0) {
+        if ((rc = enqueue_exhaustive(w.blocks, total, w.fallback, w.counters + 8, w.key, nullptr, s))) return rc;
+    } else {
+        // the fallback list's wide proposals: the network's top fallback_k, bucketed by mode and encoded exactly
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+        const dim3 pgrid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(((uint64_t)total + 63) / 64, (uint64_t)cus)));
+        zt_wide_args P{ w.blocks, w.fallback, w.counters + 8, fallback_k, st.w, st.p + oB1, st.p + oB2, st.p + oB3, st.cfg, w.cand2 };
+        if ((rc = launch("zt_mlp_propose_wide", s, [&] { hipLaunchKernelGGL(zt_mlp_propose_wide, pgrid, dim3(256), 0, s, P); }))) return rc;
+        const uint64_t max_items = (uint64_t)total * fallback_k;
+        const dim3 bgrid((unsigned)std::min<uint64_t>((uint64_t)grid_for(max_items, 256), 4096));
+        for (uint32_t pass = 0; pass < 2; pass++) {
+            if ((rc = launch("zt_bc7_bucket_wide", s, [&] {
+                    hipLaunchKernelGGL(zt_bc7_bucket_wide, bgrid, dim3(256), 0, s, (const uint32_t *)w.cand2, (const uint32_t *)w.fallback,
+                                       (const unsigned *)(w.counters + 8), fallback_k, w.counters2, w.items2, pass); })))
+                return rc;
+            if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters2); })))
+                return rc;
+        }
+        if ((rc = enqueue_props(w.blocks, w.items2, w.counters2, w.key, nullptr, max_items, s))) return rc;
+    }
     // fallback blocks re-encode their winner; the others copy their winning proposal's stored encoding
     if ((rc = launch("zt_bc7_finish", s, [&] {
             hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total / 16 + 1, 256)), dim3(256), 0, s, J, (const uint4 *)w.blocks, total,

@@ -32,6 +32,17 @@ struct zt_propose_args {
     float *risk;           // [n], or null
     float *logits;         // [n * 288] logits of the kept W3 rows (the candidate slots ascending, then the risk slot), or null
 };
+// the wide proposal of a block list (zt_mlp.hip): k candidate slots per listed block
+struct zt_wide_args {
+    const uint4 *blocks;      // 64 B per block
+    const uint32_t *list;     // block indices
+    const unsigned *count;    // list length (device)
+    int k;                    // 1..285
+    const uint16_t *w;
+    const float *b1, *b2, *b3;
+    const int *cfg;
+    uint32_t *cand;           // [count * k], unordered per block
+};
 // bf16 GEMM (zt_mlp.hip): C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]
 struct zt_gemm_args {
     const uint16_t *A, *B;
@@ -98,12 +109,15 @@ __global__ void zt_mlp_adam(float *p, const float *gr, float *m1, float *m2, int
 __global__ void zt_mlp_init(float *p, int n, uint32_t seed, uint16_t *wbf, const int *bf_index, const float *scale);
 __global__ void zt_mlp_propose4(zt_propose_args P);    // topk <= 4
 __global__ void zt_mlp_propose16(zt_propose_args P);   // topk <= 16
+__global__ void zt_mlp_propose_wide(zt_wide_args P);
 __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index);
 // learned-proposal BC7 (zt_bc7_full.hip)
 __global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);
 __global__ void zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K, float threshold, unsigned *counters,
                               uint2 *items, uint32_t pass, uint32_t *fallback, uint32_t *slot);
 __global__ void zt_bc7_bucket_scan(unsigned *counters);
+__global__ void zt_bc7_bucket_wide(const uint32_t *cand, const uint32_t *list, const unsigned *count, int K, unsigned *counters, uint2 *items,
+                                   uint32_t pass);
 __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
 #define ZT_PROP_DECL(NAME) __global__ void NAME(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key, uint4 *enc);
 ZT_PROP_DECL(zt_bc7_prop_m0) ZT_PROP_DECL(zt_bc7_prop_m1) ZT_PROP_DECL(zt_bc7_prop_m2) ZT_PROP_DECL(zt_bc7_prop_m3)

@@ -342,25 +342,43 @@ ZT_DEV bf16x4_t epilogue(f32x4_t d, const float *bias) {
     if (x3 < 0.0f) x3 = 0.0f;
     return bf16x4_t{ (short)to_bf16(x0), (short)to_bf16(x1), (short)to_bf16(x2), (short)to_bf16(x3) };
 }
-// one layer, NJ output row tiles over NK K steps: out[ct][j] = epilogue(sum_k W[j][k] in[ct][k])
-template <int NJ, int NK>
-ZT_DEV void layer(const uint4 *Wf, const float *bias, const bf16x4_t (&in)[ZT_PR_NC][NK], bf16x4_t (&out)[ZT_PR_NC][NJ], int lane, int q) {
+// one layer, NJ output row tiles over NK K steps, NC column tiles: out[ct][j] = epilogue(sum_k W[j][k] in[ct][k])
+template <int NJ, int NK, int NC>
+ZT_DEV void layer(const uint4 *Wf, const float *bias, const bf16x4_t (&in)[NC][NK], bf16x4_t (&out)[NC][NJ], int lane, int q) {
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        f32x4_t acc[ZT_PR_NC];
+        f32x4_t acc[NC];
 #pragma unroll
-        for (int ct = 0; ct < ZT_PR_NC; ct++) acc[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+        for (int ct = 0; ct < NC; ct++) acc[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
         for (int p = 0; p < NK / 2; p++) {
             const uint4 a = Wf[(j * (NK / 2) + p) * 64 + lane];
 #pragma unroll
-            for (int ct = 0; ct < ZT_PR_NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), in[ct][2 * p], acc[ct], 0, 0, 0);
+            for (int ct = 0; ct < NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), in[ct][2 * p], acc[ct], 0, 0, 0);
 #pragma unroll
-            for (int ct = 0; ct < ZT_PR_NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), in[ct][2 * p + 1], acc[ct], 0, 0, 0);
+            for (int ct = 0; ct < NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), in[ct][2 * p + 1], acc[ct], 0, 0, 0);
         }
 #pragma unroll
-        for (int ct = 0; ct < ZT_PR_NC; ct++) out[ct][j] = epilogue(acc[ct], bias + 16 * j + 4 * q);
+        for (int ct = 0; ct < NC; ct++) out[ct][j] = epilogue(acc[ct], bias + 16 * j + 4 * q);
     }
+}
+// layer-3 logits of kept row tile t (bias added): lane (r, q) gets rows 16 t + 4 q + i of each column tile's block r
+template <int NC>
+ZT_DEV void logits_tile(const uint4 *W3f, const float *B3s, const bf16x4_t (&h2)[NC][8], int t, int lane, int q, float (&v)[NC][4]) {
+    f32x4_t z[NC];
+#pragma unroll
+    for (int ct = 0; ct < NC; ct++) z[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const uint4 a = W3f[(t * 4 + p) * 64 + lane];
+#pragma unroll
+        for (int ct = 0; ct < NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), h2[ct][2 * p], z[ct], 0, 0, 0);
+#pragma unroll
+        for (int ct = 0; ct < NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), h2[ct][2 * p + 1], z[ct], 0, 0, 0);
+    }
+    const float4 b = *(const float4 *)(B3s + 16 * t + 4 * q);
+#pragma unroll
+    for (int ct = 0; ct < NC; ct++) { v[ct][0] = z[ct][0] + b.x; v[ct][1] = z[ct][1] + b.y; v[ct][2] = z[ct][2] + b.z; v[ct][3] = z[ct][3] + b.w; }
 }
 // sorted (largest first) top-KT of a stream of (logit, row) in ascending row order; ties keep the earlier row first
 template <int KT>
@@ -391,52 +409,65 @@ struct TopList {
     }
 };
 
+// the network in LDS for a kernel's life
+struct PropLds {
+    uint4 W1f[128 / 16 * 2 * 64];       // 16 KiB
+    uint4 W2f[128 / 16 * 4 * 64];       // 32 KiB
+    uint4 W3f[ZT_PR_U / 16 * 4 * 64];   // 72 KiB
+    float B1[128], B2[128], B3[ZT_PR_U];
+};
+ZT_DEV void stage_network(PropLds &L, const uint16_t *w, const float *b1, const float *b2, const float *b3) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    stage_fragments<64>(L.W1f, w, 128, tid, nth, false);
+    stage_fragments<128>(L.W2f, w + 64 * 128, 128, tid, nth, false);
+    stage_fragments<128>(L.W3f, w + 64 * 128 + 128 * 128, ZT_PR_U, tid, nth, true);
+    for (int i = tid; i < 128; i += nth) { L.B1[i] = b1[i]; L.B2[i] = b2[i]; }
+    for (int u = tid; u < ZT_PR_U; u += nth) { const int id = prop_slot(u); L.B3[u] = id < 0 ? 0.0f : b3[id]; }
+    __syncthreads();
+}
+// layer 1's B operands (K steps 0..3) for the 16 blocks of a column tile: lane (r, q) holds pixels q, 4 + q, 8 + q,
+// 12 + q of block `row` (zeros when !valid); the block's channel sums and ranges combine its four lanes
+ZT_DEV void tile_features(const uint4 *blocks, uint32_t row, bool valid, int q, int mode, bf16x4_t (&x)[4]) {
+    uint32_t px[4];
+    const uint32_t *src = (const uint32_t *)blocks + (uint64_t)row * 16 + q;
+#pragma unroll
+    for (int k = 0; k < 4; k++) px[k] = valid ? src[4 * k] : 0u;
+    int sum[4], lo[4], hi[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        sum[c] = 0; lo[c] = 255; hi[c] = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) { const int v = chan(px[k], c); sum[c] += v; lo[c] = min(lo[c], v); hi[c] = max(hi[c], v); }
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            sum[c] += __shfl_xor(sum[c], off);
+            lo[c] = min(lo[c], __shfl_xor(lo[c], off));
+            hi[c] = max(hi[c], __shfl_xor(hi[c], off));
+        }
+    }
+    const FeatScale f = feature_scale(sum, lo, hi, mode);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        x[k] = bf16x4_t{ (short)feature(f, px[k], 0), (short)feature(f, px[k], 1), (short)feature(f, px[k], 2), (short)feature(f, px[k], 3) };
+}
+
 template <int KT>
 ZT_DEV void propose_body(const zt_propose_args &P) {
-    __shared__ uint4 W1f[128 / 16 * 2 * 64];       // 16 KiB
-    __shared__ uint4 W2f[128 / 16 * 4 * 64];       // 32 KiB
-    __shared__ uint4 W3f[ZT_PR_U / 16 * 4 * 64];   // 72 KiB
-    __shared__ __align__(16) float B1s[128], B2s[128], B3s[ZT_PR_U];
+    __shared__ PropLds L;
+    stage_network(L, P.w, P.b1, P.b2, P.b3);
     const int tid = threadIdx.x, nth = blockDim.x;
-    stage_fragments<64>(W1f, P.w, 128, tid, nth, false);
-    stage_fragments<128>(W2f, P.w + 64 * 128, 128, tid, nth, false);
-    stage_fragments<128>(W3f, P.w + 64 * 128 + 128 * 128, ZT_PR_U, tid, nth, true);
-    for (int i = tid; i < 128; i += nth) { B1s[i] = P.b1[i]; B2s[i] = P.b2[i]; }
-    for (int u = tid; u < ZT_PR_U; u += nth) { const int id = prop_slot(u); B3s[u] = id < 0 ? 0.0f : P.b3[id]; }
-    __syncthreads();
     const int lane = tid & 63, r = lane & 15, q = lane >> 4, mode = P.cfg[0], K = P.topk;
     const uint32_t waves = gridDim.x * (nth >> 6), groups = (P.n + 16 * ZT_PR_NC - 1) / (16 * ZT_PR_NC);
     for (uint32_t g = blockIdx.x * (nth >> 6) + (tid >> 6); g < groups; g += waves) {
-        // features: lane (r, q) holds pixels q, 4 + q, 8 + q, 12 + q of block r = layer 1's B operand of K steps 0..3
         bf16x4_t x[ZT_PR_NC][4];
 #pragma unroll
         for (int ct = 0; ct < ZT_PR_NC; ct++) {
             const uint32_t row = (g * ZT_PR_NC + ct) * 16 + r;
-            uint32_t px[4];
-            const uint32_t *src = (const uint32_t *)P.blocks + (uint64_t)row * 16 + q;
-#pragma unroll
-            for (int k = 0; k < 4; k++) px[k] = row < P.n ? src[4 * k] : 0u;
-            int sum[4], lo[4], hi[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                sum[c] = 0; lo[c] = 255; hi[c] = 0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) { const int v = chan(px[k], c); sum[c] += v; lo[c] = min(lo[c], v); hi[c] = max(hi[c], v); }
-#pragma unroll
-                for (int off = 16; off <= 32; off <<= 1) {
-                    sum[c] += __shfl_xor(sum[c], off);
-                    lo[c] = min(lo[c], __shfl_xor(lo[c], off));
-                    hi[c] = max(hi[c], __shfl_xor(hi[c], off));
-                }
-            }
-            const FeatScale f = feature_scale(sum, lo, hi, mode);
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                x[ct][k] = bf16x4_t{ (short)feature(f, px[k], 0), (short)feature(f, px[k], 1), (short)feature(f, px[k], 2), (short)feature(f, px[k], 3) };
+            tile_features(P.blocks, row, row < P.n, q, mode, x[ct]);
         }
         bf16x4_t h1[ZT_PR_NC][8], h2[ZT_PR_NC][8];
-        layer<8, 4>(W1f, B1s, x, h1, lane, q);
-        layer<8, 8>(W2f, B2s, h1, h2, lane, q);
+        layer<8, 4, ZT_PR_NC>(L.W1f, L.B1, x, h1, lane, q);
+        layer<8, 8, ZT_PR_NC>(L.W2f, L.B2, h1, h2, lane, q);
         // layer 3, one row tile at a time, straight into the top lists
         TopList<KT> T[ZT_PR_NC];
         float riskv[ZT_PR_NC];
@@ -444,31 +475,21 @@ ZT_DEV void propose_body(const zt_propose_args &P) {
         for (int ct = 0; ct < ZT_PR_NC; ct++) { T[ct].init(); riskv[ct] = 0.0f; }
 #pragma unroll
         for (int t = 0; t < ZT_PR_U / 16; t++) {
-            f32x4_t z[ZT_PR_NC];
-#pragma unroll
-            for (int ct = 0; ct < ZT_PR_NC; ct++) z[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const uint4 a = W3f[(t * 4 + p) * 64 + lane];
-#pragma unroll
-                for (int ct = 0; ct < ZT_PR_NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), h2[ct][2 * p], z[ct], 0, 0, 0);
-#pragma unroll
-                for (int ct = 0; ct < ZT_PR_NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), h2[ct][2 * p + 1], z[ct], 0, 0, 0);
-            }
-            const float4 b = *(const float4 *)(B3s + 16 * t + 4 * q);
+            float v[ZT_PR_NC][4];
+            logits_tile<ZT_PR_NC>(L.W3f, L.B3, h2, t, lane, q, v);
 #pragma unroll
             for (int ct = 0; ct < ZT_PR_NC; ct++) {
-                const float v[4] = { z[ct][0] + b.x, z[ct][1] + b.y, z[ct][2] + b.z, z[ct][3] + b.w };
                 const uint32_t row = (g * ZT_PR_NC + ct) * 16 + r;
-                if (P.logits && row < P.n) *(float4 *)(P.logits + (uint64_t)row * ZT_PR_U + 16 * t + 4 * q) = make_float4(v[0], v[1], v[2], v[3]);
+                if (P.logits && row < P.n)
+                    *(float4 *)(P.logits + (uint64_t)row * ZT_PR_U + 16 * t + 4 * q) = make_float4(v[ct][0], v[ct][1], v[ct][2], v[ct][3]);
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int u = 16 * t + 4 * q + i;
                     if (t == ZT_PR_U / 16 - 1) {   // the last tile holds rows 272..287: slots up to 284, the risk row 285
-                        if (u == 285) riskv[ct] = v[i];
-                        T[ct].insert(u < 285 ? v[i] : -INFINITY, u);
+                        if (u == 285) riskv[ct] = v[ct][i];
+                        T[ct].insert(u < 285 ? v[ct][i] : -INFINITY, u);
                     } else {
-                        T[ct].insert(v[i], u);
+                        T[ct].insert(v[ct][i], u);
                     }
                 }
             }
@@ -495,8 +516,98 @@ ZT_DEV void propose_body(const zt_propose_args &P) {
         }
     }
 }
+
+// order-preserving map of a float to uint32 (larger float -> larger key; -inf below every finite value)
+ZT_DEV uint32_t ord_key(float v) { const uint32_t b = __float_as_uint(v); return (b & 0x80000000u) ? ~b : (b | 0x80000000u); }
+
+// The wide proposal of the listed blocks (the learned encode's fallback): each block's k candidate slots of largest
+// logit among the 285, ties at the k-th logit to the lowest slots, written unordered to cand[i * k ..] for list
+// entry i.  The k-th largest logit key T comes from a bisection over the 32-bit order-preserving keys (count of keys
+// >= a trial value, per lane then over the block's four lanes); then every key > T is taken and, of the keys == T,
+// the lowest slots until k are taken.
+ZT_DEV void propose_wide_body(const zt_wide_args &P) {
+    __shared__ PropLds L;
+    stage_network(L, P.w, P.b1, P.b2, P.b3);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int lane = tid & 63, r = lane & 15, q = lane >> 4, mode = P.cfg[0], K = P.k;
+    const uint32_t n = P.count[0], waves = gridDim.x * (nth >> 6), groups = (n + 15) / 16;
+    for (uint32_t g = blockIdx.x * (nth >> 6) + (tid >> 6); g < groups; g += waves) {
+        const uint32_t i_list = g * 16 + r;
+        const bool valid = i_list < n;
+        const uint32_t row = valid ? P.list[i_list] : 0u;
+        bf16x4_t x[1][4];
+        tile_features(P.blocks, row, valid, q, mode, x[0]);
+        bf16x4_t h1[1][8], h2[1][8];
+        layer<8, 4, 1>(L.W1f, L.B1, x, h1, lane, q);
+        layer<8, 8, 1>(L.W2f, L.B2, h1, h2, lane, q);
+        uint32_t key[ZT_PR_U / 16][4];   // rows 285..287 (not candidates) get the key of -inf
+#pragma unroll
+        for (int t = 0; t < ZT_PR_U / 16; t++) {
+            float v[1][4];
+            logits_tile<1>(L.W3f, L.B3, h2, t, lane, q, v);
+#pragma unroll
+            for (int i = 0; i < 4; i++) key[t][i] = (t == ZT_PR_U / 16 - 1 && 4 * q + i >= 285 - 16 * t) ? ord_key(-INFINITY) : ord_key(v[0][i]);
+        }
+        // bisection: T = the largest key value with at least K keys >= it (the K-th largest key of the block)
+        uint32_t T = 0;
+        for (int bit = 31; bit >= 0; bit--) {
+            const uint32_t c = T | (1u << bit);
+            int cnt = 0;
+#pragma unroll
+            for (int t = 0; t < ZT_PR_U / 16; t++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) cnt += key[t][i] >= c ? 1 : 0;
+            cnt += __shfl_xor(cnt, 16);
+            cnt += __shfl_xor(cnt, 32);
+            if (cnt >= K) T = c;
+        }
+        int gt = 0, eq = 0;
+#pragma unroll
+        for (int t = 0; t < ZT_PR_U / 16; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) { gt += key[t][i] > T ? 1 : 0; eq += key[t][i] == T ? 1 : 0; }
+        int gt_b = gt + __shfl_xor(gt, 16), eq_b = eq + __shfl_xor(eq, 16);
+        gt_b += __shfl_xor(gt_b, 32);
+        eq_b += __shfl_xor(eq_b, 32);
+        const int need = K - gt_b;   // keys == T to take, 1 <= need <= eq_b
+        // taken ties as bits (t * 4 + i) of this lane; all of them unless the block has more ties than it needs
+        uint32_t take[3] = { 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu };
+        if (__any(eq_b > need)) {   // rare: walk the block's slots in ascending order (t, q, i) and take the first `need`
+            take[0] = take[1] = take[2] = 0u;
+            int rem = need;
+#pragma unroll
+            for (int t = 0; t < ZT_PR_U / 16; t++)
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const bool took = q == qq && key[t][i] == T && rem > 0;
+                        if (took) take[(t * 4 + i) >> 5] |= 1u << ((t * 4 + i) & 31);
+                        rem -= __shfl(took ? 1 : 0, r + 16 * qq);
+                    }
+        }
+        // this lane's selections, their offset among the block's four lanes, then the stores
+        int mine = 0;
+#pragma unroll
+        for (int t = 0; t < ZT_PR_U / 16; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) mine += (key[t][i] > T || (key[t][i] == T && ((take[(t * 4 + i) >> 5] >> ((t * 4 + i) & 31)) & 1u))) ? 1 : 0;
+        const int m16 = __shfl_xor(mine, 16), m32 = __shfl_xor(mine, 32), m48 = __shfl_xor(m16, 32);
+        // exclusive prefix over q: lane q's partners are q ^ 1 (m16), q ^ 2 (m32), q ^ 3 (m48)
+        const int off = q == 0 ? 0 : q == 1 ? m16 : q == 2 ? m32 + m48 : m16 + m32 + m48;
+        if (valid) {
+            uint32_t *dst = P.cand + (uint64_t)i_list * K + off;
+#pragma unroll
+            for (int t = 0; t < ZT_PR_U / 16; t++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (key[t][i] > T || (key[t][i] == T && ((take[(t * 4 + i) >> 5] >> ((t * 4 + i) & 31)) & 1u))) *dst++ = (uint32_t)prop_slot(16 * t + 4 * q + i);
+        }
+    }
+}
 }  // namespace
 
 // K <= 4 and K <= 16
 extern "C" __global__ void __launch_bounds__(512) zt_mlp_propose4(zt_propose_args P) { propose_body<4>(P); }
 extern "C" __global__ void __launch_bounds__(512) zt_mlp_propose16(zt_propose_args P) { propose_body<16>(P); }
+extern "C" __global__ void __launch_bounds__(256) zt_mlp_propose_wide(zt_wide_args P) { propose_wide_body(P); }

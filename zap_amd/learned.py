@@ -149,9 +149,10 @@ def _time(fn, steps: int) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def evaluate(lib, chain: Chain, state, topk: int, thresholds, steps: int = 5) -> dict:
+def evaluate(lib, chain: Chain, state, topk: int, thresholds, steps: int = 5, fallback_k: int = 0) -> dict:
     """Encode time (ms, inputs resident) and chain PSNR for the plain search, the exhaustive search and the learned
-    proposal at each threshold; 'fallback' is the fraction of blocks the learned encode searched exhaustively."""
+    proposal at each threshold; 'fallback' is the fraction of blocks the learned encode sent to its fallback (the
+    exhaustive search with fallback_k = 0, else the network's top fallback_k)."""
     import torch
     imgs = chain.images()
     ws = torch.empty(lib.workspace_size(imgs), dtype=torch.uint8, device="cuda")
@@ -161,10 +162,10 @@ def evaluate(lib, chain: Chain, state, topk: int, thresholds, steps: int = 5) ->
     fws = lib.bc7_full_workspace(imgs)
     res["full"] = {"ms": _time(lambda: lib.bc7_full_dev(imgs, ws=fws), steps)}
     res["full"]["psnr"] = chain.psnr()
-    lws = lib.bc7_learned_workspace(imgs, topk)
+    lws = lib.bc7_learned_workspace(imgs, topk, fallback_k)
     res["learned"] = []
     for thr in thresholds:
-        ms = _time(lambda: lib.bc7_learned_dev(imgs, state, topk, float(thr), lws), steps)
+        ms = _time(lambda: lib.bc7_learned_dev(imgs, state, topk, float(thr), lws, fallback_k), steps)
         fb = int(lws[:256].view(torch.int32)[8].item())
         p = chain.psnr()
         res["learned"].append({"threshold": float(thr), "ms": ms, "psnr": p, "fallback": fb / chain.blocks,
@@ -174,14 +175,14 @@ def evaluate(lib, chain: Chain, state, topk: int, thresholds, steps: int = 5) ->
     return res
 
 
-def calibrate(lib, state, topk: int, textures=CALIB, max_gap_db: float = 0.04, log=print) -> dict:
-    """Fastest threshold (predicted SSE a block may give away before it is searched exhaustively) whose chain-PSNR
-    gap to the exhaustive search stays within max_gap_db on every held-out texture; speed is pooled over them.
+def calibrate(lib, state, topk: int, textures=CALIB, max_gap_db: float = 0.04, fallback_k: int = 0, log=print) -> dict:
+    """Fastest threshold (predicted SSE a block may give away before it falls back) whose chain-PSNR gap to the
+    exhaustive search stays within max_gap_db on every held-out texture; speed is pooled over them.
     The margin under 0.05 dB leaves room for texture-to-texture variation."""
     grid = [1e30, 64, 32, 16, 12, 8, 6, 5, 4, 3.5, 3, 2.5, 2, 1.75, 1.5, 1.25, 1, 0.75, 0.5, 0.25, -1]
     per = []
     for side, seed in textures:
-        per.append(evaluate(lib, Chain(lib, _texture((side, seed))), state, topk, grid))
+        per.append(evaluate(lib, Chain(lib, _texture((side, seed))), state, topk, grid, fallback_k=fallback_k))
     px = sum(r["pixels"] for r in per)
     full_ms = sum(r["full"]["ms"] for r in per)
     rows = []
@@ -197,8 +198,8 @@ def calibrate(lib, state, topk: int, textures=CALIB, max_gap_db: float = 0.04, l
         for r in rows:
             log(f"  thr {r['threshold']:8.4g}: {r['mpix_s']:9.1f} Mpix/s  gaps {r['gaps_db']} dB  fallback {r['fallback']:.3f}", flush=True)
         log(f"  exhaustive {full:.1f} Mpix/s -> threshold {best['threshold']}", flush=True)
-    return {"threshold": best["threshold"], "topk": topk, "textures": [list(t) for t in textures], "exhaustive_mpix_s": full,
-            "rows": rows}
+    return {"threshold": best["threshold"], "topk": topk, "fallback_k": fallback_k, "textures": [list(t) for t in textures],
+            "exhaustive_mpix_s": full, "rows": rows}
 
 
 def save(lib, state, meta: dict, path: str = MODEL_PATH) -> None:
@@ -230,9 +231,20 @@ if __name__ == "__main__":
     ap.add_argument("--temperature", type=float, default=0.0, help="soft-target temperature in SSE units (0: one-hot labels)")
     ap.add_argument("--risk-weight", type=float, default=0.2)
     ap.add_argument("--features", type=int, default=4, help="input normalisation (zaptex_mlp_set_features)")
+    ap.add_argument("--fallback-k", type=int, default=64,
+                    help="candidates (the network's top-k) for blocks over the risk threshold; 0: the exhaustive search")
     ap.add_argument("--out", default=MODEL_PATH, help="model file (.npy; the .json beside it holds the metadata)")
+    ap.add_argument("--recalibrate", action="store_true", help="keep the weights of --out, only recalibrate its threshold")
     a = ap.parse_args()
     lib = binding.load()
+    if a.recalibrate:
+        st, meta = load(lib, a.out)
+        cal = calibrate(lib, st, int(meta["topk"]), fallback_k=a.fallback_k)
+        meta.update({"threshold": cal["threshold"], "fallback_k": a.fallback_k,
+                     "calibration": {k: cal[k] for k in ("textures", "exhaustive_mpix_s", "rows")}})
+        save(lib, st, meta, a.out)
+        print(json.dumps({"threshold": cal["threshold"], "fallback_k": a.fallback_k}), flush=True)
+        raise SystemExit(0)
     t0 = time.time()
     mix = tuple(tuple(int(v) for v in m.split("x")) for m in a.mix.split(","))
     xb, yb, db = training_set(lib, mix)
@@ -245,11 +257,11 @@ if __name__ == "__main__":
     print(f"trained in {t_train:.1f}s", flush=True)
     n_blocks = int(xb.shape[0])
     del xb, yb, db
-    cal = calibrate(lib, st, a.topk)
+    cal = calibrate(lib, st, a.topk, fallback_k=a.fallback_k)
     save(lib, st, {"blocks": n_blocks, "mix": [list(m) for m in mix], "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
                    "temperature": a.temperature, "risk_weight": a.risk_weight, "features": a.features,
                    "label_seconds": round(t_label, 1), "train_seconds": round(t_train, 1), "history": hist,
                    "train_seeds": f"{TRAIN_SEED0}..{TRAIN_SEED0 + sum(c for _, c in mix) - 1}",
-                   "threshold": cal["threshold"], "topk": a.topk,
+                   "threshold": cal["threshold"], "topk": a.topk, "fallback_k": a.fallback_k,
                    "calibration": {k: cal[k] for k in ("textures", "exhaustive_mpix_s", "rows")}}, a.out)
     print(json.dumps({"last_epoch": hist[-1], "threshold": cal["threshold"]}), flush=True)
