@@ -354,14 +354,14 @@ ZT_EXH(zt_bc7_exh_m6, 6, 0)
 ZT_EXH(zt_bc7_exh_m7, 7, 0)
 #undef ZT_EXH
 
-// The winner of every block (or, with `list`, of the count[0] listed blocks): its candidate re-encoded (one
-// candidate per block, so the all-mode switch costs little), the 16 bytes stored, and the candidate slot written to
+// The winner of every block (or, with `items`, of the count[0] blocks items[i].x, best sorted by winning mode so a
+// wave runs one mode's code): its candidate re-encoded, the 16 bytes stored, and the candidate slot written to
 // best_cand (optional).
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key,
-                                                                  uint32_t *best_cand, const uint32_t *list, const unsigned *count) {
-    const uint32_t step = gridDim.x * blockDim.x, n = list ? count[0] : total;
+                                                                  uint32_t *best_cand, const uint2 *items, const unsigned *count) {
+    const uint32_t step = gridDim.x * blockDim.x, n = items ? count[0] : total;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-        const uint32_t g = list ? list[i] : i;
+        const uint32_t g = items ? items[i].x : i;
         const uint32_t k = key[g];
         if (k == 0xFFFFFFFFu) continue;
         uint32_t px[16];
@@ -403,11 +403,37 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_gather(zt_jobs jobs, ui
     }
 }
 
-// pass 0: count proposals per mode (counters[0..7]); pass 1: scatter them into mode-contiguous runs starting
-// at counters[16 + mode] (prefix sums filled in between by zt_bc7_bucket_scan), record each proposal's item index
-// in slot[block * K + k] (all 0xFFFFFFFF for a fallback block), and put high-risk blocks into the fallback list
-// (counters[8]; they get no proposals).  Each workgroup tallies in LDS and reserves its ranges with
-// one global atomic per mode, so the global counters see a few atomics per workgroup, not one per proposal.
+// This lane's slot among the workgroup's items of class m (0 <= m < NM) in LDS tallies cnt[]: one LDS atomic per wave
+// and class present (a ballot per class), not one per item on a handful of shared addresses.
+template <int NM>
+ZT_DEV unsigned lds_tally(unsigned *cnt, int m, bool valid) {
+    const int lane = __lane_id();
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned local = 0;
+#pragma unroll
+    for (int mm = 0; mm < NM; mm++) {
+        const uint64_t b = __ballot(valid && m == mm);
+        if (!b) continue;
+        const int leader = __ffsll((unsigned long long)b) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[mm], (unsigned)__popcll(b));
+        base = __shfl(base, leader);
+        if (valid && m == mm) local = base + (unsigned)__popcll(b & below);
+    }
+    return local;
+}
+
+// Bucketing in two passes over the same grid.  Pass 0 tallies each workgroup's items per class (mode 0..7; class 8:
+// fallback blocks) in LDS and adds the tallies to the class's counter of shard blockIdx % 8 -- each (class, shard)
+// counter on its own 256-byte line, since atomics on one line complete one after another.  zt_bc7_bucket_scan
+// turns the counters into run starts (class-contiguous runs, shard sub-runs in order) and fills the summary words;
+// pass 1 recomputes the same tallies and reserves its ranges from the same counters.
+ZT_DEV unsigned *bucket_shard(unsigned *counters, int m) {
+    return counters + ZT_BUCKET_SHARDS_AT + (m * ZT_SHARDS + (int)(blockIdx.x & (ZT_SHARDS - 1))) * ZT_SHARD_STRIDE;
+}
+// the proposals of every block: K candidates each, or none for a fallback block (risk > threshold), which goes to the
+// fallback list instead; pass 1 also records each proposal's item index in slot[block * K + k] (0xFFFFFFFF for a
+// fallback block)
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K,
                                                                   float threshold, unsigned *counters, uint2 *items, uint32_t pass,
                                                                   uint32_t *fallback, uint32_t *slot) {
@@ -420,16 +446,17 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *
         const bool valid = g < total;
         const bool fb = valid && risk[g] > threshold;
         uint32_t c[16];
-        unsigned local[16], lfb = 0;
+        unsigned local[16];
         for (int k = 0; k < K; k++) {
             c[k] = valid && !fb ? cand[(uint64_t)g * K + k] : 0xFFFFFFFFu;
-            if (c[k] != 0xFFFFFFFFu) local[k] = atomicAdd(&cnt[c[k] >> 6], 1u);
+            local[k] = lds_tally<8>(cnt, (int)(c[k] >> 6), c[k] != 0xFFFFFFFFu);
         }
-        if (pass == 1 && fb) lfb = atomicAdd(&cnt[8], 1u);
+        const unsigned lfb = lds_tally<1>(cnt + 8, 0, fb);
         __syncthreads();
         if (threadIdx.x < 9 && cnt[threadIdx.x]) {
-            if (pass == 0) { if (threadIdx.x < 8) atomicAdd(&counters[threadIdx.x], cnt[threadIdx.x]); }
-            else base[threadIdx.x] = atomicAdd(&counters[threadIdx.x == 8 ? 8 : 16 + threadIdx.x], cnt[threadIdx.x]);
+            unsigned *ctr = bucket_shard(counters, threadIdx.x);
+            if (pass == 0) atomicAdd(ctr, cnt[threadIdx.x]);
+            else base[threadIdx.x] = atomicAdd(ctr, cnt[threadIdx.x]);
         }
         __syncthreads();
         if (pass == 1) {
@@ -442,10 +469,11 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket(const uint32_t *
         __syncthreads();
     }
 }
-// the wide proposals of the fallback list, bucketed like zt_bc7_bucket (pass 0 counts per mode, pass 1 scatters into
-// the runs zt_bc7_bucket_scan lays out): item e < count[0] * K is candidate cand[e] of block list[e / K]
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket_wide(const uint32_t *cand, const uint32_t *list, const unsigned *count, int K,
-                                                                       unsigned *counters, uint2 *items, uint32_t pass) {
+// the wide proposals of the fallback list: item e < count[0] * K is candidate cand[e] of block list[e / K]; without
+// cand (K = 1), the block's current winner key[list[e]] & 511
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket_wide(const uint32_t *cand, const uint32_t *key, const uint32_t *list,
+                                                                       const unsigned *count, int K, unsigned *counters, uint2 *items,
+                                                                       uint32_t pass) {
     __shared__ unsigned cnt[8], base[8];
     const uint64_t n = (uint64_t)count[0] * (uint64_t)K, step = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += step) {
@@ -453,22 +481,36 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_bucket_wide(const uint3
         __syncthreads();
         const uint64_t e = b0 + threadIdx.x;
         const bool valid = e < n;
-        uint32_t c = 0, local = 0;
-        if (valid) { c = cand[e]; local = atomicAdd(&cnt[c >> 6], 1u); }
+        const uint32_t c = !valid ? 0u : cand ? cand[e] : key[list[e]] & 511u;
+        const uint32_t local = lds_tally<8>(cnt, (int)(c >> 6), valid);
         __syncthreads();
         if (threadIdx.x < 8 && cnt[threadIdx.x]) {
-            if (pass == 0) atomicAdd(&counters[threadIdx.x], cnt[threadIdx.x]);
-            else base[threadIdx.x] = atomicAdd(&counters[16 + threadIdx.x], cnt[threadIdx.x]);
+            unsigned *ctr = bucket_shard(counters, threadIdx.x);
+            if (pass == 0) atomicAdd(ctr, cnt[threadIdx.x]);
+            else base[threadIdx.x] = atomicAdd(ctr, cnt[threadIdx.x]);
         }
         __syncthreads();
         if (pass == 1 && valid) items[base[c >> 6] + local] = make_uint2(list[e / (uint64_t)K], c);
         __syncthreads();
     }
 }
+// shard counts -> run starts: counters[m] = items of mode m, counters[24 + m] = its run's start, counters[8] =
+// fallback blocks; each shard counter becomes its sub-run's start (the fallback list starts at 0)
 extern "C" __global__ void zt_bc7_bucket_scan(unsigned *counters) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        unsigned s = 0;
-        for (int m = 0; m < 8; m++) { counters[24 + m] = s; counters[16 + m] = s; s += counters[m]; }
+        unsigned run = 0;
+        for (int m = 0; m < 9; m++) {
+            if (m == 8) run = 0;
+            const unsigned start = run;
+            for (int s = 0; s < ZT_SHARDS; s++) {
+                unsigned *ctr = counters + ZT_BUCKET_SHARDS_AT + (m * ZT_SHARDS + s) * ZT_SHARD_STRIDE;
+                const unsigned c = *ctr;
+                *ctr = run;
+                run += c;
+            }
+            counters[m] = run - start;
+            if (m < 8) counters[24 + m] = start;
+        }
     }
 }
 

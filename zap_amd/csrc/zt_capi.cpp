@@ -602,7 +602,7 @@ int zaptex_bc7_full_dev(const zaptex_image *imgs, int n, uint32_t *best_cand, ui
     if ((rc = enqueue_exhaustive(blocks, total, nullptr, nullptr, key, errs, s))) return rc;
     if ((rc = launch("zt_bc7_finish", s, [&] {
             hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total, 256)), dim3(256), 0, s, J, (const uint4 *)blocks, total, (const uint32_t *)key, best_cand,
-                               (const uint32_t *)nullptr, (const unsigned *)nullptr); })))
+                               (const uint2 *)nullptr, (const unsigned *)nullptr); })))
         return rc;
     if (!deltas) return ZAPTEX_OK;
     return launch("zt_bc7_deltas", s, [&] {
@@ -857,23 +857,27 @@ int zaptex_bc7_gather_dev(const zaptex_image *imgs, int n, void *blocks, void *s
 // ---- learned-proposal BC7 encode
 namespace {
 // per block: its pixels (64 B), K proposals (item 8 B, encoding 16 B, candidate 4 B, item index 4 B), key, risk and
-// a fallback-list entry; with a wide fallback (K2 > 0) also K2 wide candidates (4 B) and items (8 B)
+// a fallback-list entry and its mode-sorted finish item (8 B); with a wide fallback (K2 > 0) also K2 wide candidates
+// (4 B) and items (8 B)
 struct LearnedWs {
     uint4 *blocks, *enc;
-    uint2 *items, *items2;
+    uint2 *items, *items2, *items3;
     uint32_t *cand, *slot, *fallback, *key, *cand2;
     float *risk;
-    unsigned *counters, *counters2;
+    unsigned *counters, *counters2, *counters3;
 };
+constexpr uint64_t kBucketBytes = ZT_BUCKET_WORDS * 4;   // one bucketing counter array (a multiple of 256 B)
 size_t learned_ws_bytes(uint64_t total, int K, int K2) {
-    return 256 + 512 + total * (64 + 32 * (uint64_t)K + 4 + 4 + 4) + total * 12 * (uint64_t)K2 + 1024;
+    return 256 + 3 * kBucketBytes + total * (64 + 32 * (uint64_t)K + 4 + 4 + 4 + 8) + total * 12 * (uint64_t)K2 + 1024;
 }
 LearnedWs learned_ws(void *ws, uint64_t total, int K, int K2) {
     uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
     LearnedWs v;
-    v.counters = (unsigned *)b; b += 256;
-    v.counters2 = (unsigned *)b; b += 256;
+    v.counters = (unsigned *)b; b += kBucketBytes;
+    v.counters2 = (unsigned *)b; b += kBucketBytes;
+    v.counters3 = (unsigned *)b; b += kBucketBytes;
     v.blocks = (uint4 *)b; b += total * 64;
+    v.items3 = (uint2 *)b; b += total * 8;
     v.enc = (uint4 *)b; b += total * 16 * K;
     v.items = (uint2 *)b; b += total * 8 * K;
     v.items2 = (uint2 *)b; b += total * 8 * (uint64_t)K2;
@@ -924,7 +928,9 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
     LearnedWs w = learned_ws(ws, total, topk, fallback_k);
     MlpState st = mlp_view((void *)state);
     int rc;
-    if ((rc = launch("zt_zero_u32", s, [&] { hipLaunchKernelGGL(zt_zero_u32, dim3(1), dim3(128), 0, s, w.counters, 128); }))) return rc;
+    if ((rc = launch("zt_zero_u32", s, [&] {
+            hipLaunchKernelGGL(zt_zero_u32, dim3((3 * ZT_BUCKET_WORDS + 255) / 256), dim3(256), 0, s, w.counters, 3 * ZT_BUCKET_WORDS); })))
+        return rc;
     if ((rc = launch("zt_fill_u32", s, [&] { hipLaunchKernelGGL(zt_fill_u32, dim3(grid_for(total, 256)), dim3(256), 0, s, w.key, (uint64_t)total, 0xFFFFFFFFu); })))
         return rc;
     if ((rc = launch("zt_bc7_gather", s, [&] { hipLaunchKernelGGL(zt_bc7_gather, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, w.blocks); })))
@@ -952,18 +958,27 @@ int zaptex_bc7_learned_dev(const zaptex_image *imgs, int n, const void *state, i
         const dim3 bgrid((unsigned)std::min<uint64_t>((uint64_t)grid_for(max_items, 256), 4096));
         for (uint32_t pass = 0; pass < 2; pass++) {
             if ((rc = launch("zt_bc7_bucket_wide", s, [&] {
-                    hipLaunchKernelGGL(zt_bc7_bucket_wide, bgrid, dim3(256), 0, s, (const uint32_t *)w.cand2, (const uint32_t *)w.fallback,
-                                       (const unsigned *)(w.counters + 8), fallback_k, w.counters2, w.items2, pass); })))
+                    hipLaunchKernelGGL(zt_bc7_bucket_wide, bgrid, dim3(256), 0, s, (const uint32_t *)w.cand2, (const uint32_t *)nullptr,
+                                       (const uint32_t *)w.fallback, (const unsigned *)(w.counters + 8), fallback_k, w.counters2, w.items2, pass); })))
                 return rc;
             if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters2); })))
                 return rc;
         }
         if ((rc = enqueue_props(w.blocks, w.items2, w.counters2, w.key, nullptr, max_items, s))) return rc;
     }
-    // fallback blocks re-encode their winner; the others copy their winning proposal's stored encoding
+    // fallback blocks re-encode their winner, sorted by its mode; the others copy their winning proposal's stored encoding
+    const dim3 fgrid((unsigned)grid_for(total / 16 + 1, 256));
+    for (uint32_t pass = 0; pass < 2; pass++) {
+        if ((rc = launch("zt_bc7_bucket_wide", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_bucket_wide, fgrid, dim3(256), 0, s, (const uint32_t *)nullptr, (const uint32_t *)w.key,
+                                   (const uint32_t *)w.fallback, (const unsigned *)(w.counters + 8), 1, w.counters3, w.items3, pass); })))
+            return rc;
+        if (pass == 0 && (rc = launch("zt_bc7_bucket_scan", s, [&] { hipLaunchKernelGGL(zt_bc7_bucket_scan, dim3(1), dim3(64), 0, s, w.counters3); })))
+            return rc;
+    }
     if ((rc = launch("zt_bc7_finish", s, [&] {
-            hipLaunchKernelGGL(zt_bc7_finish, dim3(grid_for(total / 16 + 1, 256)), dim3(256), 0, s, J, (const uint4 *)w.blocks, total,
-                               (const uint32_t *)w.key, (uint32_t *)nullptr, (const uint32_t *)w.fallback, (const unsigned *)(w.counters + 8)); })))
+            hipLaunchKernelGGL(zt_bc7_finish, fgrid, dim3(256), 0, s, J, (const uint4 *)w.blocks, total, (const uint32_t *)w.key, (uint32_t *)nullptr,
+                               (const uint2 *)w.items3, (const unsigned *)(w.counters + 8)); })))
         return rc;
     return launch("zt_bc7_finish_learned", s, [&] {
         hipLaunchKernelGGL(zt_bc7_finish_learned, dim3(grid_for(total, 256)), dim3(256), 0, s, J, total, topk, (const uint32_t *)w.cand,

@@ -93,7 +93,7 @@ __global__ void zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint
 ZT_EXH_DECL(zt_bc7_exh_m0) ZT_EXH_DECL(zt_bc7_exh_m1) ZT_EXH_DECL(zt_bc7_exh_m2) ZT_EXH_DECL(zt_bc7_exh_m3)
 ZT_EXH_DECL(zt_bc7_exh_m4i0) ZT_EXH_DECL(zt_bc7_exh_m4i1) ZT_EXH_DECL(zt_bc7_exh_m5) ZT_EXH_DECL(zt_bc7_exh_m6) ZT_EXH_DECL(zt_bc7_exh_m7)
 #undef ZT_EXH_DECL
-__global__ void zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key, uint32_t *best_cand, const uint32_t *list,
+__global__ void zt_bc7_finish(zt_jobs jobs, const uint4 *blocks, uint32_t total, const uint32_t *key, uint32_t *best_cand, const uint2 *items,
                               const unsigned *count);
 __global__ void zt_bc7_deltas(const uint32_t *errs, const uint32_t *key, uint32_t total, uint16_t *deltas);
 
@@ -111,13 +111,16 @@ __global__ void zt_mlp_propose4(zt_propose_args P);    // topk <= 4
 __global__ void zt_mlp_propose16(zt_propose_args P);   // topk <= 16
 __global__ void zt_mlp_propose_wide(zt_wide_args P);
 __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, const int *bf_index);
-// learned-proposal BC7 (zt_bc7_full.hip)
+// learned-proposal BC7 (zt_bc7_full.hip).  A bucketing counter array: [0..7] items per mode, [8] fallback blocks,
+// [24..31] each mode's run start, then one counter per (class 0..8, shard) on its own line from ZT_BUCKET_SHARDS_AT.
+#define ZT_BUCKET_SHARDS_AT 64
+#define ZT_BUCKET_WORDS (ZT_BUCKET_SHARDS_AT + 9 * ZT_SHARDS * ZT_SHARD_STRIDE)
 __global__ void zt_bc7_gather(zt_jobs jobs, uint32_t total, uint4 *blocks);
 __global__ void zt_bc7_bucket(const uint32_t *cand, const float *risk, uint32_t total, int K, float threshold, unsigned *counters,
                               uint2 *items, uint32_t pass, uint32_t *fallback, uint32_t *slot);
 __global__ void zt_bc7_bucket_scan(unsigned *counters);
-__global__ void zt_bc7_bucket_wide(const uint32_t *cand, const uint32_t *list, const unsigned *count, int K, unsigned *counters, uint2 *items,
-                                   uint32_t pass);
+__global__ void zt_bc7_bucket_wide(const uint32_t *cand, const uint32_t *key, const uint32_t *list, const unsigned *count, int K,
+                                   unsigned *counters, uint2 *items, uint32_t pass);
 __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
 #define ZT_PROP_DECL(NAME) __global__ void NAME(const uint4 *blocks, const uint2 *items, const unsigned *counters, uint32_t *key, uint4 *enc);
 ZT_PROP_DECL(zt_bc7_prop_m0) ZT_PROP_DECL(zt_bc7_prop_m1) ZT_PROP_DECL(zt_bc7_prop_m2) ZT_PROP_DECL(zt_bc7_prop_m3)
