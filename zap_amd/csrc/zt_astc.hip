@@ -252,25 +252,6 @@ ZT_DEV uint32_t two_means(const uint32_t px[16]) {  // zap_tex.h:1263-1283
     return m;
 }
 
-// the 4 stored 2-partition patterns nearest (Hamming, either labelling) to the block's 2-means split
-// (zap_tex.h:1328-1336): mask | seed << 16 each, all ones where fewer than 4 patterns exist
-ZT_DEV void nearest_patterns(const uint32_t px[16], uint32_t out[4]) {
-    const uint32_t km = two_means(px);
-    int cand[4] = { -1, -1, -1, -1 }, cd[4] = { 99, 99, 99, 99 };
-    const int np = g_astc.np;
-    for (int pi = 0; pi < np && cd[3]; pi++) {  // stops once 4 exact matches are found
-        int dd = __popc((uint32_t)g_astc.pmask[pi] ^ km), di = 16 - dd, dist = dd < di ? dd : di;
-        if (dist < cd[3]) {  // insertion into the sorted 4-list (strict <: earlier patterns stay first)
-            if (dist < cd[0]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = cd[0]; cand[1] = cand[0]; cd[0] = dist; cand[0] = pi; }
-            else if (dist < cd[1]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = cd[1]; cand[2] = cand[1]; cd[1] = dist; cand[1] = pi; }
-            else if (dist < cd[2]) { cd[3] = cd[2]; cand[3] = cand[2]; cd[2] = dist; cand[2] = pi; }
-            else { cd[3] = dist; cand[3] = pi; }
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) out[j] = cand[j] < 0 ? 0xFFFFFFFFu : (uint32_t)g_astc.pmask[cand[j]] | (uint32_t)g_astc.pseed[cand[j]] << 16;
-}
-
 // ---- decoder (zap_tex.h:1135-1168): the encoder's subset plus void-extent blocks; anything else is magenta
 ZT_DEV void decode_block(const uint8_t *blk, uint32_t out[16]) {
     BitReader R = reader_at(blk);
@@ -480,15 +461,50 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_pick1(zt_jobs jobs, ui
     }
 }
 
+// 16 lanes per listed block (4 blocks per wave): each lane computes the block's 2-means split (zap_tex.h:1263-1283)
+// and scores patterns lane, lane + 16, ... as (Hamming distance under either labelling) << 10 | pattern, keeping
+// its 4 smallest; the 16 lanes then merge theirs.  The 4 smallest keys are the reference's scan result
+// (zap_tex.h:1328-1336: ascending distance, earlier patterns first among equal distances).
 extern "C" __global__ void __launch_bounds__(256) zt_astc_search(zt_astc_ws W) {
-    const uint32_t n = shard_total(list_counts(W, 2)), step = gridDim.x * blockDim.x;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
+    const unsigned *counts = list_counts(W, 2);
+    const uint32_t n = shard_total(counts), groups = gridDim.x * (blockDim.x >> 4);
+    const int sub = threadIdx.x & 15, np = g_astc.np;
+    const uint32_t e0 = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
+    const uint32_t trips = (n + groups - 1) / groups;   // the same for every lane: the shuffles below need whole waves
+    for (uint32_t t = 0; t < trips; t++) {
+        const uint32_t e = e0 + t * groups;
+        const bool valid = e < n;
+        if (!__any(valid)) continue;   // whole idle waves skip the work (a wave's groups hold consecutive entries)
         uint32_t px[16];
-        load_px(W, W.lp[shard_slot(list_counts(W, 2), W.cap, e)], px);
-        uint32_t c[4];
-        nearest_patterns(px, c);
+        load_px(W, valid ? W.lp[shard_slot(counts, W.cap, e)] : 0u, px);   // idle groups read block 0 and store nothing
+        const uint32_t km = two_means(px);
+        uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0xFFFFFFFFu, b3 = 0xFFFFFFFFu;   // ascending
+#pragma unroll 8
+        for (int j = 0; j < 1024 / 16; j++) {
+            const int pi = sub + 16 * j;
+            if (pi >= np) break;
+            const int dd = __popc((uint32_t)g_astc.pmask[pi] ^ km), dist = min(dd, 16 - dd);
+            const uint32_t x = (uint32_t)dist << 10 | (uint32_t)pi;
+            b3 = max(b2, min(b3, x));
+            b2 = max(b1, min(b2, x));
+            b1 = max(b0, min(b1, x));
+            b0 = min(b0, x);
+        }
+        uint32_t out[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) W.cand[(uint64_t)e * 4 + j] = c[j];
+        for (int r = 0; r < 4; r++) {
+            uint32_t m = b0;
+#pragma unroll
+            for (int off = 8; off >= 1; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off));
+            out[r] = m;
+            const bool pop = b0 == m;
+            b0 = pop ? b1 : b0; b1 = pop ? b2 : b1; b2 = pop ? b3 : b2; b3 = pop ? 0xFFFFFFFFu : b3;
+        }
+        if (valid && sub < 4) {
+            const uint32_t k = sub == 0 ? out[0] : sub == 1 ? out[1] : sub == 2 ? out[2] : out[3];
+            const int pi = (int)(k & 1023u);
+            W.cand[(uint64_t)e * 4 + sub] = k == 0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)g_astc.pmask[pi] | (uint32_t)g_astc.pseed[pi] << 16;
+        }
     }
 }
 
