@@ -199,7 +199,7 @@ def configs_leg(lib, golden: dict, images: dict, steps: int) -> dict:
         for k in kernels:
             t, n = lib.profile_get(k)
             if n:
-                kern[k] = round(t / 5, 5)   # ms per encode (zt_astc_single and zt_astc_pair launch per configuration)
+                kern[k] = round(t / 5, 5)   # ms per encode
         lib.profile(False)
         out[name] = {"pixels": w * h, "blocks": ((w + 3) // 4) * ((h + 3) // 4), "ms": round(ms, 4),
                      "mpix_s": round(w * h / (ms / 1e3) / 1e6, 1), "byte_exact_vs_reference": exact, "kernel_ms_per_encode": kern}

@@ -4,10 +4,9 @@
 // start, quantisation through the configuration's 8-bit -> ISE map, a weight argmin and <= 3 least-squares
 // refinements; opaque blocks with SSE > 192 then try the 4 stored 2-partition patterns nearest to a 2-means
 // split.  Configurations are compile-time (their colour ranges are fixed by the spec rule, checked at set-up),
-// so every per-texel loop unrolls into registers.  The encoder runs in stages (zt_kernels.h): one kernel per
-// configuration over a list of the blocks that try it (one lane per block or per (block, pattern)), so each
-// kernel holds one trial's registers and no wave mixes opaque and translucent blocks; picks replay the
-// reference's comparison order.
+// so every per-texel loop unrolls into registers.  The encoder runs in stages (zt_kernels.h): the trials are
+// items (configuration, listed block) or (configuration, block, pattern), configuration-major, so each lane holds
+// one trial's registers and a wave runs one configuration's code; picks replay the reference's comparison order.
 #include "zt_astc_setup.h"
 #include "zt_common.h"
 #include "zt_kernels.h"
@@ -354,38 +353,32 @@ ZT_DEV void list_append(const zt_astc_ws &W, int k, uint32_t *list, bool pred, u
     const int slot = wave_append(pred, c);
     if (slot >= 0) list[(uint32_t)((c - list_counts(W, k)) / ZT_SHARD_STRIDE) * W.cap + (uint32_t)slot] = g;
 }
-template <int K> ZT_DEV void single_body(const zt_astc_ws &W, const AstcLds &T) {
-    const unsigned *counts = list_counts(W, K < 3 ? 0 : 1);
-    const uint32_t n = shard_total(counts), step = gridDim.x * blockDim.x;
-    const uint32_t *list = K < 3 ? W.lo : W.lt;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
-        const uint32_t g = list[shard_slot(counts, W.cap, e)];
-        uint32_t px[16];
-        load_px(W, g, px);
-        int ise[12];
-        uint32_t w[4];
-        uint64_t lo, hi;
-        const int err = trial<K>(px, 0, ise, w, T);
-        pack<K>(0, ise, w, lo, hi);
-        W.err1[(uint64_t)g * 3 + K % 3] = err;
-        W.blk1[(uint64_t)g * 3 + K % 3] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
-    }
+// configuration K's trial of list entry e (opaque list for K < 3, translucent otherwise)
+template <int K> ZT_DEV void single_item(const zt_astc_ws &W, const AstcLds &T, uint32_t e) {
+    const uint32_t g = (K < 3 ? W.lo : W.lt)[shard_slot(list_counts(W, K < 3 ? 0 : 1), W.cap, e)];
+    uint32_t px[16];
+    load_px(W, g, px);
+    int ise[12];
+    uint32_t w[4];
+    uint64_t lo, hi;
+    const int err = trial<K>(px, 0, ise, w, T);
+    pack<K>(0, ise, w, lo, hi);
+    W.err1[(uint64_t)g * 3 + K % 3] = err;
+    W.blk1[(uint64_t)g * 3 + K % 3] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
-template <int K> ZT_DEV void pair_body(const zt_astc_ws &W, const AstcLds &T) {
-    const uint32_t n = shard_total(list_counts(W, 2)) * 4u, step = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-        const uint32_t c = W.cand[i], o = (i >> 2) * 8 + (i & 3) * 2 + (K - 6);
-        if (c == 0xFFFFFFFFu) { W.err2[o] = 0x7FFFFFFF; continue; }
-        uint32_t px[16];
-        load_px(W, W.lp[shard_slot(list_counts(W, 2), W.cap, i >> 2)], px);
-        int ise[12];
-        uint32_t w[4];
-        uint64_t lo, hi;
-        const int err = trial<K>(px, c & 0xFFFFu, ise, w, T);
-        pack<K>((int)(c >> 16), ise, w, lo, hi);
-        W.err2[o] = err;
-        W.blk2[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
-    }
+// configuration K's trial of pattern candidate i (= listed block i / 4, its pattern i % 4)
+template <int K> ZT_DEV void pair_item(const zt_astc_ws &W, const AstcLds &T, uint32_t i) {
+    const uint32_t c = W.cand[i], o = (i >> 2) * 8 + (i & 3) * 2 + (K - 6);
+    if (c == 0xFFFFFFFFu) { W.err2[o] = 0x7FFFFFFF; return; }
+    uint32_t px[16];
+    load_px(W, W.lp[shard_slot(list_counts(W, 2), W.cap, i >> 2)], px);
+    int ise[12];
+    uint32_t w[4];
+    uint64_t lo, hi;
+    const int err = trial<K>(px, c & 0xFFFFu, ise, w, T);
+    pack<K>((int)(c >> 16), ise, w, lo, hi);
+    W.err2[o] = err;
+    W.blk2[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
 // the 1-partition winner: first smallest error in configuration order
 ZT_DEV int first_best(const zt_astc_ws &W, uint32_t g, int &slot) {
@@ -425,20 +418,28 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_classify(zt_jobs jobs,
     }
 }
 
-// one kernel per configuration, so that each gets the registers of its own trial only; at least 4 waves per SIMD
-// (a handful of values spill to reach them)
-#define ZT_ASTC_SINGLE(K)                                                                     \
-    extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_single##K(zt_astc_ws W) {     \
-        __shared__ AstcLds T;                                                                \
-        stage_tables(T);                                                                     \
-        single_body<K>(W, T);                                                                \
+// every 1-partition trial in one launch: items [0, 3 n0) are configurations 0..2 over the opaque list, items
+// [3 n0, 3 n0 + 3 n1) configurations 3..5 over the translucent list, configuration-major, so a wave runs one
+// configuration's code (at most a few waves straddle two).  Each item holds one trial's registers only; at least 4
+// waves per SIMD (a handful of values spill to reach them).
+extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_single(zt_astc_ws W) {
+    __shared__ AstcLds T;
+    stage_tables(T);
+    const uint32_t n0 = shard_total(list_counts(W, 0)), n1 = shard_total(list_counts(W, 1));
+    const uint32_t n = 3 * n0 + 3 * n1, step = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const bool op = i < 3 * n0;
+        const uint32_t j = op ? i : i - 3 * n0, m = op ? n0 : n1, c = j / m, e = j - c * m;
+        switch (op ? c : 3 + c) {
+        case 0: single_item<0>(W, T, e); break;
+        case 1: single_item<1>(W, T, e); break;
+        case 2: single_item<2>(W, T, e); break;
+        case 3: single_item<3>(W, T, e); break;
+        case 4: single_item<4>(W, T, e); break;
+        default: single_item<5>(W, T, e); break;
+        }
     }
-ZT_ASTC_SINGLE(0)
-ZT_ASTC_SINGLE(1)
-ZT_ASTC_SINGLE(2)
-ZT_ASTC_SINGLE(3)
-ZT_ASTC_SINGLE(4)
-ZT_ASTC_SINGLE(5)
+}
 
 extern "C" __global__ void __launch_bounds__(256) zt_astc_pick1(zt_jobs jobs, uint32_t total, zt_astc_ws W) {
     const uint32_t step = gridDim.x * blockDim.x;
@@ -508,14 +509,17 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_search(zt_astc_ws W) {
     }
 }
 
-#define ZT_ASTC_PAIR(K)                                                                       \
-    extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_pair##K(zt_astc_ws W) {       \
-        __shared__ AstcLds T;                                                                \
-        stage_tables(T);                                                                     \
-        pair_body<K>(W, T);                                                                  \
+// every 2-partition trial in one launch: items [0, 4 n) are configuration 6 over the (listed block, pattern)
+// candidates, items [4 n, 8 n) configuration 7
+extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_pair(zt_astc_ws W) {
+    __shared__ AstcLds T;
+    stage_tables(T);
+    const uint32_t n4 = shard_total(list_counts(W, 2)) * 4u, step = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n4; i += step) {
+        if (i < n4) pair_item<6>(W, T, i);
+        else pair_item<7>(W, T, i - n4);
     }
-ZT_ASTC_PAIR(6)
-ZT_ASTC_PAIR(7)
+}
 
 extern "C" __global__ void __launch_bounds__(256) zt_astc_pick2(zt_jobs jobs, zt_astc_ws W) {
     const uint32_t n = shard_total(list_counts(W, 2)), step = gridDim.x * blockDim.x;

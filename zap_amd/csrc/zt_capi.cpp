@@ -267,14 +267,12 @@ int enqueue_astc(const zt_jobs &J, uint32_t tot, const zt_astc_ws &W, hipStream_
             hipLaunchKernelGGL(zt_zero_u32, dim3((kAstcCounters + 255) / 256), dim3(256), 0, s, W.counters, (int)kAstcCounters); })))
         return rc;
     if ((rc = launch("zt_astc_classify", s, [&] { hipLaunchKernelGGL(zt_astc_classify, dim3(g), dim3(256), 0, s, J, tot, W); }))) return rc;
-    void (*single[6])(zt_astc_ws) = { zt_astc_single0, zt_astc_single1, zt_astc_single2, zt_astc_single3, zt_astc_single4, zt_astc_single5 };
-    for (int k = 0; k < 6; k++)
-        if ((rc = launch("zt_astc_single", s, [&] { hipLaunchKernelGGL(single[k], dim3(g), dim3(256), 0, s, W); }))) return rc;
+    if ((rc = launch("zt_astc_single", s, [&] { hipLaunchKernelGGL(zt_astc_single, dim3(grid_for((uint64_t)tot * 3, 256)), dim3(256), 0, s, W); })))
+        return rc;
     if ((rc = launch("zt_astc_pick1", s, [&] { hipLaunchKernelGGL(zt_astc_pick1, dim3(g), dim3(256), 0, s, J, tot, W); }))) return rc;
     if ((rc = launch("zt_astc_search", s, [&] { hipLaunchKernelGGL(zt_astc_search, dim3(g), dim3(256), 0, s, W); }))) return rc;
-    const int g4 = grid_for((uint64_t)tot * 4, 256);
-    if ((rc = launch("zt_astc_pair", s, [&] { hipLaunchKernelGGL(zt_astc_pair6, dim3(g4), dim3(256), 0, s, W); }))) return rc;
-    if ((rc = launch("zt_astc_pair", s, [&] { hipLaunchKernelGGL(zt_astc_pair7, dim3(g4), dim3(256), 0, s, W); }))) return rc;
+    if ((rc = launch("zt_astc_pair", s, [&] { hipLaunchKernelGGL(zt_astc_pair, dim3(grid_for((uint64_t)tot * 8, 256)), dim3(256), 0, s, W); })))
+        return rc;
     return launch("zt_astc_pick2", s, [&] { hipLaunchKernelGGL(zt_astc_pick2, dim3(g), dim3(256), 0, s, J, W); });
 }
 

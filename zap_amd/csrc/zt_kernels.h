@@ -136,12 +136,12 @@ __global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, i
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
 // ASTC 4x4 encode in stages (workspace regions, each sized for `total` blocks):
 //   classify   gather each block once (px), list opaque and translucent blocks
-//   single     one launch per 1-partition configuration K (0..2 over the opaque list, 3..5 over the translucent
-//              one): error and packed block per (block, K % 3)
+//   single     every 1-partition trial in one launch, configuration-major items (K 0..2 over the opaque list,
+//              3..5 over the translucent one): error and packed block per (block, K % 3)
 //   pick1      the reference's first smallest error; blocks that go on to the 2-partition search are listed,
 //              the others are stored
-//   search     2-means split and the 4 stored patterns nearest to it, per listed block
-//   pair       one launch per 2-partition configuration (6, 7) over (listed block, pattern) items
+//   search     2-means split and the 4 stored patterns nearest to it, 16 lanes per listed block
+//   pair       every 2-partition trial in one launch: configurations 6 then 7 over (listed block, pattern) items
 //   pick2      the first smallest error over the 1-partition winner and the 8 pattern trials, stored
 struct zt_astc_ws {
     unsigned *counters;     // sharded list lengths (ZT_SHARDS x ZT_SHARD_STRIDE each): opaque, translucent, listed
@@ -155,16 +155,10 @@ struct zt_astc_ws {
     uint4 *blk2;            // [listed * 8]
 };
 __global__ void zt_astc_classify(zt_jobs jobs, uint32_t total, zt_astc_ws W);
-__global__ void zt_astc_single0(zt_astc_ws W);
-__global__ void zt_astc_single1(zt_astc_ws W);
-__global__ void zt_astc_single2(zt_astc_ws W);
-__global__ void zt_astc_single3(zt_astc_ws W);
-__global__ void zt_astc_single4(zt_astc_ws W);
-__global__ void zt_astc_single5(zt_astc_ws W);
+__global__ void zt_astc_single(zt_astc_ws W);
 __global__ void zt_astc_pick1(zt_jobs jobs, uint32_t total, zt_astc_ws W);
 __global__ void zt_astc_search(zt_astc_ws W);
-__global__ void zt_astc_pair6(zt_astc_ws W);
-__global__ void zt_astc_pair7(zt_astc_ws W);
+__global__ void zt_astc_pair(zt_astc_ws W);
 __global__ void zt_astc_pick2(zt_jobs jobs, zt_astc_ws W);
 
 __global__ void zt_bc_decode(const uint8_t *bc, int w, int h, int fmt, uint8_t *rgba, uint64_t stride);
