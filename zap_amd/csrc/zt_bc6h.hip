@@ -32,24 +32,49 @@ ZT_DEV int quant(float u) {  // zap_tex.h:841-851
     return best;
 }
 
-// zap_tex.h:854-871: best 4-bit index per pixel for endpoints e, SSE in half-bit space
-ZT_DEV uint64_t fit(const uint32_t T[16][3], const int e[2][3], uint64_t &idx) {
+// v_dot2_u32_u16: a.lo * b.lo + a.hi * b.hi + c, modulo 2^32
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+ZT_DEV uint32_t dot2(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2_t, a), __builtin_bit_cast(u16x2_t, b), c, false);
+}
+// the block's half values in LDS (a lane's pixel i at word i * ZT_B6_WG, so a wave's reads hit consecutive banks),
+// two channels per word: T01(i) = h0 | h1 << 16, T2(i) = h2 (each h <= 0x7BFF)
+#define ZT_B6_WG 256
+struct HalfPx {
+    uint32_t *t01, *t2;
+    __device__ __forceinline__ uint32_t T01(int i) const { return t01[i * ZT_B6_WG]; }
+    __device__ __forceinline__ uint32_t T2(int i) const { return t2[i * ZT_B6_WG]; }
+};
+ZT_DEV int hch(const HalfPx &H, int i, int c) { return c == 0 ? (int)(H.T01(i) & 0xFFFFu) : c == 1 ? (int)(H.T01(i) >> 16) : (int)H.T2(i); }
+ZT_DEV float uval(const HalfPx &H, int i, int c) { return (float)hch(H, i, c) * 64.0f / 31.0f; }   // zap_tex.h:878
+
+// zap_tex.h:854-871: best 4-bit index per pixel for endpoints e, SSE in half-bit space.  Palette values p <= 0x7BFF,
+// so 2p fits 16 bits and |t - p|^2 = |t|^2 + |p|^2 - t . 2p is formed with two v_dot2_u32_u16; every true squared
+// error is below 3 * 0x7BFF^2 < 2^32, so the uint32 result is exact whatever the intermediate wraps.
+ZT_DEV uint64_t fit(const HalfPx &H, const int e[2][3], uint64_t &idx) {
     int u0[3], u1[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) { u0[c] = unquant(e[0][c]); u1[c] = unquant(e[1][c]); }
+    uint32_t P01[16], P2[16], PP[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int w = w4(j);
+        uint32_t p[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) p[c] = (uint32_t)finish(((64 - w) * u0[c] + w * u1[c] + 32) >> 6);
+        P01[j] = 2 * p[0] | (2 * p[1]) << 16;
+        P2[j] = 2 * p[2];
+        PP[j] = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    }
     uint64_t err = 0, bits = 0;
+#pragma unroll 2
     for (int i = 0; i < 16; i++) {
+        const uint32_t t01 = H.T01(i), t2 = H.T2(i), tt = dot2(t01, t01, dot2(t2, t2, 0u));
         uint32_t be = 0xFFFFFFFFu;
         int bi = 0;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-            const int w = w4(j);
-            uint32_t d = 0;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                int x = (int)T[i][c] - finish(((64 - w) * u0[c] + w * u1[c] + 32) >> 6);
-                d += (uint32_t)(x * x);
-            }
+            const uint32_t d = tt + PP[j] - dot2(t01, P01[j], dot2(t2, P2[j], 0u));
             if (d < be) { be = d; bi = j; }
         }
         bits |= (uint64_t)bi << (4 * i);
@@ -59,15 +84,15 @@ ZT_DEV uint64_t fit(const uint32_t T[16][3], const int e[2][3], uint64_t &idx) {
     return err;
 }
 
-ZT_DEV void encode_block(const uint32_t T[16][3], uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:873-931
-    float U[16][3], mean[3] = { 0.0f, 0.0f, 0.0f }, cv[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f }, ax[3] = { 1.0f, 1.0f, 1.0f };
+ZT_DEV void encode_block(const HalfPx &H, uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:873-931
+    float mean[3] = { 0.0f, 0.0f, 0.0f }, cv[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f }, ax[3] = { 1.0f, 1.0f, 1.0f };
 #pragma unroll
     for (int i = 0; i < 16; i++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) { U[i][c] = (float)T[i][c] * 64.0f / 31.0f; mean[c] += U[i][c] / 16.0f; }
+        for (int c = 0; c < 3; c++) mean[c] += uval(H, i, c) / 16.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        float r = U[i][0] - mean[0], g = U[i][1] - mean[1], b = U[i][2] - mean[2];
+        float r = uval(H, i, 0) - mean[0], g = uval(H, i, 1) - mean[1], b = uval(H, i, 2) - mean[2];
         cv[0] += r * r; cv[1] += r * g; cv[2] += r * b; cv[3] += g * g; cv[4] += g * b; cv[5] += b * b;
     }
     for (int it = 0; it < 6; it++) {
@@ -82,11 +107,11 @@ ZT_DEV void encode_block(const uint32_t T[16][3], uint64_t &olo, uint64_t &ohi) 
     float bl[3] = { 1e30f, 1e30f, 1e30f }, bh[3] = { 0.0f, 0.0f, 0.0f };
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        float d = (U[i][0] - mean[0]) * ax[0] + (U[i][1] - mean[1]) * ax[1] + (U[i][2] - mean[2]) * ax[2];
+        float d = (uval(H, i, 0) - mean[0]) * ax[0] + (uval(H, i, 1) - mean[1]) * ax[1] + (uval(H, i, 2) - mean[2]) * ax[2];
         if (d < tl) tl = d;
         if (d > th) th = d;
 #pragma unroll
-        for (int c = 0; c < 3; c++) { if (U[i][c] < bl[c]) bl[c] = U[i][c]; if (U[i][c] > bh[c]) bh[c] = U[i][c]; }
+        for (int c = 0; c < 3; c++) { const float u = uval(H, i, c); if (u < bl[c]) bl[c] = u; if (u > bh[c]) bh[c] = u; }
     }
     uint64_t best = ~0ull, bidx = 0;
     int be[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
@@ -98,7 +123,7 @@ ZT_DEV void encode_block(const uint32_t T[16][3], uint64_t &olo, uint64_t &ohi) 
             e[0][c] = quant(lo); e[1][c] = quant(hi);
         }
         uint64_t idx;
-        uint64_t err = fit(T, e, idx);
+        uint64_t err = fit(H, e, idx);
         for (int it = 0; it < 3 && err > 0; it++) {
             float aa = 0.0f, ab = 0.0f, bb = 0.0f, xa[3] = { 0.0f, 0.0f, 0.0f }, xb[3] = { 0.0f, 0.0f, 0.0f };
 #pragma unroll
@@ -106,7 +131,7 @@ ZT_DEV void encode_block(const uint32_t T[16][3], uint64_t &olo, uint64_t &ohi) 
                 float w = (float)w4((int)((idx >> (4 * i)) & 15)) / 64.0f, v = 1 - w;
                 aa += v * v; ab += v * w; bb += w * w;
 #pragma unroll
-                for (int c = 0; c < 3; c++) { xa[c] += v * U[i][c]; xb[c] += w * U[i][c]; }
+                for (int c = 0; c < 3; c++) { const float u = uval(H, i, c); xa[c] += v * u; xb[c] += w * u; }
             }
             float det = aa * bb - ab * ab;
             if (det < 1e-6f && det > -1e-6f) break;
@@ -114,7 +139,7 @@ ZT_DEV void encode_block(const uint32_t T[16][3], uint64_t &olo, uint64_t &ohi) 
 #pragma unroll
             for (int c = 0; c < 3; c++) { ne[0][c] = quant((xa[c] * bb - xb[c] * ab) / det); ne[1][c] = quant((xb[c] * aa - xa[c] * ab) / det); }
             uint64_t ni;
-            uint64_t e2 = fit(T, ne, ni);
+            uint64_t e2 = fit(H, ne, ni);
             if (e2 >= err) break;
             err = e2; idx = ni;
 #pragma unroll
@@ -143,23 +168,25 @@ ZT_DEV void encode_block(const uint32_t T[16][3], uint64_t &olo, uint64_t &ohi) 
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(256) zt_bc6h_encode(zt_jobs jobs, uint32_t total) {
+extern "C" __global__ void __launch_bounds__(ZT_B6_WG) zt_bc6h_encode(zt_jobs jobs, uint32_t total) {
+    __shared__ uint32_t s_t01[16 * ZT_B6_WG], s_t2[16 * ZT_B6_WG];   // each lane's own pixels (no sharing, no barrier)
     const uint32_t step = gridDim.x * blockDim.x;
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
         const zt_job J = job_at(jobs, zt_find_job(jobs, g));
         uint32_t b = g - J.first;
         int bx = (int)(b % (uint32_t)J.bw), by = (int)(b / (uint32_t)J.bw);
-        uint32_t T[16][3];
+        const HalfPx H{ &s_t01[threadIdx.x], &s_t2[threadIdx.x] };
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             int x = bx * 4 + (i & 3), y = by * 4 + (i >> 2);
             if (x >= J.w) x = J.w - 1;
             if (y >= J.h) y = J.h - 1;
             float4 p = *(const float4 *)(J.src + (uint64_t)y * J.stride + (uint64_t)x * 16);
-            T[i][0] = half_of(p.x); T[i][1] = half_of(p.y); T[i][2] = half_of(p.z);
+            H.t01[i * ZT_B6_WG] = half_of(p.x) | half_of(p.y) << 16;
+            H.t2[i * ZT_B6_WG] = half_of(p.z);
         }
         uint64_t lo, hi;
-        encode_block(T, lo, hi);
+        encode_block(H, lo, hi);
         store16(J.dst + (uint64_t)b * 16, lo, hi);
     }
 }
