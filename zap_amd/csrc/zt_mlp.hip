@@ -315,7 +315,11 @@ ZT_DEV int prop_slot(int u) {
     return u < 16 ? u : u < 80 ? 64 + (u - 16) : u < 144 ? 128 + (u - 80) : u < 208 ? 192 + (u - 144) : u < 216 ? 256 + (u - 208)
          : u < 220 ? 320 + (u - 216) : u == 220 ? 384 : u < 285 ? 448 + (u - 221) : u == 285 ? ZT_RISK_SLOT : -1;
 }
-ZT_DEV bf16x4_t bf4(uint32_t lo, uint32_t hi) { return bf16x4_t{ (short)(lo & 0xFFFF), (short)(lo >> 16), (short)(hi & 0xFFFF), (short)(hi >> 16) }; }
+ZT_DEV bf16x4_t bf4(uint32_t lo, uint32_t hi) { return __builtin_bit_cast(bf16x4_t, make_uint2(lo, hi)); }
+ZT_DEV uint32_t bf16_pair(float a, float b) {
+    const uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b);
+    return perm(ub + 0x7FFFu + ((ub >> 16) & 1u), ua + 0x7FFFu + ((ua >> 16) & 1u), 0x07060302u);
+}
 // weights [rows x IN] (bf16, row-major) -> LDS fragments: for row tile j, K-step pair p and lane l, 16 bytes =
 // A operands of K steps 2p and 2p + 1 (row 16 j + l % 16, columns 16 (2p + s) + 4 (l / 16) + 0..3)
 template <int IN>
@@ -340,7 +344,7 @@ ZT_DEV bf16x4_t epilogue(f32x4_t d, const float *bias) {
     if (x1 < 0.0f) x1 = 0.0f;
     if (x2 < 0.0f) x2 = 0.0f;
     if (x3 < 0.0f) x3 = 0.0f;
-    return bf16x4_t{ (short)to_bf16(x0), (short)to_bf16(x1), (short)to_bf16(x2), (short)to_bf16(x3) };
+    return bf4(bf16_pair(x0, x1), bf16_pair(x2, x3));
 }
 // one layer, NJ output row tiles over NK K steps, NC column tiles: out[ct][j] = epilogue(sum_k W[j][k] in[ct][k])
 template <int NJ, int NK, int NC>
@@ -395,11 +399,12 @@ struct TopList {
         for (int i = 0; i < KT; i++) c[i] = x > t[i];
 #pragma unroll
         for (int i = KT - 1; i > 0; i--) {
-            u[i] = c[i - 1] ? u[i - 1] : c[i] ? ux : u[i];
-            t[i] = c[i - 1] ? t[i - 1] : c[i] ? x : t[i];
+            const int v = c[i] ? ux : u[i];
+            u[i] = c[i - 1] ? u[i - 1] : v;
+            t[i] = __builtin_amdgcn_fmed3f(t[i - 1], t[i], x);
         }
         u[0] = c[0] ? ux : u[0];
-        t[0] = c[0] ? x : t[0];
+        t[0] = fmaxf(t[0], x);
     }
     __device__ __forceinline__ void pop_if(bool pop) {
 #pragma unroll
