@@ -124,6 +124,18 @@ def test_strided_subimage_and_bands(zt, oracle):
         assert np.array_equal(whole, bands), f
 
 
+def test_host_calls_from_threads(zt):
+    """The host entry points are callable from several threads at once (the reference's callers encode bands on
+    threads, viewer.cpp:461-470): each thread's output equals the same call made alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = [(synth.c2_bc7(128, 40 + i)[: 64 + 8 * i, : 96 - 4 * i], f) for i in range(8) for f in ("bc1", "bc3", "bc7")]
+    alone = [zt.bc_encode(np.ascontiguousarray(img), f) for img, f in jobs]
+    with ThreadPoolExecutor(6) as ex:
+        together = list(ex.map(lambda j: zt.bc_encode(np.ascontiguousarray(j[0]), j[1]), jobs * 3))
+    for k, out in enumerate(together):
+        assert np.array_equal(out, alone[k % len(jobs)]), k
+
+
 def test_decoders_match_oracle(zt, oracle):
     rng = np.random.default_rng(3)
     raw = rng.integers(0, 256, 16 * 300, dtype=np.uint8)

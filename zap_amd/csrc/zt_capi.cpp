@@ -100,9 +100,9 @@ const uint32_t kSrgbLinHash = 0xab9a3c1bu, kSrgbThrHash = 0x4f5c9c4fu;
 struct DeviceState {
     bool ready = false;
     hipStream_t stream = nullptr;
-    // staging for the host entry points
-    void *in = nullptr, *out = nullptr, *ws = nullptr;
-    size_t in_cap = 0, out_cap = 0, ws_cap = 0;
+    // staging for the host entry points (device buffers, and pinned host memory for rows that are not packed)
+    void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr;
+    size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0;
 };
 DeviceState g_dev[64];
 
@@ -112,6 +112,40 @@ int ensure_buf(void *&p, size_t &cap, size_t need) {
     size_t n = std::max(align_up(need, 1 << 20), (size_t)1 << 20);
     if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; return fail(ZAPTEX_E_NOMEM, "hipMalloc(%zu) failed", n); }
     cap = n;
+    return ZAPTEX_OK;
+}
+
+// Host <-> device rows for the host entry points.  Packed rows move in one copy; rows with gaps between them are
+// packed through pinned staging memory by the CPU, because a 2D copy from pageable memory is staged one row at a
+// time (a 4096-row image took 60 ms that way instead of 5).
+int ensure_pinned(DeviceState *st, size_t need) {
+    if (need <= st->pin_cap) return ZAPTEX_OK;
+    HIP_TRY(hipStreamSynchronize(st->stream));   // the old buffer may still feed a copy
+    if (st->pin) { (void)hipHostFree(st->pin); st->pin = nullptr; st->pin_cap = 0; }
+    size_t n = std::max(align_up(need, 1 << 20), (size_t)1 << 20);
+    if (hipHostMalloc(&st->pin, n, hipHostMallocDefault) != hipSuccess) { st->pin = nullptr; return fail(ZAPTEX_E_NOMEM, "hipHostMalloc(%zu) failed", n); }
+    st->pin_cap = n;
+    return ZAPTEX_OK;
+}
+int rows_to_device(DeviceState *st, void *dst, const void *src, size_t spitch, size_t row, size_t h) {
+    if (spitch == row) { HIP_TRY(hipMemcpyAsync(dst, src, row * h, hipMemcpyHostToDevice, st->stream)); return ZAPTEX_OK; }
+    if (int rc = ensure_pinned(st, row * h)) return rc;
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    for (size_t y = 0; y < h; y++) memcpy((uint8_t *)st->pin + y * row, (const uint8_t *)src + y * spitch, row);
+    HIP_TRY(hipMemcpyAsync(dst, st->pin, row * h, hipMemcpyHostToDevice, st->stream));
+    return ZAPTEX_OK;
+}
+// ends with the stream synchronised
+int rows_to_host(DeviceState *st, void *dst, size_t dpitch, const void *src, size_t row, size_t h) {
+    if (dpitch == row) {
+        HIP_TRY(hipMemcpyAsync(dst, src, row * h, hipMemcpyDeviceToHost, st->stream));
+        HIP_TRY(hipStreamSynchronize(st->stream));
+        return ZAPTEX_OK;
+    }
+    if (int rc = ensure_pinned(st, row * h)) return rc;
+    HIP_TRY(hipMemcpyAsync(st->pin, src, row * h, hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    for (size_t y = 0; y < h; y++) memcpy((uint8_t *)dst + y * dpitch, (const uint8_t *)st->pin + y * row, row);
     return ZAPTEX_OK;
 }
 
@@ -367,7 +401,7 @@ int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, i
     if (int rc = ensure_buf(st->out, st->out_cap, ob)) return rc;
     if (n7 || n14 || na) if (int rc = ensure_buf(st->ws, st->ws_cap, ws_total(n7, n14, na))) return rc;
     im.src = st->in; im.dst = st->out;
-    HIP_TRY(hipMemcpy2DAsync(st->in, row, src, stride, row, (size_t)h, hipMemcpyHostToDevice, st->stream));
+    if (int rc = rows_to_device(st, st->in, src, stride, row, (size_t)h)) return rc;
     if (int rc = enqueue(&im, 1, st->ws, st->ws_cap, st->stream)) return rc;
     HIP_TRY(hipMemcpyAsync(out, st->out, ob, hipMemcpyDeviceToHost, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
@@ -405,11 +439,9 @@ int zaptex_mip_next(const uint8_t *src, int w, int h, size_t stride, int srgb, u
     size_t row = (size_t)w * 4, orow = (size_t)nw * 4;
     if (int rc = ensure_buf(st->in, st->in_cap, row * h)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, orow * nh)) return rc;
-    HIP_TRY(hipMemcpy2DAsync(st->in, row, src, stride, row, (size_t)h, hipMemcpyHostToDevice, st->stream));
+    if (int rc = rows_to_device(st, st->in, src, stride, row, (size_t)h)) return rc;
     if (int rc = enqueue_mip(st->in, w, h, row, srgb, st->out, orow, st->stream)) return rc;
-    HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, st->out, orow, orow, (size_t)nh, hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    return ZAPTEX_OK;
+    return rows_to_host(st, dst, dst_stride, st->out, orow, (size_t)nh);
 }
 
 int zaptex_mip_next_f(const float *src, int w, int h, size_t stride, float *dst, size_t dst_stride) {
@@ -422,13 +454,11 @@ int zaptex_mip_next_f(const float *src, int w, int h, size_t stride, float *dst,
     size_t row = (size_t)w * 16, orow = (size_t)nw * 16;
     if (int rc = ensure_buf(st->in, st->in_cap, row * h)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, orow * nh)) return rc;
-    HIP_TRY(hipMemcpy2DAsync(st->in, row, src, stride * 4, row, (size_t)h, hipMemcpyHostToDevice, st->stream));
+    if (int rc = rows_to_device(st, st->in, src, stride * 4, row, (size_t)h)) return rc;
     dim3 grid((nw + 63) / 64, (nh + 3) / 4), block(64, 4);
     hipLaunchKernelGGL(zt_mip_level_f, grid, block, 0, st->stream, (const float *)st->in, w, h, (uint64_t)w * 4, (float *)st->out, (uint64_t)nw * 4);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpy2DAsync(dst, dst_stride * 4, st->out, orow, orow, (size_t)nh, hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    return ZAPTEX_OK;
+    return rows_to_host(st, dst, dst_stride * 4, st->out, orow, (size_t)nh);
 }
 
 int zaptex_bc_decode(const void *bc, int w, int h, int fmt, uint8_t *rgba, size_t stride) {
@@ -443,9 +473,7 @@ int zaptex_bc_decode(const void *bc, int w, int h, int fmt, uint8_t *rgba, size_
     int bw = (w + 3) / 4, bh = (h + 3) / 4;
     hipLaunchKernelGGL(zt_bc_decode, dim3((bw * bh + 255) / 256), dim3(256), 0, st->stream, (const uint8_t *)st->in, w, h, fmt, (uint8_t *)st->out, (uint64_t)row);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpy2DAsync(rgba, stride, st->out, row, row, (size_t)h, hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    return ZAPTEX_OK;
+    return rows_to_host(st, rgba, stride, st->out, row, (size_t)h);
 }
 
 int zaptex_bc6h_decode(const void *bc, int w, int h, uint16_t *rgba, size_t stride) {
@@ -460,9 +488,7 @@ int zaptex_bc6h_decode(const void *bc, int w, int h, uint16_t *rgba, size_t stri
     int bw = (w + 3) / 4, bh = (h + 3) / 4;
     hipLaunchKernelGGL(zt_bc6h_decode, dim3((bw * bh + 255) / 256), dim3(256), 0, st->stream, (const uint8_t *)st->in, w, h, (uint16_t *)st->out, (uint64_t)w * 4);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpy2DAsync(rgba, stride * 2, st->out, row, row, (size_t)h, hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    return ZAPTEX_OK;
+    return rows_to_host(st, rgba, stride * 2, st->out, row, (size_t)h);
 }
 
 int zaptex_astc_decode(const void *bc, int w, int h, uint8_t *rgba, size_t stride) {
@@ -477,9 +503,7 @@ int zaptex_astc_decode(const void *bc, int w, int h, uint8_t *rgba, size_t strid
     int bw = (w + 3) / 4, bh = (h + 3) / 4;
     hipLaunchKernelGGL(zt_astc_decode, dim3((bw * bh + 255) / 256), dim3(256), 0, st->stream, (const uint8_t *)st->in, w, h, (uint8_t *)st->out, (uint64_t)row);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpy2DAsync(rgba, stride, st->out, row, row, (size_t)h, hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    return ZAPTEX_OK;
+    return rows_to_host(st, rgba, stride, st->out, row, (size_t)h);
 }
 
 static int shuffle(const void *in, size_t n, int fmt, void *out, int split) {
