@@ -149,7 +149,7 @@ def batch_leg(lib, steps: int = 5) -> dict:
 
 
 CONFIG_WORK = [  # (name, format, golden key, kernels)
-    ("c1_bc1_1024", "bc1", "c1_1024_bc1", ["zt_bc14_encode", "zt_bc1_climb"]),
+    ("c1_bc1_1024", "bc1", "c1_1024_bc1", ["zt_bc14_encode", "zt_bc1_climb", "zt_bc1_finish"]),
     ("c3_bc6h_2048", "bc6h", "c3_bc6h_2048", ["zt_bc6h_encode"]),
     ("c3_astc_2048", "astc", "c3_astc_2048", ["zt_astc_classify", "zt_astc_single", "zt_astc_pick1", "zt_astc_search",
                                               "zt_astc_pair", "zt_astc_pick2"]),

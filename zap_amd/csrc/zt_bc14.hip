@@ -23,9 +23,10 @@ ZT_DEV uint32_t to565(float r, float g, float b) {  // zap_tex.h:45-49
 
 // SSE and indices of a canonical (c0 >= c1) 4-colour endpoint pair; c0 == c1 is solid with all indices 0
 // (zap_tex.h:52-94 with force4 = 1).  idx: 2 bits per pixel.  pm = the block's pixels with alpha cleared,
-// spp = sum of |pm|^2 (both per block).  key_j = 8 p.pal_j - 4 |pal_j|^2 + (3 - j) = 4 S_j + (3 - j) with
-// S_j = 2 p.pal_j - |pal_j|^2 = |p|^2 - SSE_j: the largest key is the smallest SSE, lowest index on ties;
-// key >> 2 = S_j exactly, and the low 2 bits are 3 - j.
+// spp = sum of |pm|^2 (both per block).  key_j = 128 p.pal_j - 64 |pal_j|^2 + (3 - j) = 64 S_j + (3 - j) with
+// S_j = 2 p.pal_j - |pal_j|^2 = |p|^2 - SSE_j: the largest key is the smallest SSE, lowest index on ties, and its
+// low 2 bits are 3 - j.  The 16 keys' tie terms add up to at most 48 < 64, so (sum of keys) >> 6 = sum of S_j
+// exactly (|key| < 2^25, the sum < 2^31).
 ZT_DEV int bc1_trial(const uint32_t pm[16], int spp, uint32_t c0, uint32_t c1, uint32_t &idx) {
     const uint32_t a = rgb565_expand(c0), b = rgb565_expand(c1);
     if (c0 == c1) {
@@ -42,24 +43,53 @@ ZT_DEV int bc1_trial(const uint32_t pm[16], int spp, uint32_t c0, uint32_t c1, u
         p2 |= (uint32_t)((2 * x + y + 1) / 3) << (8 * k);
         p3 |= (uint32_t)((x + 2 * y + 1) / 3) << (8 * k);
     }
-    const int K0 = no_fold(3 - 4 * (int)dot4(a, a)), K1 = no_fold(2 - 4 * (int)dot4(b, b)), K2 = no_fold(1 - 4 * (int)dot4(p2, p2)),
-              K3 = no_fold(-4 * (int)dot4(p3, p3));
+    const int K0 = no_fold(3 - 64 * (int)dot4(a, a)), K1 = no_fold(2 - 64 * (int)dot4(b, b)), K2 = no_fold(1 - 64 * (int)dot4(p2, p2)),
+              K3 = no_fold(-64 * (int)dot4(p3, p3));
     int sum = 0;
     uint32_t inv = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int k0 = (int)(dot4(pm[i], a) << 3) + K0, k1 = (int)(dot4(pm[i], b) << 3) + K1;
-        const int k2 = (int)(dot4(pm[i], p2) << 3) + K2, k3 = (int)(dot4(pm[i], p3) << 3) + K3;
+        const int k0 = (int)(dot4(pm[i], a) << 7) + K0, k1 = (int)(dot4(pm[i], b) << 7) + K1;
+        const int k2 = (int)(dot4(pm[i], p2) << 7) + K2, k3 = (int)(dot4(pm[i], p3) << 7) + K3;
         const int best = max(max(k0, k1), max(k2, k3));
         inv |= (uint32_t)(best & 3) << (2 * i);
-        sum += best >> 2;
+        sum += best;
     }
     idx = ~inv;
-    return spp - sum;
+    return spp - (sum >> 6);
 }
 ZT_DEV int bc1_attempt(const uint32_t pm[16], int spp, uint32_t &c0, uint32_t &c1, uint32_t &idx) {  // zap_tex.h:90-94
     if (c0 < c1) { uint32_t t = c0; c0 = c1; c1 = t; }
     return bc1_trial(pm, spp, c0, c1, idx);
+}
+// bc1_attempt's SSE alone (the hill climb compares errors; the winner's indices are recomputed once, in
+// zt_bc1_finish): the same keys and maxima without the index bits
+ZT_DEV int bc1_attempt_err(const uint32_t pm[16], int spp, uint32_t &c0, uint32_t &c1) {
+    if (c0 < c1) { uint32_t t = c0; c0 = c1; c1 = t; }
+    const uint32_t a = rgb565_expand(c0), b = rgb565_expand(c1);
+    if (c0 == c1) {
+        uint32_t pa = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) pa = __builtin_amdgcn_udot4(pm[i], a, pa, false);
+        return spp - 2 * (int)pa + 16 * (int)dot4(a, a);
+    }
+    uint32_t p2 = 0, p3 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int x = chan(a, k), y = chan(b, k);
+        p2 |= (uint32_t)((2 * x + y + 1) / 3) << (8 * k);
+        p3 |= (uint32_t)((x + 2 * y + 1) / 3) << (8 * k);
+    }
+    const int K0 = no_fold(3 - 64 * (int)dot4(a, a)), K1 = no_fold(2 - 64 * (int)dot4(b, b)), K2 = no_fold(1 - 64 * (int)dot4(p2, p2)),
+              K3 = no_fold(-64 * (int)dot4(p3, p3));
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int k0 = (int)(dot4(pm[i], a) << 7) + K0, k1 = (int)(dot4(pm[i], b) << 7) + K1;
+        const int k2 = (int)(dot4(pm[i], p2) << 7) + K2, k3 = (int)(dot4(pm[i], p3) << 7) + K3;
+        sum += max(max(k0, k1), max(k2, k3));
+    }
+    return spp - (sum >> 6);
 }
 // alpha-cleared pixels and their sum of squares
 ZT_DEV int bc1_prepare(const uint32_t px[16], uint32_t pm[16]) {
@@ -139,8 +169,8 @@ ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32
 // hill-climb state of one block: +-1 on each 565 channel of each endpoint (order: endpoint, channel, -1 then +1),
 // passes of 12 moves while a pass improved, at most 8 (zap_tex.h:152-166)
 struct Climb {
-    uint32_t b0, b1, bidx;
-    int best, pass, mv, improved;
+    uint32_t b0, b1, bidx;   // bidx: the start's indices (valid while moved == 0)
+    int best, pass, mv, improved, moved;
 };
 // the next valid move's endpoint pair (cheap; no trial); false when the climb is over.  Stopping as soon as the
 // SSE is 0 gives the reference's result: no later move can improve on 0.
@@ -163,9 +193,9 @@ ZT_DEV bool climb_next(Climb &s, uint32_t &n0, uint32_t &n1) {
     }
     return false;
 }
-// fold a trial (endpoints as bc1_attempt left them, canonical order) into the state
-ZT_DEV void climb_apply(Climb &s, uint32_t n0, uint32_t n1, int ne, uint32_t ni) {
-    if (ne < s.best) { s.best = ne; s.b0 = n0; s.b1 = n1; s.bidx = ni; s.improved = 1; }
+// fold a trial (endpoints as bc1_attempt_err left them, canonical order) into the state
+ZT_DEV void climb_apply(Climb &s, uint32_t n0, uint32_t n1, int ne) {
+    if (ne < s.best) { s.best = ne; s.b0 = n0; s.b1 = n1; s.improved = 1; s.moved = 1; }
 }
 ZT_DEV uint64_t bc1_pack(uint32_t c0, uint32_t c1, uint32_t idx) { return (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)idx << 32; }
 
@@ -268,8 +298,9 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, u
 // runs one trial on (nearly) every lane.  Non-BC1 blocks are skipped at refill.
 // Refills wait until at least refill_min lanes are idle (or none is busy): a refill costs every lane of the wave
 // a job lookup and a 16-pixel gather, so doing it for one lane per step would cost more than the idle lanes.
-extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min,
-                                                                 uint32_t range) {
+// Trials compute errors only.  A block the climb never moved is stored with its start's indices; a moved block
+// leaves (c0 | c1 << 16, SSE, 0, job | 2^31) in climb[] for zt_bc1_finish, which computes its indices once.
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -279,8 +310,9 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uin
         bool active = false;
         uint32_t px[16];  // alpha cleared (bc1_prepare)
         int spp = 0;
-        Climb s = { 0, 0, 0, 0, 0, 0, 0 };
+        Climb s = { 0, 0, 0, 0, 0, 0, 0, 0 };
         uint8_t *dst = nullptr;
+        uint32_t cur = 0, job = 0;
         for (;;) {
             const uint64_t need = __ballot(!active);
             const int idle = __popcll(need);
@@ -293,7 +325,8 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uin
                 if (J.fmt == 0 || J.fmt == 1) {
                     const uint32_t b = mine - J.first;
                     dst = J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8);
-                    s.b0 = st.x & 0xFFFFu; s.b1 = st.x >> 16; s.best = (int)st.y; s.bidx = st.z; s.pass = 0; s.mv = 0; s.improved = 0;
+                    s.b0 = st.x & 0xFFFFu; s.b1 = st.x >> 16; s.best = (int)st.y; s.bidx = st.z; s.pass = 0; s.mv = 0; s.improved = 0; s.moved = 0;
+                    cur = mine; job = st.w;
                     gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
                     spp = bc1_prepare(px, px);
                     active = true;
@@ -302,14 +335,32 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uin
             if (__ballot(active) == 0 && next >= end) break;
             uint32_t n0 = 0, n1 = 0;
             const bool has = active && climb_next(s, n0, n1);
-            if (active && !has) { store8(dst, bc1_pack(s.b0, s.b1, s.bidx)); active = false; }
+            if (active && !has) {
+                if (s.moved) climb[cur] = make_uint4(s.b0 | s.b1 << 16, (uint32_t)s.best, 0u, job | 0x80000000u);
+                else store8(dst, bc1_pack(s.b0, s.b1, s.bidx));
+                active = false;
+            }
             if (has) {
-                // the full trial: an error-only trial plus indices for improvements would run both on nearly
-                // every step, since some lane of the wave improves on most steps
-                uint32_t ni = 0;
-                const int ne = bc1_attempt(px, spp, n0, n1, ni);
-                climb_apply(s, n0, n1, ne, ni);
+                const int ne = bc1_attempt_err(px, spp, n0, n1);   // puts (n0, n1) in canonical order first
+                climb_apply(s, n0, n1, ne);
             }
         }
+    }
+}
+
+// the indices of every block the hill climb moved (climb[g].w has bit 31 set): one full trial of its final pair
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_finish(zt_jobs jobs, uint32_t total, const uint4 *climb) {
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const uint4 st = climb[g];
+        if (!(st.w >> 31)) continue;
+        const zt_job J = job_at(jobs, (int)(st.w & 0x7FFFFFFFu));
+        const uint32_t b = g - J.first;
+        uint32_t px[16];
+        gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+        const int spp = bc1_prepare(px, px);
+        uint32_t c0 = st.x & 0xFFFFu, c1 = st.x >> 16, idx = 0;
+        (void)bc1_trial(px, spp, c0, c1, idx);   // (c0, c1) is canonical already
+        store8(J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8), bc1_pack(c0, c1, idx));
     }
 }

@@ -258,9 +258,11 @@ int enqueue_bc14(const zt_jobs &J, uint32_t tot, uint4 *climb, hipStream_t s) {
     // ranges of up to ZT_CLIMB_RANGE blocks, smaller for small jobs so that about 4096 waves share the work
     const uint32_t range = (uint32_t)std::min<uint64_t>(ZT_CLIMB_RANGE, std::max<uint64_t>(64, ((tot + 4095) / 4096 + 63) / 64 * 64));
     const uint64_t waves = (tot + range - 1) / range;
-    return launch("zt_bc1_climb", s, [&] {
-        hipLaunchKernelGGL(zt_bc1_climb, dim3(groups_for(CFG_CLIMB, waves, wc / 64)), dim3(wc), 0, s, J, tot, (const uint4 *)climb, g_climb_refill,
-                           range); });
+    if (int rc = launch("zt_bc1_climb", s, [&] {
+            hipLaunchKernelGGL(zt_bc1_climb, dim3(groups_for(CFG_CLIMB, waves, wc / 64)), dim3(wc), 0, s, J, tot, climb, g_climb_refill, range); }))
+        return rc;
+    return launch("zt_bc1_finish", s, [&] {
+        hipLaunchKernelGGL(zt_bc1_finish, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, (const uint4 *)climb); });
 }
 
 // the reference BC7 search (mode 6, then 5, then 1) over one job table; ws >= ws_bytes_for(tot)

@@ -132,7 +132,8 @@ __global__ void zt_bc7_finish_learned(zt_jobs jobs, uint32_t total, int K, const
 // largest block range per wave of zt_bc1_climb (a wave's lanes refill from its range)
 #define ZT_CLIMB_RANGE 1024u
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb);
-__global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, const uint4 *climb, int refill_min, uint32_t range);
+__global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range);
+__global__ void zt_bc1_finish(zt_jobs jobs, uint32_t total, const uint4 *climb);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
 // ASTC 4x4 encode in stages (workspace regions, each sized for `total` blocks):
 //   classify   gather each block once (px), list opaque and translucent blocks
