@@ -78,6 +78,14 @@ template <int NP> ZT_DEV int best_key(uint32_t p, const uint32_t pal[NP], const 
     for (int j = 1; j < NP; j++) best = max(best, (int)(dot4(p, pal[j]) << 5) + K[j]);
     return best;
 }
+// the same key for a one-channel palette (entries are plain values, v a plain byte): 32 v p_j + K_j as one 24-bit
+// multiply-add per entry instead of a v_dot4_u32_u8 and a shift-add
+template <int NP> ZT_DEV int best_key1(int v, const int P32[NP], const int K[NP]) {
+    int best = __mul24(v, P32[0]) + K[0];
+#pragma unroll
+    for (int j = 1; j < NP; j++) best = max(best, __mul24(v, P32[j]) + K[j]);
+    return best;
+}
 template <int NP> ZT_DEV void palette_keys(const uint32_t pal[NP], int K[NP]) {
 #pragma unroll
     for (int j = 0; j < NP; j++) K[j] = no_fold(15 - j - 16 * (int)dot4(pal[j], pal[j]));
@@ -342,11 +350,13 @@ template <int NC, int C0> ZT_DEV int fit_2bit(const uint32_t r[16], int spp, con
     int K[4];
     build_palette<NC, 2>(e0, e1, pal);
     palette_keys<4>(pal, K);
-    int sk = 0;
+    int sk = 0, P32[4];
     uint32_t bits = 0;
 #pragma unroll
+    for (int j = 0; j < 4; j++) P32[j] = (int)pal[j] << 5;   // used when NC == 1
+#pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int key = best_key<4>(sub_px<NC, C0>(r[i]), pal, K);   // low nibble 15 - j, j < 4
+        const int key = NC == 1 ? best_key1<4>((int)sub_px<NC, C0>(r[i]), P32, K) : best_key<4>(sub_px<NC, C0>(r[i]), pal, K);   // low nibble 15 - j, j < 4
         sk += key;
         bits |= (uint32_t)(key & 3) << (2 * i);
     }

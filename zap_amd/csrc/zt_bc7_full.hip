@@ -177,11 +177,13 @@ template <int NC, int C0, int IB> ZT_DEV int fit_nbit(const uint32_t r[16], int 
     int K[NP];
     build_palette<NC, IB>(e0, e1, pal);
     palette_keys<NP>(pal, K);
-    int sk = 0;
+    int sk = 0, P32[NP];
     uint64_t bits = 0;
 #pragma unroll
+    for (int j = 0; j < NP; j++) P32[j] = (int)pal[j] << 5;   // used when NC == 1
+#pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int key = best_key<NP>(sub_px<NC, C0>(r[i]), pal, K);
+        const int key = NC == 1 ? best_key1<NP>((int)sub_px<NC, C0>(r[i]), P32, K) : best_key<NP>(sub_px<NC, C0>(r[i]), pal, K);
         sk += key;
         bits |= (uint64_t)(key & 15) << (4 * i);
     }
