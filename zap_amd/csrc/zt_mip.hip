@@ -107,9 +107,32 @@ ZT_DEV void tiles_body(const uint8_t *src, int w, int h, uint64_t stride, int sr
     if (srgb)
         for (int i = tid; i < 256; i += nt) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
     __syncthreads();
-    // level 1: 64 x 64 pixels of the tile, 16 per thread, 2 x 8-byte loads per pixel pair row
-    {
-        const int w1 = L.w[0], h1 = L.h[0];
+    // level 1: 64 x 64 pixels of the tile.  A tile inside both level-1 sides with 16-byte aligned rows and 256
+    // threads loads straight through: 8 pixel pairs per thread, each two 16-byte reads (4 source pixels of two
+    // rows), all 16 issued before the first is used, so a workgroup keeps 64 KB in flight.  Other tiles take 16
+    // pixels per thread with 8-byte reads.
+    const int w1 = L.w[0], h1 = L.h[0];
+    if (nt == 256 && (tx + 1) * 64 <= w1 && (ty + 1) * 64 <= h1 && ((((uintptr_t)src) | stride) & 15) == 0) {
+        uint4 r0[8], r1[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int pp = k * 256 + tid, x = tx * 64 + (pp & 31) * 2, y = ty * 64 + (pp >> 5);
+            const uint8_t *a = src + (uint64_t)(2 * y) * stride + (uint64_t)x * 8;
+            r0[k] = *(const uint4 *)a;
+            r1[k] = *(const uint4 *)(a + stride);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int pp = k * 256 + tid, lx = (pp & 31) * 2, ly = pp >> 5, x = tx * 64 + lx, y = ty * 64 + ly;
+            const uint32_t o0 = box4(r0[k].x, r0[k].y, r1[k].x, r1[k].y, srgb, lin, thr);
+            const uint32_t o1 = box4(r0[k].z, r0[k].w, r1[k].z, r1[k].w, srgb, lin, thr);
+            t0[ly * 64 + lx] = o0;
+            t0[ly * 64 + lx + 1] = o1;
+            uint8_t *d = L.p[0] + ((uint64_t)y * w1 + x) * 4;
+            if ((((uintptr_t)d) & 7) == 0) *(uint2 *)d = make_uint2(o0, o1);
+            else { *(uint32_t *)d = o0; *(uint32_t *)(d + 4) = o1; }
+        }
+    } else {
         for (int p = tid; p < 64 * 64; p += nt) {
             const int lx = p & 63, ly = p >> 6, x = tx * 64 + lx, y = ty * 64 + ly;
             uint32_t o = 0;
