@@ -224,7 +224,7 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="train the BC7 proposal network on the GPU")
     ap.add_argument("--mix", default=",".join(f"{s}x{c}" for s, c in TRAIN_MIX), help="training textures: side x count, ...")
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--topk", type=int, default=4)
