@@ -27,7 +27,7 @@ from . import binding, synth
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "models", "bc7_proposal.npy")
 TRAIN_SEED0 = 1000                       # training textures use seeds TRAIN_SEED0, TRAIN_SEED0 + 1, ...
-TRAIN_MIX = ((4096, 4), (2048, 16), (1024, 32))  # (side, count): about 14M blocks with the mip chains
+TRAIN_MIX = ((4096, 8), (2048, 32), (1024, 64))  # (side, count): about 28M blocks with the mip chains
 CALIB = ((4096, 500), (1024, 501), (512, 502))   # held-out (side, seed) textures for the threshold
 
 
