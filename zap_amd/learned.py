@@ -27,7 +27,7 @@ from . import binding, synth
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "models", "bc7_proposal.npy")
 TRAIN_SEED0 = 1000                       # training textures use seeds TRAIN_SEED0, TRAIN_SEED0 + 1, ...
-TRAIN_MIX = ((4096, 8), (2048, 32), (1024, 64))  # (side, count): about 28M blocks with the mip chains
+TRAIN_MIX = ((4096, 16), (2048, 64), (1024, 128))  # (side, count): about 56M blocks with the mip chains
 CALIB = ((4096, 500), (1024, 501), (512, 502))   # held-out (side, seed) textures for the threshold
 
 
@@ -179,7 +179,7 @@ def calibrate(lib, state, topk: int, textures=CALIB, max_gap_db: float = 0.04, f
     """Fastest threshold (predicted SSE a block may give away before it falls back) whose chain-PSNR gap to the
     exhaustive search stays within max_gap_db on every held-out texture; speed is pooled over them.
     The margin under 0.05 dB leaves room for texture-to-texture variation."""
-    grid = [1e30, 64, 32, 16, 12, 8, 6, 5, 4, 3.5, 3, 2.5, 2, 1.75, 1.5, 1.25, 1, 0.75, 0.5, 0.25, -1]
+    grid = [1e30, 64, 32, 16, 12, 8, 6, 5, 4, 3.5, 3, 2.5, 2, 1.75, 1.5, 1.25, 1.125, 1, 0.9, 0.8, 0.75, 0.625, 0.5, 0.25, -1]
     per = []
     for side, seed in textures:
         per.append(evaluate(lib, Chain(lib, _texture((side, seed))), state, topk, grid, fallback_k=fallback_k))
