@@ -106,8 +106,8 @@ int zaptex_mlp_set_features(void *state, int mode, void *stream);
 /* logits [batch x 512] for batch blocks of 16 RGBA8 pixels (64 bytes each) */
 int zaptex_mlp_forward(const void *state, const void *blocks, int batch, float *logits, void *ws, size_t ws_bytes, void *stream);
 /* The proposal itself, fused (features, three layers and top-k in one kernel, no workspace): cand [n x topk] gets
- * each block's topk candidate slots of largest logit among the 285 the search uses (largest first, lowest slot on
- * ties), risk [n] = exp(logit 385) - 1 (0 below 0, 2^20 from 13.9 up).  The logits are bit-identical to
+ * each block's topk candidate slots of largest logit among the 285 the search uses, largest first (logits are
+ * ranked on their float32 bits above the 9 lowest, so logits equal down to those bits rank by ascending slot), risk [n] = exp(logit 385) - 1 (0 below 0, 2^20 from 13.9 up).  The logits are bit-identical to
  * zaptex_mlp_forward's.  topk 1..16. */
 int zaptex_mlp_propose(const void *state, const void *blocks, int n, int topk, uint32_t *cand, float *risk, void *stream);
 /* test hook: the fused kernel's logits of the kept output rows (the 285 candidate slots ascending, the risk slot,

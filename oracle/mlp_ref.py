@@ -78,6 +78,18 @@ def targets(B: int, labels: np.ndarray | None = None, deltas: np.ndarray | None 
     return e / e.sum(1, keepdims=True)
 
 
+def propose_topk(Z: np.ndarray, k: int) -> np.ndarray:
+    """[B, k] used slots as the fused proposal kernel ranks them (zaptex_mlp_propose): each used slot's float32
+    logit with its low 9 bits replaced by 511 - u, or by u for a negative logit (u = the slot's rank among the
+    used slots), compared as floats.  Larger logits first; logits equal above their low 9 bits by ascending slot."""
+    slots = np.nonzero(VALID)[0]
+    v = np.ascontiguousarray(Z[:, slots], dtype=np.float32).view(np.uint32)
+    u = np.arange(slots.size, dtype=np.uint32)[None, :]
+    code = np.where(v >> np.uint32(31), u, np.uint32(511) - u).astype(np.uint32)
+    key = ((v & ~np.uint32(511)) | code).view(np.float32).astype(np.float64)
+    return slots[np.argsort(-key, axis=1, kind="stable")[:, :k]]
+
+
 def topk(Z: np.ndarray, k: int) -> np.ndarray:
     """[B, k] used slots by descending logit, lowest slot first on ties."""
     Zm = np.where(VALID[None, :], Z, -np.inf)

@@ -41,7 +41,8 @@ def test_forward_matches_oracle(zt, feat):
 @pytest.mark.parametrize("feat,topk,risk_shift,B", [(0, 4, 0.0, 1000), (4, 1, 30.0, 37), (3, 16, -30.0, 517), (2, 7, 0.0, 5)])
 def test_propose_matches_forward_topk(zt, feat, topk, risk_shift, B):
     """The fused proposal kernel equals the three-GEMM forward followed by a top-k over the used slots: same
-    candidates (largest logit first, lowest slot on ties) and risk = expm1 of the risk logit, clamped."""
+    candidates (ranked by mlp_ref.propose_topk's keys: largest logit first, logits equal above their low 9 bits by
+    ascending slot) and risk = expm1 of the risk logit, clamped."""
     import torch
     params, blocks, labels, st, tb, tl = _setup(zt, B, 11 + topk, feat)
     params[mlp_ref.OFF["b3"][0] + mlp_ref.RISK] += risk_shift   # push the risk logit into either clamp
@@ -63,10 +64,7 @@ def test_propose_matches_forward_topk(zt, feat, topk, risk_shift, B):
     lgn = lg.cpu().numpy()
     assert np.array_equal(lgn[:, :286].view(np.uint32), kept.view(np.uint32))
     assert (lgn[:, 286:] == 0).all()
-    for b in range(B):
-        v = zn[b, slots]
-        order = np.lexsort((slots, -v))[:topk]
-        assert cand[b].cpu().numpy().tolist() == slots[order].tolist(), b
+    assert np.array_equal(cand.cpu().numpy(), mlp_ref.propose_topk(zn, topk))
     r = zn[:, mlp_ref.RISK].astype(np.float64)
     exp = np.where(r <= 0, 0.0, np.where(r >= 13.9, 1048576.0, np.expm1(r)))
     got = risk.cpu().numpy().astype(np.float64)

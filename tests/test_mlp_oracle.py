@@ -86,6 +86,28 @@ def test_topk_order_and_ties():
     assert list(mlp_ref.topk(z, 4)[0]) == [70, 99, 100, 0]
 
 
+def test_propose_topk_keys():
+    """The fused kernel's ranking (mlp_ref.propose_topk) is the float order of the logits wherever they differ
+    above their low 9 bits, and the lowest slot first inside such a tie, for either sign."""
+    z = np.zeros((1, 512))
+    z[0, 70] = 3.0; z[0, 20] = 5.0; z[0, 100] = 2.0; z[0, 99] = 2.0; z[0, 385] = 9.0
+    assert list(mlp_ref.propose_topk(z, 4)[0]) == [70, 99, 100, 0]           # unused slots 20 and 385 never rank
+    rng = np.random.default_rng(5)
+    used = np.nonzero(mlp_ref.VALID)[0]
+    zs = np.zeros((64, 512), np.float32)
+    zs[:, used] = rng.permutation(np.linspace(-20, 20, 64 * used.size, dtype=np.float32)).reshape(64, used.size)
+    assert np.array_equal(mlp_ref.propose_topk(zs, 16), mlp_ref.topk(zs, 16))  # spacing 1.4e-4: far above the 9 bits
+    near = np.zeros((1, 512), np.float32)
+    for lo, hi, bits in [(used[5], used[9], 0x3FC00100), (used[40], used[41], 0xBFC00100)]:   # +-1.50003
+        near[:] = -50.0
+        a = np.array([bits], np.uint32).view(np.float32)[0]
+        b = np.nextafter(a, np.float32(np.inf), dtype=np.float32)             # larger by one ulp, same 9-bit bucket
+        near[0, lo], near[0, hi] = a, b
+        assert list(mlp_ref.propose_topk(near, 2)[0]) == [lo, hi]            # tie: lowest slot first
+        near[0, hi] = a + np.float32(1e-3)                                    # above the 9 bits: the larger first
+        assert list(mlp_ref.propose_topk(near, 2)[0]) == [hi, lo]
+
+
 def test_feature_normalisations():
     rng = np.random.default_rng(3)
     blocks = rng.integers(0, 256, (40, 64), dtype=np.uint8)

@@ -303,10 +303,9 @@ extern "C" __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, 
 // The weights stay in LDS for the kernel's life in fragment order (each wave reads 64 consecutive 16-byte operand
 // pairs: no bank conflicts): W1, W2 and, of W3, only the rows the proposals read (the 285 candidate slots in
 // ascending order, then the risk slot; 2 zero rows pad to 18 MFMA tiles).  A wave takes 32 blocks per pass, two
-// 16-block column tiles sharing every weight fragment.  Layer-3 logits stream into a sorted top-KT list per lane
-// (lane (r, q) sees kept rows 16 t + 4 q + i in ascending order, so a strict > keeps the lowest slot first among
-// equal logits); the four lanes of a block then merge their lists K times by (largest logit, lowest slot) -- the
-// order of a top-k over all 285 slots with ties to the lower slot.
+// 16-block column tiles sharing every weight fragment.  Layer-3 logits become distinct ranking keys (prop_key: the
+// logit down to its 9 low bits, then the lower row) and stream into a sorted top-KT list of keys per lane, one
+// median-of-three per entry; the four lanes of a block then merge their lists K times by the largest key.
 #define ZT_PR_U 288   // W3 rows kept
 #define ZT_PR_NC 2    // 16-block column tiles per wave pass
 namespace {
@@ -384,33 +383,34 @@ ZT_DEV void logits_tile(const uint4 *W3f, const float *B3s, const bf16x4_t (&h2)
 #pragma unroll
     for (int ct = 0; ct < NC; ct++) { v[ct][0] = z[ct][0] + b.x; v[ct][1] = z[ct][1] + b.y; v[ct][2] = z[ct][2] + b.z; v[ct][3] = z[ct][3] + b.w; }
 }
-// sorted (largest first) top-KT of a stream of (logit, row) in ascending row order; ties keep the earlier row first
+// Ranking key of logit v on kernel row u (u < 512): v's bits with the low 9 replaced by 511 - u, or by u when v is
+// negative.  Compared as floats, keys order like the logits down to those 9 bits (2^-14 of the logit) and then by
+// ascending row; the keys of one block are distinct, so a top list needs no row payload.
+ZT_DEV float prop_key(float v, int u) {
+    const uint32_t b = __float_as_uint(v), s = (uint32_t)((int32_t)b >> 31);
+    return __uint_as_float((b & ~511u) | ((s ^ (511u - (uint32_t)u)) & 511u));
+}
+ZT_DEV int prop_key_row(float k) {
+    const uint32_t b = __float_as_uint(k), c = b & 511u;
+    return (int)((b >> 31) ? c : 511u - c);
+}
+// sorted (largest first) top-KT of a stream of keys: one median-of-three per entry
 template <int KT>
 struct TopList {
     float t[KT];
-    int u[KT];
     __device__ __forceinline__ void init() {
 #pragma unroll
-        for (int i = 0; i < KT; i++) { t[i] = -INFINITY; u[i] = 0x7FFFFFFF; }
+        for (int i = 0; i < KT; i++) t[i] = -INFINITY;
     }
-    __device__ __forceinline__ void insert(float x, int ux) {
-        bool c[KT];
+    __device__ __forceinline__ void insert(float x) {
 #pragma unroll
-        for (int i = 0; i < KT; i++) c[i] = x > t[i];
-#pragma unroll
-        for (int i = KT - 1; i > 0; i--) {
-            const int v = c[i] ? ux : u[i];
-            u[i] = c[i - 1] ? u[i - 1] : v;
-            t[i] = __builtin_amdgcn_fmed3f(t[i - 1], t[i], x);
-        }
-        u[0] = c[0] ? ux : u[0];
+        for (int i = KT - 1; i > 0; i--) t[i] = __builtin_amdgcn_fmed3f(t[i - 1], t[i], x);
         t[0] = fmaxf(t[0], x);
     }
     __device__ __forceinline__ void pop_if(bool pop) {
 #pragma unroll
-        for (int i = 0; i + 1 < KT; i++) { t[i] = pop ? t[i + 1] : t[i]; u[i] = pop ? u[i + 1] : u[i]; }
+        for (int i = 0; i + 1 < KT; i++) t[i] = pop ? t[i + 1] : t[i];
         t[KT - 1] = pop ? -INFINITY : t[KT - 1];
-        u[KT - 1] = pop ? 0x7FFFFFFF : u[KT - 1];
     }
 };
 
@@ -492,9 +492,9 @@ ZT_DEV void propose_body(const zt_propose_args &P) {
                     const int u = 16 * t + 4 * q + i;
                     if (t == ZT_PR_U / 16 - 1) {   // the last tile holds rows 272..287: slots up to 284, the risk row 285
                         if (u == 285) riskv[ct] = v[ct][i];
-                        T[ct].insert(u < 285 ? v[ct][i] : -INFINITY, u);
+                        T[ct].insert(u < 285 ? prop_key(v[ct][i], u) : -INFINITY);
                     } else {
-                        T[ct].insert(v[ct][i], u);
+                        T[ct].insert(prop_key(v[ct][i], u));
                     }
                 }
             }
@@ -502,17 +502,12 @@ ZT_DEV void propose_body(const zt_propose_args &P) {
 #pragma unroll
         for (int ct = 0; ct < ZT_PR_NC; ct++) {
             const uint32_t row = (g * ZT_PR_NC + ct) * 16 + r;
-            for (int k = 0; k < K; k++) {
+            for (int k = 0; k < K; k++) {   // the block's largest key over its four lanes' lists, K times
                 float hv = T[ct].t[0];
-                int hu = T[ct].u[0];
-#pragma unroll
-                for (int off = 16; off <= 32; off <<= 1) {
-                    const float ov = __shfl_xor(hv, off);
-                    const int ou = __shfl_xor(hu, off);
-                    if (ov > hv || (ov == hv && ou < hu)) { hv = ov; hu = ou; }
-                }
-                if (P.cand && q == 0 && row < P.n) P.cand[(uint64_t)row * K + k] = (uint32_t)prop_slot(hu);
-                T[ct].pop_if(hu == T[ct].u[0]);
+                hv = fmaxf(hv, __shfl_xor(hv, 16));
+                hv = fmaxf(hv, __shfl_xor(hv, 32));
+                if (P.cand && q == 0 && row < P.n) P.cand[(uint64_t)row * K + k] = (uint32_t)prop_slot(prop_key_row(hv));
+                T[ct].pop_if(__float_as_uint(hv) == __float_as_uint(T[ct].t[0]));
             }
             if (P.risk && q == 3 && row < P.n) {
                 const float x0 = riskv[ct];
