@@ -891,26 +891,28 @@ struct LearnedWs {
     unsigned *counters, *counters2, *counters3;
 };
 constexpr uint64_t kBucketBytes = ZT_BUCKET_WORDS * 4;   // one bucketing counter array (a multiple of 256 B)
+// 14 regions, each starting on a 256-byte boundary
 size_t learned_ws_bytes(uint64_t total, int K, int K2) {
-    return 256 + 3 * kBucketBytes + total * (64 + 32 * (uint64_t)K + 4 + 4 + 4 + 8) + total * 12 * (uint64_t)K2 + 1024;
+    return 256 + 3 * kBucketBytes + total * (64 + 32 * (uint64_t)K + 4 + 4 + 4 + 8) + total * 12 * (uint64_t)K2 + 15 * 256;
 }
 LearnedWs learned_ws(void *ws, uint64_t total, int K, int K2) {
     uint8_t *b = (uint8_t *)align_up((uintptr_t)ws, 256);
+    auto take = [&b](uint64_t bytes) { uint8_t *p = b; b = (uint8_t *)align_up((uintptr_t)(b + bytes), 256); return (void *)p; };
     LearnedWs v;
-    v.counters = (unsigned *)b; b += kBucketBytes;
-    v.counters2 = (unsigned *)b; b += kBucketBytes;
-    v.counters3 = (unsigned *)b; b += kBucketBytes;
-    v.blocks = (uint4 *)b; b += total * 64;
-    v.items3 = (uint2 *)b; b += total * 8;
-    v.enc = (uint4 *)b; b += total * 16 * K;
-    v.items = (uint2 *)b; b += total * 8 * K;
-    v.items2 = (uint2 *)b; b += total * 8 * (uint64_t)K2;
-    v.cand = (uint32_t *)b; b += total * 4 * K;
-    v.slot = (uint32_t *)b; b += total * 4 * K;
-    v.cand2 = (uint32_t *)b; b += total * 4 * (uint64_t)K2;
-    v.key = (uint32_t *)b; b += total * 4;
-    v.risk = (float *)b; b += total * 4;
-    v.fallback = (uint32_t *)b;
+    v.counters = (unsigned *)take(kBucketBytes);
+    v.counters2 = (unsigned *)take(kBucketBytes);
+    v.counters3 = (unsigned *)take(kBucketBytes);
+    v.blocks = (uint4 *)take(total * 64);
+    v.items3 = (uint2 *)take(total * 8);
+    v.enc = (uint4 *)take(total * 16 * K);
+    v.items = (uint2 *)take(total * 8 * K);
+    v.items2 = (uint2 *)take(total * 8 * (uint64_t)K2);
+    v.cand = (uint32_t *)take(total * 4 * K);
+    v.slot = (uint32_t *)take(total * 4 * K);
+    v.cand2 = (uint32_t *)take(total * 4 * (uint64_t)K2);
+    v.key = (uint32_t *)take(total * 4);
+    v.risk = (float *)take(total * 4);
+    v.fallback = (uint32_t *)take(total * 4);
     return v;
 }
 typedef void (*PropFn)(const uint4 *, const uint2 *, const unsigned *, uint32_t *, uint4 *);
