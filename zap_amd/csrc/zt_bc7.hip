@@ -154,7 +154,7 @@ extern "C" __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 }
 
 // ---- stage 1 split over three kernels for M1K <= 8 (every lane busy):
-//   rank  one wave per list-1 entry: partition estimates, wave rank, top-M1K partitions in rank order
+//   rank  one wave per list-1 entry: partition estimates, the top-M1K partitions in rank order by M1K wave minima
 //   fit   one lane per (entry, candidate, subset): the exact subset fit; SSE and the subset's quantised state
 //   pick  one lane per entry: first candidate in rank order with the smallest SSE below the current one, packed
 // Capacity: entry e owns cand[8e..8e+7] and items [16e, 16e + 16).
@@ -182,12 +182,19 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L,
         if (est == 0.0f) est = 0.0f;
         uint32_t fb = __float_as_uint(est);
         fb = (fb & 0x80000000u) ? ~fb : (fb | 0x80000000u);
-        int rank = 0;
-        for (int k = 0; k < 64; k++) {
-            uint32_t ofb = __shfl(fb, k);
-            rank += ofb < fb || (ofb == fb && k < lane);
+        // the m1k smallest (estimate, partition) pairs in ascending order, one wave minimum each: the partition of
+        // rank k is the lowest lane holding the smallest key among the lanes not yet taken
+        uint64_t taken = 0;
+        for (int k = 0; k < m1k; k++) {
+            const bool free = !((taken >> lane) & 1ull);
+            const uint32_t key = free ? fb : 0xFFFFFFFFu;
+            uint32_t mn = key;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+            const int sel = __ffsll((unsigned long long)__ballot(free && key == mn)) - 1;
+            taken |= 1ull << sel;
+            if (lane == 0) cand[(uint64_t)e * 8 + k] = (uint8_t)sel;
         }
-        if (rank < m1k) cand[(uint64_t)e * 8 + rank] = (uint8_t)lane;
     }
 }
 
