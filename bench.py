@@ -17,7 +17,8 @@ Also reported, measured inside this run:
                 launch shapes from the autotuner (zap_amd/tuning/mi355x.json)
   learned       the learned-proposal encode (zap_amd/models/bc7_proposal.npy, top-4 exact + exhaustive fallback at
                 the model's calibrated threshold) against the exhaustive all-mode search on the same chain:
-                Mpix/s of both, chain PSNR of both (tex_bench.c's RGBA definition) and the gap
+                Mpix/s of both, chain PSNR of both (tex_bench.c's RGBA definition) and the gap; the network
+                kernel's TFLOP/s against the dense bf16 MFMA peak ("network")
   configs       BC1 of the 1024^2 image, BC6H of the 2048^2 HDR field, ASTC 4x4 of the 2048^2 texture: device
                 Mpix/s, golden-hash check, kernel times, and the reference encoder's rate on the host cores
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -38,6 +39,7 @@ sys.path.insert(0, ROOT)
 PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz: one scalar op per lane per clock
 PEAK_WAVE_INSTS = 256 * 2.4e9                # one wave64 VALU instruction per CU per clock (4 SIMD16 x 4 cycles)
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0                    # dense bf16 MFMA
 MIP_CHAIN_BYTES = 89_478_484                 # read the 4096^2 base once + write every generated level once
 # zt_bc7_stage6 on this workload, rocprofv3 --pmc per dispatch (profiles/zt_bc7_stage6_c2_chain.md):
 # FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the counter reads for a plain 256 MiB device copy
@@ -115,6 +117,24 @@ def learned_leg(lib, c2, plain_mpix_s: float | None = None, steps: int = 5) -> d
     if plain_mpix_s:
         out["plain_mpix_s"] = round(plain_mpix_s, 1)
         out["speed_vs_plain"] = round(lr["mpix_s"] / plain_mpix_s, 4)
+    # the network kernel against the bf16 MFMA peak: 2 (64*128 + 128*128 + 128*288) FLOP per block on the rows it
+    # keeps, over its mean duration from the library's HIP events on its launch stream
+    import torch
+    imgs = chain.images()
+    lws = lib.bc7_learned_workspace(imgs, meta["topk"], fk)
+    name = "zt_mlp_propose4" if meta["topk"] <= 4 else "zt_mlp_propose16"
+    lib.profile(True)
+    lib.profile_reset()
+    for _ in range(3):
+        lib.bc7_learned_dev(imgs, state, meta["topk"], float(meta["threshold"]), lws, fk)
+    torch.cuda.synchronize()
+    t, n = lib.profile_get(name)
+    lib.profile(False)
+    if n:
+        flop = 2 * (64 * 128 + 128 * 128 + 128 * 288) * chain.blocks
+        tf = flop / (t / n / 1e3) / 1e12
+        out["network"] = {"kernel": name, "bound": "mfma", "achieved": round(tf, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(tf / PEAK_BF16_TFLOPS, 4), "flop_per_launch": flop, "ms_per_launch": round(t / n, 5)}
     log(f"learned: {out['mpix_s']} Mpix/s ({out['speedup_vs_exhaustive']}x exhaustive {out['exhaustive_mpix_s']}), "
         f"gap {out['gap_db']} dB, fallback {out['fallback']}")
     return out
