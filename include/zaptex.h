@@ -78,6 +78,14 @@ size_t zaptex_workspace_size(const zaptex_image *imgs, int n);
  * zaptex_workspace_size bytes (256-byte aligned), owned by the caller; calls sharing a workspace must not
  * overlap. */
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream);
+/* zaptex_encode_dev without a workspace argument and without any allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The workspace is a scratch array inside the library, ZAPTEX_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends; images whose zaptex_workspace_size exceeds it are rejected with
+ * ZAPTEX_E_ARG.  Every call on one GPU shares that GPU's array, so calls that overlap on one GPU (another thread,
+ * another stream) race and must be ordered by the caller.  Kernel timing (ZAPTEX_OPT_PROFILE) is skipped. */
+#define ZAPTEX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int zaptex_encode_dev_noalloc(const zaptex_image *imgs, int n, void *stream);
 /* Exhaustive all-mode BC7 search (no reference counterpart: the reference tries modes 6, 5 and 1 only).
  * 285 candidates per block (mode*64 + slot); never worse than zaptex_encode_dev's reference search.
  * best_cand (device, one uint32 per block of all images in order, may be NULL) receives the chosen candidate;

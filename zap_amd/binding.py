@@ -102,7 +102,7 @@ class Library:
             "zaptex_astc_encode": [vp, i, i, sz, vp], "zaptex_astc_decode": [vp, i, i, vp, sz],
             "zaptex_mip_next": [vp, i, i, sz, i, vp, sz], "zaptex_mip_next_f": [vp, i, i, sz, vp, sz],
             "zaptex_bc_split": [vp, sz, i, vp], "zaptex_bc_merge": [vp, sz, i, vp],
-            "zaptex_encode_dev": [C.POINTER(ZapImage), i, vp, sz, vp],
+            "zaptex_encode_dev": [C.POINTER(ZapImage), i, vp, sz, vp], "zaptex_encode_dev_noalloc": [C.POINTER(ZapImage), i, vp],
             "zaptex_mip_next_dev": [vp, i, i, sz, i, vp, sz, vp], "zaptex_mip_next_f_dev": [vp, i, i, sz, vp, sz, vp],
             "zaptex_mip_chain_dev": [vp, i, i, sz, i, C.POINTER(vp), i, vp],
             "zaptex_bc_decode_dev": [vp, i, i, i, vp, sz, vp],
@@ -255,6 +255,12 @@ class Library:
         ws_ptr = workspace.data_ptr() if workspace is not None else None
         ws_len = workspace.numel() * workspace.element_size() if workspace is not None else 0
         self._check(self.lib.zaptex_encode_dev(imgs, len(imgs), ws_ptr, ws_len, self._stream(stream)), "zaptex_encode_dev")
+
+    NOALLOC_SCRATCH_BYTES = 64 << 20   # ZAPTEX_NOALLOC_SCRATCH_BYTES
+
+    def encode_dev_noalloc(self, imgs, stream=None) -> None:
+        """zaptex_encode_dev on the library's own scratch array: no workspace, no allocation."""
+        self._check(self.lib.zaptex_encode_dev_noalloc(imgs, len(imgs), self._stream(stream)), "zaptex_encode_dev_noalloc")
 
     def bc7_full_workspace(self, imgs, with_deltas: bool = False):
         import torch

@@ -38,6 +38,8 @@ int groups_for(int c, uint64_t items, uint64_t per_group) {
 // batch graph is being captured)
 bool g_profile = false;
 bool g_capturing = false;
+// set while a no-alloc call enqueues, so that kernel timing (which creates HIP events) stays off
+thread_local bool t_noalloc = false;
 std::mutex g_prof_mu;
 struct ProfPending { const char *name; hipEvent_t a, b; };
 struct ProfTotal { const char *name; double ms; long n; };
@@ -60,7 +62,7 @@ int fail(int code, const char *fmt, ...) {
 // launch `fn` on stream s; with profiling on, bracket it with events
 template <typename F> int launch(const char *name, hipStream_t s, F &&fn) {
     hipEvent_t a = nullptr, b = nullptr;
-    const bool prof = g_profile && !g_capturing;
+    const bool prof = g_profile && !g_capturing && !t_noalloc;
     if (prof) {
         HIP_TRY(hipEventCreate(&a)); HIP_TRY(hipEventCreate(&b));
         HIP_TRY(hipEventRecord(a, s));
@@ -97,9 +99,15 @@ uint32_t fnv1a(const void *p, size_t n) {
 }
 const uint32_t kSrgbLinHash = 0xab9a3c1bu, kSrgbThrHash = 0x4f5c9c4fu;
 
+// The encoders' workspace for zaptex_encode_dev_noalloc: a file-scope device array, one copy per GPU, held until
+// the process ends.  Host code never passes its name to a kernel (on the host the name is a stand-in, not a
+// device address); it uses the per-device address that ensure_ready reads with hipGetSymbolAddress.
+constexpr size_t kScratchBytes = ZAPTEX_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kScratchBytes];
 struct DeviceState {
     bool ready = false;
     hipStream_t stream = nullptr;
+    void *scratch = nullptr;   // this device's address of `scratch`
     // staging for the host entry points (device buffers, and pinned host memory for rows that are not packed)
     void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr;
     size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0;
@@ -171,6 +179,8 @@ int ensure_ready(DeviceState *&st) {
     zt_astc_setup setup;
     zt_astc_setup_build(&setup);
     HIP_TRY(zt_astc_upload_tables(&setup, sizeof setup));
+    HIP_TRY(hipGetSymbolAddress(&st->scratch, HIP_SYMBOL(scratch)));
+    if (!st->scratch || ((uintptr_t)st->scratch) % 256) return fail(ZAPTEX_E_DEVICE, "scratch array of device %d is not 256-byte aligned", dev);
     HIP_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
     st->ready = true;
     return ZAPTEX_OK;
@@ -556,6 +566,31 @@ int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t w
         if (int rc = ensure_ready(st)) return rc;
     }
     return enqueue(imgs, n, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+// zaptex_encode_dev with the library's per-device scratch array as its workspace: no allocation of any kind
+// (set-up on a device's first call reads the array's address; profiling events are not recorded)
+int zaptex_encode_dev_noalloc(const zaptex_image *imgs, int n, void *stream) {
+    if (!imgs || n < 1 || n > ZT_INLINE_JOBS) return fail(ZAPTEX_E_ARG, "image count %d outside 1..%d", n, ZT_INLINE_JOBS);
+    uint64_t total = 0;
+    for (int i = 0; i < n; i++) {
+        if (int rc = check_image(imgs[i])) return rc;
+        total += nblocks(imgs[i]);
+    }
+    if (total >= (1ull << 31)) return fail(ZAPTEX_E_ARG, "too many blocks in one call");
+    const size_t need = zaptex_workspace_size(imgs, n);
+    if (need > kScratchBytes)
+        return fail(ZAPTEX_E_ARG, "these images need %zu bytes of workspace, above the %zu-byte scratch of zaptex_encode_dev_noalloc "
+                    "(encode them with zaptex_encode_dev and a workspace of zaptex_workspace_size bytes)", need, kScratchBytes);
+    DeviceState *st;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (int rc = ensure_ready(st)) return rc;
+    }
+    t_noalloc = true;
+    const int rc = enqueue(imgs, n, st->scratch, kScratchBytes, (hipStream_t)stream);
+    t_noalloc = false;
+    return rc;
 }
 
 namespace {
