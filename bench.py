@@ -99,10 +99,11 @@ def cpu_baseline(levels_np, threads: int, repeats: int = 5, configs: dict | None
     return res
 
 
-def learned_leg(lib, c2, plain_mpix_s: float | None = None, steps: int = 5) -> dict:
-    """Learned proposal vs exhaustive all-mode search on the 4096^2 chain (mip chain regenerated on the GPU).  The
-    plain search's rate (the headline value) is set beside it: the learned encode trades speed against that
-    heuristic for near-exhaustive quality."""
+def learned_leg(lib, c2, steps: int = 5) -> dict:
+    """Learned proposal vs exhaustive all-mode search on the 4096^2 chain (mip chain regenerated on the GPU), with
+    the shipped fallback (the network's top fallback_k) and with the exhaustive search as the fallback
+    (fallback_k = 0).  The plain search's encode-only rate on the same chain is set beside it: the learned encode
+    trades speed against that heuristic for near-exhaustive quality."""
     from zap_amd import learned
     state, meta = learned.load(lib)
     chain = learned.Chain(lib, c2)
@@ -113,10 +114,13 @@ def learned_leg(lib, c2, plain_mpix_s: float | None = None, steps: int = 5) -> d
            "mpix_s": round(lr["mpix_s"], 1), "exhaustive_mpix_s": round(full["mpix_s"], 1),
            "speedup_vs_exhaustive": round(lr["mpix_s"] / full["mpix_s"], 3), "psnr_db": round(lr["psnr"], 4),
            "exhaustive_psnr_db": round(full["psnr"], 4), "gap_db": round(lr["gap_db"], 4), "fallback": round(lr["fallback"], 4),
-           "plain_psnr_db": round(plain["psnr"], 4), "ms": round(lr["ms"], 3), "exhaustive_ms": round(full["ms"], 3)}
-    if plain_mpix_s:
-        out["plain_mpix_s"] = round(plain_mpix_s, 1)
-        out["speed_vs_plain"] = round(lr["mpix_s"] / plain_mpix_s, 4)
+           "plain_psnr_db": round(plain["psnr"], 4), "ms": round(lr["ms"], 3), "exhaustive_ms": round(full["ms"], 3),
+           "plain_encode_mpix_s": round(plain["mpix_s"], 1), "speed_vs_plain": round(lr["mpix_s"] / plain["mpix_s"], 4)}
+    if fk:
+        r0 = learned.evaluate(lib, chain, state, meta["topk"], [meta["threshold"]], steps, fallback_k=0)["learned"][0]
+        out["exhaustive_fallback"] = {"fallback_k": 0, "mpix_s": round(r0["mpix_s"], 1), "ms": round(r0["ms"], 3),
+                                      "speedup_vs_exhaustive": round(r0["mpix_s"] / full["mpix_s"], 3), "psnr_db": round(r0["psnr"], 4),
+                                      "gap_db": round(r0["gap_db"], 4), "fallback": round(r0["fallback"], 4)}
     # the network kernel against the bf16 MFMA peak: 2 (64*128 + 128*128 + 128*288) FLOP per block on the rows it
     # keeps, over its mean duration from the library's HIP events on its launch stream
     import torch
@@ -153,8 +157,15 @@ def batch_leg(lib, steps: int = 5) -> dict:
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     bt = B.Batch(lib, [(bases[i], w, h, fmt, srgb, None) for i, (w, h, fmt, srgb, kind) in enumerate(specs)])
-    for _ in range(2):
-        bt.run()
+    bt.run()
+    torch.cuda.synchronize()
+    # every texture's levels and blocks against the reference's hashes, before timing
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "c4_batch.json")))["textures"]
+    got = bt.digests()
+    bad = [i for i, (g, d) in enumerate(zip(gold, got)) if (g["mips"], g["blocks"]) != d]
+    exact = not bad and len(gold) == len(got)
+    log(f"batch: {len(got) - len(bad)}/{len(gold)} textures byte-identical to the reference")
+    bt.run()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -165,7 +176,8 @@ def batch_leg(lib, steps: int = 5) -> dict:
     ms = e0.elapsed_time(e1) / steps
     fmts = {f: sum(1 for s in specs if s[2] == f) for f in ("bc1", "bc3", "bc5", "bc7")}
     out = {"textures": len(specs), "formats": fmts, "pixels": bt.pixels, "blocks": bt.blocks, "ms": round(ms, 3),
-           "mpix_s": round(bt.pixels / (ms / 1e3) / 1e6, 1), "graph_launches_per_step": 1, "launch_shapes": shapes,
+           "mpix_s": round(bt.pixels / (ms / 1e3) / 1e6, 1), "byte_exact_vs_reference": exact, "textures_differing": bad[:16],
+           "graph_launches_per_step": 1, "launch_shapes": shapes,
            "generate_s": round(gen_s, 2)}
     bt.close()
     autotune.reset(lib)
@@ -369,7 +381,7 @@ def main() -> int:
         log(f"host entry (H2D + encode + D2H per level): {e2e_mpix} Mpix/s")
 
     autotune.reset(lib)
-    learned_res = None if args.no_learned else learned_leg(lib, c2, value)
+    learned_res = None if args.no_learned else learned_leg(lib, c2)
     batch_res = None if args.no_batch else batch_leg(lib)
     cfg_imgs = None if args.no_configs else config_images()
     configs_res = None if args.no_configs else configs_leg(lib, golden, cfg_imgs, 20)

@@ -29,7 +29,26 @@ unsigned long long zto_ops_get(void) { return g_ops; }
 void zto_ops_reset(void) { g_ops = 0; g_stage[0] = g_stage[1] = g_stage[2] = g_stage[3] = 0; }
 /* BC7 ops by search stage: [0] mode 6 + opaque test, [1] mode 5, [2] mode-1 ranking + fits, [3] blocks reaching mode 1 */
 void zto_stage_ops(unsigned long long out[4]) { for (int k = 0; k < 4; k++) out[k] = g_stage[k]; }
+/* Unclamped (int) casts, per site (0: BC1 zap__to565, 1: BC6H zap__bc6_q, 2: ASTC zap__astc_quant): the largest
+ * |argument| that fits int32 and the number that do not (NaN included); and least-squares steps that pass the
+ * reference's |det| >= 1e-6 guard with a nearly singular system (0: BC1 |det| < 1e-2, 1: ASTC |det| < 1e-4). */
+static _Thread_local double g_cast_max[3];
+static _Thread_local unsigned long long g_cast_bad[3], g_ls_near[2];
+#define CAST_NOTE(site, f)                                                                 \
+    do {                                                                                   \
+        double a_ = fabs((double)(f));                                                     \
+        if (a_ != a_ || a_ >= 2147483648.0) g_cast_bad[site]++;                            \
+        else if (a_ > g_cast_max[site]) g_cast_max[site] = a_;                             \
+    } while (0)
+#define LS_NEAR(k, det, lim) ((det) < (lim) && (det) > -(lim) ? (void)g_ls_near[k]++ : (void)0)
+void zto_cast_stats(double maxabs[3], unsigned long long bad[3], unsigned long long near[2]) {
+    for (int k = 0; k < 3; k++) { maxabs[k] = g_cast_max[k]; bad[k] = g_cast_bad[k]; }
+    near[0] = g_ls_near[0]; near[1] = g_ls_near[1];
+}
+void zto_cast_reset(void) { for (int k = 0; k < 3; k++) { g_cast_max[k] = 0; g_cast_bad[k] = 0; } g_ls_near[0] = g_ls_near[1] = 0; }
 #else
+#define CAST_NOTE(site, f) ((void)0)
+#define LS_NEAR(k, det, lim) ((void)0)
 #define OPS(n) ((void)0)
 #define STAGE_BEGIN() ((void)0)
 #define STAGE_END(k) ((void)0)
@@ -39,7 +58,9 @@ typedef uint8_t px4[4];
 
 /* x86 cvttss2si semantics for (int)f: a value whose truncation leaves int32 (or NaN) gives INT_MIN.
  * The reference casts unclamped least-squares results (zap_tex.h:46, :842, :1187); this pins that case. */
-static int cvt_i32(float f) {
+static int cvt_i32(float f, int site) {
+    CAST_NOTE(site, f);
+    (void)site;
     if (f >= -2147483648.0f && f < 2147483648.0f) return (int)f;
     return INT_MIN;
 }
@@ -64,7 +85,7 @@ static void unpack565(unsigned c, int rgb[3]) { /* zap_tex.h:41-44 */
 }
 static int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
 static uint16_t pack565(float r, float g, float b) { /* zap_tex.h:45-49 */
-    int R = cvt_i32(r * 31.0f / 255.0f + 0.5f), G = cvt_i32(g * 63.0f / 255.0f + 0.5f), B = cvt_i32(b * 31.0f / 255.0f + 0.5f);
+    int R = cvt_i32(r * 31.0f / 255.0f + 0.5f, 0), G = cvt_i32(g * 63.0f / 255.0f + 0.5f, 0), B = cvt_i32(b * 31.0f / 255.0f + 0.5f, 0);
     OPS(3 * 4 + 3 * 2 + 2);
     return (uint16_t)(clampi(R, 0, 31) << 11 | clampi(G, 0, 63) << 5 | clampi(B, 0, 31));
 }
@@ -159,6 +180,7 @@ static int bc1_block(const px4 *px, uint16_t *oc0, uint16_t *oc1, uint32_t *oidx
             float det = aa * bb - ab * ab;
             OPS(16 * (1 + 6 + 3 + 12) + 3 + 2);
             if (det < 1e-6f && det > -1e-6f) break;
+            LS_NEAR(0, det, 1e-2f);
             float A[3], B[3];
             for (int k = 0; k < 3; k++) { A[k] = (sa[k] * bb - sb[k] * ab) / det; B[k] = (sb[k] * aa - sa[k] * ab) / det; }
             OPS(3 * 8);
@@ -876,7 +898,7 @@ static float float_of(uint16_t h) { /* zap_tex.h:828-835 */
 static int bc6_unquant(int e) { return e == 0 ? 0 : e == 1023 ? 0xFFFF : ((e << 16) + 0x8000) >> 10; }
 static int bc6_finish(int u) { return (u * 31) >> 6; }
 static int bc6_quant(float u) { /* zap_tex.h:841-851 */
-    int g = cvt_i32((u - 32.0f) / 64.0f + 0.5f), best = 0;
+    int g = cvt_i32((u - 32.0f) / 64.0f + 0.5f, 1), best = 0;
     float bd = 1e30f;
     for (long long t = (long long)g - 1; t <= (long long)g + 1; t++) {
         int e = t < 0 ? 0 : t > 1023 ? 1023 : (int)t;
@@ -1164,7 +1186,7 @@ static void astc_quantise(const astc_config *c, const float lo[4], const float h
     float s0 = lo[0] + lo[1] + lo[2], s1 = hi[0] + hi[1] + hi[2];
     const float *a = s1 >= s0 ? lo : hi, *b = s1 >= s0 ? hi : lo;
     for (int k = 0; k < c->nv / 2; k++) {
-        int x0 = cvt_i32(a[k] + 0.5f), x1 = cvt_i32(b[k] + 0.5f);
+        int x0 = cvt_i32(a[k] + 0.5f, 2), x1 = cvt_i32(b[k] + 0.5f, 2);
         x0 = clampi(x0, 0, 255); x1 = clampi(x1, 0, 255);
         ise[2 * k] = c->q[x0]; ise[2 * k + 1] = c->q[x1];
         uv[2 * k] = colour_unquant(c->r, ise[2 * k]); uv[2 * k + 1] = colour_unquant(c->r, ise[2 * k + 1]);
@@ -1213,6 +1235,7 @@ static int astc_trial(const px4 *px, const astc_config *c, const int *part, int 
             float det = aa * bb - ab * ab;
             OPS(5);
             if (det < 1e-6f && det > -1e-6f) continue;
+            LS_NEAR(1, det, 1e-4f);
             for (int ch = 0; ch < nc; ch++) { nlo[ch] = (xa[ch] * bb - xb[ch] * ab) / det; nhi[ch] = (xb[ch] * aa - xa[ch] * ab) / det; }
             OPS(nc * 8);
             for (int ch = nc; ch < 4; ch++) nlo[ch] = nhi[ch] = 255;

@@ -100,6 +100,35 @@ def test_c4_slice_equals_single_calls(zt):
     b.close()
 
 
+def test_c4_full_batch_matches_reference_hashes(zt):
+    """The whole 512-texture workload in one graph launch: every texture's generated levels and every level's
+    blocks hash to the compiled reference's bytes (tests/golden/c4_batch.json, made by make_c4_golden.py), with
+    the autotuned launch shapes the benchmark uses."""
+    import hashlib
+    import json
+    import os
+
+    import torch
+
+    from zap_amd import autotune
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c4_batch.json")))["textures"]
+    specs, bases = batch.c4_textures(512, 4)
+    assert [(w, h, fmt, srgb) for w, h, fmt, srgb, _ in specs] == [(g["w"], g["h"], g["fmt"], g["srgb"]) for g in gold]
+    autotune.apply(zt, "c4")
+    try:
+        b = batch.Batch(zt, [(bases[i], w, h, fmt, srgb, None) for i, (w, h, fmt, srgb, kind) in enumerate(specs)])
+        b.run()
+        torch.cuda.synchronize()
+        got = b.digests()
+        b.close()
+    finally:
+        autotune.reset(zt)
+    bad = [i for i, (g, (m, o)) in enumerate(zip(gold, got)) if (g["mips"], g["blocks"]) != (m, o)]
+    # a mismatch names whether the generated base already differs
+    bases_bad = [i for i in bad if hashlib.sha256(bases[i].cpu().numpy().tobytes()).hexdigest() != gold[i]["base"]]
+    assert not bad, (bad[:10], "bases differ:", bases_bad[:10])
+
+
 def test_launch_shapes_do_not_change_bytes(zt):
     import torch
     b, _ = _make(zt, SMALL[:10], seed0=700)

@@ -38,3 +38,21 @@ def test_trained_network_beats_untrained(zt, model, chain):
     assert trained["fallback"] == 0.0 and untrained["fallback"] == 0.0
     assert trained["gap_db"] < 0.3, trained
     assert trained["gap_db"] < 0.25 * untrained["gap_db"], (trained, untrained)
+
+
+@pytest.fixture(scope="module")
+def chain4096(zt):
+    return learned.Chain(zt, synth.c2_bc7(4096, 2))
+
+
+@pytest.mark.parametrize("fallback_k", [64, 0])
+def test_learned_gap_on_the_4096_benchmark_chain(zt, model, chain4096, fallback_k):
+    """The learned-proposal workload itself: the 4096^2 seed-2 texture and its 13-level chain (never trained or
+    calibrated on), at the model's calibrated threshold, with the shipped fallback (the network's top 64) and with
+    the exhaustive search as the fallback; the chain's PSNR stays within 0.05 dB of the exhaustive search's."""
+    state, meta = model
+    r = learned.evaluate(zt, chain4096, state, meta["topk"], [meta["threshold"]], steps=1, fallback_k=fallback_k)
+    lr = r["learned"][0]
+    assert lr["gap_db"] <= 0.05, lr
+    assert 0 < lr["fallback"] <= 0.05, lr                  # >= 95% of blocks: 4 exact candidates instead of 285
+    assert lr["psnr"] > r["plain"]["psnr"] + 0.5, (lr, r["plain"])   # far above the reference's own search

@@ -32,6 +32,16 @@ def test_small_fixtures(zt, golden):
         assert zt.mip_next(img, 1).tobytes().hex() == ent["mip_srgb"], key
 
 
+def test_near_singular_blocks(zt, golden):
+    """Blocks whose BC1 least-squares step passes the det guard with a nearly singular system (the reference's
+    largest unclamped casts), as the compiled reference encodes them."""
+    ent = golden["near_singular"]
+    img = np.frombuffer(bytes.fromhex(ent["input"]), np.uint8).reshape(4, 4 * ent["blocks"], 4)
+    for f in ("bc1", "bc3"):
+        assert zt.bc_encode(img, f).tobytes().hex() == ent[f], f
+    assert zt.astc_encode(img).tobytes().hex() == ent["astc"]
+
+
 def test_adversarial_blocks(zt, golden):
     blocks = synth.blocks_to_image(synth.random_blocks(8192, 7))
     for f in FMTS:

@@ -47,3 +47,32 @@ def test_c2_mips_and_bc7_lower_levels(oracle, golden):
     # BC7 of levels 2.. (1024^2 and below) keeps the CPU suite short; the GPU suite checks all 13 levels
     for lv in range(2, len(chain)):
         assert sha(oracle.bc_encode(chain[lv], "bc7")) == golden["outputs"]["c2_bc7_chain"][lv], lv
+
+
+def test_near_singular_fixture(oracle, golden):
+    ent = golden["near_singular"]
+    assert ent["blocks"] >= 32
+    img = np.frombuffer(bytes.fromhex(ent["input"]), np.uint8).reshape(4, 4 * ent["blocks"], 4)
+    for f in ("bc1", "bc3"):
+        assert oracle.bc_encode(img, f).tobytes().hex() == ent[f], f
+    assert oracle.astc_encode(img).tobytes().hex() == ent["astc"]
+
+
+def test_c4_batch_hashes_of_the_smallest_textures(oracle):
+    """The batch workload's golden file agrees with the oracle (generator + mip chain + encoder) on its 12
+    smallest textures; the GPU test checks all 512."""
+    import hashlib
+    import json
+    import os
+
+    from zap_amd import batch
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c4_batch.json")))["textures"]
+    specs = batch.c4_specs(512, 4)
+    assert len(gold) == 512
+    for i in sorted(range(512), key=lambda i: specs[i][0] * specs[i][1])[:12]:
+        w, h, fmt, srgb, kind = specs[i]
+        base = batch.texture_cpu(w, h, kind, 400000 + i)
+        chain = oracle.mip_chain(base, srgb)
+        assert sha(base) == gold[i]["base"], i
+        assert hashlib.sha256(b"".join(l.tobytes() for l in chain[1:])).hexdigest() == gold[i]["mips"], i
+        assert hashlib.sha256(b"".join(oracle.bc_encode(l, fmt).tobytes() for l in chain)).hexdigest() == gold[i]["blocks"], i

@@ -83,6 +83,19 @@ class Batch:
         o = it["layout"][2][lv]
         return it["out"][o:o + binding.bc_size(w, h, it["fmt"])]
 
+    def digests(self) -> list[tuple[str, str]]:
+        """Per texture, SHA-256 of its generated levels 1.. concatenated and of every level's blocks concatenated."""
+        import hashlib
+        out = []
+        for i, it in enumerate(self.items):
+            mh, bh = hashlib.sha256(), hashlib.sha256()
+            for lv in range(1, it["levels"]):
+                mh.update(self.level(i, lv).cpu().numpy().tobytes())
+            for lv in range(it["levels"]):
+                bh.update(self.blocks_of(i, lv).cpu().numpy().tobytes())
+            out.append((mh.hexdigest(), bh.hexdigest()))
+        return out
+
     def close(self) -> None:
         if self.handle:
             self.lib.batch_destroy(self.handle)

@@ -170,6 +170,40 @@ def random_blocks(n: int, seed: int, kind: str = "mixed") -> np.ndarray:
     return out.reshape(n, 16, 4)
 
 
+def degenerate_blocks(n: int, seed: int) -> np.ndarray:
+    """n near-degenerate 4x4 RGBA blocks (n, 16, 4), the inputs that drive the encoders' least-squares solves
+    towards a singular system: solid with +-1 on one channel, two colours, collinear steps (small and large),
+    solid with one outlier, solid with +-3 noise; families cycle with the block index."""
+    k = np.arange(n, dtype=np.int64)
+    r = hash32(seed, k[:, None], np.arange(80)[None, :]).astype(np.int64)
+    base = r[:, 0:4] & 255
+    px = np.repeat(base[:, None, :], 16, axis=1)
+    fam = k % 6
+    i16 = np.arange(16)
+    one = r[:, 4] % 3                                                    # 0: +-1 on one channel
+    noise1 = (r[:, 16:32] % 3) - 1
+    b0 = np.nonzero(fam == 0)[0]
+    px[b0[:, None], i16[None, :], one[b0][:, None]] = base[b0, one[b0]][:, None] + noise1[b0]
+    c2 = r[:, 5:9] & 255                                                 # 1: two colours
+    m2 = (r[:, 32:48] & 255) < (r[:, 9:10] & 255)
+    sel = fam == 1
+    px[sel] = np.where(m2[sel][:, :, None], c2[sel][:, None, :], px[sel])
+    d_small = (r[:, 10:13] % 17) - 8                                     # 2: collinear steps
+    s4 = r[:, 48:64] % 4
+    sel = fam == 2
+    px[sel, :, :3] = base[sel][:, None, :3] + s4[sel][:, :, None] * d_small[sel][:, None, :]
+    d_big = (r[:, 13:16] % 81) - 40                                      # 4: two levels far apart
+    s2 = (r[:, 64:80] & 1) * (1 + r[:, 14:15] % 3)
+    sel = fam == 4
+    px[sel, :, :3] = base[sel][:, None, :3] + s2[sel][:, :, None] * d_big[sel][:, None, :]
+    sel = fam == 3                                                       # 3: one outlier
+    out_i = r[:, 15] % 16
+    px[sel, out_i[sel], :] = r[sel][:, 40:44] & 255
+    sel = fam == 5                                                       # 5: +-3 noise
+    px[sel, :, :3] += ((r[sel][:, 16:64] % 7) - 3).reshape(-1, 16, 3)
+    return np.clip(px, 0, 255).astype(np.uint8)
+
+
 def blocks_to_image(blocks: np.ndarray) -> np.ndarray:
     """(n,16,4) blocks -> 4 x (4n) RGBA image whose k-th 4x4 block is blocks[k]."""
     n = blocks.shape[0]
