@@ -60,6 +60,14 @@ int zaptex_mip_next_f(const float *src, int w, int h, size_t stride, float *dst,
 int zaptex_bc_split(const void *bc, size_t n, int fmt, void *out);
 int zaptex_bc_merge(const void *planes, size_t n, int fmt, void *out);
 
+/* ---------------- texture files (zap.c:309-355, the `zap tex` command; zap.c is the reference's CLI, not zap_tex.h)
+ * The bytes `zap tex` writes for an RGBA8 image: for BC1/BC3/BC4/BC5/BC7 a DDS file (the DX10 extension for BC7 and
+ * sRGB formats, DXGI 71/77/80/83/98, +1 for sRGB) holding level 0 and, with `mips`, every level of the box chain
+ * down to 1x1 (filtered like zap_mip_next with `srgb`; BC4/BC5 are never sRGB); for ZAPTEX_ASTC a .astc file
+ * holding one level (`mips` ignored, as there).  The chain is generated and encoded on the device. */
+size_t zaptex_tex_file_size(int w, int h, int fmt, int mips, int srgb);   /* 0 for invalid arguments */
+int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, int mips, int srgb, void *out, size_t out_size);
+
 /* ---------------- device-resident entry points (no reference counterpart: the reference is CPU-only;
  * these are what the host entry points run, exposed for callers whose data already lives in HBM) */
 

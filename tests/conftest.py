@@ -47,3 +47,23 @@ def ref64():
 def zt():
     import zap_amd
     return zap_amd.load()
+
+
+def _build_c(src: str, out: str, lang: str) -> str:
+    """Compile a C test program against include/ and libzaptex when it is missing or older than its inputs."""
+    lib = os.path.join(ROOT, "zap_amd", "_lib", "libzaptex.so")
+    deps = [src, lib, os.path.join(ROOT, "include", "zap_tex.h"), os.path.join(ROOT, "include", "zaptex.h")]
+    if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(d) for d in deps if os.path.exists(d)):
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        cc = ["gcc", "-std=c11"] if lang == "c" else ["g++", "-std=c++17", "-x", "c++"]
+        subprocess.run([*cc, "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"), src, "-L", os.path.dirname(lib),
+                        "-lzaptex", "-Wl,-rpath," + os.path.dirname(lib), "-Wl,-rpath-link,/opt/rocm/lib", "-o", out], check=True)
+    return out
+
+
+@pytest.fixture(scope="session")
+def dropin_caller():
+    """tests/dropin_caller.c built as C11 and as C++17 against the drop-in zap_tex.h: (c binary, c++ binary)."""
+    src = os.path.join(ROOT, "tests", "dropin_caller.c")
+    return (_build_c(src, os.path.join(ROOT, "tests", "_bin", "dropin_caller"), "c"),
+            _build_c(src, os.path.join(ROOT, "tests", "_bin", "dropin_caller_cxx"), "c++"))

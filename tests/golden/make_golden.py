@@ -3,7 +3,7 @@
 Runs the compiled reference (oracle/_ref, built from zap_tex.h with -std=c11 -O2 -ffp-contract=off) in this
 container only; the GPU tests compare against the committed hashes and never need the reference at run time.
 Usage: python tests/golden/make_golden.py   (about a minute on 8 cores)
-       python tests/golden/make_golden.py --only near_singular   (rewrites that fixture alone)
+       python tests/golden/make_golden.py --only near_singular containers   (rewrites those fixtures alone)
 """
 from __future__ import annotations
 
@@ -89,14 +89,54 @@ def near_singular(ref, n_max: int = 192) -> dict:
             "bc3": ref.bc_encode(img, "bc3").tobytes().hex(), "astc": ref.astc_encode(img).tobytes().hex()}
 
 
+# `zap tex` runs: (name, image key, CLI arguments before "w h in out")
+CONTAINER_RUNS = [
+    ("c1_bc1", "c1_1024", ["-f", "bc1"]), ("c1_bc1_mips_srgb", "c1_1024", ["-f", "bc1", "-m", "-S"]),
+    ("c1_bc3_mips", "c1_1024", ["-f", "bc3", "-m"]), ("c1_bc4_mips_srgb", "c1_1024", ["-f", "bc4", "-m", "-S"]),
+    ("c1_bc5_mips", "c1_1024", ["-f", "bc5", "-m"]), ("c1_bc7", "c1_1024", ["-f", "bc7"]),
+    ("c2_bc7_mips", "c2_4096", ["-f", "bc7", "-m"]), ("c2_bc7_mips_srgb", "c2_4096", ["-f", "bc7", "-m", "-S"]),
+    ("c3_astc", "c3_astc_2048", ["-f", "astc"]), ("c3_astc_mips", "c3_astc_2048", ["-f", "astc", "-m"]),
+    ("odd_bc3_mips_srgb", "small_37x21", ["-f", "bc3", "-m", "-S"]), ("odd_bc7_mips", "small_5x3", ["-f", "bc7", "-m"]),
+    ("odd_bc1_mips", "small_1x1", ["-f", "bc1", "-m"]),
+]
+
+
+def container_images() -> dict:
+    return {"c1_1024": synth.c1_bc1(1024, 1), "c2_4096": synth.c2_bc7(4096, 2), "c3_astc_2048": synth.c3_astc(2048, 3),
+            "small_37x21": small_image(37, 21, 40 + 37 * 7 + 21), "small_5x3": small_image(5, 3, 40 + 5 * 7 + 3),
+            "small_1x1": small_image(1, 1, 40 + 1 * 7 + 1)}
+
+
+def containers() -> dict:
+    """SHA-256, size and header of the files the reference CLI (oracle/_ref/zapcli, built from zap.c) writes."""
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "oracle", "_ref", "zapcli")
+    imgs = container_images()
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for key, img in imgs.items():
+            img.tofile(os.path.join(tmp, key + ".rgba"))
+        for name, key, args in CONTAINER_RUNS:
+            h, w = imgs[key].shape[:2]
+            dst = os.path.join(tmp, name + (".astc" if "astc" in args else ".dds"))
+            subprocess.run([cli, "tex", *args, str(w), str(h), os.path.join(tmp, key + ".rgba"), dst], check=True, capture_output=True)
+            data = open(dst, "rb").read()
+            hn = 16 if "astc" in args else 148 if data[84:88] == b"DX10" else 128
+            out[name] = {"image": key, "args": args, "bytes": len(data), "sha256": hashlib.sha256(data).hexdigest(),
+                         "header": data[:hn].hex(), "payload_sha256": hashlib.sha256(data[hn:]).hexdigest()}
+    return out
+
+
 def main():
-    if sys.argv[1:] == ["--only", "near_singular"]:
+    if sys.argv[1:2] == ["--only"]:
         with open(OUT) as fh:
             g = json.load(fh)
-        g["near_singular"] = near_singular(loader.reference())
+        for part in sys.argv[2:]:
+            g[part] = {"near_singular": lambda: near_singular(loader.reference()), "containers": containers}[part]()
+            print(f"{part}: rewritten")
         with open(OUT, "w") as fh:
             json.dump(g, fh, indent=1, sort_keys=True)
-        print(f"near_singular: {g['near_singular']['blocks']} blocks")
         return
     ref = loader.reference()
     ref64 = loader.reference(m1k64=True)
@@ -156,6 +196,7 @@ def main():
         ent["mip_srgb"] = ref.mip_next(img, 1).tobytes().hex()
         g["small"][key] = ent
     g["near_singular"] = near_singular(ref)
+    g["containers"] = containers()
     with open(OUT, "w") as fh:
         json.dump(g, fh, indent=1, sort_keys=True)
     print(f"wrote {OUT} in {time.time() - t0:.1f}s")

@@ -145,6 +145,10 @@ class Library:
         L.zaptex_get_launch.restype = C.c_int
         L.zaptex_bc7_full_workspace_size.argtypes = [C.POINTER(ZapImage), i, i]
         L.zaptex_bc7_full_workspace_size.restype = C.c_size_t
+        L.zaptex_tex_file_size.argtypes = [i, i, i, i, i]
+        L.zaptex_tex_file_size.restype = C.c_size_t
+        L.zaptex_tex_file.argtypes = [vp, i, i, sz, i, f, i, i, vp, sz]
+        L.zaptex_tex_file.restype = C.c_int
         L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
         L.zaptex_workspace_size.restype = C.c_size_t
         L.zaptex_last_error.restype = C.c_char_p
@@ -222,6 +226,16 @@ class Library:
         out = np.zeros_like(planes)
         self._check(self.lib.zaptex_bc_merge(_ptr(planes), planes.size, fmt_id(fmt), _ptr(out)), "zap_bc_merge")
         return out
+
+    def tex_file(self, img: np.ndarray, fmt, mips: bool = False, srgb: bool = False, rdo: float = 0.0) -> bytes:
+        """The file `zap tex` writes (zap.c:309-355): DDS for BC1/3/4/5/7 (with mips: the whole chain), .astc for ASTC."""
+        img = np.ascontiguousarray(img)
+        h, w = img.shape[:2]
+        f = fmt_id(fmt)
+        n = int(self.lib.zaptex_tex_file_size(w, h, f, int(mips), int(srgb)))
+        out = np.zeros(max(1, n), dtype=np.uint8)
+        self._check(self.lib.zaptex_tex_file(_ptr(img), w, h, img.strides[0], f, rdo, int(mips), int(srgb), _ptr(out), n), "zaptex_tex_file")
+        return out[:n].tobytes()
 
     def set_option(self, key: int, value: int) -> None:
         self._check(self.lib.zaptex_set_option(key, value), "zaptex_set_option")

@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "_lib", "libzaptex.so")
+TOOL = os.path.join(HERE, "_lib", "zaptex-tex")   # the `zap tex` command on libzaptex (csrc/zaptex_tex.c)
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ZAPTEX_ARCH", "gfx950")
 
@@ -49,7 +51,19 @@ def build(jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_tool()
     return LIB
+
+
+def build_tool() -> str:
+    """zap_amd/_lib/zaptex-tex: plain C11 against include/zaptex.h, linked to the library beside it."""
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-Werror", "-I", INCLUDE, os.path.join(CSRC, "zaptex_tex.c"),
+           "-L", os.path.dirname(LIB), "-lzaptex", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", TOOL + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"zaptex-tex build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(TOOL + ".tmp", TOOL)
+    return TOOL
 
 
 if __name__ == "__main__":

@@ -109,8 +109,8 @@ struct DeviceState {
     hipStream_t stream = nullptr;
     void *scratch = nullptr;   // this device's address of `scratch`
     // staging for the host entry points (device buffers, and pinned host memory for rows that are not packed)
-    void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr;
-    size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0;
+    void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr, *mip = nullptr;
+    size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0, mip_cap = 0;
 };
 DeviceState g_dev[64];
 
@@ -420,9 +420,126 @@ int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, i
     return ZAPTEX_OK;
 }
 
+// ---- texture files as `zap tex` writes them (zap.c:309-355)
+void put32(uint8_t *p, uint32_t v) { for (int i = 0; i < 4; i++) p[i] = (uint8_t)(v >> (8 * i)); }
+int level_count(int w, int h, int mips) {
+    int n = 1;
+    while (mips && (w > 1 || h > 1)) { w = w > 1 ? w / 2 : 1; h = h > 1 ? h / 2 : 1; n++; }
+    return n;
+}
+// DDS file header for BC1/3/4/5/7: "DDS " + DDS_HEADER (124 B), + DDS_HEADER_DXT10 (20 B) for BC7 or an sRGB
+// format; returns its length (128 or 148).  BC4/BC5 are never sRGB (the caller clears the flag).
+size_t dds_header(uint8_t hdr[148], int w, int h, int fmt, int levels, int mips, int srgb) {
+    static const char kFourCC[5][5] = { "DXT1", "DXT5", "BC4U", "BC5U", "DX10" };
+    static const uint32_t kDxgi[5] = { 71, 77, 80, 83, 98 };   // DXGI_FORMAT_BC1/3/4/5/7_UNORM; +1 is the _SRGB one
+    const bool dx10 = fmt == ZAPTEX_BC7 || srgb;
+    memset(hdr, 0, 148);
+    memcpy(hdr, "DDS ", 4);
+    put32(hdr + 4, 124);
+    put32(hdr + 8, 0x1 | 0x2 | 0x4 | 0x1000 | 0x80000 | (mips ? 0x20000 : 0));  // CAPS HEIGHT WIDTH PIXELFORMAT LINEARSIZE (MIPMAPCOUNT)
+    put32(hdr + 12, (uint32_t)h);
+    put32(hdr + 16, (uint32_t)w);
+    put32(hdr + 20, (uint32_t)out_bytes(w, h, fmt));                              // level 0's bytes
+    put32(hdr + 28, (uint32_t)levels);
+    put32(hdr + 76, 32);                                                          // DDS_PIXELFORMAT: size, FOURCC flag
+    put32(hdr + 80, 0x4);
+    memcpy(hdr + 84, dx10 ? "DX10" : kFourCC[fmt], 4);
+    put32(hdr + 108, 0x1000 | (mips ? 0x400008 : 0));                             // TEXTURE (+ COMPLEX | MIPMAP)
+    if (!dx10) return 128;
+    put32(hdr + 128, kDxgi[fmt] + (srgb ? 1u : 0u));
+    put32(hdr + 132, 3);                                                          // TEXTURE2D
+    put32(hdr + 140, 1);                                                          // array size
+    return 148;
+}
+// .astc file header: magic, 4x4x1 block, then width, height and depth as 24-bit little-endian values
+void astc_header(uint8_t hdr[16], int w, int h) {
+    static const uint8_t kMagic[4] = { 0x13, 0xAB, 0xA1, 0x5C };
+    memcpy(hdr, kMagic, 4);
+    hdr[4] = 4; hdr[5] = 4; hdr[6] = 1;
+    for (int k = 0; k < 3; k++) { hdr[7 + k] = (uint8_t)(w >> (8 * k)); hdr[10 + k] = (uint8_t)(h >> (8 * k)); hdr[13 + k] = (uint8_t)(1 >> (8 * k)); }
+}
+size_t tex_file_bytes(int w, int h, int fmt, int mips, int srgb) {
+    if (fmt == ZAPTEX_ASTC) return 16 + out_bytes(w, h, ZAPTEX_BC7);
+    if (fmt == ZAPTEX_BC4 || fmt == ZAPTEX_BC5) srgb = 0;
+    size_t total = fmt == ZAPTEX_BC7 || srgb ? 148 : 128;
+    for (int l = 0, n = level_count(w, h, mips), lw = w, lh = h; l < n; l++, lw = lw > 1 ? lw / 2 : 1, lh = lh > 1 ? lh / 2 : 1)
+        total += out_bytes(lw, lh, fmt);
+    return total;
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t zaptex_tex_file_size(int w, int h, int fmt, int mips, int srgb) {
+    if (w < 1 || h < 1 || w > (1 << 16) || h > (1 << 16) || (!is_bcn(fmt) && fmt != ZAPTEX_ASTC)) return 0;
+    return tex_file_bytes(w, h, fmt, mips, srgb);
+}
+
+int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, int mips, int srgb, void *out, size_t out_size) {
+    if ((!is_bcn(fmt) && fmt != ZAPTEX_ASTC) || w < 1 || h < 1 || w > (1 << 16) || h > (1 << 16) || !rgba || !out || stride < (size_t)w * 4)
+        return fail(ZAPTEX_E_ARG, "bad arguments");
+    const size_t need = tex_file_bytes(w, h, fmt, mips, srgb);
+    if (out_size < need) return fail(ZAPTEX_E_ARG, "output buffer %zu < %zu bytes", out_size, need);
+    uint8_t *o = (uint8_t *)out;
+    if (fmt == ZAPTEX_ASTC) {   // one level, like the reference (its -m is ignored)
+        astc_header(o, w, h);
+        return host_encode(rgba, w, h, stride, 4, ZAPTEX_ASTC, o + 16);
+    }
+    if (rdo > 0) return fail(ZAPTEX_E_UNSUPPORTED, "rdo > 0 (rate-distortion optimisation) is not implemented by libzaptex");
+    if (fmt == ZAPTEX_BC4 || fmt == ZAPTEX_BC5) srgb = 0;   // channel data, never sRGB
+    const int levels = level_count(w, h, mips);
+    uint8_t hdr[148];   // the file can be shorter than 148 bytes (a 1x1 BC1 file is 136)
+    const size_t hn = dds_header(hdr, w, h, fmt, levels, mips, srgb);
+    memcpy(o, hdr, hn);
+    // the whole chain on the device: base in once, levels 1.. generated in HBM, every level encoded, blocks out once
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState *st;
+    if (int rc = ensure_ready(st)) return rc;
+    std::vector<int> lw(1, w), lh(1, h);
+    for (int l = 1; l < levels; l++) {
+        lw.push_back(lw.back() > 1 ? lw.back() / 2 : 1);
+        lh.push_back(lh.back() > 1 ? lh.back() / 2 : 1);
+    }
+    // device offsets, 256-byte aligned: generated level l (>= 1) at moff[l - 1], level l's blocks at ooff[l]
+    std::vector<size_t> moff, ooff;
+    size_t mip_bytes = 0, blk_bytes = 0;
+    for (int l = 0; l < levels; l++) {
+        if (l) { moff.push_back(mip_bytes); mip_bytes += align_up((size_t)lw[l] * lh[l] * 4, 256); }
+        ooff.push_back(blk_bytes);
+        blk_bytes += align_up(out_bytes(lw[l], lh[l], fmt), 256);
+    }
+    const size_t row = (size_t)w * 4;
+    if (int rc = ensure_buf(st->in, st->in_cap, row * (size_t)h)) return rc;
+    if (int rc = ensure_buf(st->out, st->out_cap, blk_bytes)) return rc;
+    if (levels > 1) if (int rc = ensure_buf(st->mip, st->mip_cap, mip_bytes)) return rc;
+    uint64_t ws_need = 0;
+    for (int l0 = 0; l0 < levels; l0 += ZT_INLINE_JOBS) {
+        uint64_t n = 0;
+        for (int l = l0; l < std::min(levels, l0 + ZT_INLINE_JOBS); l++) n += (uint64_t)((lw[l] + 3) / 4) * ((lh[l] + 3) / 4);
+        ws_need = std::max<uint64_t>(ws_need, fmt == ZAPTEX_BC7 ? ws_total(n, 0) : ws_total(0, n));
+    }
+    if (int rc = ensure_buf(st->ws, st->ws_cap, ws_need)) return rc;
+    if (int rc = rows_to_device(st, st->in, rgba, stride, row, (size_t)h)) return rc;
+    std::vector<void *> lvl;
+    for (int l = 1; l < levels; l++) lvl.push_back((uint8_t *)st->mip + moff[l - 1]);
+    if (levels > 1) if (int rc = enqueue_mip_chain(st->in, w, h, row, srgb, lvl.data(), levels - 1, st->stream)) return rc;
+    for (int l0 = 0; l0 < levels; l0 += ZT_INLINE_JOBS) {
+        zaptex_image im[ZT_INLINE_JOBS];
+        int n = 0;
+        for (int l = l0; l < std::min(levels, l0 + ZT_INLINE_JOBS); l++, n++)
+            im[n] = zaptex_image{ l ? lvl[l - 1] : st->in, (uint8_t *)st->out + ooff[l], (size_t)lw[l] * 4, lw[l], lh[l], fmt };
+        if (int rc = enqueue(im, n, st->ws, st->ws_cap, st->stream)) return rc;
+    }
+    size_t pos = hn;
+    for (int l = 0; l < levels; l++) {
+        const size_t b = out_bytes(lw[l], lh[l], fmt);
+        HIP_TRY(hipMemcpyAsync(o + pos, (uint8_t *)st->out + ooff[l], b, hipMemcpyDeviceToHost, st->stream));
+        pos += b;
+    }
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    return ZAPTEX_OK;
+}
 
 int zaptex_bc_encode(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, void *out) {
     if (!is_bcn(fmt)) return fail(ZAPTEX_E_ARG, "unknown zap_bc_format %d", fmt);
