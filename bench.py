@@ -7,9 +7,12 @@ already in HBM.  value = 22,369,621 source pixels per step / step time, in Mpix/
 checked against the reference's golden hashes before timing.
 
 Also reported, measured inside this run:
-  roofline      the dominant kernel (BC7 mode-6 stage) against the VALU lane-op peak: algorithmic ops per
-                launch (tests/golden/ops.json, counted by the oracle on this exact input) / its average
-                duration from HIP events on its launch stream; the mip chain's HBM GB/s is given beside it
+  roofline      the dominant kernel (BC7 mode-6 stage) against the VALU issue ceiling measured in this run: its
+                VALU wave instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU on this build and workload) over its
+                average duration from HIP events on its launch stream, set against the fastest issue rate of an
+                inline-asm probe (libztprobe.so); traffic = FETCH_SIZE + WRITE_SIZE per launch, each scaled by what it
+                reads for a plain copy of known size; the oracle's scalar-op rate is given beside it as work, and the
+                mip chain's GB/s against the HBM peak and the plain-copy rate measured in this run
   cpu_baseline  the reference encoder (oracle/_ref, built from zap_tex.h) on this host's cores, banded into
                 8-block rows like the reference viewer, over the same 13-level chain
   batch         the C4 workload: 512 seeded textures (sides 64..4096 incl. non-powers of two; 30% BC1, 15% BC3,
@@ -36,16 +39,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes x 2.4 GHz: one scalar op per lane per clock
-PEAK_WAVE_INSTS = 256 * 2.4e9                # one wave64 VALU instruction per CU per clock (4 SIMD16 x 4 cycles)
+N_CU, N_XCD = 256, 8                         # MI355X: GRBM_GUI_ACTIVE sums the 8 XCDs' busy cycles
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0                    # dense bf16 MFMA
 MIP_CHAIN_BYTES = 89_478_484                 # read the 4096^2 base once + write every generated level once
-# zt_bc7_stage6 on this workload, rocprofv3 --pmc per dispatch (profiles/zt_bc7_stage6_c2_chain.md):
-# FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the counter reads for a plain 256 MiB device copy
-# (FETCH 131,087 KiB -> x2.0, WRITE 262,144 KiB -> x1.0); SQ_INSTS_VALU in wave64 instructions.
-STAGE6_COUNTERS = {"fetch_kib": 43_973.4, "fetch_scale": 2.0, "write_kib": 47_247.0, "write_scale": 1.0,
-                   "sq_insts_valu": 136_058_480}
 
 
 def log(*a):
@@ -97,6 +94,135 @@ def cpu_baseline(levels_np, threads: int, repeats: int = 5, configs: dict | None
                     ts.append(time.perf_counter() - t0)
             res["configs_mpix_s"][name] = round(img.shape[0] * img.shape[1] / statistics.median(ts) / 1e6, 2)
     return res
+
+
+VALU_PROBE_KINDS = ("v_add_u32", "v_dot4_u32_u8", "v_fma_f32", "v_max_i32", "v_lshl_add_u32", "v_perm_b32")
+COPY_PROBE_BYTES = 256 << 20
+
+
+def _probe_lib():
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "zap_amd", "_lib", "libztprobe.so"))
+    lib.ztprobe_valu.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    lib.ztprobe_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+    return lib
+
+
+def copy_probe(steps: int = 10) -> dict:
+    """A plain 256 MiB device copy (zt_copy_probe): its rate, 2 x bytes / time, is the practical HBM bar."""
+    import torch
+    lib = _probe_lib()
+    a = torch.empty(COPY_PROBE_BYTES, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    st = torch.cuda.current_stream().cuda_stream
+    assert lib.ztprobe_copy(a.data_ptr(), b.data_ptr(), COPY_PROBE_BYTES, st) == 0
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        lib.ztprobe_copy(a.data_ptr(), b.data_ptr(), COPY_PROBE_BYTES, st)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"bytes": COPY_PROBE_BYTES, "ms": round(ms, 4), "gb_s": round(2 * COPY_PROBE_BYTES / (ms / 1e3) / 1e9, 1)}
+
+
+def valu_probe(groups: int = 256 * 8, iters: int = 4096) -> dict:
+    """The chip's VALU issue ceiling, measured: independent inline-asm chains of one instruction (32 wave64
+    instructions per wave and round), 8 workgroups of 4 waves per CU, timed with HIP events on the launch stream.
+    Returns wave instructions per second per kind and the fastest."""
+    import torch
+    lib = _probe_lib()
+    out = torch.empty(groups * 256, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    insts = groups * 4 * iters * 32
+    res = {"groups": groups, "waves": groups * 4, "iters": iters, "wave_insts_per_launch": insts}
+    for kind, name in enumerate(VALU_PROBE_KINDS):
+        assert lib.ztprobe_valu(kind, groups, iters, out.data_ptr(), stream.cuda_stream) == 0
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            lib.ztprobe_valu(kind, groups, iters, out.data_ptr(), stream.cuda_stream)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        res[name] = {"ms": round(ms, 4), "wave_insts_per_s": round(insts / (ms / 1e3) / 1e9, 2)}   # G wave instructions / s
+        log(f"valu probe {name}: {ms:.3f} ms, {res[name]['wave_insts_per_s']} G wave-insts/s")
+    res["ceiling_g_wave_insts_per_s"] = max(res[n]["wave_insts_per_s"] for n in VALU_PROBE_KINDS)
+    return res
+
+
+COUNTER_PASSES = (("SQ_INSTS_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE", "SQ_BUSY_CYCLES"), ("FETCH_SIZE",), ("WRITE_SIZE",))
+
+
+def counters_child(c2_path: str) -> int:
+    """The program the counter passes profile: one plain copy of COPY_PROBE_BYTES (calibrates the byte counters),
+    then two steps of the workload (mip chain + BC7 encode of the 4096^2 chain) with the bench's launch shapes."""
+    import numpy as np
+    import torch
+
+    import zap_amd
+    from zap_amd import autotune, binding
+    lib = zap_amd.load()
+    autotune.apply(lib, "c2")
+    plib = _probe_lib()
+    a = torch.empty(COPY_PROBE_BYTES, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    plib.ztprobe_copy(a.data_ptr(), b.data_ptr(), COPY_PROBE_BYTES, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    del a, b
+    c2 = np.load(c2_path)
+    H, W = c2.shape[:2]
+    sizes = binding.mip_sizes(W, H)
+    base = torch.from_numpy(c2).cuda()
+    levels = [torch.empty((h, w, 4), dtype=torch.uint8, device="cuda") for (w, h) in sizes[1:]]
+    outs = [torch.empty(binding.bc_size(w, h, "bc7"), dtype=torch.uint8, device="cuda") for (w, h) in sizes]
+    imgs = lib.images(((base if i == 0 else levels[i - 1]), outs[i], w, h, w * 4, "bc7") for i, (w, h) in enumerate(sizes))
+    ws = torch.empty(lib.workspace_size(imgs), dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        lib.mip_chain_dev(base, W, H, W * 4, levels, srgb=0)
+        lib.encode_dev(imgs, ws)
+    torch.cuda.synchronize()
+    log("counters child: done")
+    return 0
+
+
+def rocprof_counters(c2_path: str, timeout: int = 180) -> dict:
+    """Per-dispatch means of the COUNTER_PASSES counters for every kernel of counters_child, one rocprofv3 --pmc
+    run per pass (a pass holds what one run can collect); {"error": ...} when the profiler is missing or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return {"error": "rocprofv3 not found"}
+    out: dict = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for k, counters in enumerate(COUNTER_PASSES):
+            d = os.path.join(tmp, f"p{k}")
+            cmd = [exe, "--pmc", *counters, "-d", d, "-o", "c", "--output-format", "csv", "--",
+                   sys.executable, os.path.abspath(__file__), "--counters-child", c2_path]
+            log(f"counters: pass {k + 1}/{len(COUNTER_PASSES)} ({' '.join(counters)})")
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return {"error": f"rocprofv3 pass {counters} timed out"}
+            files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+            if r.returncode != 0 and not files:
+                return {"error": f"rocprofv3 pass {counters} failed (rc {r.returncode}): {r.stderr[-300:]}"}
+            per = {}
+            for row in csv.DictReader(open(files[0])):
+                name = row["Kernel_Name"].split("(")[0].replace("void ", "")
+                key = (name, row["Dispatch_Id"])
+                per.setdefault(key, {})[row["Counter_Name"]] = float(row["Counter_Value"])
+                per[key]["_ns"] = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+            for (name, _), vals in per.items():
+                agg = out.setdefault(name, {})
+                for c, v in vals.items():
+                    agg.setdefault(c, []).append(v)
+    return {name: {c: sum(v) / len(v) for c, v in cs.items()} for name, cs in out.items()}
 
 
 def learned_leg(lib, c2, steps: int = 5) -> dict:
@@ -268,17 +394,20 @@ def main() -> int:
     ap.add_argument("--no-host", action="store_true", help="skip the host-entry (PCIe-inclusive) leg")
     ap.add_argument("--batch-only", action="store_true", help="only build and time the 512-texture batch (for profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--counters-child", metavar="C2_NPY", help=argparse.SUPPRESS)
+    ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes (traffic, VALU instructions)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU issue probe (zap_amd/_lib/libztprobe.so) and print its rates")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the profiler's HBM byte counters)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy probe: 3 x 268435456 B device copies")
+        print(json.dumps(copy_probe(3)), flush=True)
+        return 0
+    if args.counters_child:
+        return counters_child(args.counters_child)
+    if args.valu_probe:
+        print(json.dumps({"valu": valu_probe(), "copy": copy_probe()}), flush=True)
         return 0
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs the hot path on a single MI355X (no multi-GPU path); use --gpus 1")
@@ -359,15 +488,59 @@ def main() -> int:
               "stage1": group_ms(["zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_stage1"]),
               "mips": group_ms(["zt_mip_tiles", "zt_mip_tail", "zt_mip_level"])}
     s6 = kern["zt_bc7_stage6"]["ms_per_launch"]
-    achieved = ops["ops_stage6"] / (s6 / 1e3) / 1e12
     mip_gbs = MIP_CHAIN_BYTES / (stages["mips"] / 1e3) / 1e9
     stage_tops = {k: round(ops[f"ops_{k}"] / (v / 1e3) / 1e12, 3) for k, v in stages.items() if k != "mips" and v > 0}
     log(json.dumps(kern))
     enc_ms = stages["stage6"] + stages["stage5"] + stages["stage1"] + group_ms(["zt_zero_u32"])
     enc_tops = ops["ops_total"] / (enc_ms / 1e3) / 1e12
-    c = STAGE6_COUNTERS
-    traffic = int((c["fetch_kib"] * c["fetch_scale"] + c["write_kib"] * c["write_scale"]) * 1024)
-    valu_issue = c["sq_insts_valu"] / (s6 / 1e3) / PEAK_WAVE_INSTS
+
+    # ceilings measured on this box now: the fastest VALU issue rate of the probe, the plain-copy bandwidth
+    probe = valu_probe()
+    copy = copy_probe()
+    ceiling = probe["ceiling_g_wave_insts_per_s"]
+    # stage 6's VALU instructions and HBM bytes per launch: rocprofv3 counter passes over this build and workload
+    if args.no_counters:
+        counters = {"error": "skipped (--no-counters)"}
+    else:
+        import tempfile
+        fd, c2_path = tempfile.mkstemp(suffix=".npy")
+        os.close(fd)
+        np.save(c2_path, c2)
+        try:
+            counters = rocprof_counters(c2_path)
+        finally:
+            os.unlink(c2_path)
+    s6c, cpc = counters.get("zt_bc7_stage6"), counters.get("zt_copy_probe")
+    roof = {"bound": "valu", "kernel": "zt_bc7_stage6", "unit": "G VALU wave-instructions/s", "peak": ceiling,
+            "peak_is": "the fastest issue rate of the VALU probe (zt_valu_probe, inline-asm chains) measured in this run",
+            "ms_per_launch": s6, "achieved": None, "frac": None, "traffic": None}
+    if s6c and cpc and "SQ_INSTS_VALU" in s6c:
+        insts = s6c["SQ_INSTS_VALU"]
+        achieved = insts / (s6 / 1e3) / 1e9
+        scale_f = COPY_PROBE_BYTES / (cpc["FETCH_SIZE"] * 1024)
+        scale_w = COPY_PROBE_BYTES / (cpc["WRITE_SIZE"] * 1024)
+        cycles = s6c["GRBM_GUI_ACTIVE"] / N_XCD
+        roof.update({"achieved": round(achieved, 2), "frac": round(achieved / ceiling, 4),
+                     "traffic": int(s6c["FETCH_SIZE"] * 1024 * scale_f + s6c["WRITE_SIZE"] * 1024 * scale_w),
+                     "sq_insts_valu": int(insts), "sq_waves": int(s6c["SQ_WAVES"]),
+                     "fetch_kib": round(s6c["FETCH_SIZE"], 1), "write_kib": round(s6c["WRITE_SIZE"], 1),
+                     "byte_scale": {"fetch": round(scale_f, 4), "write": round(scale_w, 4), "copy_bytes": COPY_PROBE_BYTES},
+                     "clock_ghz_profiled": round(cycles / (s6c["_ns"] / 1e9) / 1e9, 3),
+                     "issue_per_cu_clock": round(insts / N_CU / cycles, 4),
+                     "counters_source": "rocprofv3 --pmc passes in this run: " + "; ".join(" ".join(p) for p in COUNTER_PASSES)})
+    else:
+        roof["counters_source"] = f"unavailable: {counters.get('error', 'stage-6 or copy counters missing')}"
+    roof.update({"valu_probe": probe,
+                 "algorithmic": {"ops_per_launch": ops["ops_stage6"], "tops": round(ops["ops_stage6"] / (s6 / 1e3) / 1e12, 3),
+                                 "is": "the oracle's scalar ops (tests/golden/ops.json) per second: a work rate, not a utilisation "
+                                       "(the kernel does four multiply-adds in one v_dot4_u32_u8)",
+                                 "stage_tops": stage_tops, "encoder_tops": round(enc_tops, 3), "encoder_ms": round(enc_ms, 5)},
+                 "mips": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "practical_peak": copy["gb_s"],
+                          "frac_of_practical": round(mip_gbs / copy["gb_s"], 4), "bytes_per_chain": MIP_CHAIN_BYTES,
+                          "ms": round(stages["mips"], 5)}})
+    log(f"roofline: stage 6 {roof['achieved']} of {ceiling} G wave-insts/s (frac {roof['frac']}), traffic {roof['traffic']}; "
+        f"mips {mip_gbs:.0f} GB/s of {copy['gb_s']} copy")
 
     # end to end through the drop-in's host entry (zap_bc_encode: host buffers in, PCIe both ways), per level
     levels_np = [c2] + [l.cpu().numpy() for l in levels]
@@ -399,15 +572,7 @@ def main() -> int:
                    "value_is": "the plain search (the reference's mode 6/5/1 search, byte-exact); learned-proposal figures in 'learned'",
                    "global_batch": 1, "seq_len": None, "parallelism": "single GPU", "bc7_m1k": lib.get_option(1),
                    "byte_exact_vs_reference": exact},
-        "roofline": {"bound": "valu", "kernel": "zt_bc7_stage6", "achieved": round(achieved, 3), "peak": round(PEAK_VALU_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(achieved / PEAK_VALU_TOPS, 4), "traffic": traffic,
-                     "ops_per_launch": ops["ops_stage6"], "ms_per_launch": s6,
-                     "valu_issue_frac": round(valu_issue, 4), "stage_tops": stage_tops,
-                     "encoder": {"ops": ops["ops_total"], "ms": round(enc_ms, 5), "achieved": round(enc_tops, 3),
-                                 "frac": round(enc_tops / PEAK_VALU_TOPS, 4)},
-                     "mips": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                              "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "bytes_per_chain": MIP_CHAIN_BYTES,
-                              "ms": round(stages["mips"], 5)}},
+        "roofline": roof,
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
         "host_entry_mpix_s": e2e_mpix,
         "launch_shapes": shapes_c2,

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "_lib", "libzaptex.so")
+PROBE = os.path.join(HERE, "_lib", "libztprobe.so")   # measurement kernels for the bench's roofline denominators
 TOOL = os.path.join(HERE, "_lib", "zaptex-tex")   # the `zap tex` command on libzaptex (csrc/zaptex_tex.c)
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -52,7 +53,17 @@ def build(jobs: int | None = None) -> str:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_tool()
+    build_probe()
     return LIB
+
+
+def build_probe() -> str:
+    cmd = [HIPCC, *FLAGS, "-shared", os.path.join(CSRC, "zt_probe.hip"), "-o", PROBE + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libztprobe build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(PROBE + ".tmp", PROBE)
+    return PROBE
 
 
 def build_tool() -> str:
