@@ -157,6 +157,31 @@ def test_decoders_match_oracle(zt, oracle):
     img = synth.c3_astc(64, 9)
     bc = zt.astc_encode(img)
     assert np.array_equal(zt.astc_decode(bc, 64, 64), oracle.astc_decode(bc, 64, 64))
+    # every BC7 mode: 256 random blocks each, mode bits forced (random bytes reach modes 6 and 7 only ~1 in 100)
+    blk = rng.integers(0, 256, (8, 256, 16), dtype=np.uint8)
+    for m in range(8):
+        blk[m, :, 0] = (blk[m, :, 0] & np.uint8(0xFF << (m + 1) & 0xFF)) | np.uint8(1 << m)
+    raw7 = np.ascontiguousarray(blk.reshape(-1))
+    assert np.array_equal(zt.bc_decode(raw7, 4 * 64, 4 * 32, "bc7"), oracle.bc_decode(raw7, 4 * 64, 4 * 32, "bc7"))
+    # ASTC blocks in the decoder's subset: 4x4 single-plane modes (bits 1:0 != 0, 3:2 = 0, 6:5 = 2, 8:7 = 0, bit 10 = 0),
+    # one or two partitions, endpoint modes 8 / 12 (some mixed, which decode to magenta), the rest random; about a
+    # third leave room for their colour values (the rest decode to magenta too)
+    n = 16384
+    a = rng.integers(0, 256, (n, 16), dtype=np.uint8)
+    lo = a[:, :8].copy().view(np.uint64)[:, 0]
+    modes = np.array([m for m in range(2048) if m & 3 and not (m >> 2) & 3 and (m >> 5) & 3 == 2 and not (m >> 7) & 3 and not (m >> 10) & 1],
+                     np.uint64)
+    mode = modes[rng.integers(0, len(modes), n)]
+    parts = rng.integers(0, 2, n).astype(np.uint64)
+    cem = np.where(rng.integers(0, 2, n) == 1, 12, 8).astype(np.uint64)
+    lo = (lo & ~np.uint64(0x1FFF)) | mode | parts << np.uint64(11)              # mode, partition count
+    one, two = parts == 0, parts == 1
+    lo[one] = (lo[one] & ~np.uint64(0xF << 13)) | cem[one] << np.uint64(13)       # endpoint mode
+    cf = (cem[two] << np.uint64(2)) | (rng.integers(0, 8, int(two.sum())) == 0).astype(np.uint64)
+    lo[two] = (lo[two] & ~np.uint64(0x3F << 23)) | cf << np.uint64(23)           # seed (bits 13..22) stays random
+    a[:, :8] = lo.view(np.uint8).reshape(n, 8)
+    rawa = np.ascontiguousarray(a.reshape(-1))
+    assert np.array_equal(zt.astc_decode(rawa, 4 * 128, 4 * 128), oracle.astc_decode(rawa, 4 * 128, 4 * 128))
 
 
 def test_split_merge_match_oracle(zt, oracle):

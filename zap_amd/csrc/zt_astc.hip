@@ -80,7 +80,7 @@ template <int K> ZT_DEV int fit_weights(const uint32_t px[16], int spp, const in
     int best[16];
 #pragma unroll
     for (int j = 0; j < L; j++) {
-        const int wt = zt_weight_unquant(j, WB);
+        const int wt = zt_weight_value(j, WB);
         uint32_t pj[P];
         int kj[P];
 #pragma unroll
@@ -146,7 +146,7 @@ template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 if (!((m >> i) & 1)) continue;
-                float f = (float)zt_weight_unquant(wbyte(w, i), WB) / 64.0f, g = 1 - f;
+                float f = (float)zt_weight_value(wbyte(w, i), WB) / 64.0f, g = 1 - f;
                 aa += g * g; ab += g * f; bb += f * f;
 #pragma unroll
                 for (int c = 0; c < NC; c++) { xa[c] += g * chanf(px[i], c); xb[c] += f * chanf(px[i], c); }
@@ -251,70 +251,74 @@ ZT_DEV uint32_t two_means(const uint32_t px[16]) {  // zap_tex.h:1263-1283
     return m;
 }
 
-// ---- decoder (zap_tex.h:1135-1168): the encoder's subset plus void-extent blocks; anything else is magenta
+// ---- decoder (zap_tex.h:1135-1168): void-extent blocks and the 4x4 single-plane modes with plain-bit weights and
+// colour endpoint modes 8 / 12 (one or two partitions); anything else decodes to magenta.  Every lookup is a set-up
+// table: the block mode's weight bits, the trit/quint digits of a transfer block, each range's unquantised colour
+// values, the weight values, the 2-partition pattern of the seed.
+
+// n values of ISE range r starting at block bit `pos`: each group of 5 (trits) or 3 (quints) values interleaves
+// its plain fields with the transfer bits of one trit/quint block, 2,2,1,2,1 or 3,2,2 bits after the fields
+ZT_DEV void ise_read(uint64_t lo, uint64_t hi, int pos, const zt_ise_range &r, int n, int v[16]) {
+    const int gs = r.trit ? 5 : r.quint ? 3 : 1;
+    const uint32_t xbits = r.trit ? 0x12122u : 0x223u;   // transfer bits after field j: nibble j
+    for (int g = 0; g < n; g += gs) {
+        const int cnt = min(gs, n - g);
+        int X = 0, xp = 0;
+        for (int j = 0; j < cnt; j++) {
+            v[g + j] = (int)get_bits(lo, hi, pos, r.bits);
+            pos += r.bits;
+            if (gs > 1) {
+                const int xb = (int)(xbits >> (4 * j) & 15u);
+                X |= (int)get_bits(lo, hi, pos, xb) << xp;
+                pos += xb; xp += xb;
+            }
+        }
+        if (gs == 1) continue;
+        const uint32_t dig = r.trit ? g_astc.tdec[X] : g_astc.qdec[X];
+        for (int j = 0; j < cnt; j++) v[g + j] |= (int)(r.trit ? dig >> (2 * j) & 3u : dig >> (3 * j) & 7u) << r.bits;
+    }
+}
+
 ZT_DEV void decode_block(const uint8_t *blk, uint32_t out[16]) {
-    BitReader R = reader_at(blk);
-    const int mode = R.take(11);
+    const uint4 raw = *(const uint4 *)blk;
+    const uint64_t lo = (uint64_t)raw.x | (uint64_t)raw.y << 32, hi = (uint64_t)raw.z | (uint64_t)raw.w << 32;
     for (int i = 0; i < 16; i++) out[i] = 0xFFFF00FFu;
-    if ((mode & 0x1FF) == 0x1FC) {
+    const int mode = (int)(lo & 0x7FFu);
+    if ((mode & 0x1FF) == 0x1FC) {   // void extent: the UNORM16 colour at bits 64..127, high byte per channel
         uint32_t v = 0;
-        for (int c = 0; c < 4; c++) v |= (uint32_t)((int)get_bits(R.lo, R.hi, 64 + 16 * c, 16) >> 8) << (8 * c);
+        for (int c = 0; c < 4; c++) v |= (uint32_t)(hi >> (16 * c + 8) & 0xFFu) << (8 * c);
         for (int i = 0; i < 16; i++) out[i] = v;
         return;
     }
-    const int Rr = (mode & 1) << 1 | (mode >> 1 & 1) << 2 | (mode >> 4 & 1), H = mode >> 9 & 1;
-    if (!(mode & 3) || (mode >> 2 & 3) || (mode >> 5 & 3) != 2 || (mode >> 7 & 3) || (mode >> 10 & 1)) return;
-    const int WL0[8] = { 0, 0, 2, 3, 4, 5, 6, 8 }, WL1[8] = { 0, 0, 10, 12, 16, 20, 24, 32 };
-    const int levels = H ? WL1[Rr] : WL0[Rr];
-    const int wb = levels == 2 ? 1 : levels == 4 ? 2 : levels == 8 ? 3 : levels == 16 ? 4 : levels == 32 ? 5 : 0;
-    const int parts = R.take(2) + 1;
-    int seed = 0, cem;
+    const int wb = g_astc.mode_wb[mode], parts = (int)(lo >> 11 & 3u) + 1;
     if (!wb || parts > 2) return;
+    int seed = 0, cem, pos;
     if (parts == 2) {
-        seed = R.take(10);
-        int cf = R.take(6);
-        if (cf & 3) return;
-        cem = cf >> 2;
-    } else cem = R.take(4);
+        const int cf = (int)(lo >> 23 & 0x3Fu);
+        if (cf & 3) return;   // mixed endpoint modes are outside the subset
+        seed = (int)(lo >> 13 & 0x3FFu); cem = cf >> 2; pos = 29;
+    } else {
+        cem = (int)(lo >> 13 & 15u); pos = 17;
+    }
     const int nv = cem == 8 ? 6 : cem == 12 ? 8 : 0;
     if (!nv) return;
-    const int ri = zt_colour_range(nv * parts, 128 - R.pos - 16 * wb);
+    const int ri = zt_colour_range(nv * parts, 128 - pos - 16 * wb);
     if (ri < 0) return;
-    const zt_ise_range r = zt_ise(ri);
     int v[16];
-    const int gs = r.trit ? 5 : r.quint ? 3 : 1, n = nv * parts;
-    for (int g = 0; g < n; g += gs) {
-        int cnt = n - g < gs ? n - g : gs, m[5], X = 0, xp = 0, t[5];
-        for (int k = 0; k < cnt; k++) {
-            m[k] = R.take(r.bits);
-            if (gs > 1) {
-                int xb = r.trit ? (k == 0 || k == 1 || k == 3 ? 2 : 1) : (k == 0 ? 3 : 2);
-                X |= R.take(xb) << xp; xp += xb;
-            }
-        }
-        if (r.trit) zt_trits(X, t);
-        else if (r.quint) zt_quints(X, t);
-        for (int k = 0; k < cnt; k++) v[g + k] = gs > 1 ? t[k] << r.bits | m[k] : m[k];
-    }
-    for (int i = 0; i < n; i++) v[i] = zt_colour_unquant(r, v[i]);
-    int e[2][2][4];
+    ise_read(lo, hi, pos, zt_ise(ri), nv * parts, v);
+    for (int i = 0; i < nv * parts; i++) v[i] = g_astc.colour[ri][v[i]];
+    int e0[2][4], e1[2][4];
     for (int k = 0; k < parts; k++) {
-        const int *q = v + k * nv;
-        int a0 = cem == 12 ? q[6] : 255, a1 = cem == 12 ? q[7] : 255;
-        if (q[1] + q[3] + q[5] >= q[0] + q[2] + q[4]) {
-            e[k][0][0] = q[0]; e[k][0][1] = q[2]; e[k][0][2] = q[4]; e[k][0][3] = a0;
-            e[k][1][0] = q[1]; e[k][1][1] = q[3]; e[k][1][2] = q[5]; e[k][1][3] = a1;
-        } else {
-            e[k][0][0] = (q[1] + q[5]) >> 1; e[k][0][1] = (q[3] + q[5]) >> 1; e[k][0][2] = q[5]; e[k][0][3] = a1;
-            e[k][1][0] = (q[0] + q[4]) >> 1; e[k][1][1] = (q[2] + q[4]) >> 1; e[k][1][2] = q[4]; e[k][1][3] = a0;
-        }
+        if (cem == 8) endpoints<8>(v + k * nv, e0[k], e1[k]);
+        else endpoints<12>(v + k * nv, e0[k], e1[k]);
     }
+    // weights fill the block from bit 127 down: reversed, they read forward from bit 0
+    const uint64_t wlo = __brevll(hi), whi = __brevll(lo);
+    const uint32_t pm = parts == 2 ? g_astc.part2[seed] : 0u;
     for (int i = 0; i < 16; i++) {
-        int w = 0, k = parts == 2 ? zt_astc_partition(seed, i & 3, i >> 2, 2) : 0;
-        for (int j = 0; j < wb; j++) { int bit = 127 - (i * wb + j); w |= (int)get_bits(R.lo, R.hi, bit, 1) << j; }
-        w = zt_weight_unquant(w, wb);
+        const int k = (int)(pm >> i & 1u), w = g_astc.weight[wb][get_bits(wlo, whi, i * wb, wb)];
         uint32_t o = 0;
-        for (int c = 0; c < 4; c++) o |= (uint32_t)lerp(e[k][0][c], e[k][1][c], w) << (8 * c);
+        for (int c = 0; c < 4; c++) o |= (uint32_t)lerp(e0[k][c], e1[k][c], w) << (8 * c);
         out[i] = o;
     }
 }

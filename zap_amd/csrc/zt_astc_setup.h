@@ -1,7 +1,7 @@
-// zt_astc_setup.h — ASTC 4x4 LDR tables and spec functions shared by the host table builder and the
-// gfx950 kernels (zap_tex.h:977-1305).  The encoder's per-call setup in the reference (ISE packing tables,
-// 8 trial configurations with their 8-bit -> ISE quantisation maps, the 822 usable 2-partition patterns) is
-// built once per device here and kept in device memory.
+// zt_astc_setup.h — ASTC 4x4 LDR tables shared by the host table builder and the gfx950 kernels
+// (zap_tex.h:977-1305).  Everything the encoder and decoder look up is built once per device from the ASTC
+// specification's own data tables (integer-sequence ranges, colour-unquantisation patterns, trit/quint codings,
+// block-mode layout) and kept in device memory: the kernels decode by table lookup, never by per-value bit logic.
 #pragma once
 #include <stdint.h>
 
@@ -11,73 +11,115 @@
 #define ZT_HD inline
 #endif
 
-// integer-sequence-encoding ranges, ASTC specification order: levels, trits, quints, bits
+// ---- integer-sequence-encoding ranges, specification order (range index 0..20)
 struct zt_ise_range { int levels, trit, quint, bits; };
+// (levels, trit?, quint?, plain bits) per range index; every range is 2^bits, 3 * 2^bits or 5 * 2^bits levels
+#define ZT_ISE_TABLE                                                                                                         \
+    { { 2, 0, 0, 1 }, { 3, 1, 0, 0 }, { 4, 0, 0, 2 }, { 5, 0, 1, 0 }, { 6, 1, 0, 1 }, { 8, 0, 0, 3 }, { 10, 0, 1, 1 },     \
+      { 12, 1, 0, 2 }, { 16, 0, 0, 4 }, { 20, 0, 1, 2 }, { 24, 1, 0, 3 }, { 32, 0, 0, 5 }, { 40, 0, 1, 3 }, { 48, 1, 0, 4 }, \
+      { 64, 0, 0, 6 }, { 80, 0, 1, 4 }, { 96, 1, 0, 5 }, { 128, 0, 0, 7 }, { 160, 0, 1, 5 }, { 192, 1, 0, 6 }, { 256, 0, 0, 8 } }
 ZT_HD zt_ise_range zt_ise(int i) {
-    switch (i) {
-    case 0: return { 2, 0, 0, 1 };    case 1: return { 3, 1, 0, 0 };    case 2: return { 4, 0, 0, 2 };
-    case 3: return { 5, 0, 1, 0 };    case 4: return { 6, 1, 0, 1 };    case 5: return { 8, 0, 0, 3 };
-    case 6: return { 10, 0, 1, 1 };   case 7: return { 12, 1, 0, 2 };   case 8: return { 16, 0, 0, 4 };
-    case 9: return { 20, 0, 1, 2 };   case 10: return { 24, 1, 0, 3 };  case 11: return { 32, 0, 0, 5 };
-    case 12: return { 40, 0, 1, 3 };  case 13: return { 48, 1, 0, 4 };  case 14: return { 64, 0, 0, 6 };
-    case 15: return { 80, 0, 1, 4 };  case 16: return { 96, 1, 0, 5 };  case 17: return { 128, 0, 0, 7 };
-    case 18: return { 160, 0, 1, 5 }; case 19: return { 192, 1, 0, 6 }; default: return { 256, 0, 0, 8 };
-    }
+    const zt_ise_range t[21] = ZT_ISE_TABLE;
+    return t[i < 0 ? 0 : i > 20 ? 20 : i];
 }
-ZT_HD int zt_ise_bits(zt_ise_range r, int n) { return n * r.bits + (r.trit ? (8 * n + 4) / 5 : r.quint ? (7 * n + 2) / 3 : 0); }
-// largest colour range (>= 6 levels) whose encoding fits (zap_tex.h:996-999); -1 if none
+ZT_HD int zt_ceil_div(int a, int b) { return (a + b - 1) / b; }
+// bits of an n-value sequence: n plain fields, plus ceil(8n/5) trit or ceil(7n/3) quint transfer bits
+ZT_HD int zt_ise_bits(zt_ise_range r, int n) {
+    return n * r.bits + (r.trit ? zt_ceil_div(8 * n, 5) : 0) + (r.quint ? zt_ceil_div(7 * n, 3) : 0);
+}
+// colour endpoints take the finest range (of at least 6 levels, range index >= 4) whose sequence fits the bits left
+// (zap_tex.h:996-999); -1 when none fits
 ZT_HD int zt_colour_range(int nvals, int avail) {
-    for (int i = 20; i >= 4; i--) if (zt_ise_bits(zt_ise(i), nvals) <= avail) return i;
-    return -1;
+    int best = -1;
+    for (int i = 4; i <= 20; i++)
+        if (zt_ise_bits(zt_ise(i), nvals) <= avail) best = i;
+    return best;
 }
-ZT_HD int zt_replicate(int v, int n, int to) {  // zap_tex.h:1001-1005
-    int x = 0;
-    for (int sh = to - n; sh > -n; sh -= n) x |= sh >= 0 ? v << sh : v >> -sh;
-    return x & ((1 << to) - 1);
+// an n-bit value repeated MSB-first to fill `to` bits (the ASTC bit-replication rule)
+ZT_HD int zt_widen(int v, int n, int to) {
+    int x = 0, have = 0;
+    while (have < to) { x = x << n | v; have += n; }
+    return x >> (have - to);
 }
-ZT_HD int zt_colour_unquant(zt_ise_range r, int v) {  // zap_tex.h:1008-1027
-    if (!r.trit && !r.quint) return zt_replicate(v, r.bits, 8);
-    int n = r.bits, m = v & ((1 << n) - 1), D = v >> n;
-    int a = m & 1, b = m >> 1 & 1, c = m >> 2 & 1, d = m >> 3 & 1, e = m >> 4 & 1, f = m >> 5 & 1, A = a ? 0x1FF : 0, B = 0, C = 0;
-    switch (r.levels) {
-    case 6: C = 204; break;
-    case 10: C = 113; break;
-    case 12: B = b << 8 | b << 4 | b << 2 | b << 1; C = 93; break;
-    case 20: B = b << 8 | b << 3 | b << 2; C = 54; break;
-    case 24: B = c << 8 | b << 7 | c << 3 | b << 2 | c << 1 | b; C = 44; break;
-    case 40: B = c << 8 | b << 7 | c << 2 | b << 1 | c; C = 26; break;
-    case 48: B = d << 8 | c << 7 | b << 6 | d << 2 | c << 1 | b; C = 22; break;
-    case 80: B = d << 8 | c << 7 | b << 6 | d << 1 | c; C = 13; break;
-    case 96: B = e << 8 | d << 7 | c << 6 | b << 5 | e << 1 | d; C = 11; break;
-    case 160: B = e << 8 | d << 7 | c << 6 | b << 5 | e; C = 6; break;
-    case 192: B = f << 8 | e << 7 | d << 6 | c << 5 | b << 4 | f; C = 5; break;
+// a weight of n plain bits (n = 1..5) on the 0..64 scale: widened to 6 bits, values above 32 moved up by one
+ZT_HD int zt_weight_value(int w, int n) {
+    const int x = zt_widen(w, n, 6);
+    return x + (x > 32);
+}
+
+// ---- the specification's tables in spec notation, turned into lookup tables by zt_astc_setup_build
+namespace zt_spec {
+// colour unquantisation of trit/quint ranges: T = D * C + B, then T ^ A, result (A & 0x80) | (T >> 2), where D is the
+// trit/quint digit, A replicates the value's lowest plain bit to 9 bits and B is the 9-bit pattern below, MSB first,
+// over the plain bits a (bit 0, in A) and b..f (bits 1..5)
+struct UnquantRow { int levels, C; const char *B; };
+const UnquantRow kUnquant[11] = { { 6, 204, "000000000" }, { 10, 113, "000000000" }, { 12, 93, "b000b0bb0" },
+                                  { 20, 54, "b0000bb00" }, { 24, 44, "cb000cbcb" }, { 40, 26, "cb0000cbc" },
+                                  { 48, 22, "dcb000dcb" }, { 80, 13, "dcb0000dc" }, { 96, 11, "edcb000ed" },
+                                  { 160, 6, "edcb0000e" }, { 192, 5, "fedcb000f" } };
+inline int field(int x, int hi, int lo) { return x >> lo & ((1 << (hi - lo + 1)) - 1); }
+inline int colour_value(zt_ise_range r, int v) {
+    if (!r.trit && !r.quint) return zt_widen(v, r.bits, 8);
+    const int plain = v & ((1 << r.bits) - 1), D = v >> r.bits;
+    const UnquantRow *row = kUnquant;
+    while (row->levels != r.levels) row++;
+    int B = 0;
+    for (int k = 0; k < 9; k++) {
+        const char ch = row->B[k];
+        B = B << 1 | (ch == '0' ? 0 : plain >> (ch - 'a') & 1);
     }
-    int T = (D * C + B) ^ A;
+    const int A = (plain & 1) ? 0x1FF : 0, T = (D * row->C + B) ^ A;
     return (A & 0x80) | (T >> 2);
 }
-ZT_HD int zt_weight_unquant(int w, int n) { int x = zt_replicate(w, n, 6); return x > 32 ? x + 1 : x; }
-
-ZT_HD void zt_trits(int T, int t[5]) {  // 8 bits -> 5 trits (spec C.2.12; zap_tex.h:1032-1042)
+// the five trits of an 8-bit trit block T[7:0] (decoding procedure of the specification's trit table)
+inline void trit_digits(int T, int t[5]) {
     int C;
-    if ((T >> 2 & 7) == 7) { C = (T >> 5 & 7) << 2 | (T & 3); t[4] = 2; t[3] = 2; }
-    else {
-        C = T & 31;
-        if ((T >> 5 & 3) == 3) { t[4] = 2; t[3] = T >> 7 & 1; } else { t[4] = T >> 7 & 1; t[3] = T >> 5 & 3; }
+    if (field(T, 4, 2) == 7) {
+        C = field(T, 7, 5) << 2 | field(T, 1, 0);
+        t[4] = 2; t[3] = 2;
+    } else {
+        C = field(T, 4, 0);
+        if (field(T, 6, 5) == 3) { t[4] = 2; t[3] = field(T, 7, 7); }
+        else { t[4] = field(T, 7, 7); t[3] = field(T, 6, 5); }
     }
-    if ((C & 3) == 3) { t[2] = 2; t[1] = C >> 4 & 1; t[0] = (C >> 3 & 1) << 1 | ((C >> 2 & 1) & ~(C >> 3) & 1); }
-    else if ((C >> 2 & 3) == 3) { t[2] = 2; t[1] = 2; t[0] = C & 3; }
-    else { t[2] = C >> 4 & 1; t[1] = C >> 2 & 3; t[0] = (C >> 1 & 1) << 1 | ((C & 1) & ~(C >> 1) & 1); }
+    if (field(C, 1, 0) == 3) {
+        t[2] = 2; t[1] = field(C, 4, 4);
+        t[0] = field(C, 3, 3) << 1 | (field(C, 2, 2) & (field(C, 3, 3) ^ 1));
+    } else if (field(C, 3, 2) == 3) {
+        t[2] = 2; t[1] = 2; t[0] = field(C, 1, 0);
+    } else {
+        t[2] = field(C, 4, 4); t[1] = field(C, 3, 2);
+        t[0] = field(C, 1, 1) << 1 | (field(C, 0, 0) & (field(C, 1, 1) ^ 1));
+    }
 }
-ZT_HD void zt_quints(int Q, int q[3]) {  // 7 bits -> 3 quints (zap_tex.h:1044-1053)
-    if ((Q >> 1 & 3) == 3 && (Q >> 5 & 3) == 0) {
-        q[2] = (Q & 1) << 2 | ((Q >> 4 & 1) & ~Q & 1) << 1 | ((Q >> 3 & 1) & ~Q & 1); q[1] = 4; q[0] = 4;
+// the three quints of a 7-bit quint block Q[6:0] (decoding procedure of the specification's quint table)
+inline void quint_digits(int Q, int q[3]) {
+    if (field(Q, 2, 1) == 3 && field(Q, 6, 5) == 0) {
+        q[2] = field(Q, 0, 0) << 2 | (field(Q, 4, 4) & (field(Q, 0, 0) ^ 1)) << 1 | (field(Q, 3, 3) & (field(Q, 0, 0) ^ 1));
+        q[1] = 4; q[0] = 4;
         return;
     }
     int C;
-    if ((Q >> 1 & 3) == 3) { q[2] = 4; C = (Q >> 3 & 3) << 3 | (~(Q >> 5) & 3) << 1 | (Q & 1); }
-    else { q[2] = Q >> 5 & 3; C = Q & 31; }
-    if ((C & 7) == 5) { q[1] = 4; q[0] = C >> 3 & 3; } else { q[1] = C >> 3 & 3; q[0] = C & 7; }
+    if (field(Q, 2, 1) == 3) { q[2] = 4; C = field(Q, 4, 3) << 3 | (field(Q, 6, 5) ^ 3) << 1 | field(Q, 0, 0); }
+    else { q[2] = field(Q, 6, 5); C = field(Q, 4, 0); }
+    if (field(C, 2, 0) == 5) { q[1] = 4; q[0] = field(C, 4, 3); }
+    else { q[1] = field(C, 4, 3); q[0] = field(C, 2, 0); }
 }
+// weight bits per texel of an 11-bit block mode, for the modes this codec reads: a 4x4 weight grid, one plane,
+// plain-bit weight ranges (2, 4, 8, 16 or 32 levels); 0 for every other mode.  Spec layout of modes with
+// bits[1:0] != 0: R = {bit 1, bit 0, bit 4}, H = bit 9, dual plane = bit 10, and for bits[3:2] == 0 a grid of
+// (bits[8:7] + 4) x (bits[6:5] + 2); (H, R) pick the weight range.
+inline int block_mode_weight_bits(int mode) {
+    if (field(mode, 1, 0) == 0 || field(mode, 3, 2) != 0) return 0;
+    if (field(mode, 8, 7) + 4 != 4 || field(mode, 6, 5) + 2 != 4 || field(mode, 10, 10)) return 0;
+    const int R = field(mode, 1, 1) << 2 | field(mode, 0, 0) << 1 | field(mode, 4, 4), H = field(mode, 9, 9);
+    const int low[8] = { 0, 0, 2, 3, 4, 5, 6, 8 }, high[8] = { 0, 0, 10, 12, 16, 20, 24, 32 };
+    const int levels = H ? high[R] : low[R];
+    for (int b = 1; b <= 5; b++) if (levels == 1 << b) return b;
+    return 0;
+}
+}  // namespace zt_spec
+
 ZT_HD uint32_t zt_part_hash(uint32_t p) {  // spec C.2.21 (zap_tex.h:1098-1101)
     p ^= p >> 15; p -= p << 17; p += p << 7; p += p << 4; p ^= p >> 5; p += p << 16; p ^= p >> 7; p ^= p >> 3; p ^= p << 6; p ^= p >> 17;
     return p;
@@ -106,29 +148,64 @@ ZT_HD int zt_astc_partition(int seed, int x, int y, int count) {  // small-block
 struct zt_astc_cfg { int range, levels, trit, quint, bits, wb, cem, nv, parts; };
 struct zt_astc_setup {
     zt_astc_cfg cfg[8];
-    uint8_t q[8][256];     // 8-bit value -> nearest ISE value of the cfg's colour range
-    uint8_t unq[8][256];   // ISE value -> 0..255
-    uint8_t tenc[256];     // 5 trits (base-3 digits, t0 lowest) -> packed 8-bit code (smallest code wins)
-    uint8_t qenc[128];     // 3 quints -> 7-bit code
-    uint16_t pmask[1024];  // usable 2-partition patterns: bit i = texel i in partition 1
+    uint8_t q[8][256];        // 8-bit value -> nearest ISE value of the cfg's colour range
+    uint8_t unq[8][256];      // ISE value -> 0..255, per cfg
+    uint8_t colour[21][256];  // ISE value -> 0..255, every range (the decoder's)
+    uint8_t weight[6][32];    // plain-bit weight w of n bits (n = 1..5) -> 0..64
+    uint16_t tdec[256];       // trit block -> its 5 trits, 2 bits each (trit 0 lowest)
+    uint16_t qdec[128];       // quint block -> its 3 quints, 3 bits each
+    uint8_t tenc[256];        // 5 trits (base-3 digits, t0 lowest) -> the smallest trit block decoding to them
+    uint8_t qenc[128];        // 3 quints -> the smallest quint block
+    uint8_t mode_wb[2048];    // block mode -> weight bits of the supported 4x4 modes, 0 otherwise
+    uint16_t part2[1024];     // 2-partition pattern of every seed: bit i = texel i in partition 1
+    uint16_t pmask[1024];     // usable 2-partition patterns (neither all 0 nor all 1), in seed order
     uint16_t pseed[1024];
     int np;
 };
 
 inline void zt_astc_setup_build(zt_astc_setup *s) {
-    for (int T = 255; T >= 0; T--) { int t[5]; zt_trits(T, t); s->tenc[t[0] + 3 * t[1] + 9 * t[2] + 27 * t[3] + 81 * t[4]] = (uint8_t)T; }
-    for (int Q = 127; Q >= 0; Q--) { int q[3]; zt_quints(Q, q); s->qenc[q[0] + 5 * q[1] + 25 * q[2]] = (uint8_t)Q; }
+    for (int T = 0; T < 256; T++) {
+        int t[5];
+        zt_spec::trit_digits(T, t);
+        s->tdec[T] = (uint16_t)(t[0] | t[1] << 2 | t[2] << 4 | t[3] << 6 | t[4] << 8);
+    }
+    for (int Q = 0; Q < 128; Q++) {
+        int q[3];
+        zt_spec::quint_digits(Q, q);
+        s->qdec[Q] = (uint16_t)(q[0] | q[1] << 3 | q[2] << 6);
+    }
+    // encode tables: the inverse of the decode tables, the smallest block where several decode alike
+    bool tset[243] = {}, qset[125] = {};
+    for (int T = 0; T < 256; T++) {
+        int id = 0;
+        for (int k = 4; k >= 0; k--) id = id * 3 + (s->tdec[T] >> (2 * k) & 3);
+        if (!tset[id]) { tset[id] = true; s->tenc[id] = (uint8_t)T; }
+    }
+    for (int Q = 0; Q < 128; Q++) {
+        int id = 0;
+        for (int k = 2; k >= 0; k--) id = id * 5 + (s->qdec[Q] >> (3 * k) & 7);
+        if (!qset[id]) { qset[id] = true; s->qenc[id] = (uint8_t)Q; }
+    }
+    for (int r = 0; r < 21; r++)
+        for (int v = 0; v < 256; v++) s->colour[r][v] = (uint8_t)(v < zt_ise(r).levels ? zt_spec::colour_value(zt_ise(r), v) : 0);
+    for (int n = 0; n < 6; n++)
+        for (int w = 0; w < 32; w++) s->weight[n][w] = (uint8_t)(n && w < (1 << n) ? zt_weight_value(w, n) : 0);
+    for (int m = 0; m < 2048; m++) s->mode_wb[m] = (uint8_t)zt_spec::block_mode_weight_bits(m);
+    // the encoder's trial configurations: (colour endpoint mode, weight bits, partitions)
     const int spec[8][3] = { { 8, 4, 1 }, { 8, 3, 1 }, { 8, 5, 1 }, { 12, 4, 1 }, { 12, 3, 1 }, { 12, 2, 1 }, { 8, 3, 2 }, { 8, 2, 2 } };
     for (int k = 0; k < 8; k++) {
         zt_astc_cfg &c = s->cfg[k];
         c.cem = spec[k][0]; c.wb = spec[k][1]; c.parts = spec[k][2]; c.nv = c.cem == 8 ? 6 : 8;
         c.range = zt_colour_range(c.nv * c.parts, 128 - (c.parts == 1 ? 17 : 29) - 16 * c.wb);
-        zt_ise_range r = zt_ise(c.range);
+        const zt_ise_range r = zt_ise(c.range);
         c.levels = r.levels; c.trit = r.trit; c.quint = r.quint; c.bits = r.bits;
-        for (int v = 0; v < 256; v++) s->unq[k][v] = (uint8_t)(v < r.levels ? zt_colour_unquant(r, v) : 0);
-        for (int x = 0; x < 256; x++) {
+        for (int v = 0; v < 256; v++) s->unq[k][v] = s->colour[c.range][v];
+        for (int x = 0; x < 256; x++) {   // nearest level, the lowest ISE value on ties
             int best = 0, bd = 1 << 30;
-            for (int v = 0; v < r.levels; v++) { int d = zt_colour_unquant(r, v) - x; d = d < 0 ? -d : d; if (d < bd) { bd = d; best = v; } }
+            for (int v = 0; v < r.levels; v++) {
+                const int d = s->colour[c.range][v] > x ? s->colour[c.range][v] - x : x - s->colour[c.range][v];
+                if (d < bd) { bd = d; best = v; }
+            }
             s->q[k][x] = (uint8_t)best;
         }
     }
@@ -136,6 +213,7 @@ inline void zt_astc_setup_build(zt_astc_setup *s) {
     for (int sd = 0; sd < 1024; sd++) {
         unsigned m = 0;
         for (int i = 0; i < 16; i++) m |= (unsigned)zt_astc_partition(sd, i & 3, i >> 2, 2) << i;
+        s->part2[sd] = (uint16_t)m;
         if (m && m != 0xFFFFu) { s->pmask[s->np] = (uint16_t)m; s->pseed[s->np++] = (uint16_t)sd; }
     }
 }
