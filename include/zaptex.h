@@ -25,7 +25,7 @@ enum {
     ZAPTEX_E_ARG = 1,         /* invalid argument (size, format, alignment, workspace too small) */
     ZAPTEX_E_DEVICE = 2,      /* no HIP device, or a HIP runtime / kernel launch error */
     ZAPTEX_E_NOMEM = 3,       /* device or pinned allocation failed */
-    ZAPTEX_E_UNSUPPORTED = 4  /* a reference feature outside this library: rdo > 0 (rate-distortion mode) */
+    ZAPTEX_E_UNSUPPORTED = 4  /* a reference feature outside this library (none of zap_tex.h's entry points today) */
 };
 
 /* formats: zap_bc_format values (zap_tex.h:32), plus the two non-BCn encoders */
@@ -35,12 +35,15 @@ enum { ZAPTEX_BC1 = 0, ZAPTEX_BC3 = 1, ZAPTEX_BC4 = 2, ZAPTEX_BC5 = 3, ZAPTEX_BC
 enum {
     ZAPTEX_OPT_BC7_M1K = 1,  /* mode-1 partitions fully fitted after ranking, 1..64 (zap_tex.h:750-752, default 4) */
     ZAPTEX_OPT_PROFILE = 2,  /* 1: time every kernel with HIP events on its launch stream (zaptex_profile_*) */
-    ZAPTEX_OPT_CLIMB_REFILL = 3  /* BC1 hill climb: idle lanes (1..64) a wave gathers before it refills them */
+    ZAPTEX_OPT_CLIMB_REFILL = 3, /* BC1 hill climb: idle lanes (1..64) a wave gathers before it refills them */
+    ZAPTEX_OPT_RDO_SEGMENT = 4,  /* rdo > 0: blocks per speculative segment, 16..2^24, or 0 = automatic (default) */
+    ZAPTEX_OPT_RDO_WARMUP = 5    /* rdo > 0: blocks a segment's speculative walk starts early, 0..2^24 (default 512);
+                                    segment and warm-up change speed only, never the bytes */
 };
 
 /* ---------------- host entry points (drop-in bodies of zap_tex.h) */
 
-/* zap_tex.h:1474 zap_bc_encode. rdo > 0 returns ZAPTEX_E_UNSUPPORTED. */
+/* zap_tex.h:1474 zap_bc_encode, rdo > 0 included (zap_tex.h:223-306, :786-815). */
 int zaptex_bc_encode(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, void *out);
 /* zap_tex.h:1526 zap_bc_decode */
 int zaptex_bc_decode(const void *bc, int w, int h, int fmt, uint8_t *rgba, size_t stride);
@@ -86,6 +89,14 @@ size_t zaptex_workspace_size(const zaptex_image *imgs, int n);
  * zaptex_workspace_size bytes (256-byte aligned), owned by the caller; calls sharing a workspace must not
  * overlap. */
 int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t ws_bytes, void *stream);
+/* zap_tex.h:1474 zap_bc_encode with rdo (zap_tex.h:223-306, :786-815) on device memory: zaptex_encode_dev, then the
+ * rate-distortion pass of every image (BCn formats only).  Each image is its own zap_bc_encode call: candidates come
+ * from its own previous blocks, in raster order.  The pass cuts every 8-byte block stream into segments walked in
+ * parallel from guessed starting states, then checks the guesses in order and re-walks a segment from the true state
+ * until it rejoins the guess, so the bytes are the reference's whatever the guesses.  The workspace is device memory
+ * of zaptex_rdo_workspace_size bytes (zaptex_workspace_size when rdo <= 0). */
+size_t zaptex_rdo_workspace_size(const zaptex_image *imgs, int n);
+int zaptex_encode_rdo_dev(const zaptex_image *imgs, int n, float rdo, void *workspace, size_t ws_bytes, void *stream);
 /* zaptex_encode_dev without a workspace argument and without any allocation (no counterpart in the reference;
  * for callers that cannot allocate device memory: code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  The workspace is a scratch array inside the library, ZAPTEX_NOALLOC_SCRATCH_BYTES per

@@ -64,6 +64,7 @@ class TexLib:
             fn = getattr(L, p + name); fn.restype = C.POINTER(C.c_float)
         if prefix == "zto_":
             L.zto_bc_encode_rows.argtypes = [vp, i, i, sz, i, vp, i, i]
+            L.zto_bc_encode_rdo.argtypes = [vp, i, i, sz, i, f, vp]
             L.zto_bc6h_encode_rows.argtypes = [vp, i, i, sz, vp, i, i]
             L.zto_astc_encode_rows.argtypes = [vp, i, i, sz, vp, i, i]
             L.zto_bc7_block.argtypes = [vp, vp, C.POINTER(i)]; L.zto_bc7_block.restype = i
@@ -86,7 +87,7 @@ class TexLib:
             L.zref_bc_merge.argtypes = [vp, sz, i, vp]
 
     # ---- encoders (img: HxWx4 uint8 / float32, C-contiguous rows; stride may exceed W*4)
-    def bc_encode(self, img: np.ndarray, fmt, w: int | None = None, h: int | None = None) -> np.ndarray:
+    def bc_encode(self, img: np.ndarray, fmt, w: int | None = None, h: int | None = None, rdo: float = 0.0) -> np.ndarray:
         fmt = FMT.get(fmt, fmt)
         h = img.shape[0] if h is None else h
         w = img.shape[1] if w is None else w
@@ -94,7 +95,9 @@ class TexLib:
         out = np.zeros(bc_size(w, h, fmt), dtype=np.uint8)
         stride = img.strides[0]
         if self.prefix == "zref_":
-            self._enc(_ptr(img), w, h, stride, fmt, 0.0, _ptr(out))
+            self._enc(_ptr(img), w, h, stride, fmt, float(rdo), _ptr(out))
+        elif rdo > 0:
+            self.lib.zto_bc_encode_rdo(_ptr(img), w, h, stride, fmt, float(rdo), _ptr(out))
         else:
             self._enc(_ptr(img), w, h, stride, fmt, _ptr(out))
         return out

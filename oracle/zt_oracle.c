@@ -1394,6 +1394,205 @@ void zto_bc_encode(const uint8_t *rgba, int w, int h, size_t stride, int fmt, vo
     zto_bc_encode_rows(rgba, w, h, stride, fmt, out, 0, (h + 3) / 4);
 }
 
+/* ======================================================================= RDO (rdo > 0)
+ * zap_tex.h:223-306 (rate model, BC1 and BC4 sub-blocks), :520-579 (mode-6 repack, refit of indices or
+ * endpoints), :786-815 (BC7), :1474-1506 (driver: raster order, one state per sub-block stream). */
+typedef struct { int dist[2]; } rdo_state;
+
+/* zap_tex.h:234-242: 0 when the field continues the current match, the cost of a new match when it equals a field
+ * of the previous 16 blocks (the nearest one), else its size in literals */
+static float field_rate(const uint8_t *f, int n, const uint8_t *base, size_t cur, size_t stride, size_t off, int dist, int *newdist) {
+    *newdist = dist;
+    if (dist > 0 && (size_t)dist <= cur && !memcmp(f, base + (cur - (size_t)dist) * stride + off, (size_t)n)) return 0.0f;
+    for (size_t j = 1; j <= 16 && j <= cur; j++)
+        if (!memcmp(f, base + (cur - j) * stride + off, (size_t)n)) { *newdist = (int)j; return n < 3 ? (float)n : 3.0f; }
+    return (float)n;
+}
+
+/* zap_tex.h:245-276: candidates from each of the previous 16 blocks: its endpoints (indices refitted), this block's
+ * endpoints with its indices, its whole block; 3-colour candidates that would use the transparent index are skipped */
+static void bc1_rdo(const px4 *px, int force4, float lambda, uint8_t *base, size_t cur, size_t stride, size_t off, rdo_state *rs) {
+    uint16_t c0, c1;
+    uint32_t idx;
+    int e0 = bc1_block(px, &c0, &c1, &idx);
+    uint8_t best[8], cand[8];
+    put_bc1(best, c0, c1, idx);
+    int d0, d1, bd0 = 0, bd1 = 0;
+    float bj = (float)e0 + lambda * (field_rate(best, 4, base, cur, stride, off, rs->dist[0], &bd0) +
+                                     field_rate(best + 4, 4, base, cur, stride, off + 4, rs->dist[1], &bd1));
+    for (size_t j = 1; j <= 16 && j <= cur; j++) {
+        const uint8_t *pb = base + (cur - j) * stride + off;
+        uint16_t p0 = (uint16_t)ld_le(pb, 2), p1 = (uint16_t)ld_le(pb + 2, 2);
+        uint32_t pi = (uint32_t)ld_le(pb + 4, 4), fi;
+        for (int kind = 0; kind < 3; kind++) {
+            int e;
+            if (kind == 0) { e = bc1_indices(px, p0, p1, force4, &fi); put_bc1(cand, p0, p1, fi); }
+            else if (kind == 1) { e = bc1_score(px, c0, c1, force4, pi); put_bc1(cand, c0, c1, pi); }
+            else { e = bc1_score(px, p0, p1, force4, pi); memcpy(cand, pb, 8); }
+            if (!force4 && ld_le(cand, 2) <= ld_le(cand + 2, 2)) {
+                uint32_t ci = (uint32_t)ld_le(cand + 4, 4);
+                int bad = 0;
+                for (int i = 0; i < 16; i++) bad |= ((ci >> (2 * i)) & 3) == 3;
+                if (bad) continue;
+            }
+            float jj = (float)e;
+            if (jj >= bj) continue;
+            jj += lambda * (field_rate(cand, 4, base, cur, stride, off, rs->dist[0], &d0) +
+                            field_rate(cand + 4, 4, base, cur, stride, off + 4, rs->dist[1], &d1));
+            if (jj < bj) { bj = jj; memcpy(best, cand, 8); bd0 = d0; bd1 = d1; }
+        }
+    }
+    rs->dist[0] = bd0; rs->dist[1] = bd1;
+    memcpy(base + cur * stride + off, best, 8);
+}
+
+/* zap_tex.h:279-306 */
+static void bc4_rdo(const uint8_t v[16], float lambda, uint8_t *base, size_t cur, size_t stride, size_t off, rdo_state *rs) {
+    uint8_t a0, a1;
+    uint64_t idx;
+    int e0 = bc4_block(v, &a0, &a1, &idx);
+    uint8_t best[8], cand[8];
+    put_bc4(best, a0, a1, idx);
+    int d0, d1, bd0 = 0, bd1 = 0;
+    float bj = (float)e0 + lambda * (field_rate(best, 2, base, cur, stride, off, rs->dist[0], &bd0) +
+                                     field_rate(best + 2, 6, base, cur, stride, off + 2, rs->dist[1], &bd1));
+    for (size_t j = 1; j <= 16 && j <= cur; j++) {
+        const uint8_t *pb = base + (cur - j) * stride + off;
+        uint64_t pi = ld_le(pb + 2, 6), fi;
+        for (int kind = 0; kind < 3; kind++) {
+            int e;
+            if (kind == 0) { e = bc4_indices(v, pb[0], pb[1], &fi); put_bc4(cand, pb[0], pb[1], fi); }
+            else if (kind == 1) { e = bc4_score(v, a0, a1, pi); put_bc4(cand, a0, a1, pi); }
+            else { e = bc4_score(v, pb[0], pb[1], pi); memcpy(cand, pb, 8); }
+            float jj = (float)e;
+            if (jj >= bj) continue;
+            jj += lambda * (field_rate(cand, 2, base, cur, stride, off, rs->dist[0], &d0) +
+                            field_rate(cand + 2, 6, base, cur, stride, off + 2, rs->dist[1], &d1));
+            if (jj < bj) { bj = jj; memcpy(best, cand, 8); bd0 = d0; bd1 = d1; }
+        }
+    }
+    rs->dist[0] = bd0; rs->dist[1] = bd1;
+    memcpy(base + cur * stride + off, best, 8);
+}
+
+/* mode-6 block fields (zap_tex.h:520-543) */
+static void mode6_pack(int q[2][4], const int pb[2], const int *idx, uint8_t out[16]) {
+    int pos = 0;
+    memset(out, 0, 16);
+    putbits(out, &pos, 1 << 6, 7);
+    for (int c = 0; c < 4; c++) for (int e = 0; e < 2; e++) putbits(out, &pos, (unsigned)q[e][c], 7);
+    putbits(out, &pos, (unsigned)pb[0], 1); putbits(out, &pos, (unsigned)pb[1], 1);
+    for (int i = 0; i < 16; i++) putbits(out, &pos, (unsigned)idx[i], i ? 4 : 3);
+}
+static int is_mode6(const uint8_t *b) { return (b[0] & 0x7F) == 0x40; }
+static void mode6_unpack(const uint8_t *b, int q[2][4], int pb[2], int idx[16]) {
+    int pos = 7;
+    for (int c = 0; c < 4; c++) for (int e = 0; e < 2; e++) q[e][c] = (int)getbits(b, &pos, 7);
+    pb[0] = (int)getbits(b, &pos, 1); pb[1] = (int)getbits(b, &pos, 1);
+    for (int i = 0; i < 16; i++) idx[i] = (int)getbits(b, &pos, i ? 4 : 3);
+}
+static void mode6_palette(int q[2][4], const int pb[2], int pal[16][4]) { /* zap_tex.h:538-542 */
+    int e[2][4];
+    for (int k = 0; k < 2; k++) for (int c = 0; c < 4; c++) e[k][c] = expand_bits(q[k][c] << 1 | pb[k], 8);
+    for (int j = 0; j < 16; j++) for (int c = 0; c < 4; c++) pal[j][c] = ((64 - W4[j]) * e[0][c] + W4[j] * e[1][c] + 32) >> 6;
+}
+/* zap_tex.h:545-555: best indices for fixed endpoints, pixel 0 limited to 0..7 */
+static int mode6_refit_indices(const px4 *px, int q[2][4], const int pb[2], int idx[16]) {
+    int pal[16][4], err = 0;
+    mode6_palette(q, pb, pal);
+    for (int i = 0; i < 16; i++) {
+        int bi = 0, be = 1 << 30;
+        for (int j = 0; j < (i ? 16 : 8); j++) { int d = sse_n(px[i], pal[j], 4); if (d < be) { be = d; bi = j; } }
+        idx[i] = bi; err += be;
+    }
+    return err;
+}
+/* zap_tex.h:557-579: least-squares endpoints for fixed indices, each endpoint's p-bit chosen on its own */
+static int mode6_refit_endpoints(const px4 *px, const int idx[16], int q[2][4], int pb[2]) {
+    float aa = 0, ab = 0, bb = 0, ax[4] = { 0, 0, 0, 0 }, bx[4] = { 0, 0, 0, 0 }, lo[4], hi[4];
+    for (int i = 0; i < 16; i++) {
+        float w = W4[idx[i]] / 64.0f, v = 1 - w;
+        aa += v * v; ab += v * w; bb += w * w;
+        for (int c = 0; c < 4; c++) { ax[c] += v * px[i][c]; bx[c] += w * px[i][c]; }
+    }
+    float det = aa * bb - ab * ab;
+    for (int c = 0; c < 4; c++) {
+        if (det < 1e-6f && det > -1e-6f) lo[c] = hi[c] = (ax[c] + bx[c]) / 16.0f;
+        else { lo[c] = (ax[c] * bb - bx[c] * ab) / det; hi[c] = (bx[c] * aa - ax[c] * ab) / det; }
+        lo[c] = lo[c] < 0 ? 0 : lo[c] > 255 ? 255 : lo[c]; hi[c] = hi[c] < 0 ? 0 : hi[c] > 255 ? 255 : hi[c];
+    }
+    int e0[4], e1[4], t0[4], t1[4], b0 = 1 << 30, b1 = 1 << 30;
+    for (int p = 0; p < 2; p++) {
+        int x = quant_pbit(lo, 4, 8, p, t0, e0), y = quant_pbit(hi, 4, 8, p, t1, e1);
+        if (x < b0) { b0 = x; memcpy(q[0], t0, sizeof t0); pb[0] = p; }
+        if (y < b1) { b1 = y; memcpy(q[1], t1, sizeof t1); pb[1] = p; }
+    }
+    int pal[16][4], err = 0;
+    mode6_palette(q, pb, pal);
+    for (int i = 0; i < 16; i++) err += sse_n(px[i], pal[idx[i]], 4);
+    return err;
+}
+/* zap_tex.h:788-790: bytes 0-7 and 9-15 are the two fields, byte 8 is always paid */
+static float bc7_rate(const uint8_t *c, const uint8_t *base, size_t cur, const rdo_state *rs, int d[2]) {
+    return field_rate(c, 8, base, cur, 16, 0, rs->dist[0], &d[0]) + field_rate(c + 9, 7, base, cur, 16, 9, rs->dist[1], &d[1]) + 1;
+}
+/* zap_tex.h:792-815: the whole previous block, or (mode 6) its endpoints with refitted indices, or its indices with
+ * re-solved endpoints */
+static void bc7_rdo(const px4 *px, float lambda, uint8_t *base, size_t cur, rdo_state *rs) {
+    uint8_t best[16], cand[16];
+    int e0 = bc7_block(px, best), bd[2] = { 0, 0 }, d[2];
+    if (lambda > 0) {
+        float bj = (float)e0 + lambda * bc7_rate(best, base, cur, rs, bd);
+        for (size_t j = 1; j <= 16 && j <= cur; j++) {
+            const uint8_t *pb = base + (cur - j) * 16;
+            int pq[2][4], ppb[2], pidx[16], nq[2][4], npb[2], nidx[16];
+            int six = is_mode6(pb);
+            if (six) mode6_unpack(pb, pq, ppb, pidx);
+            for (int kind = 0; kind < (six ? 3 : 1); kind++) {
+                int e;
+                if (kind == 0) { e = bc7_decoded_sse(px, pb); memcpy(cand, pb, 16); }
+                else if (kind == 1) { e = mode6_refit_indices(px, pq, ppb, nidx); mode6_pack(pq, ppb, nidx, cand); }
+                else { e = mode6_refit_endpoints(px, pidx, nq, npb); mode6_pack(nq, npb, pidx, cand); }
+                if ((float)e >= bj) continue;
+                float jj = (float)e + lambda * bc7_rate(cand, base, cur, rs, d);
+                if (jj < bj) { bj = jj; memcpy(best, cand, 16); bd[0] = d[0]; bd[1] = d[1]; }
+            }
+        }
+        rs->dist[0] = bd[0]; rs->dist[1] = bd[1];
+    }
+    memcpy(base + cur * 16, best, 16);
+}
+
+/* zap_bc_encode with rdo (zap_tex.h:1474-1506): raster order; BC3 = alpha BC4 stream (state 0) + 4-colour BC1
+ * stream (state 1), BC5 = R and G BC4 streams */
+void zto_bc_encode_rdo(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, void *out_) {
+    uint8_t *out = (uint8_t *)out_;
+    int bw = (w + 3) / 4, bh = (h + 3) / 4;
+    size_t bs = fmt == 0 || fmt == 2 ? 8 : 16;
+    rdo_state rs[2] = { { { 1, 1 } }, { { 1, 1 } } };
+    p3_init();
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            px4 px[16];
+            uint8_t ch[2][16];
+            gather(rgba, w, h, stride, bx, by, px);
+            size_t cur = (size_t)by * (size_t)bw + (size_t)bx;
+            if (fmt == 0) bc1_rdo((const px4 *)px, 0, rdo, out, cur, bs, 0, &rs[0]);
+            else if (fmt == 2) {
+                for (int i = 0; i < 16; i++) ch[0][i] = px[i][0];
+                bc4_rdo(ch[0], rdo, out, cur, bs, 0, &rs[0]);
+            } else if (fmt == 1) {
+                for (int i = 0; i < 16; i++) ch[0][i] = px[i][3];
+                bc4_rdo(ch[0], rdo, out, cur, bs, 0, &rs[0]);
+                bc1_rdo((const px4 *)px, 1, rdo, out, cur, bs, 8, &rs[1]);
+            } else if (fmt == 3) {
+                for (int i = 0; i < 16; i++) { ch[0][i] = px[i][0]; ch[1][i] = px[i][1]; }
+                bc4_rdo(ch[0], rdo, out, cur, bs, 0, &rs[0]);
+                bc4_rdo(ch[1], rdo, out, cur, bs, 8, &rs[1]);
+            } else bc7_rdo((const px4 *)px, rdo, out, cur, &rs[0]);
+        }
+}
+
 /* per-block BC7 stage probe: returns the chosen mode (6, 5 or 1) and the final SSE */
 int zto_bc7_block(const uint8_t px64[64], uint8_t out[16], int *err) {
     p3_init();

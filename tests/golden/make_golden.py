@@ -3,7 +3,7 @@
 Runs the compiled reference (oracle/_ref, built from zap_tex.h with -std=c11 -O2 -ffp-contract=off) in this
 container only; the GPU tests compare against the committed hashes and never need the reference at run time.
 Usage: python tests/golden/make_golden.py   (about a minute on 8 cores)
-       python tests/golden/make_golden.py --only near_singular containers   (rewrites those fixtures alone)
+       python tests/golden/make_golden.py --only near_singular containers rdo   (rewrites those fixtures alone)
 """
 from __future__ import annotations
 
@@ -128,12 +128,28 @@ def containers() -> dict:
     return out
 
 
+RDO_LAMBDAS = (10.0, 30.0, 100.0, 300.0)   # tex_bench.c:199-225 (0 is the plain encoding)
+
+
+def rdo_hashes(threads: int = 8) -> dict:
+    """SHA-256 of the reference's rdo > 0 encodes (single calls: RDO's raster order cannot be banded): the C1 1024^2
+    image in every format and level 2 (1024^2) of the C2 chain in BC7, at each of RDO_LAMBDAS."""
+    ref = loader.reference()
+    c1 = synth.c1_bc1(1024, 1)
+    c2l2 = ref.mip_chain(synth.c2_bc7(4096, 2), 0)[2]
+    jobs = [(f"c1_1024_{f}_rdo{int(l)}", c1, f, l) for f in ["bc1", "bc3", "bc4", "bc5", "bc7"] for l in RDO_LAMBDAS]
+    jobs += [(f"c2_level2_bc7_rdo{int(l)}", c2l2, "bc7", l) for l in RDO_LAMBDAS]
+    with ThreadPoolExecutor(threads) as ex:
+        outs = list(ex.map(lambda j: sha(ref.bc_encode(j[1], j[2], rdo=j[3])), jobs))
+    return {"inputs": {"c1_1024": sha(c1), "c2_level2": sha(c2l2)}, "outputs": {j[0]: o for j, o in zip(jobs, outs)}}
+
+
 def main():
     if sys.argv[1:2] == ["--only"]:
         with open(OUT) as fh:
             g = json.load(fh)
         for part in sys.argv[2:]:
-            g[part] = {"near_singular": lambda: near_singular(loader.reference()), "containers": containers}[part]()
+            g[part] = {"near_singular": lambda: near_singular(loader.reference()), "containers": containers, "rdo": rdo_hashes}[part]()
             print(f"{part}: rewritten")
         with open(OUT, "w") as fh:
             json.dump(g, fh, indent=1, sort_keys=True)
@@ -197,6 +213,7 @@ def main():
         g["small"][key] = ent
     g["near_singular"] = near_singular(ref)
     g["containers"] = containers()
+    g["rdo"] = rdo_hashes()
     with open(OUT, "w") as fh:
         json.dump(g, fh, indent=1, sort_keys=True)
     print(f"wrote {OUT} in {time.time() - t0:.1f}s")

@@ -21,10 +21,15 @@ def test_dropin_header_wraps_every_reference_entry_point():
         assert f" {fn}(" in text, fn
 
 
-def test_rdo_is_rejected_not_emulated(zt):
+def test_rdo_runs_on_the_device_only(zt):
+    """rdo > 0 is a device path like the rest: without a device it fails, it never runs on the CPU; a non-BCn image is
+    refused before any device work."""
     img = np.zeros((4, 4, 4), np.uint8)
-    with pytest.raises(binding.ZapTexError, match="unsupported"):
-        zt.bc_encode(img, "bc1", rdo=10.0)
+    if zt.device_count() == 0:
+        with pytest.raises(binding.ZapTexError, match="device"):
+            zt.bc_encode(img, "bc1", rdo=10.0)
+    bad = (binding.ZapImage * 1)(binding.ZapImage(img.ctypes.data, img.ctypes.data, 16, 4, 4, binding.ASTC))
+    assert zt.lib.zaptex_encode_rdo_dev(bad, 1, 10.0, None, 0, None) == 1
 
 
 def test_bad_arguments(zt):

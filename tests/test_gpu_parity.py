@@ -196,7 +196,3 @@ def test_split_merge_match_oracle(zt, oracle):
         assert np.array_equal(sp, exp), f
         assert np.array_equal(zt.bc_merge(sp, f), raw), f
 
-
-def test_rdo_rejected_on_device(zt):
-    with pytest.raises(binding.ZapTexError, match="unsupported"):
-        zt.bc_encode(np.zeros((8, 8, 4), np.uint8), "bc7", rdo=30.0)

@@ -103,6 +103,7 @@ class Library:
             "zaptex_mip_next": [vp, i, i, sz, i, vp, sz], "zaptex_mip_next_f": [vp, i, i, sz, vp, sz],
             "zaptex_bc_split": [vp, sz, i, vp], "zaptex_bc_merge": [vp, sz, i, vp],
             "zaptex_encode_dev": [C.POINTER(ZapImage), i, vp, sz, vp], "zaptex_encode_dev_noalloc": [C.POINTER(ZapImage), i, vp],
+            "zaptex_encode_rdo_dev": [C.POINTER(ZapImage), i, f, vp, sz, vp],
             "zaptex_mip_next_dev": [vp, i, i, sz, i, vp, sz, vp], "zaptex_mip_next_f_dev": [vp, i, i, sz, vp, sz, vp],
             "zaptex_mip_chain_dev": [vp, i, i, sz, i, C.POINTER(vp), i, vp],
             "zaptex_bc_decode_dev": [vp, i, i, i, vp, sz, vp],
@@ -149,6 +150,8 @@ class Library:
         L.zaptex_tex_file_size.restype = C.c_size_t
         L.zaptex_tex_file.argtypes = [vp, i, i, sz, i, f, i, i, vp, sz]
         L.zaptex_tex_file.restype = C.c_int
+        L.zaptex_rdo_workspace_size.argtypes = [C.POINTER(ZapImage), i]
+        L.zaptex_rdo_workspace_size.restype = C.c_size_t
         L.zaptex_workspace_size.argtypes = [C.POINTER(ZapImage), i]
         L.zaptex_workspace_size.restype = C.c_size_t
         L.zaptex_last_error.restype = C.c_char_p
@@ -269,6 +272,15 @@ class Library:
         ws_ptr = workspace.data_ptr() if workspace is not None else None
         ws_len = workspace.numel() * workspace.element_size() if workspace is not None else 0
         self._check(self.lib.zaptex_encode_dev(imgs, len(imgs), ws_ptr, ws_len, self._stream(stream)), "zaptex_encode_dev")
+
+    def rdo_workspace_size(self, imgs) -> int:
+        return int(self.lib.zaptex_rdo_workspace_size(imgs, len(imgs)))
+
+    def encode_rdo_dev(self, imgs, rdo: float, workspace=None, stream=None) -> None:
+        """zaptex_encode_dev, then the rate-distortion pass of every image (zap_bc_encode with rdo, per image)."""
+        ws_ptr = workspace.data_ptr() if workspace is not None else None
+        ws_len = workspace.numel() * workspace.element_size() if workspace is not None else 0
+        self._check(self.lib.zaptex_encode_rdo_dev(imgs, len(imgs), float(rdo), ws_ptr, ws_len, self._stream(stream)), "zaptex_encode_rdo_dev")
 
     NOALLOC_SCRATCH_BYTES = 64 << 20   # ZAPTEX_NOALLOC_SCRATCH_BYTES
 

@@ -26,7 +26,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-SOURCES = ["zt_bc7.hip", "zt_bc7_full.hip", "zt_mlp.hip", "zt_bc14.hip", "zt_bc6h.hip", "zt_astc.hip", "zt_mip.hip", "zt_decode.hip", "zt_capi.cpp"]
+SOURCES = ["zt_bc7.hip", "zt_bc7_full.hip", "zt_mlp.hip", "zt_bc14.hip", "zt_bc6h.hip", "zt_astc.hip", "zt_mip.hip", "zt_decode.hip", "zt_rdo.hip", "zt_capi.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -109,8 +109,8 @@ struct DeviceState {
     hipStream_t stream = nullptr;
     void *scratch = nullptr;   // this device's address of `scratch`
     // staging for the host entry points (device buffers, and pinned host memory for rows that are not packed)
-    void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr, *mip = nullptr;
-    size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0, mip_cap = 0;
+    void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr, *mip = nullptr, *rdo = nullptr;
+    size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0, mip_cap = 0, rdo_cap = 0;
 };
 DeviceState g_dev[64];
 
@@ -353,6 +353,81 @@ int enqueue(const zaptex_image *imgs, int n, void *ws, size_t ws_bytes, hipStrea
     return ZAPTEX_OK;
 }
 
+// ---- rate-distortion passes (rdo > 0) over images whose plain encoding is already enqueued into their dst: one
+// stream per BC1, BC4 or BC7 image, two per BC3 (alpha BC4 half, 4-colour BC1 half) or BC5 (R and G halves).  Each
+// stream is cut into segments walked speculatively, one wave each, then checked and repaired in order.
+int g_rdo_seg = 0;      // segment length in blocks (0: automatic)
+int g_rdo_warm = 512;   // warm-up blocks before a segment
+uint32_t rdo_seg_len(uint64_t nb) {
+    if (g_rdo_seg) return (uint32_t)g_rdo_seg;
+    return (uint32_t)std::max<uint64_t>(64, ((nb + 1023) / 1024 + 15) / 16 * 16);   // about 1,024 waves per stream
+}
+int rdo_streams_of(int fmt) { return fmt == ZAPTEX_BC3 || fmt == ZAPTEX_BC5 ? 2 : 1; }
+// scratch of one image: the plain copy, then per stream the per-block distances, the segment states and ends
+size_t rdo_scratch_image(const zaptex_image &im) {
+    const uint64_t nb = nblocks(im), nseg = (nb + rdo_seg_len(nb) - 1) / rdo_seg_len(nb);
+    const size_t cache = nseg > 1 ? align_up(nb * ZT_RDO_LANES * 16, 256) + align_up(nb * ZT_RDO_LANES * 4, 256) : 0;
+    return align_up(image_out_bytes(im), 256) +
+           (size_t)rdo_streams_of(im.fmt) * (align_up(nb * 2, 256) + align_up(nseg * sizeof(zt_rdo_state), 256) + align_up(nseg * 4, 256) + cache);
+}
+size_t rdo_scratch_bytes(const zaptex_image *imgs, int n) {
+    size_t t = 0;
+    for (int i = 0; i < n; i++) t += rdo_scratch_image(imgs[i]);
+    return t;
+}
+int enqueue_rdo(const zaptex_image *imgs, int n, float rdo, void *scratch, size_t scratch_bytes, hipStream_t s) {
+    if (!(rdo > 0)) return ZAPTEX_OK;   // zap_tex.h:249, :284, :794: rdo <= 0 (or NaN) is the plain encoding
+    for (int i = 0; i < n; i++)
+        if (!is_bcn(imgs[i].fmt)) return fail(ZAPTEX_E_ARG, "rdo applies to BC1/BC3/BC4/BC5/BC7 only (image %d)", i);
+    if (scratch_bytes < rdo_scratch_bytes(imgs, n) || !scratch) return fail(ZAPTEX_E_ARG, "rdo scratch %zu < %zu bytes", scratch_bytes, rdo_scratch_bytes(imgs, n));
+    zt_rdo_streams S;
+    memset(&S, 0, sizeof S);
+    int k = 0;
+    uint64_t segs = 0;
+    auto flush = [&]() -> int {
+        if (!k) return ZAPTEX_OK;
+        const int cnt = k;
+        const uint64_t g = segs;
+        k = 0; segs = 0;
+        int rc = launch("zt_rdo_spec", s, [&] { hipLaunchKernelGGL(zt_rdo_spec, dim3((unsigned)g), dim3(64), 0, s, S, rdo); });
+        if (rc) return rc;
+        return launch("zt_rdo_fix", s, [&] { hipLaunchKernelGGL(zt_rdo_fix, dim3(cnt), dim3(64), 0, s, S, rdo); });
+    };
+    uint8_t *p = (uint8_t *)scratch;
+    for (int i = 0; i < n; i++) {
+        const zaptex_image &im = imgs[i];
+        const uint64_t nb = nblocks(im);
+        const uint32_t seg = rdo_seg_len(nb), nseg = (uint32_t)((nb + seg - 1) / seg);
+        uint8_t *plain = p;
+        p += align_up(image_out_bytes(im), 256);
+        HIP_TRY(hipMemcpyAsync(plain, im.dst, image_out_bytes(im), hipMemcpyDeviceToDevice, s));
+        const int ns = rdo_streams_of(im.fmt);
+        for (int t = 0; t < ns; t++) {
+            if (k == ZT_RDO_STREAMS) if (int rc = flush()) return rc;
+            zt_rdo_stream &r = S.s[k++];
+            r.src = (const uint8_t *)im.src; r.out = (uint8_t *)im.dst; r.plain = plain; r.stride = im.stride;
+            r.dists = (uint16_t *)p; p += align_up(nb * 2, 256);
+            r.states = (zt_rdo_state *)p; p += align_up((uint64_t)nseg * sizeof(zt_rdo_state), 256);
+            r.seg_end = (uint32_t *)p; p += align_up((uint64_t)nseg * 4, 256);
+            if (nseg > 1) {   // the speculative walk's candidates, which the ordered repair reuses
+                r.cand = (uint4 *)p; p += align_up(nb * ZT_RDO_LANES * 16, 256);
+                r.cand_err = (int *)p; p += align_up(nb * ZT_RDO_LANES * 4, 256);
+            }
+            r.w = im.w; r.h = im.h; r.bw = (im.w + 3) / 4; r.bh = (im.h + 3) / 4;
+            r.bs = (int)block_bytes(im.fmt); r.seg = seg; r.warm = (uint32_t)g_rdo_warm; r.nseg = nseg;
+            switch (im.fmt) {   // zap_tex.h:1494-1503
+            case ZAPTEX_BC1: r.kind = ZT_RDO_BC1; break;
+            case ZAPTEX_BC4: r.kind = ZT_RDO_BC4; break;
+            case ZAPTEX_BC3: if (t == 0) { r.kind = ZT_RDO_BC4; r.chan = 3; } else { r.kind = ZT_RDO_BC1; r.off = 8; r.force4 = 1; } break;
+            case ZAPTEX_BC5: r.kind = ZT_RDO_BC4; r.chan = t; r.off = 8 * t; break;
+            default: r.kind = ZT_RDO_BC7; break;
+            }
+            segs += nseg;
+        }
+    }
+    return flush();
+}
+
 int enqueue_mip(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, hipStream_t s) {
     int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
     dim3 grid((nw + 63) / 64, (nh + 3) / 4), block(64, 4);
@@ -401,7 +476,7 @@ int enqueue_mip_chain(const void *base, int w, int h, size_t stride, int srgb, v
 }
 
 // run one host-buffer image through the device: stage in, encode, copy out
-int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, int fmt, void *out) {
+int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, int fmt, void *out, float rdo = 0.0f) {
     std::lock_guard<std::mutex> lk(g_mu);
     DeviceState *st;
     if (int rc = ensure_ready(st)) return rc;
@@ -415,6 +490,10 @@ int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, i
     im.src = st->in; im.dst = st->out;
     if (int rc = rows_to_device(st, st->in, src, stride, row, (size_t)h)) return rc;
     if (int rc = enqueue(&im, 1, st->ws, st->ws_cap, st->stream)) return rc;
+    if (rdo > 0) {
+        if (int rc = ensure_buf(st->rdo, st->rdo_cap, rdo_scratch_bytes(&im, 1))) return rc;
+        if (int rc = enqueue_rdo(&im, 1, rdo, st->rdo, st->rdo_cap, st->stream)) return rc;
+    }
     HIP_TRY(hipMemcpyAsync(out, st->out, ob, hipMemcpyDeviceToHost, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
     return ZAPTEX_OK;
@@ -486,7 +565,6 @@ int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, f
         astc_header(o, w, h);
         return host_encode(rgba, w, h, stride, 4, ZAPTEX_ASTC, o + 16);
     }
-    if (rdo > 0) return fail(ZAPTEX_E_UNSUPPORTED, "rdo > 0 (rate-distortion optimisation) is not implemented by libzaptex");
     if (fmt == ZAPTEX_BC4 || fmt == ZAPTEX_BC5) srgb = 0;   // channel data, never sRGB
     const int levels = level_count(w, h, mips);
     uint8_t hdr[148];   // the file can be shorter than 148 bytes (a 1x1 BC1 file is 136)
@@ -520,6 +598,18 @@ int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, f
         ws_need = std::max<uint64_t>(ws_need, fmt == ZAPTEX_BC7 ? ws_total(n, 0) : ws_total(0, n));
     }
     if (int rc = ensure_buf(st->ws, st->ws_cap, ws_need)) return rc;
+    if (rdo > 0) {   // the rate-distortion scratch of the largest group of levels, before anything is enqueued
+        size_t rdo_need = 0;
+        for (int l0 = 0; l0 < levels; l0 += ZT_INLINE_JOBS) {
+            size_t b = 0;
+            for (int l = l0; l < std::min(levels, l0 + ZT_INLINE_JOBS); l++) {
+                const zaptex_image im{ nullptr, nullptr, (size_t)lw[l] * 4, lw[l], lh[l], fmt };
+                b += rdo_scratch_image(im);
+            }
+            rdo_need = std::max(rdo_need, b);
+        }
+        if (int rc = ensure_buf(st->rdo, st->rdo_cap, rdo_need)) return rc;
+    }
     if (int rc = rows_to_device(st, st->in, rgba, stride, row, (size_t)h)) return rc;
     std::vector<void *> lvl;
     for (int l = 1; l < levels; l++) lvl.push_back((uint8_t *)st->mip + moff[l - 1]);
@@ -530,6 +620,7 @@ int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, f
         for (int l = l0; l < std::min(levels, l0 + ZT_INLINE_JOBS); l++, n++)
             im[n] = zaptex_image{ l ? lvl[l - 1] : st->in, (uint8_t *)st->out + ooff[l], (size_t)lw[l] * 4, lw[l], lh[l], fmt };
         if (int rc = enqueue(im, n, st->ws, st->ws_cap, st->stream)) return rc;
+        if (rdo > 0) if (int rc = enqueue_rdo(im, n, rdo, st->rdo, st->rdo_cap, st->stream)) return rc;
     }
     size_t pos = hn;
     for (int l = 0; l < levels; l++) {
@@ -543,9 +634,8 @@ int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, f
 
 int zaptex_bc_encode(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, void *out) {
     if (!is_bcn(fmt)) return fail(ZAPTEX_E_ARG, "unknown zap_bc_format %d", fmt);
-    if (rdo > 0) return fail(ZAPTEX_E_UNSUPPORTED, "rdo > 0 (rate-distortion optimisation) is not implemented by libzaptex");
     if (w < 1 || h < 1 || !rgba || !out || stride < (size_t)w * 4) return fail(ZAPTEX_E_ARG, "bad arguments");
-    return host_encode(rgba, w, h, stride, 4, fmt, out);
+    return host_encode(rgba, w, h, stride, 4, fmt, out, rdo);
 }
 
 int zaptex_bc6h_encode(const float *rgba, int w, int h, size_t stride, void *out) {
@@ -683,6 +773,21 @@ int zaptex_encode_dev(const zaptex_image *imgs, int n, void *workspace, size_t w
         if (int rc = ensure_ready(st)) return rc;
     }
     return enqueue(imgs, n, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+size_t zaptex_rdo_workspace_size(const zaptex_image *imgs, int n) {
+    return align_up(zaptex_workspace_size(imgs, n), 256) + rdo_scratch_bytes(imgs, n);
+}
+
+int zaptex_encode_rdo_dev(const zaptex_image *imgs, int n, float rdo, void *workspace, size_t ws_bytes, void *stream) {
+    if (!imgs || n < 1 || n > ZT_INLINE_JOBS) return fail(ZAPTEX_E_ARG, "image count %d outside 1..%d", n, ZT_INLINE_JOBS);
+    for (int i = 0; i < n; i++)
+        if (!is_bcn(imgs[i].fmt)) return fail(ZAPTEX_E_ARG, "rdo applies to BC1/BC3/BC4/BC5/BC7 only (image %d)", i);
+    const size_t plain_ws = align_up(zaptex_workspace_size(imgs, n), 256);
+    if (rdo > 0 && ws_bytes < zaptex_rdo_workspace_size(imgs, n))
+        return fail(ZAPTEX_E_ARG, "workspace %zu < %zu bytes (zaptex_rdo_workspace_size)", ws_bytes, zaptex_rdo_workspace_size(imgs, n));
+    if (int rc = zaptex_encode_dev(imgs, n, workspace, ws_bytes, stream)) return rc;
+    return enqueue_rdo(imgs, n, rdo, (uint8_t *)workspace + plain_ws, ws_bytes - plain_ws, (hipStream_t)stream);
 }
 
 // zaptex_encode_dev with the library's per-device scratch array as its workspace: no allocation of any kind
@@ -1532,6 +1637,16 @@ int zaptex_set_option(int key, int value) {
         g_m1k = value;
         return ZAPTEX_OK;
     }
+    if (key == ZAPTEX_OPT_RDO_SEGMENT) {
+        if (value != 0 && (value < 16 || value > (1 << 24))) return fail(ZAPTEX_E_ARG, "rdo segment must be 0 (automatic) or 16..2^24 blocks");
+        g_rdo_seg = value;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_RDO_WARMUP) {
+        if (value < 0 || value > (1 << 24)) return fail(ZAPTEX_E_ARG, "rdo warm-up must be 0..2^24 blocks");
+        g_rdo_warm = value;
+        return ZAPTEX_OK;
+    }
     if (key == ZAPTEX_OPT_CLIMB_REFILL) {
         if (value < 1 || value > 64) return fail(ZAPTEX_E_ARG, "refill threshold must be 1..64");
         g_climb_refill = value;
@@ -1540,7 +1655,8 @@ int zaptex_set_option(int key, int value) {
     return fail(ZAPTEX_E_ARG, "unknown option %d", key);
 }
 int zaptex_get_option(int key) {
-    return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : key == ZAPTEX_OPT_CLIMB_REFILL ? g_climb_refill : -1;
+    return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : key == ZAPTEX_OPT_CLIMB_REFILL ? g_climb_refill
+         : key == ZAPTEX_OPT_RDO_SEGMENT ? g_rdo_seg : key == ZAPTEX_OPT_RDO_WARMUP ? g_rdo_warm : -1;
 }
 
 int zaptex_profile_collect(void) {

@@ -56,6 +56,35 @@ struct zt_gemm_args {
     int M, N, K, relu, accumulate, pad;
 };
 
+// rate-distortion streams (zt_rdo.hip): one per image, or two for BC3 / BC5 (one per 8-byte half).  A stream is cut
+// into segments of `seg` blocks walked speculatively in parallel (zt_rdo_spec), then checked and repaired in order
+// (zt_rdo_fix).
+enum { ZT_RDO_BC1 = 0, ZT_RDO_BC4 = 1, ZT_RDO_BC7 = 2 };
+struct zt_rdo_state {           // the state a segment starts from: its 16 predecessors' bytes and the match distances
+    uint4 ring[16];
+    uint32_t dist, pad[3];
+};
+struct zt_rdo_stream {
+    const uint8_t *src;         // RGBA8 rows
+    uint8_t *out;               // the image's blocks (final)
+    const uint8_t *plain;       // a copy of the image's plain encoding, laid out like out
+    uint16_t *dists;            // per block: the distances (d0 | d1 << 8) its speculative decision left
+    zt_rdo_state *states;       // per segment 1..: the state it assumed
+    uint32_t *seg_end;          // per segment: the distances it left
+    uint4 *cand;                // per block and lane 0..48: the candidate bytes of the speculative decision
+    int *cand_err;              //   and its SSE (-1: not a valid candidate)
+    uint64_t stride;
+    int w, h, bw, bh;
+    int bs, off;                // block stride (8 or 16) and this stream's byte offset in a block (0 or 8)
+    int kind, chan, force4;     // chan: BC4's channel; force4: BC1 half of BC3 (4-colour palette always)
+    uint32_t seg, warm, nseg;   // segment length (>= 16), warm-up blocks, segments
+};
+#define ZT_RDO_STREAMS 20
+#define ZT_RDO_LANES 49         // candidates per block: 48 from the previous 16 blocks, and the plain block
+struct zt_rdo_streams { zt_rdo_stream s[ZT_RDO_STREAMS]; };
+extern "C" __global__ void zt_rdo_spec(zt_rdo_streams S, float lambda);
+extern "C" __global__ void zt_rdo_fix(zt_rdo_streams S, float lambda);
+
 // destination levels of a fused mip launch (packed rows, 4 bytes per pixel)
 // one texture's level in a batched mip launch: source level (w x h, `stride` bytes per row) -> destination rows
 // packed at (w/2) * 4 bytes; `first` = index of its first destination pixel among all jobs of the launch
