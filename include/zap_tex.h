@@ -3,8 +3,8 @@
  *
  * Same types, names and signatures as the reference header (zap_tex.h:4-23, README.md:256-268), so callers
  * recompile against this include path with no source edits; outputs are byte-identical to the reference
- * encoder at rdo = 0.  The functions stay `void` like the reference's.  They never fall back to the CPU:
- * if the GPU call fails (no device, rdo > 0, bad arguments) the handler zap_tex_on_error points to is
+ * encoder, rate-distortion mode (rdo > 0) included.  The functions stay `void` like the reference's.  They never fall
+ * back to the CPU: if the GPU call fails (no device, bad arguments) the handler zap_tex_on_error points to is
  * called with the status and message; the default handler prints them and aborts.
  */
 #ifndef ZAP_TEX_H
