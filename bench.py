@@ -393,6 +393,8 @@ def main() -> int:
     ap.add_argument("--no-configs", action="store_true", help="skip the BC1 1024^2 / BC6H 2048^2 / ASTC 2048^2 legs")
     ap.add_argument("--no-host", action="store_true", help="skip the host-entry (PCIe-inclusive) leg")
     ap.add_argument("--batch-only", action="store_true", help="only build and time the 512-texture batch (for profiling)")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="only the BC1 1024^2 / BC6H 2048^2 / ASTC 2048^2 legs, --steps encodes each (for profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--counters-child", metavar="C2_NPY", help=argparse.SUPPRESS)
     ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes (traffic, VALU instructions)")
@@ -425,6 +427,11 @@ def main() -> int:
     if args.batch_only:
         print(json.dumps(batch_leg(lib, args.steps)), flush=True)
         return 0
+    if args.configs_only:
+        golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+        res = configs_leg(lib, golden, config_images(), args.steps)
+        print(json.dumps(res), flush=True)
+        return 0 if all(r["byte_exact_vs_reference"] for r in res.values()) else 1
     shapes_c2 = autotune.apply(lib, "c2")
     golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
     ops = json.load(open(os.path.join(ROOT, "tests", "golden", "ops.json")))["c2_bc7_chain"]

@@ -44,24 +44,30 @@ template <int CEM> ZT_DEV void endpoints(const int *v, int e0[4], int e1[4]) {
 // endpoint quantisation looks them up per lane)
 struct AstcLds { uint8_t q[8][256]; uint8_t unq[8][256]; };
 
-// float endpoints -> ISE values and their unquantised values (zap_tex.h:1183-1194)
-template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int *ise, int *uv, const AstcLds &T) {
+// a trial's colour endpoint values, one byte each (value v of partition k at byte k * NV + v): the ISE values in
+// q, their unquantised 8-bit values in u
+struct Ends { uint32_t q[3], u[3]; };
+ZT_DEV int byte_at(const uint32_t *a, int i) { return (int)((a[i >> 2] >> (8 * (i & 3))) & 255u); }
+ZT_DEV void byte_put(uint32_t *a, int i, int v) { a[i >> 2] = (a[i >> 2] & ~(255u << (8 * (i & 3)))) | (uint32_t)v << (8 * (i & 3)); }
+
+// float endpoints -> partition k's ISE values and their unquantised values (zap_tex.h:1183-1194)
+template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], Ends &E, int k, const AstcLds &T) {
     constexpr int NV = AC<K>::CEM == 8 ? 6 : 8;
     float s0 = lo[0] + lo[1] + lo[2], s1 = hi[0] + hi[1] + hi[2];
     const bool keep = s1 >= s0;
+    int ise[NV], uv[NV];
 #pragma unroll
-    for (int k = 0; k < NV / 2; k++) {
-        float a = keep ? lo[k] : hi[k], b = keep ? hi[k] : lo[k];
+    for (int c = 0; c < NV / 2; c++) {
+        float a = keep ? lo[c] : hi[c], b = keep ? hi[c] : lo[c];
         int x0 = clampi(cvt_i32(a + 0.5f), 0, 255), x1 = clampi(cvt_i32(b + 0.5f), 0, 255);
-        ise[2 * k] = T.q[K][x0]; ise[2 * k + 1] = T.q[K][x1];
-        uv[2 * k] = T.unq[K][ise[2 * k]]; uv[2 * k + 1] = T.unq[K][ise[2 * k + 1]];
+        ise[2 * c] = T.q[K][x0]; ise[2 * c + 1] = T.q[K][x1];
+        uv[2 * c] = T.unq[K][ise[2 * c]]; uv[2 * c + 1] = T.unq[K][ise[2 * c + 1]];
     }
-    if (uv[1] + uv[3] + uv[5] < uv[0] + uv[2] + uv[4]) {
+    const int sw = uv[1] + uv[3] + uv[5] < uv[0] + uv[2] + uv[4];
 #pragma unroll
-        for (int k = 0; k < NV / 2; k++) {
-            int t = ise[2 * k]; ise[2 * k] = ise[2 * k + 1]; ise[2 * k + 1] = t;
-            t = uv[2 * k]; uv[2 * k] = uv[2 * k + 1]; uv[2 * k + 1] = t;
-        }
+    for (int v = 0; v < NV; v++) {
+        byte_put(E.q, k * NV + v, sw ? ise[v ^ 1] : ise[v]);
+        byte_put(E.u, k * NV + v, sw ? uv[v ^ 1] : uv[v]);
     }
 }
 
@@ -70,13 +76,18 @@ template <int K> ZT_DEV void quantise(const float lo[4], const float hi[4], int 
 // SSE, lowest j on ties; IB index bits hold up to 32 entries (5-bit weights), keys stay below 2^25.  The loop runs
 // over palette entries outside and texels inside, so only the 16 running maxima stay live.  The palette's unused
 // alpha byte is 0, so texel alpha drops out of dot(p, pal_j); spp = sum |p|^2 over the used channels.
-template <int K> ZT_DEV int fit_weights(const uint32_t px[16], int spp, const int *uv, uint32_t pm, uint32_t w[4]) {
+template <int K> ZT_DEV int fit_weights(const uint32_t px[16], int spp, const Ends &E, uint32_t pm, uint32_t w[4]) {
     constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, L = 1 << WB, NV = CEM == 8 ? 6 : 8;
     constexpr int IB = WB > 4 ? WB : 4, IM = (1 << IB) - 1;
     constexpr uint32_t CM = CEM == 12 ? 0xFFFFFFFFu : 0x00FFFFFFu;
     int e0[P][4], e1[P][4];
 #pragma unroll
-    for (int k = 0; k < P; k++) endpoints<CEM>(uv + k * NV, e0[k], e1[k]);
+    for (int k = 0; k < P; k++) {
+        int v[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) v[i] = byte_at(E.u, k * NV + i);
+        endpoints<CEM>(v, e0[k], e1[k]);
+    }
     int best[16];
 #pragma unroll
     for (int j = 0; j < L; j++) {
@@ -109,63 +120,87 @@ template <int K> ZT_DEV int fit_weights(const uint32_t px[16], int spp, const in
     return spp - sk;
 }
 ZT_DEV int wbyte(const uint32_t w[4], int i) { return (int)((w[i >> 2] >> (8 * (i & 3))) & 255u); }
+// four WB-bit weight indices (one per byte) -> their values on the 0..64 scale, per byte: bit replication to 6 bits,
+// then +1 above 32 (every byte stays below 128, so no step carries into the next byte)
+template <int WB> ZT_DEV uint32_t wvals4(uint32_t x) {
+    uint32_t v;
+    if (WB == 2) v = x * 21u;
+    else if (WB == 3) v = x * 9u;
+    else if (WB == 4) v = x << 2 | ((x >> 2) & 0x03030303u);
+    else v = x << 1 | ((x >> 4) & 0x01010101u);
+    return v + (((v + 0x5F5F5F5Fu) >> 7) & 0x01010101u);
+}
+// channel planes of pixels 4r..4r+3: byte j of pl[c] = channel c of pixel 4r + j
+ZT_DEV void planes(const uint32_t px[16], int r, uint32_t pl[4]) {
+    const uint32_t a = px[4 * r], b = px[4 * r + 1], c = px[4 * r + 2], d = px[4 * r + 3];
+    const uint32_t ab01 = perm(b, a, 0x05010400u), ab23 = perm(b, a, 0x07030602u);
+    const uint32_t cd01 = perm(d, c, 0x05010400u), cd23 = perm(d, c, 0x07030602u);
+    pl[0] = perm(cd01, ab01, 0x05040100u); pl[1] = perm(cd01, ab01, 0x07060302u);
+    pl[2] = perm(cd23, ab23, 0x05040100u); pl[3] = perm(cd23, ab23, 0x07060302u);
+}
 
-// one trial configuration (zap_tex.h:1212-1244)
-template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12], uint32_t w[4], const AstcLds &T) {
-    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NC = CEM == 12 ? 4 : 3, NV = CEM == 8 ? 6 : 8;
+// one trial configuration (zap_tex.h:1212-1244).  The least-squares refit's sums are exact in fp32: with f = wv/64
+// and g = 1 - f for a weight value wv in 0..64, each g*g, g*f, f*f is a multiple of 1/4096 and each g*p, f*p a
+// multiple of 1/64, and every partial sum stays below 2^24 of those units, so summing in any order gives the
+// reference's floats.  They are summed here as integers, four texels per dot4 over byte planes of the pixels.
+template <int K> ZT_DEV int trial(uint32_t px[16], uint32_t pm, Ends &E, uint32_t w[4], const AstcLds &T) {
+    constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NC = CEM == 12 ? 4 : 3;
     constexpr uint32_t CM = NC == 4 ? 0xFFFFFFFFu : 0x00FFFFFFu;
     int spp = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) spp += (int)dot4(px[i] & CM, px[i] & CM);
-    int uv[12];
+#pragma unroll
+    for (int q = 0; q < 3; q++) { E.q[q] = 0; E.u[q] = 0; }
 #pragma unroll
     for (int k = 0; k < P; k++) {
         float lo[4], hi[4];
-        if (P == 1) pca<NC, true>(px, 0xFFFFu, 16, lo, hi);
+        if (P == 1) pca<NC, true, true>(px, 0xFFFFu, 16, lo, hi);
         else {
             uint32_t m = k ? pm : (~pm & 0xFFFFu);
             int cnt = __popc(m);
             if (!cnt) { m = 1; cnt = 1; }
-            pca<NC, false>(px, m, cnt, lo, hi);
+            pca<NC, false, true>(px, m, cnt, lo, hi);
         }
-        quantise<K>(lo, hi, ise + k * NV, uv + k * NV, T);
+        quantise<K>(lo, hi, E, k, T);
     }
-    int err = fit_weights<K>(px, spp, uv, pm, w);
+    int err = fit_weights<K>(px, spp, E, pm, w);
 #pragma unroll 1
     for (int it = 0; it < 3 && err > 0; it++) {
-        int ti[12], tuv[12];
+        Ends t = E;
         uint32_t tw[4];
-#pragma unroll
-        for (int v = 0; v < NV * P; v++) { ti[v] = ise[v]; tuv[v] = uv[v]; }
 #pragma unroll
         for (int k = 0; k < P; k++) {
             const uint32_t m = P == 1 ? 0xFFFFu : k ? pm : (~pm & 0xFFFFu);
-            float aa = 0.0f, ab = 0.0f, bb = 0.0f, xa[NC], xb[NC];
+            uint32_t s00 = 0, s01 = 0, s11 = 0, s0[NC], s1[NC];   // sums of g*g, g*f, f*f, g*p, f*p, scaled to integers
 #pragma unroll
-            for (int c = 0; c < NC; c++) { xa[c] = 0.0f; xb[c] = 0.0f; }
+            for (int c = 0; c < NC; c++) { s0[c] = 0; s1[c] = 0; }
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                if (!((m >> i) & 1)) continue;
-                float f = (float)zt_weight_value(wbyte(w, i), WB) / 64.0f, g = 1 - f;
-                aa += g * g; ab += g * f; bb += f * f;
+            for (int r = 0; r < 4; r++) {
+                // texels 4r..4r+3: 64 f and 64 g per byte, zero outside the partition
+                const uint32_t bm = P == 1 ? 0xFFFFFFFFu : ((((m >> (4 * r)) & 15u) * 0x00204081u) & 0x01010101u) * 0xFFu;
+                const uint32_t wv = wvals4<WB>(w[r]), v1 = wv & bm, v0 = (0x40404040u - wv) & bm;
+                s00 = dot4acc(v0, v0, s00); s01 = dot4acc(v0, v1, s01); s11 = dot4acc(v1, v1, s11);
+                uint32_t pl[4];
+                planes(px, r, pl);
 #pragma unroll
-                for (int c = 0; c < NC; c++) { xa[c] += g * chanf(px[i], c); xb[c] += f * chanf(px[i], c); }
+                for (int c = 0; c < NC; c++) { s0[c] = dot4acc(v0, pl[c], s0[c]); s1[c] = dot4acc(v1, pl[c], s1[c]); }
             }
-            float det = aa * bb - ab * ab;
+            const float aa = (float)s00 / 4096.0f, ab = (float)s01 / 4096.0f, bb = (float)s11 / 4096.0f;
+            const float det = aa * bb - ab * ab;
             if (det < 1e-6f && det > -1e-6f) continue;
             float nlo[4], nhi[4];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                if (c < NC) { nlo[c] = (xa[c] * bb - xb[c] * ab) / det; nhi[c] = (xb[c] * aa - xa[c] * ab) / det; }
+                const float xa = (float)s0[c < NC ? c : 0] / 64.0f, xb = (float)s1[c < NC ? c : 0] / 64.0f;
+                if (c < NC) { nlo[c] = (xa * bb - xb * ab) / det; nhi[c] = (xb * aa - xa * ab) / det; }
                 else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
             }
-            quantise<K>(nlo, nhi, ti + k * NV, tuv + k * NV, T);
+            quantise<K>(nlo, nhi, t, k, T);
         }
-        int e = fit_weights<K>(px, spp, tuv, pm, tw);
+        int e = fit_weights<K>(px, spp, t, pm, tw);
         if (e >= err) break;
         err = e;
-#pragma unroll
-        for (int v = 0; v < NV * P; v++) { ise[v] = ti[v]; uv[v] = tuv[v]; }
+        E = t;
 #pragma unroll
         for (int q = 0; q < 4; q++) w[q] = tw[q];
     }
@@ -173,7 +208,7 @@ template <int K> ZT_DEV int trial(const uint32_t px[16], uint32_t pm, int ise[12
 }
 
 // integer-sequence encode of N values (zap_tex.h:1080-1095)
-template <int K, int N> ZT_DEV void ise_put(Bits128 &b, int pos, const int *v) {
+template <int K, int N> ZT_DEV void ise_put(Bits128 &b, int pos, const uint32_t *v) {
     constexpr int TR = AC<K>::TRIT, QU = AC<K>::QUINT, BITS = AC<K>::BITS;
     constexpr int GS = TR ? 5 : QU ? 3 : 1, BASE = TR ? 3 : 5;
     const int xb[5] = { TR ? 2 : 3, 2, TR ? 1 : 2, 2, 1 };
@@ -184,18 +219,18 @@ template <int K, int N> ZT_DEV void ise_put(Bits128 &b, int pos, const int *v) {
         if (GS > 1) {
             int id = 0;
 #pragma unroll
-            for (int k = GS - 1; k >= 0; k--) id = id * BASE + (k < cnt ? v[g + k] >> BITS : 0);
+            for (int k = GS - 1; k >= 0; k--) id = id * BASE + (k < cnt ? byte_at(v, g + k) >> BITS : 0);
             X = TR ? g_astc.tenc[id] : g_astc.qenc[id];
         }
 #pragma unroll
         for (int k = 0; k < cnt; k++) {
-            if (BITS) { b.put(pos, (uint64_t)(v[g + k] & ((1 << BITS) - 1)), BITS); pos += BITS; }
+            if (BITS) { b.put(pos, (uint64_t)(byte_at(v, g + k) & ((1 << BITS) - 1)), BITS); pos += BITS; }
             if (GS > 1) { b.put(pos, (uint64_t)((X >> xp) & ((1 << xb[k]) - 1)), xb[k]); pos += xb[k]; xp += xb[k]; }
         }
     }
 }
 
-template <int K> ZT_DEV void pack(int seed, const int *ise, const uint32_t w[4], uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:1246-1260
+template <int K> ZT_DEV void pack(int seed, const Ends &E, const uint32_t w[4], uint64_t &olo, uint64_t &ohi) {  // zap_tex.h:1246-1260
     constexpr int CEM = AC<K>::CEM, WB = AC<K>::WB, P = AC<K>::PARTS, NV = CEM == 8 ? 6 : 8;
     constexpr int R = WB == 2 ? 4 : WB == 3 ? 7 : WB == 4 ? 4 : 7, H = WB >= 4 ? 1 : 0;
     constexpr uint32_t mode = (R >> 1 & 1) | (R >> 2 & 1) << 1 | (R & 1) << 4 | 2 << 5 | H << 9;
@@ -203,7 +238,7 @@ template <int K> ZT_DEV void pack(int seed, const int *ise, const uint32_t w[4],
     int pos;
     if (P == 2) { b.put(13, (uint64_t)seed, 10); b.put(23, (uint64_t)CEM << 2, 6); pos = 29; }
     else { b.put(13, (uint64_t)CEM, 4); pos = 17; }
-    ise_put<K, NV * P>(b, pos, ise);
+    ise_put<K, NV * P>(b, pos, E.q);
     Bits128 wb{ 0, 0 };
 #pragma unroll
     for (int i = 0; i < 16; i++) wb.put(i * WB, (uint64_t)wbyte(w, i), WB);
@@ -357,39 +392,46 @@ ZT_DEV void list_append(const zt_astc_ws &W, int k, uint32_t *list, bool pred, u
     const int slot = wave_append(pred, c);
     if (slot >= 0) list[(uint32_t)((c - list_counts(W, k)) / ZT_SHARD_STRIDE) * W.cap + (uint32_t)slot] = g;
 }
-// configuration K's trial of list entry e (opaque list for K < 3, translucent otherwise)
-template <int K> ZT_DEV void single_item(const zt_astc_ws &W, const AstcLds &T, uint32_t e) {
+// block of opaque-list entry e
+ZT_DEV uint32_t opaque_block(const zt_astc_ws &W, uint32_t e) { return W.lo[shard_slot(list_counts(W, 0), W.cap, e)]; }
+// configuration K's trial of list entry e (opaque list for K < 3, translucent otherwise; n0 opaque entries)
+template <int K> ZT_DEV void single_item(const zt_astc_ws &W, const AstcLds &T, uint32_t e, uint32_t n0) {
     const uint32_t g = (K < 3 ? W.lo : W.lt)[shard_slot(list_counts(W, K < 3 ? 0 : 1), W.cap, e)];
     uint32_t px[16];
     load_px(W, g, px);
-    int ise[12];
+    Ends E;
     uint32_t w[4];
     uint64_t lo, hi;
-    const int err = trial<K>(px, 0, ise, w, T);
-    pack<K>(0, ise, w, lo, hi);
-    W.err1[(uint64_t)g * 3 + K % 3] = err;
-    W.blk1[(uint64_t)g * 3 + K % 3] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    const int err = trial<K>(px, 0, E, w, T);
+    pack<K>(0, E, w, lo, hi);
+    const uint64_t o = (uint64_t)(K % 3) * W.n + (K < 3 ? e : n0 + e);
+    W.err1[o] = err;
+    W.blk1[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
 // configuration K's trial of pattern candidate i (= listed block i / 4, its pattern i % 4)
 template <int K> ZT_DEV void pair_item(const zt_astc_ws &W, const AstcLds &T, uint32_t i) {
     const uint32_t c = W.cand[i], o = (i >> 2) * 8 + (i & 3) * 2 + (K - 6);
     if (c == 0xFFFFFFFFu) { W.err2[o] = 0x7FFFFFFF; return; }
     uint32_t px[16];
-    load_px(W, W.lp[shard_slot(list_counts(W, 2), W.cap, i >> 2)], px);
-    int ise[12];
+    load_px(W, opaque_block(W, W.lp[shard_slot(list_counts(W, 2), W.cap, i >> 2)]), px);
+    Ends E;
     uint32_t w[4];
     uint64_t lo, hi;
-    const int err = trial<K>(px, c & 0xFFFFu, ise, w, T);
-    pack<K>((int)(c >> 16), ise, w, lo, hi);
+    const int err = trial<K>(px, c & 0xFFFFu, E, w, T);
+    pack<K>((int)(c >> 16), E, w, lo, hi);
     W.err2[o] = err;
     W.blk2[o] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
-// the 1-partition winner: first smallest error in configuration order
-ZT_DEV int first_best(const zt_astc_ws &W, uint32_t g, int &slot) {
-    int best = W.err1[(uint64_t)g * 3];
-    slot = 0;
+// the 1-partition winner of entry u: first smallest error in configuration order; returns its error, `at` its result
+ZT_DEV int first_best(const zt_astc_ws &W, uint32_t u, uint64_t &at) {
+    int best = W.err1[u];
+    at = u;
 #pragma unroll
-    for (int k = 1; k < 3; k++) { const int e = W.err1[(uint64_t)g * 3 + k]; if (e < best) { best = e; slot = k; } }
+    for (int k = 1; k < 3; k++) {
+        const uint64_t o = (uint64_t)k * W.n + u;
+        const int e = W.err1[o];
+        if (e < best) { best = e; at = o; }
+    }
     return best;
 }
 ZT_DEV void store_block(const zt_jobs &jobs, uint32_t g, uint4 v) {
@@ -435,34 +477,31 @@ extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_single(zt_astc_ws W
         const bool op = i < 3 * n0;
         const uint32_t j = op ? i : i - 3 * n0, m = op ? n0 : n1, c = j / m, e = j - c * m;
         switch (op ? c : 3 + c) {
-        case 0: single_item<0>(W, T, e); break;
-        case 1: single_item<1>(W, T, e); break;
-        case 2: single_item<2>(W, T, e); break;
-        case 3: single_item<3>(W, T, e); break;
-        case 4: single_item<4>(W, T, e); break;
-        default: single_item<5>(W, T, e); break;
+        case 0: single_item<0>(W, T, e, n0); break;
+        case 1: single_item<1>(W, T, e, n0); break;
+        case 2: single_item<2>(W, T, e, n0); break;
+        case 3: single_item<3>(W, T, e, n0); break;
+        case 4: single_item<4>(W, T, e, n0); break;
+        default: single_item<5>(W, T, e, n0); break;
         }
     }
 }
 
+// over entries u: the opaque list's entries, then the translucent list's
 extern "C" __global__ void __launch_bounds__(256) zt_astc_pick1(zt_jobs jobs, uint32_t total, zt_astc_ws W) {
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t g0 = blockIdx.x * blockDim.x; g0 < total; g0 += step) {
-        const uint32_t g = g0 + threadIdx.x;
+    const uint32_t step = gridDim.x * blockDim.x, n0 = shard_total(list_counts(W, 0));
+    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < total; u0 += step) {
+        const uint32_t u = u0 + threadIdx.x;
         bool search = false;
-        if (g < total) {
-            int slot;
-            const int best = first_best(W, g, slot);
-            uint32_t amin = 255;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const uint4 v = W.px[(uint64_t)g * 4 + r];
-                amin = min(min(amin, v.x >> 24), min(v.y >> 24, min(v.z >> 24, v.w >> 24)));
-            }
-            search = amin == 255 && best > 16 * 3 * 4;   // opaque and not good enough yet (zap_tex.h:1327)
-            if (!search) store_block(jobs, g, W.blk1[(uint64_t)g * 3 + slot]);
+        if (u < total) {
+            const bool opaque = u < n0;
+            const uint32_t g = opaque ? W.lo[shard_slot(list_counts(W, 0), W.cap, u)] : W.lt[shard_slot(list_counts(W, 1), W.cap, u - n0)];
+            uint64_t at;
+            const int best = first_best(W, u, at);
+            search = opaque && best > 16 * 3 * 4;   // opaque and not good enough yet (zap_tex.h:1327)
+            if (!search) store_block(jobs, g, W.blk1[at]);
         }
-        list_append(W, 2, W.lp, search, g, g0 + (threadIdx.x & ~63u));
+        list_append(W, 2, W.lp, search, u, u0 + (threadIdx.x & ~63u));
     }
 }
 
@@ -481,7 +520,7 @@ extern "C" __global__ void __launch_bounds__(256) zt_astc_search(zt_astc_ws W) {
         const bool valid = e < n;
         if (!__any(valid)) continue;   // whole idle waves skip the work (a wave's groups hold consecutive entries)
         uint32_t px[16];
-        load_px(W, valid ? W.lp[shard_slot(counts, W.cap, e)] : 0u, px);   // idle groups read block 0 and store nothing
+        load_px(W, valid ? opaque_block(W, W.lp[shard_slot(counts, W.cap, e)]) : 0u, px);   // idle groups read block 0, store nothing
         const uint32_t km = two_means(px);
         uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0xFFFFFFFFu, b3 = 0xFFFFFFFFu;   // ascending
 #pragma unroll 8
@@ -528,10 +567,10 @@ extern "C" __global__ void __launch_bounds__(256, 4) zt_astc_pair(zt_astc_ws W) 
 extern "C" __global__ void __launch_bounds__(256) zt_astc_pick2(zt_jobs jobs, zt_astc_ws W) {
     const uint32_t n = shard_total(list_counts(W, 2)), step = gridDim.x * blockDim.x;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
-        const uint32_t g = W.lp[shard_slot(list_counts(W, 2), W.cap, e)];
-        int slot;
-        int best = first_best(W, g, slot);
-        uint4 v = W.blk1[(uint64_t)g * 3 + slot];
+        const uint32_t u = W.lp[shard_slot(list_counts(W, 2), W.cap, e)], g = opaque_block(W, u);
+        uint64_t at;
+        int best = first_best(W, u, at);
+        uint4 v = W.blk1[at];
         for (int j = 0; j < 4; j++) {   // the reference's order: pattern j with configuration 6, then 7
             if (W.cand[(uint64_t)e * 4 + j] == 0xFFFFFFFFu) break;
 #pragma unroll

@@ -232,6 +232,7 @@ zt_astc_ws astc_ws_in(void *ws, uint64_t n) {
     W.cand = (uint32_t *)p; p += n * 16;
     const uint64_t r = astc_list_rows(n);
     W.cap = (uint32_t)(r / ZT_SHARDS);
+    W.n = (uint32_t)n;
     W.lo = (uint32_t *)p; p += r * 4;
     W.lt = (uint32_t *)p; p += r * 4;
     W.lp = (uint32_t *)p;

@@ -167,19 +167,22 @@ __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
 // ASTC 4x4 encode in stages (workspace regions, each sized for `total` blocks):
 //   classify   gather each block once (px), list opaque and translucent blocks
 //   single     every 1-partition trial in one launch, configuration-major items (K 0..2 over the opaque list,
-//              3..5 over the translucent one): error and packed block per (block, K % 3)
-//   pick1      the reference's first smallest error; blocks that go on to the 2-partition search are listed,
-//              the others are stored
+//              3..5 over the translucent one): error and packed block per (K % 3, list entry), so a wave's
+//              lanes write consecutive results
+//   pick1      over the list entries: the reference's first smallest error; opaque blocks that go on to the
+//              2-partition search are listed, the others are stored
 //   search     2-means split and the 4 stored patterns nearest to it, 16 lanes per listed block
 //   pair       every 2-partition trial in one launch: configurations 6 then 7 over (listed block, pattern) items
 //   pick2      the first smallest error over the 1-partition winner and the 8 pattern trials, stored
 struct zt_astc_ws {
     unsigned *counters;     // sharded list lengths (ZT_SHARDS x ZT_SHARD_STRIDE each): opaque, translucent, listed
     uint32_t cap;           // slots per shard of each list
-    uint32_t *lo, *lt, *lp; // block lists (opaque, translucent, listed for the 2-partition search)
+    uint32_t n;             // blocks
+    uint32_t *lo, *lt;      // block lists: opaque, translucent
+    uint32_t *lp;           // opaque-list entries listed for the 2-partition search
     uint4 *px;              // [block * 4]: 16 pixels
-    int *err1;              // [block * 3]
-    uint4 *blk1;            // [block * 3]
+    int *err1;              // [(configuration % 3) * n + u]: u = opaque-list entry, or n_opaque + translucent-list entry
+    uint4 *blk1;            // the same
     uint32_t *cand;         // [listed * 4]: pattern mask | seed << 16 (all ones: none)
     int *err2;              // [listed * 8]: pattern j, configuration 6 + k at 2j + k
     uint4 *blk2;            // [listed * 8]

@@ -4,10 +4,19 @@
 #include "zt_common.h"
 
 namespace {
+// Marks the 16 pixel registers as rewritten (an empty asm that claims to modify them).  The optimiser then cannot
+// reuse a byte -> float conversion made before this point after it, so each pass over the pixels converts again (one
+// VALU op each) instead of keeping 16 x NC floats live across passes.
+ZT_DEV void refresh16(uint32_t *p) {
+    asm volatile("" : "+v"(p[0]), "+v"(p[1]), "+v"(p[2]), "+v"(p[3]), "+v"(p[4]), "+v"(p[5]), "+v"(p[6]), "+v"(p[7]),
+                 "+v"(p[8]), "+v"(p[9]), "+v"(p[10]), "+v"(p[11]), "+v"(p[12]), "+v"(p[13]), "+v"(p[14]), "+v"(p[15]));
+}
+
 // `sums` (optional, FULL only): the integer channel sums.  Each term p/16 and each partial sum of the reference's
 // mean is then exact (multiples of 1/16 below 256), so the mean is exactly sum * (1/16).
-template <int NC, bool FULL>
-ZT_DEV void pca(const uint32_t px[16], uint32_t mask, int cnt, float lo[4], float hi[4], const int *sums = nullptr) {
+// REFRESH: refresh16 the (then non-const) pixels before the covariance and projection passes.
+template <int NC, bool FULL, bool REFRESH = false, typename PX>
+ZT_DEV void pca(PX *px, uint32_t mask, int cnt, float lo[4], float hi[4], const int *sums = nullptr) {
     float mean[NC], cv[NC][NC], ax[NC];
     const float fc = (float)cnt;
 #pragma unroll
@@ -27,6 +36,7 @@ ZT_DEV void pca(const uint32_t px[16], uint32_t mask, int cnt, float lo[4], floa
             for (int c = 0; c < NC; c++) mean[c] += FULL ? chanf(px[k], c) / 16.0f : chanf(px[k], c) / fc;
         }
     }
+    if constexpr (REFRESH) refresh16(px);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         if (!FULL && !((mask >> k) & 1)) continue;
@@ -56,6 +66,7 @@ ZT_DEV void pca(const uint32_t px[16], uint32_t mask, int cnt, float lo[4], floa
 #pragma unroll
         for (int a = 0; a < NC; a++) ax[a] = nv[a] / m;
     }
+    if constexpr (REFRESH) refresh16(px);
     float n2 = 0.0f, tl = 1e9f, th = -1e9f;
 #pragma unroll
     for (int a = 0; a < NC; a++) n2 += ax[a] * ax[a];
