@@ -262,12 +262,13 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t sn
 }
 
 // ---- PCA start, fit, at most 2 least-squares refinements (zap_tex.h:489-515)
-template <int NC, int N, int IB, bool SHARED, bool FULL>
-ZT_DEV int encode_subset(const uint32_t px[16], const ChanWords &T, uint32_t mask, int cnt, int q[2][4], int pb[2], uint64_t &idx) {
+// R: refresh the (then non-const) pixel registers between the PCA passes (pca's REFRESH)
+template <int NC, int N, int IB, bool SHARED, bool FULL, bool R = false, typename PX>
+ZT_DEV int encode_subset(PX *px, const ChanWords &T, uint32_t mask, int cnt, int q[2][4], int pb[2], uint64_t &idx) {
     int P[NC];
     chan_sums<NC, 0, FULL>(T, mask, P);
     float lo[4], hi[4];
-    pca<NC, FULL>(px, mask, cnt, lo, hi, P);
+    pca<NC, FULL, R>(px, mask, cnt, lo, hi, P);
     const int spp = subset_sq<NC, 0>(px, mask);
     const uint64_t snib = FULL ? ~0ull : subset_nibbles(mask);
     int err = fit_subset<NC, N, IB, SHARED>(px, mask, spp, snib, lo, hi, q, pb, idx);

@@ -43,12 +43,12 @@ ZT_DEV int fit_subset_plain(const uint32_t px[16], uint32_t mask, int spp, uint6
     idx = bits ^ snib;
     return spp - ((sk - nibble_sum(bits)) >> 4);
 }
-template <int NC, int N, int IB>
-ZT_DEV int encode_subset_plain(const uint32_t px[16], const ChanWords &T, uint32_t mask, int cnt, int q[2][4], uint64_t &idx) {
+template <int NC, int N, int IB, bool R = false, typename PX>
+ZT_DEV int encode_subset_plain(PX *px, const ChanWords &T, uint32_t mask, int cnt, int q[2][4], uint64_t &idx) {
     int P[NC];
     chan_sums<NC, 0, false>(T, mask, P);
     float lo[4], hi[4];
-    pca<NC, false>(px, mask, cnt, lo, hi);
+    pca<NC, false, R>(px, mask, cnt, lo, hi);
     const int spp = subset_sq<NC, 0>(px, mask);
     const uint64_t snib = subset_nibbles(mask);
     int err = fit_subset_plain<NC, N, IB>(px, mask, spp, snib, lo, hi, q, idx);
@@ -98,8 +98,9 @@ ZT_DEV uint64_t nibble_fill(uint32_t mask, int bits) {
     return r;
 }
 
-// modes 0, 1, 2, 3, 6, 7 for one partition: SSE, packed block
-template <int M> ZT_DEV int cand_subsets(const uint32_t px[16], int part, uint64_t &olo, uint64_t &ohi) {
+// modes 0, 1, 2, 3, 6, 7 for one partition: SSE, packed block.  R: refresh the pixel registers between the PCA
+// passes (fewer VGPRs: the search kernels run at higher occupancy; px must then be modifiable)
+template <int M, bool R = false, typename PX> ZT_DEV int cand_subsets(PX *px, int part, uint64_t &olo, uint64_t &ohi) {
     using T = B7<M>;
     constexpr int NC = T::AB ? 4 : 3, HASP = T::EPB || T::SPB, N = T::CB + HASP;
     int q[T::NS][2][4], pb[T::NS][2];
@@ -114,8 +115,8 @@ template <int M> ZT_DEV int cand_subsets(const uint32_t px[16], int part, uint64
         const int cnt = __popc(masks[s]);
         uint64_t idx;
         pb[s][0] = pb[s][1] = 0;
-        if (HASP) err += encode_subset<NC, N, T::IB, (bool)T::SPB, (T::NS == 1)>(px, CW, masks[s], cnt, q[s], pb[s], idx);
-        else err += encode_subset_plain<NC, N, T::IB>(px, CW, masks[s], cnt, q[s], idx);
+        if (HASP) err += encode_subset<NC, N, T::IB, (bool)T::SPB, (T::NS == 1), R>(px, CW, masks[s], cnt, q[s], pb[s], idx);
+        else err += encode_subset_plain<NC, N, T::IB, R>(px, CW, masks[s], cnt, q[s], idx);
         full |= idx;
     }
     if (NC == 3) {
@@ -228,7 +229,7 @@ ZT_DEV int part_nbit(const uint32_t r[16], const ChanWords &T, const int P[4], i
 }
 
 // modes 4 (ISB = index selector) and 5: rotation `rot`
-template <int M, int ISB> ZT_DEV int cand_rotation(const uint32_t px[16], int rot, uint64_t &olo, uint64_t &ohi) {
+template <int M, int ISB, bool R = false> ZT_DEV int cand_rotation(const uint32_t px[16], int rot, uint64_t &olo, uint64_t &ohi) {
     constexpr int CB = M == 4 ? 5 : 7, AB = M == 4 ? 6 : 8, IBC = (M == 4 && ISB) ? 3 : 2, IBA = (M == 4 && !ISB) ? 3 : 2;
     uint32_t r[16];
 #pragma unroll
@@ -240,7 +241,7 @@ template <int M, int ISB> ZT_DEV int cand_rotation(const uint32_t px[16], int ro
     chan_words(r, CW);
     int P[4];
     chan_sums<4, 0, true>(CW, 0xFFFFu, P);
-    pca<3, true>(r, 0xFFFFu, 16, lo, hi, P);
+    pca<3, true, R>(r, 0xFFFFu, 16, lo, hi, P);
     int err = part_nbit<3, 0, IBC>(r, CW, P, CB, lo, hi, cq, ci);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -296,10 +297,11 @@ namespace {
 // the slots of mode M handled by one launch (mode 4 is split by its index selector ISB: slots 2 r + ISB)
 template <int M, int ISB> constexpr int kSlots = M == 0 ? 16 : (M == 4 || M == 5) ? 4 : M == 6 ? 1 : 64;
 template <int M, int ISB> ZT_DEV int slot_of(int s) { return M == 4 ? 2 * s + ISB : s; }
-template <int M, int ISB> ZT_DEV int eval_mode(const uint32_t px[16], int slot, uint64_t &lo, uint64_t &hi) {
-    if constexpr (M == 4) return cand_rotation<4, ISB>(px, slot >> 1, lo, hi);
-    else if constexpr (M == 5) return cand_rotation<5, 0>(px, slot, lo, hi);
-    else return cand_subsets<M>(px, slot, lo, hi);
+// the search kernels' evaluation: pixel registers refreshed between PCA passes (px is the caller's own copy)
+template <int M, int ISB> ZT_DEV int eval_mode(uint32_t px[16], int slot, uint64_t &lo, uint64_t &hi) {
+    if constexpr (M == 4) return cand_rotation<4, ISB, true>(px, slot >> 1, lo, hi);
+    else if constexpr (M == 5) return cand_rotation<5, 0, true>(px, slot, lo, hi);
+    else return cand_subsets<M, true>(px, slot, lo, hi);
 }
 ZT_DEV void load_block(const uint4 *blocks, uint32_t g, uint32_t px[16]) {
     const uint4 *s = blocks + (uint64_t)g * 4;

@@ -1,11 +1,13 @@
 // zt_mlp.hip — the BC7 proposal network on gfx950: 64 -> 128 -> 128 -> 512 MLP (ReLU), BF16 weights and
-// activations on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, trained here with its own
+// activations on MFMA (gfx950's v_mfma_f32_16x16x32_bf16) with FP32 accumulation, trained here with its own
 // forward/backward/Adam kernels.  Input: the 4x4 RGBA8 block (64 values / 255); output: one logit per BC7
 // candidate slot mode*64 + slot, of which the 285 used slots are read.
 #include "zt_common.h"
 #include "zt_kernels.h"
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short s16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -15,6 +17,13 @@ ZT_DEV uint16_t to_bf16(float f) {  // round to nearest even (finite inputs)
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 ZT_DEV float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// One 16x16x32 MFMA step.  Its 32 K positions are used in one fixed order everywhere in this file: lane (r, q)
+// supplies k = 4q + 0..3 in elements 0..3 and k = 16 + 4q + 0..3 in elements 4..7 (of its 32-wide K chunk), so a
+// step's two halves are the operands two 16x16x16 steps (k = 4q + 0..3 of each 16) would take.  Every product
+// then lands in the same position in the GEMM and the fused proposal kernel, which keeps their sums identical.
+ZT_DEV bf16x8_t cat8(bf16x4_t lo, bf16x4_t hi) { return __builtin_bit_cast(bf16x8_t, (s16x8_t)__builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7)); }
+ZT_DEV f32x4_t mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 // candidate slot used by the exhaustive search (285 of 512)
 ZT_DEV bool cand_valid(int c) {
@@ -49,13 +58,13 @@ ZT_DEV bf16x4_t load_b4(const zt_gemm_args &g, int k, int n) {
 
 }  // namespace
 
-// C(m, n) = sum_k A(m, k) B(k, n): 64x64 tile per workgroup, 32x32 per wave (2x2 MFMA 16x16x16 tiles).
-// MFMA lane layout: A row = lane % 16, k = 4 (lane / 16) + v; B col = lane % 16, same k; D col = lane % 16,
+// C(m, n) = sum_k A(m, k) B(k, n): 64x64 tile per workgroup, 32x32 per wave (2x2 MFMA 16x16x32 tiles).
+// MFMA lane layout: A row = lane % 16, B col = lane % 16, k positions as in mfma32; D col = lane % 16,
 // row = 4 (lane / 16) + v.  blockIdx.z splits K (accumulate mode adds into Cf atomically).
 extern "C" __global__ void __launch_bounds__(256) zt_gemm_bf16(zt_gemm_args g) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m0 = blockIdx.y * 64 + (w >> 1) * 32, n0 = blockIdx.x * 64 + (w & 1) * 32;
-    const int kper = ((g.K + (int)gridDim.z - 1) / (int)gridDim.z + 15) / 16 * 16;
+    const int kper = ((g.K + (int)gridDim.z - 1) / (int)gridDim.z + 31) / 32 * 32;
     const int kb = blockIdx.z * kper, ke = min(g.K, kb + kper);
     const int r = lane & 15, q = lane >> 4;
     f32x4_t acc[2][2];
@@ -63,22 +72,25 @@ extern "C" __global__ void __launch_bounds__(256) zt_gemm_bf16(zt_gemm_args g) {
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
-    for (int k = kb; k < ke; k += 16) {
-        bf16x4_t a[2], b[2];
-        const int kk = k + 4 * q;
+    for (int k = kb; k < ke; k += 32) {
+        bf16x4_t a[2][2], b[2][2];   // [tile][half]: half h covers k + 16 h + 4q + 0..3
 #pragma unroll
-        for (int i = 0; i < 2; i++) a[i] = kk < ke ? load_a4(g, m0 + 16 * i + r, kk) : bf16x4_t{ 0, 0, 0, 0 };
+        for (int h = 0; h < 2; h++) {
+            const int kk = k + 16 * h + 4 * q;
 #pragma unroll
-        for (int j = 0; j < 2; j++) b[j] = kk < ke ? load_b4(g, kk, n0 + 16 * j + r) : bf16x4_t{ 0, 0, 0, 0 };
-        // zero the tail of a partial last K step on both operands
-        if (kk + 3 >= ke) {
+            for (int i = 0; i < 2; i++) a[i][h] = kk < ke ? load_a4(g, m0 + 16 * i + r, kk) : bf16x4_t{ 0, 0, 0, 0 };
 #pragma unroll
-            for (int v = 0; v < 4; v++) if (kk + v >= ke) { a[0][v] = a[1][v] = 0; b[0][v] = b[1][v] = 0; }
+            for (int j = 0; j < 2; j++) b[j][h] = kk < ke ? load_b4(g, kk, n0 + 16 * j + r) : bf16x4_t{ 0, 0, 0, 0 };
+            // zero the tail of a partial last K step on both operands
+            if (kk + 3 >= ke) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) if (kk + v >= ke) { a[0][h][v] = a[1][h][v] = 0; b[0][h][v] = b[1][h][v] = 0; }
+            }
         }
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 2; j++) acc[i][j] = mfma32(cat8(a[i][0], a[i][1]), cat8(b[j][0], b[j][1]), acc[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -295,11 +307,11 @@ extern "C" __global__ void zt_mlp_refresh(const float *p, int n, uint16_t *wbf, 
 
 // ---- inference: features, the three layers and the top-k proposals in one persistent kernel ----------------------
 // Each layer runs transposed, H^T = W X^T: the weights are the MFMA's A operand (from LDS), the activations its B
-// operand.  In the 16x16x16 layout the accumulator of output tile j (lane (r, q) holds neurons 16 j + 4 q + i of
-// block r) is then, after bias, ReLU and bf16 rounding, exactly the B operand of the next layer's K step j, so
-// activations never leave registers.  The products, their grouping into 16-term MFMA steps and the K order are
-// those of zt_gemm_bf16 (only the roles of A and B swap), and the epilogue is the same, so the logits are
-// bit-identical to the three GEMMs.
+// operand.  The accumulator of output tile j (lane (r, q) holds neurons 16 j + 4 q + i of block r) is then, after
+// bias, ReLU and bf16 rounding, exactly one half of the B operand of the next layer's 32-wide K step j / 2 (mfma32's
+// k order), so activations never leave registers.  The products, their positions in the 32-term MFMA steps and
+// the K order are those of zt_gemm_bf16 (only the roles of A and B swap), and the epilogue is the same, so the
+// logits are bit-identical to the three GEMMs.
 // The weights stay in LDS for the kernel's life in fragment order (each wave reads 64 consecutive 16-byte operand
 // pairs: no bank conflicts): W1, W2 and, of W3, only the rows the proposals read (the 285 candidate slots in
 // ascending order, then the risk slot; 2 zero rows pad to 18 MFMA tiles).  A wave takes 32 blocks per pass, two
@@ -319,8 +331,8 @@ ZT_DEV uint32_t bf16_pair(float a, float b) {
     const uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b);
     return perm(ub + 0x7FFFu + ((ub >> 16) & 1u), ua + 0x7FFFu + ((ua >> 16) & 1u), 0x07060302u);
 }
-// weights [rows x IN] (bf16, row-major) -> LDS fragments: for row tile j, K-step pair p and lane l, 16 bytes =
-// A operands of K steps 2p and 2p + 1 (row 16 j + l % 16, columns 16 (2p + s) + 4 (l / 16) + 0..3)
+// weights [rows x IN] (bf16, row-major) -> LDS fragments: for row tile j, 32-wide K step p and lane l, 16 bytes =
+// the A operand (row 16 j + l % 16, columns 32 p + 16 s + 4 (l / 16) + 0..3 for halves s = 0, 1: mfma32's k order)
 template <int IN>
 ZT_DEV void stage_fragments(uint4 *dst, const uint16_t *w, int rows_kept, int tid, int nthreads, bool w3) {
     constexpr int CP = IN / 32;
@@ -345,7 +357,8 @@ ZT_DEV bf16x4_t epilogue(f32x4_t d, const float *bias) {
     if (x3 < 0.0f) x3 = 0.0f;
     return bf4(bf16_pair(x0, x1), bf16_pair(x2, x3));
 }
-// one layer, NJ output row tiles over NK K steps, NC column tiles: out[ct][j] = epilogue(sum_k W[j][k] in[ct][k])
+// one layer, NJ output row tiles over NK 16-wide K halves (NK / 2 MFMA steps), NC column tiles:
+// out[ct][j] = epilogue(sum_k W[j][k] in[ct][k])
 template <int NJ, int NK, int NC>
 ZT_DEV void layer(const uint4 *Wf, const float *bias, const bf16x4_t (&in)[NC][NK], bf16x4_t (&out)[NC][NJ], int lane, int q) {
 #pragma unroll
@@ -355,11 +368,9 @@ ZT_DEV void layer(const uint4 *Wf, const float *bias, const bf16x4_t (&in)[NC][N
         for (int ct = 0; ct < NC; ct++) acc[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
         for (int p = 0; p < NK / 2; p++) {
-            const uint4 a = Wf[(j * (NK / 2) + p) * 64 + lane];
+            const bf16x8_t a = __builtin_bit_cast(bf16x8_t, Wf[(j * (NK / 2) + p) * 64 + lane]);
 #pragma unroll
-            for (int ct = 0; ct < NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), in[ct][2 * p], acc[ct], 0, 0, 0);
-#pragma unroll
-            for (int ct = 0; ct < NC; ct++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), in[ct][2 * p + 1], acc[ct], 0, 0, 0);
+            for (int ct = 0; ct < NC; ct++) acc[ct] = mfma32(a, cat8(in[ct][2 * p], in[ct][2 * p + 1]), acc[ct]);
         }
 #pragma unroll
         for (int ct = 0; ct < NC; ct++) out[ct][j] = epilogue(acc[ct], bias + 16 * j + 4 * q);
@@ -373,11 +384,9 @@ ZT_DEV void logits_tile(const uint4 *W3f, const float *B3s, const bf16x4_t (&h2)
     for (int ct = 0; ct < NC; ct++) z[ct] = f32x4_t{ 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
     for (int p = 0; p < 4; p++) {
-        const uint4 a = W3f[(t * 4 + p) * 64 + lane];
+        const bf16x8_t a = __builtin_bit_cast(bf16x8_t, W3f[(t * 4 + p) * 64 + lane]);
 #pragma unroll
-        for (int ct = 0; ct < NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.x, a.y), h2[ct][2 * p], z[ct], 0, 0, 0);
-#pragma unroll
-        for (int ct = 0; ct < NC; ct++) z[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf4(a.z, a.w), h2[ct][2 * p + 1], z[ct], 0, 0, 0);
+        for (int ct = 0; ct < NC; ct++) z[ct] = mfma32(a, cat8(h2[ct][2 * p], h2[ct][2 * p + 1]), z[ct]);
     }
     const float4 b = *(const float4 *)(B3s + 16 * t + 4 * q);
 #pragma unroll
@@ -430,7 +439,7 @@ ZT_DEV void stage_network(PropLds &L, const uint16_t *w, const float *b1, const 
     for (int u = tid; u < ZT_PR_U; u += nth) { const int id = prop_slot(u); L.B3[u] = id < 0 ? 0.0f : b3[id]; }
     __syncthreads();
 }
-// layer 1's B operands (K steps 0..3) for the 16 blocks of a column tile: lane (r, q) holds pixels q, 4 + q, 8 + q,
+// layer 1's B operands (16-wide K halves 0..3) for the 16 blocks of a column tile: lane (r, q) holds pixels q, 4 + q, 8 + q,
 // 12 + q of block `row` (zeros when !valid); the block's channel sums and ranges combine its four lanes
 ZT_DEV void tile_features(const uint4 *blocks, uint32_t row, bool valid, int q, int mode, bf16x4_t (&x)[4]) {
     uint32_t px[4];
@@ -547,6 +556,8 @@ ZT_DEV void propose_wide_body(const zt_wide_args &P) {
             logits_tile<1>(L.W3f, L.B3, h2, t, lane, q, v);
 #pragma unroll
             for (int i = 0; i < 4; i++) key[t][i] = (t == ZT_PR_U / 16 - 1 && 4 * q + i >= 285 - 16 * t) ? ord_key(-INFINITY) : ord_key(v[0][i]);
+            // one tile's LDS fragments at a time: hoisting all 18 tiles' loads ahead of their MFMAs spills
+            __builtin_amdgcn_sched_barrier(0);
         }
         // bisection: T = the largest key value with at least K keys >= it (the K-th largest key of the block)
         uint32_t T = 0;
