@@ -2,11 +2,13 @@
  *
  * Plain C types only; every function returns a status (ZAPTEX_OK = 0) and never falls back to the CPU:
  * without a usable HIP device the call fails with ZAPTEX_E_DEVICE and zaptex_last_error() says why.
- * Output bytes equal the reference encoder's (WatchDogStudios/zap 1.3.0, zap_tex.h) at rdo = 0.
+ * Output bytes equal the reference encoder's (WatchDogStudios/zap 1.3.0, zap_tex.h), rdo > 0 included.
  *
  * Two families of entry points:
  *   host  — pointers are ordinary (pageable) host memory, exactly like zap_tex.h; the call is synchronous.
- *           Each replaces the zap_tex.h function cited beside it.  Thread-safe (calls are serialised).
+ *           Each replaces the zap_tex.h function cited beside it.  Thread-safe: concurrent calls run side by
+ *           side on the device (up to 8 per device, each with its own stream and staging buffers; more wait).
+ *           Any size the kernels can index: sides up to 2^30, fewer than 2^31 blocks.
  *   _dev  — pointers are device memory on the current HIP device; work is enqueued on `stream`
  *           (a hipStream_t, NULL = the default stream) and the call returns without waiting.
  */

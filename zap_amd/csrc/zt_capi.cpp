@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -18,7 +19,7 @@
 namespace {
 
 thread_local char g_err[512];
-std::mutex g_mu;     // serialises host entry points (shared staging buffers)
+std::mutex g_mu;     // per-device set-up, and the host slots' busy flags
 int g_m1k = 4;
 int g_climb_refill = 16;  // zt_bc1_climb refills once this many lanes of a wave are idle
 
@@ -104,19 +105,26 @@ const uint32_t kSrgbLinHash = 0xab9a3c1bu, kSrgbThrHash = 0x4f5c9c4fu;
 // device address); it uses the per-device address that ensure_ready reads with hipGetSymbolAddress.
 constexpr size_t kScratchBytes = ZAPTEX_NOALLOC_SCRATCH_BYTES;
 __device__ __align__(256) uint8_t scratch[kScratchBytes];
-struct DeviceState {
-    bool ready = false;
+// one host entry point call's staging: its stream, device buffers, and pinned host memory for rows that are not
+// packed.  Each call holds one slot from start to finish (HostLease).
+struct HostSlot {
     hipStream_t stream = nullptr;
-    void *scratch = nullptr;   // this device's address of `scratch`
-    // staging for the host entry points (device buffers, and pinned host memory for rows that are not packed)
     void *in = nullptr, *out = nullptr, *ws = nullptr, *pin = nullptr, *mip = nullptr, *rdo = nullptr;
     size_t in_cap = 0, out_cap = 0, ws_cap = 0, pin_cap = 0, mip_cap = 0, rdo_cap = 0;
+};
+constexpr int kHostSlots = 8;   // host calls that run side by side on one device
+struct DeviceState {
+    bool ready = false;
+    hipStream_t stream = nullptr;   // the library's own set-up work (batch tables, network initialisation)
+    void *scratch = nullptr;        // this device's address of `scratch`
+    HostSlot slots[kHostSlots];
+    bool busy[kHostSlots] = {};
 };
 DeviceState g_dev[64];
 
 int ensure_buf(void *&p, size_t &cap, size_t need) {
     if (need <= cap) return ZAPTEX_OK;
-    if (p) { hipFree(p); p = nullptr; cap = 0; }
+    if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
     size_t n = std::max(align_up(need, 1 << 20), (size_t)1 << 20);
     if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; return fail(ZAPTEX_E_NOMEM, "hipMalloc(%zu) failed", n); }
     cap = n;
@@ -126,7 +134,7 @@ int ensure_buf(void *&p, size_t &cap, size_t need) {
 // Host <-> device rows for the host entry points.  Packed rows move in one copy; rows with gaps between them are
 // packed through pinned staging memory by the CPU, because a 2D copy from pageable memory is staged one row at a
 // time (a 4096-row image took 60 ms that way instead of 5).
-int ensure_pinned(DeviceState *st, size_t need) {
+int ensure_pinned(HostSlot *st, size_t need) {
     if (need <= st->pin_cap) return ZAPTEX_OK;
     HIP_TRY(hipStreamSynchronize(st->stream));   // the old buffer may still feed a copy
     if (st->pin) { (void)hipHostFree(st->pin); st->pin = nullptr; st->pin_cap = 0; }
@@ -135,7 +143,7 @@ int ensure_pinned(DeviceState *st, size_t need) {
     st->pin_cap = n;
     return ZAPTEX_OK;
 }
-int rows_to_device(DeviceState *st, void *dst, const void *src, size_t spitch, size_t row, size_t h) {
+int rows_to_device(HostSlot *st, void *dst, const void *src, size_t spitch, size_t row, size_t h) {
     if (spitch == row) { HIP_TRY(hipMemcpyAsync(dst, src, row * h, hipMemcpyHostToDevice, st->stream)); return ZAPTEX_OK; }
     if (int rc = ensure_pinned(st, row * h)) return rc;
     HIP_TRY(hipStreamSynchronize(st->stream));
@@ -144,7 +152,7 @@ int rows_to_device(DeviceState *st, void *dst, const void *src, size_t spitch, s
     return ZAPTEX_OK;
 }
 // ends with the stream synchronised
-int rows_to_host(DeviceState *st, void *dst, size_t dpitch, const void *src, size_t row, size_t h) {
+int rows_to_host(HostSlot *st, void *dst, size_t dpitch, const void *src, size_t row, size_t h) {
     if (dpitch == row) {
         HIP_TRY(hipMemcpyAsync(dst, src, row * h, hipMemcpyDeviceToHost, st->stream));
         HIP_TRY(hipStreamSynchronize(st->stream));
@@ -186,14 +194,52 @@ int ensure_ready(DeviceState *&st) {
     return ZAPTEX_OK;
 }
 
+// A host entry point's lease on one of its device's host slots: calls from different threads run side by side on
+// their own streams and staging buffers, up to kHostSlots per device; further calls wait for a slot.  (The
+// reference functions share no state, so concurrent callers are the norm, e.g. one thread per band.)
+std::condition_variable g_slot_cv;
+struct HostLease {
+    DeviceState *dev = nullptr;
+    HostSlot *h = nullptr;
+    int idx = -1;
+    int acquire() {
+        std::unique_lock<std::mutex> lk(g_mu);
+        if (int rc = ensure_ready(dev)) return rc;
+        g_slot_cv.wait(lk, [&] {
+            for (int i = 0; i < kHostSlots; i++) if (!dev->busy[i]) return true;
+            return false;
+        });
+        for (idx = 0; dev->busy[idx]; idx++) {}
+        HostSlot &slot = dev->slots[idx];
+        if (!slot.stream) HIP_TRY(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking));
+        dev->busy[idx] = true;
+        h = &slot;
+        return ZAPTEX_OK;
+    }
+    ~HostLease() {
+        if (!h) return;
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            dev->busy[idx] = false;
+        }
+        g_slot_cv.notify_one();
+    }
+};
+
 bool is_bcn(int fmt) { return fmt >= ZAPTEX_BC1 && fmt <= ZAPTEX_BC7; }
 bool valid_fmt(int fmt) { return fmt >= ZAPTEX_BC1 && fmt <= ZAPTEX_ASTC; }
 size_t image_out_bytes(const zaptex_image &im) { return out_bytes(im.w, im.h, im.fmt == ZAPTEX_BC6H || im.fmt == ZAPTEX_ASTC ? ZAPTEX_BC7 : im.fmt); }
-uint32_t nblocks(const zaptex_image &im) { return (uint32_t)(((im.w + 3) / 4) * ((im.h + 3) / 4)); }
+uint32_t nblocks(const zaptex_image &im) { return (uint32_t)((uint64_t)((im.w + 3) / 4) * (uint64_t)((im.h + 3) / 4)); }
+// Sizes the kernels can index: sides up to 2^30 and fewer than 2^31 blocks per image (block indices are 32-bit;
+// a launch's total is checked the same way).  The reference takes any size; these bounds are far beyond memory.
+constexpr int kMaxSide = 1 << 30;
+bool size_ok(int w, int h) {
+    return w >= 1 && h >= 1 && w <= kMaxSide && h <= kMaxSide && (uint64_t)((w + 3) / 4) * (uint64_t)((h + 3) / 4) < (1ull << 31);
+}
 
 int check_image(const zaptex_image &im) {
     if (!valid_fmt(im.fmt)) return fail(ZAPTEX_E_ARG, "unknown format %d", im.fmt);
-    if (im.w < 1 || im.h < 1 || im.w > (1 << 16) || im.h > (1 << 16)) return fail(ZAPTEX_E_ARG, "bad size %dx%d", im.w, im.h);
+    if (!size_ok(im.w, im.h)) return fail(ZAPTEX_E_ARG, "bad size %dx%d", im.w, im.h);
     if (!im.src || !im.dst) return fail(ZAPTEX_E_ARG, "null image pointer");
     size_t px = im.fmt == ZAPTEX_BC6H ? 16 : 4;
     if (im.stride < (size_t)im.w * px) return fail(ZAPTEX_E_ARG, "stride %zu below row size", im.stride);
@@ -431,7 +477,7 @@ int enqueue_rdo(const zaptex_image *imgs, int n, float rdo, void *scratch, size_
 
 int enqueue_mip(const void *src, int w, int h, size_t stride, int srgb, void *dst, size_t dst_stride, hipStream_t s) {
     int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
-    dim3 grid((nw + 63) / 64, (nh + 3) / 4), block(64, 4);
+    dim3 grid((nw + 63) / 64, std::min((nh + 3) / 4, 65535)), block(64, 4);
     return launch("zt_mip_level", s, [&] {
         hipLaunchKernelGGL(zt_mip_level, grid, block, 0, s, (const uint8_t *)src, w, h, (uint64_t)stride, srgb, (uint8_t *)dst, (uint64_t)dst_stride);
     });
@@ -449,7 +495,7 @@ int enqueue_mip_chain(const void *base, int w, int h, size_t stride, int srgb, v
         const int cw = lw[l], ch = lh[l];
         zt_mip_levels L;
         memset(&L, 0, sizeof L);
-        if ((cw > 128 || ch > 128) && cw >= 2 && ch >= 2) {
+        if ((cw > 128 || ch > 128) && cw >= 2 && ch >= 2 && (lh[l + 1] + 63) / 64 <= 65535) {
             int k = 0;
             while (k < 7 && l + k < nlevels && lw[l + k] >= 2 && lh[l + k] >= 2) {
                 L.p[k] = (uint8_t *)levels[l + k]; L.w[k] = lw[l + k + 1]; L.h[k] = lh[l + k + 1];
@@ -478,9 +524,9 @@ int enqueue_mip_chain(const void *base, int w, int h, size_t stride, int srgb, v
 
 // run one host-buffer image through the device: stage in, encode, copy out
 int host_encode(const void *src, int w, int h, size_t stride, size_t px_bytes, int fmt, void *out, float rdo = 0.0f) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     size_t row = (size_t)w * px_bytes, inb = row * (size_t)h;
     zaptex_image im{ nullptr, nullptr, row, w, h, fmt };
     size_t ob = image_out_bytes(im);
@@ -552,12 +598,12 @@ size_t tex_file_bytes(int w, int h, int fmt, int mips, int srgb) {
 extern "C" {
 
 size_t zaptex_tex_file_size(int w, int h, int fmt, int mips, int srgb) {
-    if (w < 1 || h < 1 || w > (1 << 16) || h > (1 << 16) || (!is_bcn(fmt) && fmt != ZAPTEX_ASTC)) return 0;
+    if (!size_ok(w, h) || (!is_bcn(fmt) && fmt != ZAPTEX_ASTC)) return 0;
     return tex_file_bytes(w, h, fmt, mips, srgb);
 }
 
 int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, float rdo, int mips, int srgb, void *out, size_t out_size) {
-    if ((!is_bcn(fmt) && fmt != ZAPTEX_ASTC) || w < 1 || h < 1 || w > (1 << 16) || h > (1 << 16) || !rgba || !out || stride < (size_t)w * 4)
+    if ((!is_bcn(fmt) && fmt != ZAPTEX_ASTC) || !size_ok(w, h) || !rgba || !out || stride < (size_t)w * 4)
         return fail(ZAPTEX_E_ARG, "bad arguments");
     const size_t need = tex_file_bytes(w, h, fmt, mips, srgb);
     if (out_size < need) return fail(ZAPTEX_E_ARG, "output buffer %zu < %zu bytes", out_size, need);
@@ -572,9 +618,9 @@ int zaptex_tex_file(const uint8_t *rgba, int w, int h, size_t stride, int fmt, f
     const size_t hn = dds_header(hdr, w, h, fmt, levels, mips, srgb);
     memcpy(o, hdr, hn);
     // the whole chain on the device: base in once, levels 1.. generated in HBM, every level encoded, blocks out once
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     std::vector<int> lw(1, w), lh(1, h);
     for (int l = 1; l < levels; l++) {
         lw.push_back(lw.back() > 1 ? lw.back() / 2 : 1);
@@ -653,9 +699,9 @@ int zaptex_mip_next(const uint8_t *src, int w, int h, size_t stride, int srgb, u
     if (w < 1 || h < 1 || !src || !dst || stride < (size_t)w * 4) return fail(ZAPTEX_E_ARG, "bad arguments");
     int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
     if (dst_stride < (size_t)nw * 4) return fail(ZAPTEX_E_ARG, "dst_stride below row size");
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     size_t row = (size_t)w * 4, orow = (size_t)nw * 4;
     if (int rc = ensure_buf(st->in, st->in_cap, row * h)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, orow * nh)) return rc;
@@ -668,14 +714,14 @@ int zaptex_mip_next_f(const float *src, int w, int h, size_t stride, float *dst,
     if (w < 1 || h < 1 || !src || !dst || stride < (size_t)w * 4) return fail(ZAPTEX_E_ARG, "bad arguments");
     int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
     if (dst_stride < (size_t)nw * 4) return fail(ZAPTEX_E_ARG, "dst_stride below row size");
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     size_t row = (size_t)w * 16, orow = (size_t)nw * 16;
     if (int rc = ensure_buf(st->in, st->in_cap, row * h)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, orow * nh)) return rc;
     if (int rc = rows_to_device(st, st->in, src, stride * 4, row, (size_t)h)) return rc;
-    dim3 grid((nw + 63) / 64, (nh + 3) / 4), block(64, 4);
+    dim3 grid((nw + 63) / 64, std::min((nh + 3) / 4, 65535)), block(64, 4);
     hipLaunchKernelGGL(zt_mip_level_f, grid, block, 0, st->stream, (const float *)st->in, w, h, (uint64_t)w * 4, (float *)st->out, (uint64_t)nw * 4);
     HIP_TRY(hipGetLastError());
     return rows_to_host(st, dst, dst_stride * 4, st->out, orow, (size_t)nh);
@@ -683,9 +729,9 @@ int zaptex_mip_next_f(const float *src, int w, int h, size_t stride, float *dst,
 
 int zaptex_bc_decode(const void *bc, int w, int h, int fmt, uint8_t *rgba, size_t stride) {
     if (!is_bcn(fmt) || w < 1 || h < 1 || !bc || !rgba || stride < (size_t)w * 4) return fail(ZAPTEX_E_ARG, "bad arguments");
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     size_t ib = out_bytes(w, h, fmt), row = (size_t)w * 4;
     if (int rc = ensure_buf(st->in, st->in_cap, ib)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, row * h)) return rc;
@@ -698,9 +744,9 @@ int zaptex_bc_decode(const void *bc, int w, int h, int fmt, uint8_t *rgba, size_
 
 int zaptex_bc6h_decode(const void *bc, int w, int h, uint16_t *rgba, size_t stride) {
     if (w < 1 || h < 1 || !bc || !rgba || stride < (size_t)w * 4) return fail(ZAPTEX_E_ARG, "bad arguments");
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     size_t ib = out_bytes(w, h, ZAPTEX_BC7), row = (size_t)w * 8;
     if (int rc = ensure_buf(st->in, st->in_cap, ib)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, row * h)) return rc;
@@ -713,9 +759,9 @@ int zaptex_bc6h_decode(const void *bc, int w, int h, uint16_t *rgba, size_t stri
 
 int zaptex_astc_decode(const void *bc, int w, int h, uint8_t *rgba, size_t stride) {
     if (w < 1 || h < 1 || !bc || !rgba || stride < (size_t)w * 4) return fail(ZAPTEX_E_ARG, "bad arguments");
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     size_t ib = out_bytes(w, h, ZAPTEX_BC7), row = (size_t)w * 4;
     if (int rc = ensure_buf(st->in, st->in_cap, ib)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, row * h)) return rc;
@@ -729,9 +775,9 @@ int zaptex_astc_decode(const void *bc, int w, int h, uint8_t *rgba, size_t strid
 static int shuffle(const void *in, size_t n, int fmt, void *out, int split) {
     if (!is_bcn(fmt) || (!in && n) || (!out && n)) return fail(ZAPTEX_E_ARG, "bad arguments");
     if (!n) return ZAPTEX_OK;
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState *st;
-    if (int rc = ensure_ready(st)) return rc;
+    HostLease lease;
+    if (int rc = lease.acquire()) return rc;
+    HostSlot *st = lease.h;
     if (int rc = ensure_buf(st->in, st->in_cap, n)) return rc;
     if (int rc = ensure_buf(st->out, st->out_cap, n)) return rc;
     HIP_TRY(hipMemcpyAsync(st->in, in, n, hipMemcpyHostToDevice, st->stream));
@@ -1289,7 +1335,7 @@ int zaptex_mip_next_f_dev(const void *src, int w, int h, size_t stride, void *ds
         if (int rc = ensure_ready(st)) return rc;
     }
     int nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
-    dim3 grid((nw + 63) / 64, (nh + 3) / 4), block(64, 4);
+    dim3 grid((nw + 63) / 64, std::min((nh + 3) / 4, 65535)), block(64, 4);
     hipLaunchKernelGGL(zt_mip_level_f, grid, block, 0, (hipStream_t)stream, (const float *)src, w, h, (uint64_t)stride, (float *)dst, (uint64_t)dst_stride);
     HIP_TRY(hipGetLastError());
     return ZAPTEX_OK;
@@ -1346,7 +1392,7 @@ void batch_layout(int w, int h, int fmt, int levels, std::vector<size_t> *mip_of
 int max_levels(int w, int h) { int n = 1; while (w > 1 || h > 1) { w = w > 1 ? w / 2 : 1; h = h > 1 ? h / 2 : 1; n++; } return n; }
 int check_texture(const zaptex_texture &t) {
     if (t.fmt < ZAPTEX_BC1 || t.fmt > ZAPTEX_BC7) return fail(ZAPTEX_E_ARG, "batch formats are BC1, BC3, BC4, BC5 and BC7 (got %d)", t.fmt);
-    if (t.w < 1 || t.h < 1 || t.w > (1 << 16) || t.h > (1 << 16)) return fail(ZAPTEX_E_ARG, "bad size %dx%d", t.w, t.h);
+    if (!size_ok(t.w, t.h)) return fail(ZAPTEX_E_ARG, "bad size %dx%d", t.w, t.h);
     if (t.levels < 1 || t.levels > max_levels(t.w, t.h)) return fail(ZAPTEX_E_ARG, "levels %d outside 1..%d", t.levels, max_levels(t.w, t.h));
     if (!t.base || !t.out || (t.levels > 1 && !t.mips)) return fail(ZAPTEX_E_ARG, "null texture pointer");
     if (t.stride < (size_t)t.w * 4 || (((uintptr_t)t.base) | t.stride) % 4) return fail(ZAPTEX_E_ARG, "base rows must be 4-byte aligned, stride >= 4 w");
@@ -1669,7 +1715,7 @@ int zaptex_profile_collect(void) {
         auto it = std::find_if(g_prof_total.begin(), g_prof_total.end(), [&](const ProfTotal &t) { return !strcmp(t.name, p.name); });
         if (it == g_prof_total.end()) g_prof_total.push_back({ p.name, ms, 1 });
         else { it->ms += ms; it->n++; }
-        hipEventDestroy(p.a); hipEventDestroy(p.b);
+        (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b);
     }
     g_prof_pending.clear();
     return ZAPTEX_OK;
