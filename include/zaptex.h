@@ -39,8 +39,14 @@ enum {
     ZAPTEX_OPT_PROFILE = 2,  /* 1: time every kernel with HIP events on its launch stream (zaptex_profile_*) */
     ZAPTEX_OPT_CLIMB_REFILL = 3, /* BC1 hill climb: idle lanes (1..64) a wave gathers before it refills them */
     ZAPTEX_OPT_RDO_SEGMENT = 4,  /* rdo > 0: blocks per speculative segment, 16..2^24, or 0 = automatic (default) */
-    ZAPTEX_OPT_RDO_WARMUP = 5    /* rdo > 0: blocks a segment's speculative walk starts early, 0..2^24 (default 512);
+    ZAPTEX_OPT_RDO_WARMUP = 5,   /* rdo > 0: blocks a segment's speculative walk starts early, 0..2^24 (default 512);
                                     segment and warm-up change speed only, never the bytes */
+    /* batches (zaptex_batch_create reads these; none changes the bytes): */
+    ZAPTEX_OPT_BATCH_PERSISTENT = 6, /* 1: one persistent kernel over a task queue; 0 (default): a graph of the
+                                        per-stage kernels (also used when ZAPTEX_OPT_BC7_M1K > 8) */
+    ZAPTEX_OPT_BATCH_MIX = 7,        /* level-0 encode tasks queued per round-0 mip task, 0..64 (default 2) */
+    ZAPTEX_OPT_BATCH_CHUNKS = 8,     /* 256-block chunks per BC1-family / BC7 stage-6 task, 1..64 (default 1) */
+    ZAPTEX_OPT_BATCH_GROUPS = 9      /* persistent workgroups per CU, 0 (default: as many as fit) or 1..32 */
 };
 
 /* ---------------- host entry points (drop-in bodies of zap_tex.h) */

@@ -152,3 +152,40 @@ def test_launch_shapes_do_not_change_bytes(zt):
     finally:
         for k, (wg, cap) in saved.items():
             zt.set_launch(k, wg, cap)
+
+
+@pytest.mark.skip(reason="the persistent batch kernel's first GPU run faulted the GPU; it stays off the GPU (and off by "
+                         "default) until the cause is found")
+@pytest.mark.parametrize("mix,chunks,groups", [(2, 1, 0), (0, 1, 1), (64, 8, 0), (3, 64, 2)])
+def test_persistent_kernel_equals_stage_graph(zt, mix, chunks, groups):
+    """The persistent batch kernel (one launch over the task queue) against the graph of per-stage kernels on the
+    same textures: queue mix, chunks per task and resident workgroups change the schedule only.  Three replays of
+    the one graph launch each give the same bytes (the kernel's last workgroup clears the queue for the next)."""
+    import torch
+    saved = {k: zt.get_option(k) for k in (6, 7, 8, 9)}
+    try:
+        zt.set_option(6, 0)
+        b, _ = _make(zt, SMALL, seed0=500, stride_pad=2)
+        b.run()
+        torch.cuda.synchronize()
+        ref = _snapshot(b)
+        b.close()
+        zt.set_option(6, 1)
+        zt.set_option(7, mix)
+        zt.set_option(8, chunks)
+        zt.set_option(9, groups)
+        b, _ = _make(zt, SMALL, seed0=500, stride_pad=2)
+        for rep in range(3):
+            for it in b.items:   # clear the outputs so every replay has to write them again
+                it["out"].zero_()
+                it["mips"].zero_()
+            b.run()
+            torch.cuda.synchronize()
+            got = _snapshot(b)
+            for i, ((m0, o0), (m1, o1)) in enumerate(zip(ref, got)):
+                assert all(np.array_equal(x, y) for x, y in zip(m0, m1)), (rep, i, "mips")
+                assert all(np.array_equal(x, y) for x, y in zip(o0, o1)), (rep, i, "blocks")
+        b.close()
+    finally:
+        for k, v in saved.items():
+            zt.set_option(k, v)

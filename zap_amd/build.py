@@ -26,7 +26,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-SOURCES = ["zt_bc7.hip", "zt_bc7_full.hip", "zt_mlp.hip", "zt_bc14.hip", "zt_bc6h.hip", "zt_astc.hip", "zt_mip.hip", "zt_decode.hip", "zt_rdo.hip", "zt_capi.cpp"]
+# zt_batch.hip includes zt_mip.hip, zt_bc14.hip and zt_bc7.hip: one translation unit for those kernels and the
+# persistent batch kernel that runs their bodies
+SOURCES = ["zt_batch.hip", "zt_bc7_full.hip", "zt_mlp.hip", "zt_bc6h.hip", "zt_astc.hip", "zt_decode.hip", "zt_rdo.hip", "zt_capi.cpp"]
 
 
 def _compile(src: str) -> str:

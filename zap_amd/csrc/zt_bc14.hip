@@ -267,12 +267,17 @@ ZT_DEV uint64_t bc4_block(const uint32_t px[16], int ch) {
 
 }  // namespace
 
+// The kernels' bodies take the workgroup index and grid size as arguments (bid, nblk), so the persistent batch
+// kernel can run any workgroup of them as a task.
+
 // BC4 / BC5 blocks and the BC3 alpha half complete here; BC1 blocks and the BC3 colour half stop before the hill
 // climb and leave their state in climb[g] = (c0 | c1 << 16, SSE, indices, job) for zt_bc1_climb; the other
 // blocks leave (0, 0, 0, job).
-extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb) {
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+namespace {
+ZT_DEV void bc14_encode_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t step = nblk * blockDim.x;
+    for (uint32_t g = bid * blockDim.x + tid; g < total; g += step) {
         const int ji = zt_find_job(jobs, g);
         const zt_job J = job_at(jobs, ji);
         uint32_t b = g - J.first;
@@ -291,6 +296,10 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, u
         }
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb) {
+    bc14_encode_body(jobs, total, climb, blockIdx.x, gridDim.x);
+}
 
 // The BC1 hill climb with lane refill.  Each wave owns ranges of `range` blocks (wave-strided; a multiple of 64, at
 // most ZT_CLIMB_RANGE, chosen by the host so that small jobs still spread over enough waves); a lane that has no
@@ -300,10 +309,12 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, u
 // a job lookup and a 16-pixel gather, so doing it for one lane per step would cost more than the idle lanes.
 // Trials compute errors only.  A block the climb never moved is stored with its start's indices; a moved block
 // leaves (c0 | c1 << 16, SSE, 0, job | 2^31) in climb[] for zt_bc1_finish, which computes its indices once.
-extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range) {
-    const int lane = threadIdx.x & 63;
+namespace {
+ZT_DEV void bc1_climb_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const int lane = tid & 63;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t wave = (bid * blockDim.x + tid) >> 6, nwaves = (nblk * blockDim.x) >> 6;
     for (uint64_t r0 = (uint64_t)wave * range; r0 < total; r0 += (uint64_t)nwaves * range) {
         const uint32_t end = (uint32_t)min<uint64_t>(total, r0 + range);
         uint32_t next = (uint32_t)r0;  // same on every lane
@@ -347,11 +358,17 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uin
         }
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range) {
+    bc1_climb_body(jobs, total, climb, refill_min, range, blockIdx.x, gridDim.x);
+}
 
 // the indices of every block the hill climb moved (climb[g].w has bit 31 set): one full trial of its final pair
-extern "C" __global__ void __launch_bounds__(256) zt_bc1_finish(zt_jobs jobs, uint32_t total, const uint4 *climb) {
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+namespace {
+ZT_DEV void bc1_finish_body(const zt_jobs &jobs, uint32_t total, const uint4 *climb, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t step = nblk * blockDim.x;
+    for (uint32_t g = bid * blockDim.x + tid; g < total; g += step) {
         const uint4 st = climb[g];
         if (!(st.w >> 31)) continue;
         const zt_job J = job_at(jobs, (int)(st.w & 0x7FFFFFFFu));
@@ -363,4 +380,8 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_finish(zt_jobs jobs, ui
         (void)bc1_trial(px, spp, c0, c1, idx);   // (c0, c1) is canonical already
         store8(J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8), bc1_pack(c0, c1, idx));
     }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_finish(zt_jobs jobs, uint32_t total, const uint4 *climb) {
+    bc1_finish_body(jobs, total, climb, blockIdx.x, gridDim.x);
 }

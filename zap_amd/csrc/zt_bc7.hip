@@ -16,11 +16,16 @@
 
 // ===================================================================================== kernels
 
+// The kernels' bodies take the workgroup index and grid size as arguments (bid, nblk), so the persistent batch
+// kernel can run any workgroup of them as a task.
+
 // Stage 6: every block. Writes the mode-6 block; blocks with SSE > 64 go to list 5 with their pixels.
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L) {
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += step) {
-        uint32_t g = base + threadIdx.x;
+namespace {
+ZT_DEV void stage6_body(const zt_jobs &jobs, uint32_t total, const zt_bc7_lists &L, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t step = nblk * blockDim.x;
+    for (uint32_t base = bid * blockDim.x; base < total; base += step) {
+        uint32_t g = base + tid;
         bool valid = g < total, pred = false;
         int err = 0, opaque = 0, j = 0;
         uint32_t px[16];
@@ -39,7 +44,7 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, ui
             opaque = amin == 255;
             pred = err > 64;
         }
-        unsigned *c = shard_counter(L.counters, base + (threadIdx.x & ~63u));
+        unsigned *c = shard_counter(L.counters, base + (tid & ~63u));
         int slot = wave_append(pred, c);
         if (slot >= 0) {
             slot += (int)((uint32_t)((c - L.counters) / ZT_SHARD_STRIDE) * L.cap5);
@@ -50,12 +55,18 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, ui
         }
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L) {
+    stage6_body(jobs, total, L, blockIdx.x, gridDim.x);
+}
 
 // Stage 5: lane = (list-5 entry, rotation). Keeps mode 5 when it beats mode 6; opaque blocks go to list 1.
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L) {
-    const uint32_t n5 = shard_total(L.counters), items = n5 * 4, step = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < items; base += step) {
-        uint32_t item = base + threadIdx.x, e = item >> 2;
+namespace {
+ZT_DEV void stage5_body(const zt_jobs &jobs, const zt_bc7_lists &L, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t n5 = shard_total(L.counters), items = n5 * 4, step = nblk * blockDim.x;
+    for (uint32_t base = bid * blockDim.x; base < items; base += step) {
+        uint32_t item = base + tid, e = item >> 2;
         int rot = (int)(item & 3);
         bool valid = item < items;
         uint32_t px[16];
@@ -84,20 +95,25 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage5(zt_jobs jobs, zt
             const zt_job J = job_at(jobs, (int)meta.z);
             store16(J.dst + (uint64_t)(meta.x - J.first) * 16, lo, hi);
         }
-        unsigned *c = shard_counter(L.counters + ZT_BC7_LIST1, base + (threadIdx.x & ~63u));   // <= 16 appends per wave
+        unsigned *c = shard_counter(L.counters + ZT_BC7_LIST1, base + (tid & ~63u));   // <= 16 appends per wave
         int slot = wave_append(winner && meta.w, c);
         if (slot >= 0)
             L.meta1[(uint32_t)((c - L.counters - ZT_BC7_LIST1) / ZT_SHARD_STRIDE) * L.cap1 + (uint32_t)slot] =
                 make_uint4(e, (uint32_t)(improved ? e5 : (int)meta.y), 0, 0);
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L) {
+    stage5_body(jobs, L, blockIdx.x, gridDim.x);
+}
 
 // Stage 1: one wave per list-1 entry, lane = partition.
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k) {
+namespace {
+ZT_DEV void stage1_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k, uint32_t bid, uint32_t nblk) {
     const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < n1; e += nw) {
+    const uint32_t nw = nblk * (blockDim.x >> 6);
+    for (uint32_t e = bid * (blockDim.x >> 6) + wv; e < n1; e += nw) {
         uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         uint4 meta = L.meta5[m.x];
         int cur = (int)m.y;
@@ -142,6 +158,10 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage1(zt_jobs jobs, zt
         }
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k) {
+    stage1_body(jobs, L, m1k, blockIdx.x, gridDim.x);
+}
 
 extern "C" __global__ void zt_zero_u32(unsigned *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,11 +178,13 @@ extern "C" __global__ void zt_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 //   fit   one lane per (entry, candidate, subset): the exact subset fit; SSE and the subset's quantised state
 //   pick  one lane per entry: first candidate in rank order with the smallest SSE below the current one, packed
 // Capacity: entry e owns cand[8e..8e+7] and items [16e, 16e + 16).
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand) {
+namespace {
+ZT_DEV void s1_rank_body(const zt_bc7_lists &L, int m1k, uint8_t *cand, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
     const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + wv; e < n1; e += nw) {
+    const int lane = tid & 63, wv = tid >> 6;
+    const uint32_t nw = nblk * (blockDim.x >> 6);
+    for (uint32_t e = bid * (blockDim.x >> 6) + wv; e < n1; e += nw) {
         const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         uint32_t px[16];
         const uint4 *s = L.px5 + (uint64_t)m.x * 4;
@@ -197,10 +219,16 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L,
         }
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand) {
+    s1_rank_body(L, m1k, cand, blockIdx.x, gridDim.x);
+}
 
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state) {
-    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), items = n1 * (uint32_t)m1k * 2, step = gridDim.x * blockDim.x;
-    for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < items; it += step) {
+namespace {
+ZT_DEV void s1_fit_body(const zt_bc7_lists &L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), items = n1 * (uint32_t)m1k * 2, step = nblk * blockDim.x;
+    for (uint32_t it = bid * blockDim.x + tid; it < items; it += step) {
         const uint32_t e = it / (2u * m1k), k = (it >> 1) % (uint32_t)m1k, sb = it & 1;
         const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         uint32_t px[16];
@@ -222,11 +250,17 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, 
         sub_state[o] = make_uint4((uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)qs, (uint32_t)(qs >> 32));
     }
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state) {
+    s1_fit_body(L, m1k, cand, sub_err, sub_state, blockIdx.x, gridDim.x);
+}
 
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err,
-                                                                   const uint4 *sub_state) {
-    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), step = gridDim.x * blockDim.x;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n1; e += step) {
+namespace {
+ZT_DEV void s1_pick_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k, const uint8_t *cand, const int *sub_err, const uint4 *sub_state,
+                         uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), step = nblk * blockDim.x;
+    for (uint32_t e = bid * blockDim.x + tid; e < n1; e += step) {
         const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
         int best = (int)m.y, bk = -1;
         for (int k = 0; k < m1k; k++) {
@@ -278,4 +312,9 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_pick(zt_jobs jobs, z
         const zt_job J = job_at(jobs, (int)meta.z);
         store16(J.dst + (uint64_t)(meta.x - J.first) * 16, b.lo, b.hi);
     }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err,
+                                                                   const uint4 *sub_state) {
+    s1_pick_body(jobs, L, m1k, cand, sub_err, sub_state, blockIdx.x, gridDim.x);
 }

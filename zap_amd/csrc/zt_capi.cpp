@@ -1403,13 +1403,15 @@ int check_texture(const zaptex_texture &t) {
 // (a fused tile pass of up to 7 levels while the source exceeds 128 on a side, single levels while a side is 1
 // and the other exceeds 128, then one workgroup for a source of at most 128 x 128); step s of every texture goes
 // into the launches of round s (one per step kind), so each texture's steps run in order.
-struct MipLaunch { int kind; size_t first; int count; uint64_t px; int srgb; };  // kind 0 tiles, 1 single level, 2 tail
+struct MipLaunch { int kind; size_t first; int count; uint64_t px; int srgb; int round; };  // kind 0 tiles, 1 single level, 2 tail
 struct MipPlan {
     std::vector<zt_mipstep> steps;   // tile and tail launches' steps, launch by launch
     std::vector<uint2> items;        // tile items (step within its launch, tile)
     std::vector<zt_mipjob> jobs;     // single-level jobs, launch by launch
     std::vector<MipLaunch> launches; // first = offset into items (kind 0), jobs (kind 1) or steps (kind 2)
     std::vector<size_t> tile_steps;  // kind-0 launches: offset of their steps
+    std::vector<std::vector<int>> level_round;  // [texture][level]: the round whose step writes it (-1: level 0)
+    int rounds = 0;
 };
 MipPlan plan_mips(const zaptex_texture *t, int n) {
     struct TexStep { int kind, l, k; };
@@ -1452,9 +1454,16 @@ MipPlan plan_mips(const zaptex_texture *t, int n) {
         return m;
     };
     MipPlan P;
+    P.rounds = (int)rounds;
+    P.level_round.resize(n);
+    for (int i = 0; i < n; i++) {
+        P.level_round[i].assign(t[i].levels, -1);
+        for (size_t r = 0; r < ts[i].size(); r++)
+            for (int j = 1; j <= ts[i][r].k; j++) P.level_round[i][ts[i][r].l + j] = (int)r;
+    }
     for (size_t r = 0; r < rounds; r++)
         for (int kind = 0; kind < 3; kind++) {
-            MipLaunch ml{ kind, 0, 0, 0, 0 };
+            MipLaunch ml{ kind, 0, 0, 0, 0, (int)r };
             ml.first = kind == 0 ? P.items.size() : kind == 1 ? P.jobs.size() : P.steps.size();
             const size_t step0 = P.steps.size();
             for (int i = 0; i < n; i++) {
@@ -1483,6 +1492,190 @@ MipPlan plan_mips(const zaptex_texture *t, int n) {
             P.launches.push_back(ml);
         }
     return P;
+}
+
+// ---- the persistent batch kernel's queue (zt_batch.hip)
+// Encode jobs of a family in queue order: level 0 of every texture first, then the levels written by mip round 0,
+// round 1, ... (texture and level order within a round), so the level-0 chunks need no mips and every later chunk
+// waits for one round's counter.
+struct FamJob { int round, tex, lvl; uint32_t blocks; };
+std::vector<FamJob> family_jobs(const zaptex_texture *t, int n, const MipPlan &P, bool bc7) {
+    std::vector<FamJob> v;
+    for (int i = 0; i < n; i++) {
+        if ((t[i].fmt == ZAPTEX_BC7) != bc7) continue;
+        int cw = t[i].w, ch = t[i].h;
+        for (int l = 0; l < t[i].levels; l++) {
+            v.push_back({ P.level_round[i][l], i, l, (uint32_t)((uint64_t)((cw + 3) / 4) * (uint64_t)((ch + 3) / 4)) });
+            cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
+        }
+    }
+    std::stable_sort(v.begin(), v.end(), [](const FamJob &a, const FamJob &b) { return a.round < b.round; });
+    return v;
+}
+
+int g_batch_persistent = 0;  // 1: batches run as one persistent kernel; 0 (default): a graph of the per-stage kernels
+int g_batch_mix = 2;         // level-0 encode tasks per round-0 mip task at the head of the queue
+int g_batch_per = 1;         // 256-block chunks per encode task (BC1-family start, BC7 stage 6)
+int g_batch_groups = 0;      // persistent workgroups per CU (0: as many as fit)
+
+// counters after the mip rounds' (round r: counter r)
+enum { PC_BC14, PC_CLIMB, PC_FINISH, PC_S6, PC_S5, PC_RANK, PC_FIT, PC_PICK, PC_N };
+struct PersistentQueue {
+    std::vector<zt_seg> segs;
+    std::vector<uint32_t> queue;   // segment << ZT_TASK_SEG_SHIFT | task within the segment
+    uint32_t nctl = 0;
+};
+// device tables the mip segments point into (null while only counting tasks)
+struct PqTables { const zt_mipstep *steps = nullptr; const uint2 *items = nullptr; const zt_mipjob *mipjobs = nullptr; };
+
+// x and y interleaved evenly (each prefix holds the two in proportion to their lengths)
+std::vector<uint32_t> even_merge(const std::vector<uint32_t> &x, const std::vector<uint32_t> &y) {
+    std::vector<uint32_t> o;
+    o.reserve(x.size() + y.size());
+    size_t i = 0, j = 0;
+    while (i < x.size() || j < y.size()) {
+        if (j == y.size() || (i < x.size() && i * y.size() <= j * x.size())) o.push_back(x[i++]);
+        else o.push_back(y[j++]);
+    }
+    return o;
+}
+// one task of a, then `rb` of b, until a runs out; then the rest of b
+std::vector<uint32_t> lead_merge(const std::vector<uint32_t> &a, const std::vector<uint32_t> &b, int rb) {
+    std::vector<uint32_t> o;
+    o.reserve(a.size() + b.size());
+    size_t j = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        o.push_back(a[i]);
+        for (int k = 0; k < rb && j < b.size(); k++) o.push_back(b[j++]);
+    }
+    while (j < b.size()) o.push_back(b[j++]);
+    return o;
+}
+
+// The queue: every task waits only for tasks placed before it (mip round r after round r - 1; a chunk of levels
+// from round r after every mip task; each encode stage after the whole stage it reads), so it drains.
+int plan_queue(const MipPlan &P, const std::vector<FamJob> &f14, const std::vector<FamJob> &f7, uint32_t tot14, uint32_t tot7,
+               const PqTables &tb, PersistentQueue &Q) {
+    Q = PersistentQueue();
+    const uint32_t R = (uint32_t)P.rounds, C = R;   // stage counter s: C + s
+    std::vector<std::vector<uint32_t>> tasks;       // per segment, its queue entries
+    auto add = [&](uint32_t kind, uint32_t vbase, uint32_t count, uint32_t per, uint32_t vgrid, uint32_t wait, uint32_t need, uint32_t done) {
+        zt_seg sg;
+        memset(&sg, 0, sizeof sg);
+        sg.kind = kind; sg.vbase = vbase; sg.count = count; sg.per = per; sg.vgrid = vgrid; sg.wait = wait; sg.need = need; sg.done = done;
+        Q.segs.push_back(sg);
+        const uint32_t id = (uint32_t)(Q.segs.size() - 1), nt = (count + per - 1) / per;
+        std::vector<uint32_t> e(nt);
+        for (uint32_t j = 0; j < nt; j++) e[j] = id << ZT_TASK_SEG_SHIFT | j;
+        tasks.push_back(std::move(e));
+        return id;
+    };
+    auto cat = [&](const std::vector<uint32_t> &ids) {
+        std::vector<uint32_t> o;
+        for (uint32_t id : ids) o.insert(o.end(), tasks[id].begin(), tasks[id].end());
+        return o;
+    };
+    // mip rounds (P.launches come in round order)
+    std::vector<uint32_t> round_tasks(R, 0);
+    std::vector<std::vector<uint32_t>> round_segs(R);
+    size_t tl = 0;
+    for (const MipLaunch &ml : P.launches) {
+        const uint32_t r = (uint32_t)ml.round, wait = r ? r - 1 : ZT_NO_WAIT, need = r ? round_tasks[r - 1] : 0;
+        uint32_t id;
+        if (ml.kind == 0) {
+            id = add(ZT_T_MIP_TILE, 0, (uint32_t)ml.count, 1, (uint32_t)ml.count, wait, need, r);
+            Q.segs[id].a = tb.steps ? (const void *)(tb.steps + P.tile_steps[tl]) : nullptr;
+            Q.segs[id].b = tb.items ? (const void *)(tb.items + ml.first) : nullptr;
+            tl++;
+        } else if (ml.kind == 1) {
+            const uint32_t g = (uint32_t)groups_for(CFG_MIPB, ml.px, 256);
+            id = add(ZT_T_MIP_LEVEL, 0, g, 1, g, wait, need, r);
+            Q.segs[id].a = tb.mipjobs ? (const void *)(tb.mipjobs + ml.first) : nullptr;
+            Q.segs[id].n = (uint32_t)ml.count; Q.segs[id].total = ml.px; Q.segs[id].srgb = (uint32_t)ml.srgb;
+        } else {
+            id = add(ZT_T_MIP_TAIL, 0, (uint32_t)ml.count, 1, (uint32_t)ml.count, wait, need, r);
+            Q.segs[id].a = tb.steps ? (const void *)(tb.steps + ml.first) : nullptr;
+        }
+        round_segs[r].push_back(id);
+        round_tasks[r] += (uint32_t)tasks[id].size();
+    }
+    // encode chunks of 256 blocks by the round of their levels; a chunk waits for the round of its last block's job
+    // (the jobs are sorted by round, so that is the latest round it reads)
+    auto chunks = [&](const std::vector<FamJob> &f, uint32_t tot, uint32_t kind, uint32_t done, std::vector<std::vector<uint32_t>> &by_round) {
+        by_round.assign(R + 1, {});
+        const uint32_t nch = (tot + 255) / 256;
+        std::vector<int> cr(nch, -1);
+        size_t j = 0;
+        uint64_t end = f[0].blocks;   // job j's blocks end here (the jobs' blocks sum to tot)
+        for (uint32_t c = 0; c < nch; c++) {
+            const uint64_t last = std::min<uint64_t>(256ull * c + 255, (uint64_t)tot - 1);
+            while (last >= end) end += f[++j].blocks;
+            cr[c] = f[j].round;
+        }
+        uint32_t ntasks = 0;
+        for (uint32_t c0 = 0; c0 < nch;) {
+            uint32_t c1 = c0;
+            while (c1 < nch && cr[c1] == cr[c0]) c1++;
+            const int r = cr[c0];
+            const uint32_t id = add(kind, c0, c1 - c0, (uint32_t)g_batch_per, nch, r < 0 ? ZT_NO_WAIT : (uint32_t)r, r < 0 ? 0 : round_tasks[r], done);
+            by_round[r + 1].push_back(id);
+            ntasks += (uint32_t)tasks[id].size();
+            c0 = c1;
+        }
+        return ntasks;
+    };
+    std::vector<std::vector<uint32_t>> c14, c7;
+    const uint32_t n14 = tot14 ? chunks(f14, tot14, ZT_T_BC14, C + PC_BC14, c14) : 0;
+    const uint32_t n7 = tot7 ? chunks(f7, tot7, ZT_T_S6, C + PC_S6, c7) : 0;
+    if (!tot14) c14.assign(R + 1, {});
+    if (!tot7) c7.assign(R + 1, {});
+    // the later stages, each after the whole stage it reads
+    std::vector<uint32_t> climb, finish, s5, rank, fit, pick;
+    if (tot14) {
+        const uint32_t range = (uint32_t)std::min<uint64_t>(ZT_CLIMB_RANGE, std::max<uint64_t>(64, ((tot14 + 4095) / 4096 + 63) / 64 * 64));
+        const uint32_t gc = (uint32_t)groups_for(CFG_CLIMB, (tot14 + range - 1) / range, 4), gf = (uint32_t)groups_for(CFG_BC14, tot14, 256);
+        const uint32_t ic = add(ZT_T_CLIMB, 0, gc, 1, gc, C + PC_BC14, n14, C + PC_CLIMB);
+        const uint32_t ifn = add(ZT_T_FINISH, 0, gf, 1, gf, C + PC_CLIMB, gc, C + PC_FINISH);
+        climb = tasks[ic]; finish = tasks[ifn];
+    }
+    if (tot7) {
+        const uint32_t g5 = (uint32_t)groups_for(CFG_S5, (uint64_t)tot7 * 4, 256), g1 = (uint32_t)groups_for(CFG_S1, (uint64_t)tot7 / 16 + 1, 4);
+        s5 = tasks[add(ZT_T_S5, 0, g5, 1, g5, C + PC_S6, n7, C + PC_S5)];
+        rank = tasks[add(ZT_T_RANK, 0, g1, 1, g1, C + PC_S5, g5, C + PC_RANK)];
+        fit = tasks[add(ZT_T_FIT, 0, g1, 1, g1, C + PC_RANK, g1, C + PC_FIT)];
+        pick = tasks[add(ZT_T_PICK, 0, g1, 1, g1, C + PC_FIT, g1, C + PC_PICK)];
+    }
+    if (Q.segs.size() > ZT_MAX_SEGS) return fail(ZAPTEX_E_ARG, "batch plan needs %zu queue segments (at most %d)", Q.segs.size(), ZT_MAX_SEGS);
+    for (const auto &e : tasks)
+        if (e.size() >= (1u << ZT_TASK_SEG_SHIFT)) return fail(ZAPTEX_E_ARG, "batch plan has a segment of %zu tasks", e.size());
+    // the order: round-0 mip tasks led through the level-0 encode chunks, the later rounds, the chunks of their
+    // levels, then the stages
+    std::vector<uint32_t> &q = Q.queue;
+    const std::vector<uint32_t> lvl0 = even_merge(cat(c14[0]), cat(c7[0]));
+    q = R ? lead_merge(cat(round_segs[0]), lvl0, g_batch_mix) : lvl0;
+    for (uint32_t r = 1; r < R; r++) { const auto v = cat(round_segs[r]); q.insert(q.end(), v.begin(), v.end()); }
+    for (uint32_t r = 0; r < R; r++) { const auto v = even_merge(cat(c14[r + 1]), cat(c7[r + 1])); q.insert(q.end(), v.begin(), v.end()); }
+    for (const auto &v : { even_merge(climb, s5), even_merge(finish, rank), fit, pick }) q.insert(q.end(), v.begin(), v.end());
+    Q.nctl = ZT_CTL_COUNTERS + R + PC_N;
+    // what keeps the kernel from waiting forever, checked before anything runs: every task appears once, and when a
+    // task that waits comes up, every task counting towards its counter is already before it and `need` is their
+    // number
+    std::vector<uint32_t> total(Q.nctl, 0), seen(Q.nctl, 0), uses(Q.segs.size(), 0);
+    size_t ntasks = 0;
+    for (size_t i = 0; i < tasks.size(); i++) { total[Q.segs[i].done] += (uint32_t)tasks[i].size(); ntasks += tasks[i].size(); }
+    if (q.size() != ntasks) return fail(ZAPTEX_E_ARG, "batch plan: %zu queue entries for %zu tasks", q.size(), ntasks);
+    for (uint32_t e : q) {
+        const uint32_t sg = e >> ZT_TASK_SEG_SHIFT, j = e & ((1u << ZT_TASK_SEG_SHIFT) - 1u);
+        if (sg >= Q.segs.size() || j >= tasks[sg].size()) return fail(ZAPTEX_E_ARG, "batch plan: bad queue entry %08x", e);
+        const zt_seg &S = Q.segs[sg];
+        if (S.wait != ZT_NO_WAIT && (S.wait >= Q.nctl || S.need != total[S.wait] || seen[S.wait] != total[S.wait]))
+            return fail(ZAPTEX_E_ARG, "batch plan: a task of segment %u waits for tasks after it", sg);
+        seen[S.done]++;
+        uses[sg]++;
+    }
+    for (size_t i = 0; i < tasks.size(); i++)
+        if (uses[i] != tasks[i].size()) return fail(ZAPTEX_E_ARG, "batch plan: segment %zu queued %u of %zu tasks", i, uses[i], tasks[i].size());
+    return ZAPTEX_OK;
 }
 
 struct BatchSizes { size_t steps, items, mipjobs, jobs14, jobs7; uint64_t blocks14, blocks7; int n14, n7; };
@@ -1517,6 +1710,9 @@ struct zaptex_batch {
     void *bc7ws = nullptr;
     size_t bc7ws_bytes = 0;
     uint4 *climb = nullptr;
+    bool persistent = false;   // one zt_batch_persistent launch (else the per-stage kernels)
+    zt_batch_args args;
+    int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1535,13 +1731,27 @@ size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels) {
     batch_layout(w, h, fmt, levels, nullptr, nullptr, nullptr, &o);
     return o;
 }
+}  // extern "C"
+namespace {
+// the persistent queue's regions: task list, segments, control words (counted with null table addresses)
+struct QueueSizes { size_t queue, segs, ctl; };
+QueueSizes queue_sizes(const PersistentQueue &Q) {
+    return { align_up(Q.queue.size() * 4, 256), align_up(Q.segs.size() * sizeof(zt_seg), 256), align_up((size_t)Q.nctl * 4, 256) };
+}
+}  // namespace
+extern "C" {
+
 size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
     if (!t || n < 1) return 0;
     for (int i = 0; i < n; i++)
-        if (t[i].w < 1 || t[i].h < 1 || t[i].levels < 1 || t[i].levels > max_levels(t[i].w, t[i].h)) return 0;
+        if (!size_ok(t[i].w, t[i].h) || t[i].levels < 1 || t[i].levels > max_levels(t[i].w, t[i].h)) return 0;
     MipPlan P = plan_mips(t, n);
     BatchSizes z = batch_sizes(t, n, P);
-    return 256 + z.steps + z.items + z.mipjobs + z.jobs14 + z.jobs7 + ws_total(z.blocks7, z.blocks14);
+    PersistentQueue Q;
+    if (z.blocks14 >= (1ull << 31) || z.blocks7 >= (1ull << 31)) return 0;
+    if (plan_queue(P, family_jobs(t, n, P, false), family_jobs(t, n, P, true), (uint32_t)z.blocks14, (uint32_t)z.blocks7, PqTables(), Q)) return 0;
+    const QueueSizes qz = queue_sizes(Q);
+    return 256 + z.steps + z.items + z.mipjobs + z.jobs14 + z.jobs7 + qz.queue + qz.segs + qz.ctl + ws_total(z.blocks7, z.blocks14);
 }
 
 int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out) {
@@ -1560,38 +1770,41 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     }
     zaptex_batch *b = new zaptex_batch();
     HIP_TRY(hipGetDevice(&b->device));
+    const std::vector<FamJob> f14 = family_jobs(t, n, P, false), f7 = family_jobs(t, n, P, true);
+    PersistentQueue Q;
+    if (int rc = plan_queue(P, f14, f7, (uint32_t)z.blocks14, (uint32_t)z.blocks7, PqTables(), Q)) { delete b; return rc; }
+    const QueueSizes qz = queue_sizes(Q);
     uint8_t *p = (uint8_t *)align_up((uintptr_t)ws, 256);
     b->steps = (zt_mipstep *)p; p += z.steps;
     b->items = (uint2 *)p; p += z.items;
     b->mipjobs = (zt_mipjob *)p; p += z.mipjobs;
     zt_job *d14 = (zt_job *)p; p += z.jobs14;
     zt_job *d7 = (zt_job *)p; p += z.jobs7;
+    uint32_t *dqueue = (uint32_t *)p; p += qz.queue;
+    zt_seg *dsegs = (zt_seg *)p; p += qz.segs;
+    unsigned *dctl = (unsigned *)p; p += qz.ctl;
     b->bc7ws = p;
     b->bc7ws_bytes = ws_bytes_for(z.blocks7);
     b->climb = (uint4 *)(p + ws_climb_offset(z.blocks7));
     b->mip_launches = P.launches;
     b->tile_steps = P.tile_steps;
-    // encode jobs per family in texture / level order (level l >= 1 read from the mip arena)
+    // encode jobs per family in queue order (family_jobs: level 0 of every texture, then by mip round); level
+    // l >= 1 is read from the texture's mip arena
     std::vector<zt_job> h14, h7;
-    for (int i = 0; i < n; i++) {
-        std::vector<size_t> mo, oo;
-        batch_layout(t[i].w, t[i].h, t[i].fmt, t[i].levels, &mo, &oo, nullptr, nullptr);
-        int cw = t[i].w, ch = t[i].h;
-        const uint8_t *src = (const uint8_t *)t[i].base;
-        uint64_t stride = t[i].stride;
-        for (int l = 0; l < t[i].levels; l++) {
+    for (int fam = 0; fam < 2; fam++)
+        for (const FamJob &fj : fam ? f7 : f14) {
+            const zaptex_texture &tx = t[fj.tex];
+            std::vector<size_t> mo, oo;
+            batch_layout(tx.w, tx.h, tx.fmt, tx.levels, &mo, &oo, nullptr, nullptr);
+            int cw = tx.w, ch = tx.h;
+            for (int l = 0; l < fj.lvl; l++) { cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1; }
             zt_job j{};
-            j.src = src; j.dst = (uint8_t *)t[i].out + oo[l]; j.stride = stride; j.w = cw; j.h = ch; j.bw = (cw + 3) / 4; j.fmt = t[i].fmt;
-            const uint32_t nb = (uint32_t)(((cw + 3) / 4) * ((ch + 3) / 4));
-            if (t[i].fmt == ZAPTEX_BC7) { j.first = b->tot7; b->tot7 += nb; h7.push_back(j); }
-            else { j.first = b->tot14; b->tot14 += nb; h14.push_back(j); }
-            if (l + 1 < t[i].levels) {
-                src = (const uint8_t *)t[i].mips + mo[l];
-                cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
-                stride = (uint64_t)cw * 4;
-            }
+            j.src = fj.lvl ? (const uint8_t *)tx.mips + mo[fj.lvl - 1] : (const uint8_t *)tx.base;
+            j.stride = fj.lvl ? (uint64_t)cw * 4 : tx.stride;
+            j.dst = (uint8_t *)tx.out + oo[fj.lvl]; j.w = cw; j.h = ch; j.bw = (cw + 3) / 4; j.fmt = tx.fmt;
+            if (fam) { j.first = b->tot7; b->tot7 += fj.blocks; h7.push_back(j); }
+            else { j.first = b->tot14; b->tot14 += fj.blocks; h14.push_back(j); }
         }
-    }
     hipStream_t s = st->stream;
     if (!P.steps.empty()) HIP_TRY(hipMemcpyAsync(b->steps, P.steps.data(), P.steps.size() * sizeof(zt_mipstep), hipMemcpyHostToDevice, s));
     if (!P.items.empty()) HIP_TRY(hipMemcpyAsync(b->items, P.items.data(), P.items.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
@@ -1601,9 +1814,37 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     HIP_TRY(hipStreamSynchronize(s));
     b->j14.ext = d14; b->j14.n = (int)h14.size();
     b->j7.ext = d7; b->j7.n = (int)h7.size();
+    // the persistent kernel (the mode-1 split it runs covers M1K <= 8, the default 4): queue, segments, counters
+    b->persistent = g_batch_persistent && g_m1k <= 8;
+    if (b->persistent) {
+        PqTables tb;
+        tb.steps = b->steps; tb.items = b->items; tb.mipjobs = b->mipjobs;
+        if (int rc = plan_queue(P, f14, f7, b->tot14, b->tot7, tb, Q)) { delete b; return rc; }
+        if (queue_sizes(Q).queue > qz.queue || queue_sizes(Q).segs > qz.segs) { delete b; return fail(ZAPTEX_E_ARG, "batch queue outgrew its workspace"); }
+        HIP_TRY(hipMemcpyAsync(dqueue, Q.queue.data(), Q.queue.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(dsegs, Q.segs.data(), Q.segs.size() * sizeof(zt_seg), hipMemcpyHostToDevice, s));
+        zt_batch_args &A = b->args;
+        memset(&A, 0, sizeof A);
+        A.queue = dqueue; A.ntasks = (uint32_t)Q.queue.size(); A.nctl = Q.nctl; A.segs = dsegs; A.ctl = dctl;
+        A.j14 = b->j14; A.j7 = b->j7; A.tot14 = b->tot14; A.tot7 = b->tot7; A.climb = b->climb;
+        A.refill_min = g_climb_refill;
+        A.range = (uint32_t)std::min<uint64_t>(ZT_CLIMB_RANGE, std::max<uint64_t>(64, ((b->tot14 + 4095) / 4096 + 63) / 64 * 64));
+        if (b->tot7) A.L = lists_in(b->bc7ws, b->tot7);   // (no BC7 blocks: no lists, null counters)
+        A.m1k = g_m1k;
+        // the queue state starts cleared; the kernel's last workgroup clears it again for the next launch
+        HIP_TRY(hipMemsetAsync(dctl, 0, (size_t)Q.nctl * 4, s));
+        if (b->tot7) HIP_TRY(hipMemsetAsync(A.L.counters, 0, ZT_BC7_COUNTERS * 4, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        int per_cu = 0, cus = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)zt_batch_persistent, 256, 0));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+        b->grid = (g_batch_groups > 0 ? g_batch_groups : std::max(1, per_cu)) * std::max(1, cus);
+    }
     // capture: mip levels in order, then the encoders (BC1-5, then BC7), as one graph
     auto body = [&](hipStream_t cs) -> int {
         int rc;
+        if (b->persistent)
+            return launch("zt_batch_persistent", cs, [&] { hipLaunchKernelGGL(zt_batch_persistent, dim3(b->grid), dim3(256), 0, cs, b->args); });
         size_t tl = 0;  // kind-0 launches seen
         for (const MipLaunch &ml : b->mip_launches) {
             if (ml.kind == 0) {
@@ -1699,11 +1940,32 @@ int zaptex_set_option(int key, int value) {
         g_climb_refill = value;
         return ZAPTEX_OK;
     }
+    if (key == ZAPTEX_OPT_BATCH_PERSISTENT) {
+        g_batch_persistent = value != 0;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_BATCH_MIX) {
+        if (value < 0 || value > 64) return fail(ZAPTEX_E_ARG, "batch mix must be 0..64");
+        g_batch_mix = value;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_BATCH_CHUNKS) {
+        if (value < 1 || value > 64) return fail(ZAPTEX_E_ARG, "batch chunks per task must be 1..64");
+        g_batch_per = value;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_BATCH_GROUPS) {
+        if (value < 0 || value > 32) return fail(ZAPTEX_E_ARG, "batch workgroups per CU must be 0..32");
+        g_batch_groups = value;
+        return ZAPTEX_OK;
+    }
     return fail(ZAPTEX_E_ARG, "unknown option %d", key);
 }
 int zaptex_get_option(int key) {
     return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : key == ZAPTEX_OPT_CLIMB_REFILL ? g_climb_refill
-         : key == ZAPTEX_OPT_RDO_SEGMENT ? g_rdo_seg : key == ZAPTEX_OPT_RDO_WARMUP ? g_rdo_warm : -1;
+         : key == ZAPTEX_OPT_RDO_SEGMENT ? g_rdo_seg : key == ZAPTEX_OPT_RDO_WARMUP ? g_rdo_warm
+         : key == ZAPTEX_OPT_BATCH_PERSISTENT ? g_batch_persistent : key == ZAPTEX_OPT_BATCH_MIX ? g_batch_mix
+         : key == ZAPTEX_OPT_BATCH_CHUNKS ? g_batch_per : key == ZAPTEX_OPT_BATCH_GROUPS ? g_batch_groups : -1;
 }
 
 int zaptex_profile_collect(void) {

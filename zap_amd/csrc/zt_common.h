@@ -50,6 +50,14 @@ ZT_DEV int cvt_i32(float f) {
     return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : (int)0x80000000u;
 }
 
+// threadIdx.x as a value the optimiser cannot move.  The persistent batch kernel runs many kernels' bodies in one
+// loop; with threadIdx.x itself, loop-invariant code motion hoists each body's thread-index arithmetic to kernel
+// entry, where it stays live through every other body (their registers would add up instead of overlapping).
+ZT_DEV uint32_t zt_tid() {
+    uint32_t v = threadIdx.x;
+    asm volatile("" : "+v"(v));
+    return v;
+}
 ZT_DEV int chan(uint32_t p, int c) { return (int)((p >> (8 * c)) & 255u); }
 ZT_DEV float chanf(uint32_t p, int c) { return (float)chan(p, c); }
 ZT_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }

@@ -107,7 +107,43 @@ struct zt_mipstep {
     zt_mip_levels L;
 };
 
+// ---- the persistent batch kernel (zt_batch.hip)
+// One launch runs a whole batch (mip chains, BC1-family and BC7 encodes of many textures) as a queue of tasks, each
+// `per` consecutive workgroups of one of the batch's kernels.  A segment is a run of tasks of one kind: workgroups
+// vbase .. vbase + count - 1 of that kernel launched with vgrid workgroups.  Before a task starts, counter `wait`
+// must reach `need`; a finished task adds 1 to counter `done`.  The host orders the queue so that every task waits
+// only for tasks before it, which keeps the queue deadlock-free with any number of resident workgroups.
+enum { ZT_T_MIP_TILE, ZT_T_MIP_LEVEL, ZT_T_MIP_TAIL, ZT_T_BC14, ZT_T_CLIMB, ZT_T_FINISH, ZT_T_S6, ZT_T_S5, ZT_T_S1, ZT_T_RANK,
+       ZT_T_FIT, ZT_T_PICK };
+#define ZT_NO_WAIT 0xFFFFFFFFu
+#define ZT_TASK_SEG_SHIFT 24   // queue entry: segment << 24 | task within the segment
+#define ZT_MAX_SEGS 256
+struct zt_seg {
+    uint32_t kind, vbase, count, per, vgrid, wait, need, done;
+    uint32_t n, srgb;          // mip level jobs: job count, any sRGB
+    const void *a, *b;         // mip: steps (tile, tail) or jobs (level); tile items
+    uint64_t total;            // mip level jobs: destination pixels
+};
+// ctl words: [0] queue head, [1] finished workgroups, [2] dependency waits that timed out (kept across launches),
+// [ZT_CTL_COUNTERS + i] counter i.  The last workgroup to finish clears the queue head, the finished count, the
+// counters and the BC7 list counters for the next launch.
+#define ZT_CTL_COUNTERS 4
+struct zt_batch_args {
+    const uint32_t *queue;
+    uint32_t ntasks, nctl;     // tasks; ctl words in use
+    const zt_seg *segs;
+    unsigned *ctl;
+    zt_jobs j14, j7;           // BC1/BC3/BC4/BC5 and BC7 job tables (device arrays)
+    uint32_t tot14, tot7;
+    uint4 *climb;
+    int refill_min;
+    uint32_t range;
+    zt_bc7_lists L;
+    int m1k;
+};
+
 extern "C" {
+__global__ void zt_batch_persistent(zt_batch_args A);
 __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
 __global__ void zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L);
 __global__ void zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k);

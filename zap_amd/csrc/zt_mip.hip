@@ -107,7 +107,7 @@ namespace {
 ZT_DEV void tiles_body(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, const zt_mip_levels &L, int tx, int ty) {
     __shared__ float lin[256], thr[256];
     __shared__ uint32_t t0[64 * 64], t1[32 * 32];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = (int)zt_tid(), nt = blockDim.x;
     if (srgb)
         for (int i = tid; i < 256; i += nt) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
     __syncthreads();
@@ -185,7 +185,7 @@ namespace {
 ZT_DEV void tail_body(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, const zt_mip_levels &L) {
     __shared__ float lin[256], thr[256];
     __shared__ uint32_t a[64 * 64], b[32 * 64];
-    const int tid = threadIdx.x;
+    const int tid = (int)zt_tid();
     if (srgb)
         for (int i = tid; i < 256; i += blockDim.x) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
     __syncthreads();
@@ -222,17 +222,24 @@ extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *sr
 
 // ---------------------------------------------------------------- batched fused steps
 // Many textures' fused steps in one launch: zt_mip_tiles_batch runs one tile item per workgroup (items name a
-// step and a tile), zt_mip_tail_batch one step per workgroup.
-extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles_batch(const zt_mipstep *steps, const uint2 *items) {
-    const uint2 it = items[blockIdx.x];  // (step, tile index in row-major tile order of the step's level 1)
+// step and a tile), zt_mip_tail_batch one step per workgroup (any workgroup size).  The bodies take the item or
+// step index (bid), so the persistent batch kernel runs them as tasks.
+namespace {
+ZT_DEV void tiles_item_body(const zt_mipstep *steps, const uint2 *items, uint32_t bid) {
+    const uint2 it = items[bid];  // (step, tile index in row-major tile order of the step's level 1)
     const zt_mipstep &S = steps[it.x];
     const int tiles_x = (S.L.w[0] + 63) / 64;
     tiles_body(S.src, S.w, S.h, S.stride, S.srgb, S.nlev, S.L, (int)(it.y % (uint32_t)tiles_x), (int)(it.y / (uint32_t)tiles_x));
 }
-extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail_batch(const zt_mipstep *steps) {
-    const zt_mipstep &S = steps[blockIdx.x];
+ZT_DEV void tail_step_body(const zt_mipstep *steps, uint32_t bid) {
+    const zt_mipstep &S = steps[bid];
     tail_body(S.src, S.w, S.h, S.stride, S.srgb, S.nlev, S.L);
 }
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles_batch(const zt_mipstep *steps, const uint2 *items) {
+    tiles_item_body(steps, items, blockIdx.x);
+}
+extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail_batch(const zt_mipstep *steps) { tail_step_body(steps, blockIdx.x); }
 
 // ---------------------------------------------------------------- batched level
 // One level of many textures in one launch: lane per destination pixel over all jobs (grid-stride); a job is one
@@ -248,13 +255,15 @@ ZT_DEV int find_mip_job(const zt_mipjob *J, int n, uint64_t p) {
 }
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(256) zt_mip_batch(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb) {
+namespace {
+ZT_DEV void mip_batch_body(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
     __shared__ float lin[256], thr[256];
     if (any_srgb)
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
+        for (int i = tid; i < 256; i += blockDim.x) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
     __syncthreads();
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += step) {
+    const uint64_t step = (uint64_t)nblk * blockDim.x;
+    for (uint64_t p = (uint64_t)bid * blockDim.x + tid; p < total; p += step) {
         const zt_mipjob J = jobs[find_mip_job(jobs, njobs, p)];
         const int nw = J.w > 1 ? J.w / 2 : 1;
         const uint64_t local = p - J.first;
@@ -265,4 +274,8 @@ extern "C" __global__ void __launch_bounds__(256) zt_mip_batch(const zt_mipjob *
         const uint32_t c = *(const uint32_t *)(r1 + x0 * 4), d = *(const uint32_t *)(r1 + x1 * 4);
         *(uint32_t *)(J.dst + local * 4) = box4(a, b, c, d, J.srgb, lin, thr);
     }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_mip_batch(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb) {
+    mip_batch_body(jobs, njobs, total, any_srgb, blockIdx.x, gridDim.x);
 }
