@@ -1,0 +1,51 @@
+"""The forms of the BC1 hill climb give the reference's bytes: one lane per block with lane refill (zt_bc1_climb,
+used for large launches such as the 512-texture batch) and four or eight lanes per block, each trying one of the
+next moves from the same endpoints (zt_bc1_climb4 / zt_bc1_climb8, used up to ZAPTEX_OPT_CLIMB4_MAX blocks).  Each
+is forced in turn on the reference's hashes of the 1024^2 BC1 workload, the adversarial block set and the
+near-singular fixture, and against the oracle on odd sizes; the grouped climbs also with other wave ranges."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from zap_amd import synth
+
+pytestmark = pytest.mark.gpu
+OPT_CLIMB4_MAX, OPT_CLIMB4_RANGE, OPT_CLIMB_GROUP = 11, 12, 13
+
+
+@pytest.fixture(params=[("one-lane", 0, 0, 4), ("four-lane", 1 << 30, 0, 4), ("four-lane-r64", 1 << 30, 64, 4),
+                        ("four-lane-r1024", 1 << 30, 1024, 4), ("eight-lane", 1 << 30, 0, 8), ("eight-lane-r40", 1 << 30, 40, 8)],
+                ids=lambda p: p[0])
+def climb(zt, request):
+    keys = (OPT_CLIMB4_MAX, OPT_CLIMB4_RANGE, OPT_CLIMB_GROUP)
+    saved = [zt.get_option(k) for k in keys]
+    for k, v in zip(keys, request.param[1:]):
+        zt.set_option(k, v)
+    yield request.param[0]
+    for k, v in zip(keys, saved):
+        zt.set_option(k, v)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_climbs_match_reference_hashes(zt, golden, climb):
+    c1 = synth.c1_bc1(1024, 1)
+    for f in ("bc1", "bc3"):
+        assert sha(zt.bc_encode(c1, f)) == golden["outputs"][f"c1_1024_{f}"], (climb, f)
+    blocks = synth.blocks_to_image(synth.random_blocks(8192, 7))
+    for f in ("bc1", "bc3"):
+        assert sha(zt.bc_encode(blocks, f)) == golden["outputs"][f"blocks_8192_{f}"], (climb, f)
+    ent = golden["near_singular"]
+    img = np.frombuffer(bytes.fromhex(ent["input"]), np.uint8).reshape(4, 4 * ent["blocks"], 4)
+    for f in ("bc1", "bc3"):
+        assert zt.bc_encode(img, f).tobytes().hex() == ent[f], (climb, f)
+
+
+@pytest.mark.parametrize("w,h,seed", [(1, 1, 1), (5, 3, 3), (129, 67, 4), (1000, 750, 6)])
+def test_climbs_match_oracle(zt, oracle, climb, w, h, seed):
+    img = np.ascontiguousarray(synth.c1_bc1(max(w, h, 4), seed)[:h, :w])
+    for f in ("bc1", "bc3"):
+        assert np.array_equal(zt.bc_encode(img, f), oracle.bc_encode(img, f)), (climb, f)

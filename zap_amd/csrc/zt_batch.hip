@@ -20,7 +20,7 @@ constexpr uint32_t kWaitSpins = 1u << 22;   // about a second of s_sleep 8: far 
 
 ZT_DEV void run_task(const zt_batch_args &A, const zt_seg &S, uint32_t vb) {
     switch (S.kind) {
-    case ZT_T_MIP_TILE: tiles_item_body((const zt_mipstep *)S.a, (const uint2 *)S.b, vb); break;
+    case ZT_T_MIP_TILE: tiles_item_body<64>((const zt_mipstep *)S.a, (const uint2 *)S.b, vb); break;
     case ZT_T_MIP_LEVEL: mip_batch_body((const zt_mipjob *)S.a, (int)S.n, S.total, (int)S.srgb, vb, S.vgrid); break;
     case ZT_T_MIP_TAIL: tail_step_body((const zt_mipstep *)S.a, vb); break;
     case ZT_T_BC14: bc14_encode_body(A.j14, A.tot14, A.climb, vb, S.vgrid); break;

@@ -363,6 +363,100 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uin
     bc1_climb_body(jobs, total, climb, refill_min, range, blockIdx.x, gridDim.x);
 }
 
+// The same hill climb with four lanes per block, for images too small to fill the GPU one lane per block (a wave's
+// time is then its slowest block's chain of trials).  Lane q of a block's group tries move mv + q of the current
+// pass from the group's endpoints; the group commits the first of them in the reference's move order that improves
+// and goes on from the move after it, or from mv + 4 when none improves.  A move tried from the endpoints the
+// reference would try it from gives the reference's result: the moves before the committed one saw no improvement
+// from those same endpoints, and the moves after it are tried again from the new ones.  Groups refill from their
+// wave's range like the lanes of zt_bc1_climb; the results go to the same places.
+namespace {
+template <int G>
+ZT_DEV void bc1_climb_group_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, uint32_t range, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const int lane = tid & 63, q = lane & (G - 1), gbase = lane & ~(G - 1);
+    const uint64_t below = gbase ? (~0ull >> (64 - gbase)) : 0ull;
+    const uint32_t wave = (bid * blockDim.x + tid) >> 6, nwaves = (nblk * blockDim.x) >> 6;
+    for (uint64_t r0 = (uint64_t)wave * range; r0 < total; r0 += (uint64_t)nwaves * range) {
+        const uint32_t end = (uint32_t)min<uint64_t>(total, r0 + range);
+        uint32_t next = (uint32_t)r0;  // same on every lane
+        bool active = false;
+        uint32_t px[16];  // alpha cleared (bc1_prepare)
+        int spp = 0, best = 0, pass = 0, mv = 0;
+        bool improved = false, moved = false;
+        uint32_t b0 = 0, b1 = 0, bidx = 0, cur = 0, job = 0;
+        uint8_t *dst = nullptr;
+        for (;;) {
+            // idle groups take the next blocks of the range in group order (one ballot over the groups' first lanes)
+            const uint64_t need = __ballot(!active && q == 0);
+            const uint32_t mine = next + (uint32_t)__popcll(need & below);
+            next += (uint32_t)__popcll(need);
+            if (!active && mine < end) {
+                const uint4 st = climb[mine];
+                const zt_job J = job_at(jobs, (int)st.w);
+                if (J.fmt == 0 || J.fmt == 1) {
+                    const uint32_t b = mine - J.first;
+                    dst = J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8);
+                    b0 = st.x & 0xFFFFu; b1 = st.x >> 16; best = (int)st.y; bidx = st.z; pass = 0; mv = 0; improved = false; moved = false;
+                    cur = mine; job = st.w;
+                    gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+                    spp = bc1_prepare(px, px);
+                    active = true;
+                }
+            }
+            if (__ballot(active) == 0 && next >= end) break;
+            // this lane's move from the group's endpoints
+            const int m = mv + q;
+            uint32_t n0 = 0, n1 = 0;
+            int ne = 0;
+            bool better = false;
+            if (active && best > 0 && m < 12) {
+                const int ep = m / 6, ch = (m >> 1) % 3, d = (m & 1) ? 1 : -1;
+                const int sh = ch == 0 ? 11 : ch == 1 ? 5 : 0, mx = ch == 1 ? 63 : 31;
+                uint32_t c = ep ? b1 : b0;
+                const int v = (int)((c >> sh) & (uint32_t)mx);
+                if (v + d >= 0 && v + d <= mx) {
+                    c = (c & ~((uint32_t)mx << sh)) | ((uint32_t)(v + d) << sh);
+                    n0 = ep ? b0 : c; n1 = ep ? c : b1;
+                    ne = bc1_attempt_err(px, spp, n0, n1);   // puts (n0, n1) in canonical order first
+                    better = ne < best;
+                }
+            }
+            // the group's first improving move, taken by every lane of the group
+            const uint32_t gb = (uint32_t)(__ballot(better) >> gbase) & ((1u << G) - 1u);
+            const int first = gb ? __ffs(gb) - 1 : 0;
+            const uint32_t pair = (uint32_t)__shfl((int)(n0 | n1 << 16), gbase + first);
+            const int pbest = __shfl(ne, gbase + first);
+            if (active) {
+                bool done = best == 0;
+                if (!done) {
+                    if (gb) { best = pbest; b0 = pair & 0xFFFFu; b1 = pair >> 16; improved = true; moved = true; mv += first + 1; }
+                    else mv += G;
+                    if (best == 0) done = true;   // no later move can improve on 0
+                    else if (mv >= 12) {
+                        if (!improved || pass == 7) done = true;
+                        else { pass++; mv = 0; improved = false; }
+                    }
+                }
+                if (done) {
+                    if (q == 0) {
+                        if (moved) climb[cur] = make_uint4(b0 | b1 << 16, (uint32_t)best, 0u, job | 0x80000000u);
+                        else store8(dst, bc1_pack(b0, b1, bidx));
+                    }
+                    active = false;
+                }
+            }
+        }
+    }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb4(zt_jobs jobs, uint32_t total, uint4 *climb, uint32_t range) {
+    bc1_climb_group_body<4>(jobs, total, climb, range, blockIdx.x, gridDim.x);
+}
+extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb8(zt_jobs jobs, uint32_t total, uint4 *climb, uint32_t range) {
+    bc1_climb_group_body<8>(jobs, total, climb, range, blockIdx.x, gridDim.x);
+}
+
 // the indices of every block the hill climb moved (climb[g].w has bit 31 set): one full trial of its final pair
 namespace {
 ZT_DEV void bc1_finish_body(const zt_jobs &jobs, uint32_t total, const uint4 *climb, uint32_t bid, uint32_t nblk) {

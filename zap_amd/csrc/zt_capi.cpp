@@ -22,6 +22,9 @@ thread_local char g_err[512];
 std::mutex g_mu;     // per-device set-up, and the host slots' busy flags
 int g_m1k = 4;
 int g_climb_refill = 16;  // zt_bc1_climb refills once this many lanes of a wave are idle
+int g_climb4_max = 1 << 20;   // BC1-family blocks up to which a launch climbs with four lanes per block (zt_bc1_climb4)
+int g_climb4_range = 0;       // zt_bc1_climb4's blocks per wave (0: automatic)
+int g_climb_group = 4;        // lanes per block of the grouped climb (4 or 8)
 
 // launch shapes of the grid-stride kernels: workgroup size (64, 128 or 256) and the cap on workgroups;
 // zaptex_set_launch changes them (the autotuner's results), a batch captures those in force when it is built
@@ -312,12 +315,24 @@ int enqueue_bc14(const zt_jobs &J, uint32_t tot, uint4 *climb, hipStream_t s) {
     if (int rc = launch("zt_bc14_encode", s, [&] {
             hipLaunchKernelGGL(zt_bc14_encode, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, climb); }))
         return rc;
-    // ranges of up to ZT_CLIMB_RANGE blocks, smaller for small jobs so that about 4096 waves share the work
-    const uint32_t range = (uint32_t)std::min<uint64_t>(ZT_CLIMB_RANGE, std::max<uint64_t>(64, ((tot + 4095) / 4096 + 63) / 64 * 64));
-    const uint64_t waves = (tot + range - 1) / range;
-    if (int rc = launch("zt_bc1_climb", s, [&] {
-            hipLaunchKernelGGL(zt_bc1_climb, dim3(groups_for(CFG_CLIMB, waves, wc / 64)), dim3(wc), 0, s, J, tot, climb, g_climb_refill, range); }))
-        return rc;
+    if (tot <= (uint64_t)g_climb4_max) {
+        // G lanes per block: ranges of g_climb4_range blocks (automatic: 64 / G blocks per wave up to 4096 waves)
+        const uint32_t G = (uint32_t)g_climb_group, per = 64 / G;
+        const uint32_t range = g_climb4_range ? (uint32_t)g_climb4_range
+                                              : (uint32_t)std::max<uint64_t>(per, ((tot + 4095) / 4096 + per - 1) / per * per);
+        const uint64_t waves = (tot + range - 1) / range;
+        if (int rc = launch("zt_bc1_climb", s, [&] {
+                if (G == 8) hipLaunchKernelGGL(zt_bc1_climb8, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, J, tot, climb, range);
+                else hipLaunchKernelGGL(zt_bc1_climb4, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, J, tot, climb, range); }))
+            return rc;
+    } else {
+        // ranges of up to ZT_CLIMB_RANGE blocks, smaller for small jobs so that about 4096 waves share the work
+        const uint32_t range = (uint32_t)std::min<uint64_t>(ZT_CLIMB_RANGE, std::max<uint64_t>(64, ((tot + 4095) / 4096 + 63) / 64 * 64));
+        const uint64_t waves = (tot + range - 1) / range;
+        if (int rc = launch("zt_bc1_climb", s, [&] {
+                hipLaunchKernelGGL(zt_bc1_climb, dim3(groups_for(CFG_CLIMB, waves, wc / 64)), dim3(wc), 0, s, J, tot, climb, g_climb_refill, range); }))
+            return rc;
+    }
     return launch("zt_bc1_finish", s, [&] {
         hipLaunchKernelGGL(zt_bc1_finish, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, (const uint4 *)climb); });
 }
@@ -483,7 +498,12 @@ int enqueue_mip(const void *src, int w, int h, size_t stride, int srgb, void *ds
     });
 }
 
-// whole chain: fused tile passes (up to 7 levels each) while the source is larger than 128 on a side, then
+// level-1 tile side of the fused mip passes (ZAPTEX_OPT_MIP_TILE): 32 (up to 6 levels per pass; the default: four
+// times the workgroups of 64, 1 us faster on the 4096^2 chain and 0.7% on the 512-texture batch) or 64 (up to 7)
+int g_mip_tile = 32;
+int tile_levels(int tile) { return tile == 32 ? 6 : 7; }
+
+// whole chain: fused tile passes (up to 7 or 6 levels each) while the source is larger than 128 on a side, then
 // one workgroup for the rest; a level with a side of 1 and the other side above 128 goes level by level
 int enqueue_mip_chain(const void *base, int w, int h, size_t stride, int srgb, void *const *levels, int nlevels, hipStream_t s) {
     std::vector<int> lw(1, w), lh(1, h);
@@ -495,15 +515,20 @@ int enqueue_mip_chain(const void *base, int w, int h, size_t stride, int srgb, v
         const int cw = lw[l], ch = lh[l];
         zt_mip_levels L;
         memset(&L, 0, sizeof L);
-        if ((cw > 128 || ch > 128) && cw >= 2 && ch >= 2 && (lh[l + 1] + 63) / 64 <= 65535) {
+        const int T = g_mip_tile;
+        if ((cw > 128 || ch > 128) && cw >= 2 && ch >= 2 && (lh[l + 1] + T - 1) / T <= 65535) {
             int k = 0;
-            while (k < 7 && l + k < nlevels && lw[l + k] >= 2 && lh[l + k] >= 2) {
+            while (k < tile_levels(T) && l + k < nlevels && lw[l + k] >= 2 && lh[l + k] >= 2) {
                 L.p[k] = (uint8_t *)levels[l + k]; L.w[k] = lw[l + k + 1]; L.h[k] = lh[l + k + 1];
                 k++;
             }
-            dim3 grid((lw[l + 1] + 63) / 64, (lh[l + 1] + 63) / 64);
-            if ((rc = launch("zt_mip_tiles", s, [&] {
-                    hipLaunchKernelGGL(zt_mip_tiles, grid, dim3(256), 0, s, (const uint8_t *)src, cw, ch, (uint64_t)ss, srgb, k, L); })))
+            dim3 grid((lw[l + 1] + T - 1) / T, (lh[l + 1] + T - 1) / T);
+            if (T == 32) {
+                if ((rc = launch("zt_mip_tiles", s, [&] {
+                        hipLaunchKernelGGL(zt_mip_tiles32, grid, dim3(256), 0, s, (const uint8_t *)src, cw, ch, (uint64_t)ss, srgb, k, L); })))
+                    return rc;
+            } else if ((rc = launch("zt_mip_tiles", s, [&] {
+                           hipLaunchKernelGGL(zt_mip_tiles, grid, dim3(256), 0, s, (const uint8_t *)src, cw, ch, (uint64_t)ss, srgb, k, L); })))
                 return rc;
             src = levels[l + k - 1]; ss = (size_t)lw[l + k] * 4; l += k;
         } else if (cw <= 128 && ch <= 128) {
@@ -1412,8 +1437,9 @@ struct MipPlan {
     std::vector<size_t> tile_steps;  // kind-0 launches: offset of their steps
     std::vector<std::vector<int>> level_round;  // [texture][level]: the round whose step writes it (-1: level 0)
     int rounds = 0;
+    int tile = 64;                   // level-1 tile side of the kind-0 launches
 };
-MipPlan plan_mips(const zaptex_texture *t, int n) {
+MipPlan plan_mips(const zaptex_texture *t, int n, int tile) {
     struct TexStep { int kind, l, k; };
     std::vector<std::vector<TexStep>> ts(n);
     std::vector<std::vector<int>> LW(n), LH(n);
@@ -1429,7 +1455,7 @@ MipPlan plan_mips(const zaptex_texture *t, int n) {
             TexStep st;
             if ((w > 128 || h > 128) && w >= 2 && h >= 2) {
                 int k = 0;
-                while (k < 7 && l + k < L - 1 && LW[i][l + k] >= 2 && LH[i][l + k] >= 2) k++;
+                while (k < tile_levels(tile) && l + k < L - 1 && LW[i][l + k] >= 2 && LH[i][l + k] >= 2) k++;
                 st = { 0, l, k };
             } else if (w <= 128 && h <= 128) {
                 st = { 2, l, L - 1 - l };
@@ -1454,6 +1480,7 @@ MipPlan plan_mips(const zaptex_texture *t, int n) {
         return m;
     };
     MipPlan P;
+    P.tile = tile;
     P.rounds = (int)rounds;
     P.level_round.resize(n);
     for (int i = 0; i < n; i++) {
@@ -1481,7 +1508,7 @@ MipPlan plan_mips(const zaptex_texture *t, int n) {
                     const uint32_t si = (uint32_t)(P.steps.size() - step0);
                     P.steps.push_back(make_step(i, st));
                     if (kind == 0) {
-                        const uint32_t tiles = (uint32_t)(((LW[i][st.l + 1] + 63) / 64) * ((LH[i][st.l + 1] + 63) / 64));
+                        const uint32_t tiles = (uint32_t)(((LW[i][st.l + 1] + tile - 1) / tile) * ((LH[i][st.l + 1] + tile - 1) / tile));
                         for (uint32_t k = 0; k < tiles; k++) P.items.push_back(make_uint2(si, k));
                     }
                 }
@@ -1517,6 +1544,8 @@ int g_batch_persistent = 0;  // 1: batches run as one persistent kernel; 0 (defa
 int g_batch_mix = 2;         // level-0 encode tasks per round-0 mip task at the head of the queue
 int g_batch_per = 1;         // 256-block chunks per encode task (BC1-family start, BC7 stage 6)
 int g_batch_groups = 0;      // persistent workgroups per CU (0: as many as fit)
+// the batch's mip tiles: the persistent kernel runs the 64-pixel tile body only
+int batch_mip_tile() { return g_batch_persistent ? 64 : g_mip_tile; }
 
 // counters after the mip rounds' (round r: counter r)
 enum { PC_BC14, PC_CLIMB, PC_FINISH, PC_S6, PC_S5, PC_RANK, PC_FIT, PC_PICK, PC_N };
@@ -1702,6 +1731,7 @@ struct zaptex_batch {
     int device = 0;
     std::vector<MipLaunch> mip_launches;
     std::vector<size_t> tile_steps;
+    int mip_tile = 64;
     zt_mipstep *steps = nullptr;
     uint2 *items = nullptr;
     zt_mipjob *mipjobs = nullptr;
@@ -1745,7 +1775,7 @@ size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
     if (!t || n < 1) return 0;
     for (int i = 0; i < n; i++)
         if (!size_ok(t[i].w, t[i].h) || t[i].levels < 1 || t[i].levels > max_levels(t[i].w, t[i].h)) return 0;
-    MipPlan P = plan_mips(t, n);
+    MipPlan P = plan_mips(t, n, batch_mip_tile());
     BatchSizes z = batch_sizes(t, n, P);
     PersistentQueue Q;
     if (z.blocks14 >= (1ull << 31) || z.blocks7 >= (1ull << 31)) return 0;
@@ -1759,7 +1789,7 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     *out = nullptr;
     for (int i = 0; i < n; i++)
         if (int rc = check_texture(t[i])) return rc;
-    MipPlan P = plan_mips(t, n);
+    MipPlan P = plan_mips(t, n, batch_mip_tile());
     BatchSizes z = batch_sizes(t, n, P);
     if (z.blocks7 >= (1ull << 31) || z.blocks14 >= (1ull << 31)) return fail(ZAPTEX_E_ARG, "too many blocks in one batch");
     if (!ws || ws_bytes < zaptex_batch_workspace_size(t, n)) return fail(ZAPTEX_E_ARG, "workspace %zu < %zu", ws_bytes, zaptex_batch_workspace_size(t, n));
@@ -1788,6 +1818,7 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     b->climb = (uint4 *)(p + ws_climb_offset(z.blocks7));
     b->mip_launches = P.launches;
     b->tile_steps = P.tile_steps;
+    b->mip_tile = P.tile;
     // encode jobs per family in queue order (family_jobs: level 0 of every texture, then by mip round); level
     // l >= 1 is read from the texture's mip arena
     std::vector<zt_job> h14, h7;
@@ -1850,7 +1881,9 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
             if (ml.kind == 0) {
                 const zt_mipstep *steps = b->steps + b->tile_steps[tl++];
                 const uint2 *items = b->items + ml.first;
-                if ((rc = launch("zt_mip_tiles_batch", cs, [&] { hipLaunchKernelGGL(zt_mip_tiles_batch, dim3(ml.count), dim3(256), 0, cs, steps, items); })))
+                if ((rc = launch("zt_mip_tiles_batch", cs, [&] {
+                         if (b->mip_tile == 32) hipLaunchKernelGGL(zt_mip_tiles32_batch, dim3(ml.count), dim3(256), 0, cs, steps, items);
+                         else hipLaunchKernelGGL(zt_mip_tiles_batch, dim3(ml.count), dim3(256), 0, cs, steps, items); })))
                     return rc;
             } else if (ml.kind == 1) {
                 const zt_mipjob *jobs = b->mipjobs + ml.first;
@@ -1940,6 +1973,26 @@ int zaptex_set_option(int key, int value) {
         g_climb_refill = value;
         return ZAPTEX_OK;
     }
+    if (key == ZAPTEX_OPT_CLIMB4_MAX) {
+        if (value < 0) return fail(ZAPTEX_E_ARG, "grouped-climb threshold must be >= 0");
+        g_climb4_max = value;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_CLIMB4_RANGE) {
+        if (value != 0 && (value < 8 || value > 65536 || value % 8)) return fail(ZAPTEX_E_ARG, "grouped-climb range must be 0 or a multiple of 8 in 8..65536");
+        g_climb4_range = value;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_CLIMB_GROUP) {
+        if (value != 4 && value != 8) return fail(ZAPTEX_E_ARG, "grouped-climb lanes per block must be 4 or 8");
+        g_climb_group = value;
+        return ZAPTEX_OK;
+    }
+    if (key == ZAPTEX_OPT_MIP_TILE) {
+        if (value != 32 && value != 64) return fail(ZAPTEX_E_ARG, "mip tile side must be 32 or 64");
+        g_mip_tile = value;
+        return ZAPTEX_OK;
+    }
     if (key == ZAPTEX_OPT_BATCH_PERSISTENT) {
         g_batch_persistent = value != 0;
         return ZAPTEX_OK;
@@ -1965,7 +2018,9 @@ int zaptex_get_option(int key) {
     return key == ZAPTEX_OPT_BC7_M1K ? g_m1k : key == ZAPTEX_OPT_PROFILE ? (int)g_profile : key == ZAPTEX_OPT_CLIMB_REFILL ? g_climb_refill
          : key == ZAPTEX_OPT_RDO_SEGMENT ? g_rdo_seg : key == ZAPTEX_OPT_RDO_WARMUP ? g_rdo_warm
          : key == ZAPTEX_OPT_BATCH_PERSISTENT ? g_batch_persistent : key == ZAPTEX_OPT_BATCH_MIX ? g_batch_mix
-         : key == ZAPTEX_OPT_BATCH_CHUNKS ? g_batch_per : key == ZAPTEX_OPT_BATCH_GROUPS ? g_batch_groups : -1;
+         : key == ZAPTEX_OPT_BATCH_CHUNKS ? g_batch_per : key == ZAPTEX_OPT_BATCH_GROUPS ? g_batch_groups
+         : key == ZAPTEX_OPT_MIP_TILE ? g_mip_tile : key == ZAPTEX_OPT_CLIMB4_MAX ? g_climb4_max
+         : key == ZAPTEX_OPT_CLIMB4_RANGE ? g_climb4_range : key == ZAPTEX_OPT_CLIMB_GROUP ? g_climb_group : -1;
 }
 
 int zaptex_profile_collect(void) {

@@ -198,6 +198,8 @@ __global__ void zt_bc7_finish_learned(zt_jobs jobs, uint32_t total, int K, const
 #define ZT_CLIMB_RANGE 1024u
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb);
 __global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range);
+__global__ void zt_bc1_climb4(zt_jobs jobs, uint32_t total, uint4 *climb, uint32_t range);   // four lanes per block
+__global__ void zt_bc1_climb8(zt_jobs jobs, uint32_t total, uint4 *climb, uint32_t range);   // eight lanes per block
 __global__ void zt_bc1_finish(zt_jobs jobs, uint32_t total, const uint4 *climb);
 __global__ void zt_bc6h_encode(zt_jobs jobs, uint32_t total);
 // ASTC 4x4 encode in stages (workspace regions, each sized for `total` blocks):
@@ -238,9 +240,11 @@ __global__ void zt_bc_shuffle(const uint8_t *in, uint8_t *out, uint64_t nb, int 
 // mips: one level (any size); tile chain (levels 1..k inside a workgroup tile); float level
 __global__ void zt_mip_level(const uint8_t *src, int w, int h, uint64_t stride, int srgb, uint8_t *dst, uint64_t dst_stride);
 __global__ void zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
+__global__ void zt_mip_tiles32(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
 __global__ void zt_mip_tail(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, zt_mip_levels L);
 __global__ void zt_mip_batch(const zt_mipjob *jobs, int njobs, uint64_t total, int any_srgb);
 __global__ void zt_mip_tiles_batch(const zt_mipstep *steps, const uint2 *items);
+__global__ void zt_mip_tiles32_batch(const zt_mipstep *steps, const uint2 *items);
 __global__ void zt_mip_tail_batch(const zt_mipstep *steps);
 __global__ void zt_mip_level_f(const float *src, int w, int h, uint64_t stride, float *dst, uint64_t dst_stride);
 }

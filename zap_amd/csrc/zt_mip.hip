@@ -101,44 +101,47 @@ ZT_DEV uint32_t box4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int srgb, c
 }
 }  // namespace
 
-// One 64 x 64 tile (tx, ty) of level 1 and the matching tiles of levels 2..nlev.  L.p[i] / L.w[i], L.h[i]:
-// level i+1 buffers (packed rows).  Called by every thread of a workgroup (it synchronises).
+// One T x T tile (tx, ty) of level 1 (T = 64 or 32) and the matching tiles of levels 2..nlev (at most
+// log2(T) + 1 levels).  L.p[i] / L.w[i], L.h[i]: level i+1 buffers (packed rows).  Called by every thread of a
+// workgroup (it synchronises).
 namespace {
+template <int T>
 ZT_DEV void tiles_body(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev, const zt_mip_levels &L, int tx, int ty) {
     __shared__ float lin[256], thr[256];
-    __shared__ uint32_t t0[64 * 64], t1[32 * 32];
+    __shared__ uint32_t t0[T * T], t1[T / 2 * T / 2];
+    constexpr int H = T / 2, NP = T * T / 2 / 256;   // pixel pairs per row of the tile; pairs per thread (256 threads)
     const int tid = (int)zt_tid(), nt = blockDim.x;
     if (srgb)
         for (int i = tid; i < 256; i += nt) { lin[i] = g_srgb_lin[i]; thr[i] = i < 255 ? g_srgb_thr[i] : 0.0f; }
     __syncthreads();
-    // level 1: 64 x 64 pixels of the tile.  A tile inside both level-1 sides with 16-byte aligned rows and 256
-    // threads loads straight through: 8 pixel pairs per thread, each two 16-byte reads (4 source pixels of two
-    // rows), all 16 issued before the first is used, so a workgroup keeps 64 KB in flight.  Other tiles take 16
-    // pixels per thread with 8-byte reads.
+    // level 1: T x T pixels of the tile.  A tile inside both level-1 sides with 16-byte aligned rows and 256
+    // threads loads straight through: NP pixel pairs per thread, each two 16-byte reads (4 source pixels of two
+    // rows), all issued before the first is used, so a workgroup keeps 4 T^2 bytes in flight.  Other tiles take
+    // their pixels one at a time with 8-byte reads.
     const int w1 = L.w[0], h1 = L.h[0];
-    if (nt == 256 && (tx + 1) * 64 <= w1 && (ty + 1) * 64 <= h1 && ((((uintptr_t)src) | stride) & 15) == 0) {
-        uint4 r0[8], r1[8];
+    if (nt == 256 && (tx + 1) * T <= w1 && (ty + 1) * T <= h1 && ((((uintptr_t)src) | stride) & 15) == 0) {
+        uint4 r0[NP], r1[NP];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int pp = k * 256 + tid, x = tx * 64 + (pp & 31) * 2, y = ty * 64 + (pp >> 5);
+        for (int k = 0; k < NP; k++) {
+            const int pp = k * 256 + tid, x = tx * T + (pp % H) * 2, y = ty * T + pp / H;
             const uint8_t *a = src + (uint64_t)(2 * y) * stride + (uint64_t)x * 8;
             r0[k] = *(const uint4 *)a;
             r1[k] = *(const uint4 *)(a + stride);
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int pp = k * 256 + tid, lx = (pp & 31) * 2, ly = pp >> 5, x = tx * 64 + lx, y = ty * 64 + ly;
+        for (int k = 0; k < NP; k++) {
+            const int pp = k * 256 + tid, lx = (pp % H) * 2, ly = pp / H, x = tx * T + lx, y = ty * T + ly;
             const uint32_t o0 = box4(r0[k].x, r0[k].y, r1[k].x, r1[k].y, srgb, lin, thr);
             const uint32_t o1 = box4(r0[k].z, r0[k].w, r1[k].z, r1[k].w, srgb, lin, thr);
-            t0[ly * 64 + lx] = o0;
-            t0[ly * 64 + lx + 1] = o1;
+            t0[ly * T + lx] = o0;
+            t0[ly * T + lx + 1] = o1;
             uint8_t *d = L.p[0] + ((uint64_t)y * w1 + x) * 4;
             if ((((uintptr_t)d) & 7) == 0) *(uint2 *)d = make_uint2(o0, o1);
             else { *(uint32_t *)d = o0; *(uint32_t *)(d + 4) = o1; }
         }
     } else {
-        for (int p = tid; p < 64 * 64; p += nt) {
-            const int lx = p & 63, ly = p >> 6, x = tx * 64 + lx, y = ty * 64 + ly;
+        for (int p = tid; p < T * T; p += nt) {
+            const int lx = p % T, ly = p / T, x = tx * T + lx, y = ty * T + ly;
             uint32_t o = 0;
             if (x < w1 && y < h1) {
                 const uint8_t *r0 = src + (uint64_t)(2 * y) * stride + (uint64_t)(2 * x) * 4, *r1 = r0 + stride;
@@ -156,8 +159,8 @@ ZT_DEV void tiles_body(const uint8_t *src, int w, int h, uint64_t stride, int sr
         }
     }
     __syncthreads();
-    // levels 2..nlev inside the tile, ping-ponging between t0 (64 wide) and t1 (32 wide)
-    int side = 32;
+    // levels 2..nlev inside the tile, ping-ponging between t0 (T wide) and t1 (T/2 wide)
+    int side = H;
     for (int l = 1; l < nlev; l++, side >>= 1) {
         uint32_t *srcT = (l & 1) ? t0 : t1, *dstT = (l & 1) ? t1 : t0;
         const int sw = side * 2;
@@ -174,10 +177,14 @@ ZT_DEV void tiles_body(const uint8_t *src, int w, int h, uint64_t stride, int sr
 }
 }  // namespace
 
-// grid: tiles of 64 x 64 level-1 pixels; block 256
+// grid: tiles of 64 x 64 (zt_mip_tiles) or 32 x 32 (zt_mip_tiles32) level-1 pixels; block 256
 extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
                                                                  zt_mip_levels L) {
-    tiles_body(src, w, h, stride, srgb, nlev, L, blockIdx.x, blockIdx.y);
+    tiles_body<64>(src, w, h, stride, srgb, nlev, L, blockIdx.x, blockIdx.y);
+}
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles32(const uint8_t *src, int w, int h, uint64_t stride, int srgb, int nlev,
+                                                                   zt_mip_levels L) {
+    tiles_body<32>(src, w, h, stride, srgb, nlev, L, blockIdx.x, blockIdx.y);
 }
 
 // one workgroup: levels from a source of at most 128 x 128, with the reference's clamping when a side is 1
@@ -225,11 +232,11 @@ extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail(const uint8_t *sr
 // step and a tile), zt_mip_tail_batch one step per workgroup (any workgroup size).  The bodies take the item or
 // step index (bid), so the persistent batch kernel runs them as tasks.
 namespace {
-ZT_DEV void tiles_item_body(const zt_mipstep *steps, const uint2 *items, uint32_t bid) {
+template <int T> ZT_DEV void tiles_item_body(const zt_mipstep *steps, const uint2 *items, uint32_t bid) {
     const uint2 it = items[bid];  // (step, tile index in row-major tile order of the step's level 1)
     const zt_mipstep &S = steps[it.x];
-    const int tiles_x = (S.L.w[0] + 63) / 64;
-    tiles_body(S.src, S.w, S.h, S.stride, S.srgb, S.nlev, S.L, (int)(it.y % (uint32_t)tiles_x), (int)(it.y / (uint32_t)tiles_x));
+    const int tiles_x = (S.L.w[0] + T - 1) / T;
+    tiles_body<T>(S.src, S.w, S.h, S.stride, S.srgb, S.nlev, S.L, (int)(it.y % (uint32_t)tiles_x), (int)(it.y / (uint32_t)tiles_x));
 }
 ZT_DEV void tail_step_body(const zt_mipstep *steps, uint32_t bid) {
     const zt_mipstep &S = steps[bid];
@@ -237,7 +244,10 @@ ZT_DEV void tail_step_body(const zt_mipstep *steps, uint32_t bid) {
 }
 }  // namespace
 extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles_batch(const zt_mipstep *steps, const uint2 *items) {
-    tiles_item_body(steps, items, blockIdx.x);
+    tiles_item_body<64>(steps, items, blockIdx.x);
+}
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles32_batch(const zt_mipstep *steps, const uint2 *items) {
+    tiles_item_body<32>(steps, items, blockIdx.x);
 }
 extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail_batch(const zt_mipstep *steps) { tail_step_body(steps, blockIdx.x); }
 
