@@ -1,4 +1,5 @@
-/* Exhaustive check of the kernels' BC7 endpoint quantisers (zap_amd/csrc/zt_bc7_core.h: div255, quant_pbit1,
+/* Exhaustive check of the kernels' BC7 endpoint quantisers (zap_amd/csrc/zt_common.h: div255, also used by the
+ * BC1 565 quantiser; zt_bc7_core.h: quant_pbit1,
  * quant_plain) against the reference's arithmetic (zap_tex.h:426-438 zap__bc7q, :613-624 zap__bc7_qn).
  *   ./quant_exhaustive [stride] [limit]
  * checks every stride-th float bit pattern: div255 for all finite floats, the quantisers for |v| <= limit

@@ -16,8 +16,8 @@ ZT_DEV uint32_t rgb565_expand(uint32_t c) {  // zap_tex.h:41-44, packed R | G <<
     uint32_t R = (c >> 11) & 31, G = (c >> 5) & 63, B = c & 31;
     return ((R << 3) | (R >> 2)) | ((G << 2) | (G >> 4)) << 8 | ((B << 3) | (B >> 2)) << 16;
 }
-ZT_DEV uint32_t to565(float r, float g, float b) {  // zap_tex.h:45-49
-    int R = cvt_i32(r * 31.0f / 255.0f + 0.5f), G = cvt_i32(g * 63.0f / 255.0f + 0.5f), B = cvt_i32(b * 31.0f / 255.0f + 0.5f);
+ZT_DEV uint32_t to565(float r, float g, float b) {  // zap_tex.h:45-49; x / 255.0f as div255 (+0.5 erases a -0 difference)
+    int R = cvt_i32(div255(r * 31.0f) + 0.5f), G = cvt_i32(div255(g * 63.0f) + 0.5f), B = cvt_i32(div255(b * 31.0f) + 0.5f);
     return (uint32_t)(clampi(R, 0, 31) << 11 | clampi(G, 0, 63) << 5 | clampi(B, 0, 31));
 }
 
@@ -199,68 +199,61 @@ ZT_DEV void climb_apply(Climb &s, uint32_t n0, uint32_t n1, int ne) {
 }
 ZT_DEV uint64_t bc1_pack(uint32_t c0, uint32_t c1, uint32_t idx) { return (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)idx << 32; }
 
-// BC4 palette of an endpoint pair (zap_tex.h:172-176): 8 interpolants when a0 > a1, else 6 plus 0 and 255
-ZT_DEV void bc4_palette(int a0, int a1, int pal[8]) {
+// BC4 palette of an endpoint pair with a0 > a1 (zap_tex.h:172-176, first branch): a0, a1 and the six interpolants
+// ((8 - i) a0 + (i - 1) a1 + 3) / 7.  The encoder never meets a0 <= a1 with a0 != a1 (a pair that would be one is
+// replaced by (hi, lo), and hi == lo is a flat block), so the other branch is the decoder's only.  The numerators are
+// at most 7 * 255 + 3 = 1788, below the 13109 up to which (n * 9363) >> 16 = floor(n / 7) exactly (checked for every
+// n by tests/test_kernel_identities.py).
+ZT_DEV void bc4_palette8(int a0, int a1, int pal[8]) {
     pal[0] = a0; pal[1] = a1;
-    if (a0 > a1) {
+    int n = 6 * a0 + a1 + 3;
 #pragma unroll
-        for (int i = 2; i < 8; i++) pal[i] = ((8 - i) * a0 + (i - 1) * a1 + 3) / 7;
-    } else {
-#pragma unroll
-        for (int i = 2; i < 6; i++) pal[i] = ((6 - i) * a0 + (i - 1) * a1 + 2) / 5;
-        pal[6] = 0; pal[7] = 255;
-    }
+    for (int i = 2; i < 8; i++, n += a1 - a0) pal[i] = (int)(__umul24((uint32_t)n, 9363u) >> 16);
 }
 
 // BC4 on channel `ch` of the block; returns the 8-byte block (a0, a1, 48 index bits).  The 9 inset pairs
 // (zap_tex.h:198-213) are compared by SSE alone, which does not depend on how ties between palette entries are
-// broken: key_j = v * 2 pal_j - pal_j^2 (one 24-bit multiply-add) and SSE = sum (v^2 - max_j key_j).  Only the
-// chosen pair's indices are then computed, with the reference's lowest-index rule.
+// broken: key_j = v * 2 pal_j - pal_j^2 (one 24-bit multiply-add: the constant goes through no_fold, or the compiler
+// splits it into a multiply and a subtract) and SSE = sum (v^2 - max_j key_j).  Only the chosen pair's indices are
+// then computed, with the reference's lowest-index rule.  A flat block (hi == lo) is (hi, hi) with every index 0,
+// which is what all nine of its pairs give.
 ZT_DEV uint64_t bc4_block(const uint32_t px[16], int ch) {
     int v[16], lo = 255, hi = 0, svv = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) { v[i] = chan(px[i], ch); lo = min(lo, v[i]); hi = max(hi, v[i]); svv += v[i] * v[i]; }
+    if (hi == lo) return (uint64_t)hi | (uint64_t)hi << 8;
     int best = 1 << 30, b0 = hi, b1 = lo;
 #pragma unroll 1
     for (int pr = 0; pr < 9; pr++) {
-            const int d0 = pr / 3, d1 = pr % 3;
-            int a0 = hi - d0, a1 = lo + d1;
-            if (a0 <= a1) { a0 = hi; a1 = lo; }
-            int e;
-            if (a0 == a1) {
-                e = 0;
+        const int d0 = pr / 3, d1 = pr % 3;
+        int a0 = hi - d0, a1 = lo + d1;
+        if (a0 <= a1) { a0 = hi; a1 = lo; }   // hi > lo here
+        int pal[8], P2[8], K[8];
+        bc4_palette8(a0, a1, pal);
 #pragma unroll
-                for (int i = 0; i < 16; i++) e += (v[i] - a0) * (v[i] - a0);
-            } else {
-                int pal[8], P2[8], K[8];
-                bc4_palette(a0, a1, pal);
-#pragma unroll
-                for (int j = 0; j < 8; j++) { P2[j] = 2 * pal[j]; K[j] = -pal[j] * pal[j]; }
-                int s = 0;
-#pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    int m = __mul24(v[i], P2[0]) + K[0];
-#pragma unroll
-                    for (int j = 1; j < 8; j++) m = max(m, __mul24(v[i], P2[j]) + K[j]);
-                    s += m;
-                }
-                e = svv - s;
-            }
-            if (e < best) { best = e; b0 = a0; b1 = a1; }
-    }
-    uint64_t bi = 0;
-    if (b0 != b1) {
-        int pal[8], P32[8], K[8];
-        bc4_palette(b0, b1, pal);
-#pragma unroll
-        for (int j = 0; j < 8; j++) { P32[j] = 32 * pal[j]; K[j] = 15 - j - 16 * pal[j] * pal[j]; }
+        for (int j = 0; j < 8; j++) { P2[j] = 2 * pal[j]; K[j] = no_fold(-pal[j] * pal[j]); }
+        int sk = 0;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            int bk = __mul24(v[i], P32[0]) + K[0];
+            int m = __mul24(v[i], P2[0]) + K[0];
 #pragma unroll
-            for (int j = 1; j < 8; j++) bk = max(bk, __mul24(v[i], P32[j]) + K[j]);
-            bi |= (uint64_t)(15 - (bk & 15)) << (3 * i);
+            for (int j = 1; j < 8; j++) m = max(m, __mul24(v[i], P2[j]) + K[j]);
+            sk += m;
         }
+        const int e = svv - sk;
+        if (e < best) { best = e; b0 = a0; b1 = a1; }
+    }
+    int pal[8], P32[8], K[8];
+    bc4_palette8(b0, b1, pal);
+#pragma unroll
+    for (int j = 0; j < 8; j++) { P32[j] = 32 * pal[j]; K[j] = no_fold(15 - j - 16 * pal[j] * pal[j]); }
+    uint64_t bi = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int bk = __mul24(v[i], P32[0]) + K[0];
+#pragma unroll
+        for (int j = 1; j < 8; j++) bk = max(bk, __mul24(v[i], P32[j]) + K[j]);
+        bi |= (uint64_t)(15 - (bk & 15)) << (3 * i);
     }
     return (uint64_t)b0 | (uint64_t)b1 << 8 | bi << 16;
 }

@@ -31,15 +31,6 @@ template <int NC, int C0> ZT_DEV uint32_t sub_px(uint32_t p) {
     return v & (uint32_t)((1ull << (8 * NC)) - 1ull);
 }
 
-// RN(v / 255) from one product and one FMA correction (the correction of a product by the correctly rounded
-// reciprocal gives the correctly rounded quotient): equal to the division for every finite float except v = -0,
-// which gives +0 (every caller adds 0.5 or subtracts a p-bit next, which erases the sign of a zero).  Checked
-// exhaustively against v / 255.0f by tests/quant_exhaustive.c.
-ZT_DEV float div255(float v) {
-    const float R = 1.0f / 255.0f, q0 = v * R;
-    return __builtin_fmaf(__builtin_fmaf(-q0, 255.0f, v), R, q0);
-}
-
 // ---- p-bit quantiser for one channel (zap_tex.h:426-438); N = bits including the p-bit, t = v / 255 * (2^N - 1)
 // (shared by both p-bits).  The reference tries g-1, g, g+1, each clamped to [0, mx], and keeps the first of the
 // smallest errors.  Here g is clamped to [1, mx-1] first, so the three candidates need no clamp: where that moves

@@ -58,6 +58,14 @@ ZT_DEV uint32_t zt_tid() {
     asm volatile("" : "+v"(v));
     return v;
 }
+// RN(v / 255) from one product and one FMA correction (the correction of a product by the correctly rounded
+// reciprocal gives the correctly rounded quotient): equal to the division for every finite float except v = -0,
+// which gives +0 (every caller adds 0.5 or subtracts a p-bit next, which erases the sign of a zero).  Checked
+// exhaustively against v / 255.0f by tests/quant_exhaustive.c.
+ZT_DEV float div255(float v) {
+    const float R = 1.0f / 255.0f, q0 = v * R;
+    return __builtin_fmaf(__builtin_fmaf(-q0, 255.0f, v), R, q0);
+}
 ZT_DEV int chan(uint32_t p, int c) { return (int)((p >> (8 * c)) & 255u); }
 ZT_DEV float chanf(uint32_t p, int c) { return (float)chan(p, c); }
 ZT_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
