@@ -56,7 +56,7 @@ ZT_DEV void stage6_body(const zt_jobs &jobs, uint32_t total, const zt_bc7_lists 
     }
 }
 }  // namespace
-extern "C" __global__ void __launch_bounds__(256) zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L) {
+extern "C" __global__ void __launch_bounds__(256, 4) zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L) {
     stage6_body(jobs, total, L, blockIdx.x, gridDim.x);
 }
 

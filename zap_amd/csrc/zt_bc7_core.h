@@ -254,7 +254,7 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t sn
 
 // ---- PCA start, fit, at most 2 least-squares refinements (zap_tex.h:489-515)
 // R: refresh the (then non-const) pixel registers between the PCA passes (pca's REFRESH)
-template <int NC, int N, int IB, bool SHARED, bool FULL, bool R = false, typename PX>
+template <int NC, int N, int IB, bool SHARED, bool FULL, bool R = false, bool RT = false, typename PX>
 ZT_DEV int encode_subset(PX *px, const ChanWords &T, uint32_t mask, int cnt, int q[2][4], int pb[2], uint64_t &idx) {
     int P[NC];
     chan_sums<NC, 0, FULL>(T, mask, P);
@@ -269,7 +269,15 @@ ZT_DEV int encode_subset(PX *px, const ChanWords &T, uint32_t mask, int cnt, int
 #pragma unroll
         for (int g = 0; g < 4; g++) KW[g] = weight_bytes<IB>(idx_bytes4(idx, g));
         float aa, ab, bb, sa[NC], sb[NC];
-        ls_sums<NC, 0>(T, KW, P, cnt, aa, ab, bb, sa, sb);
+        if constexpr (RT) {   // the channel words again, so they are not live across the fits
+            uint32_t pr[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) pr[k] = px[k];
+            refresh16(pr);
+            ChanWords T2;
+            chan_words(pr, T2);
+            ls_sums<NC, 0>(T2, KW, P, cnt, aa, ab, bb, sa, sb);
+        } else ls_sums<NC, 0>(T, KW, P, cnt, aa, ab, bb, sa, sb);
         float det = aa * bb - ab * ab;
         if (det < 1e-6f && det > -1e-6f) break;
         float nlo[4], nhi[4];
@@ -297,7 +305,7 @@ ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
     uint64_t idx;
     ChanWords T;
     chan_words(px, T);
-    int err = encode_subset<4, 8, 4, false, true>(px, T, 0xFFFFu, 16, q, pb, idx);
+    int err = encode_subset<4, 8, 4, false, true, false, true>(px, T, 0xFFFFu, 16, q, pb, idx);
     if (idx & 8) {  // anchor MSB set: swap endpoints and p-bits, invert indices
 #pragma unroll
         for (int c = 0; c < 4; c++) { int t = q[0][c]; q[0][c] = q[1][c]; q[1][c] = t; }

@@ -10,8 +10,9 @@
 
 template <int KIND> __global__ void __launch_bounds__(256) zt_valu_probe(uint32_t *out, int iters, uint32_t seed) {
     uint32_t a[8];
+    uint64_t w[8];   // the 64-bit chains of v_mad_u64_u32
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int k = 0; k < 8; k++) a[k] = t * 2654435761u + seed * (k + 1);
+    for (int k = 0; k < 8; k++) { a[k] = t * 2654435761u + seed * (k + 1); w[k] = a[k]; }
     const uint32_t b = t ^ seed, sel = 0x05040100u ^ (seed & 0x03030303u);
     for (int it = 0; it < iters; it++) {
 #pragma unroll
@@ -22,11 +23,14 @@ template <int KIND> __global__ void __launch_bounds__(256) zt_valu_probe(uint32_
             else if (KIND == 2) asm volatile("v_fma_f32 %0, %0, %1, %1" : "+v"(a[k]) : "v"(b));
             else if (KIND == 3) asm volatile("v_max_i32 %0, %0, %1" : "+v"(a[k]) : "v"(b));
             else if (KIND == 4) asm volatile("v_lshl_add_u32 %0, %0, 2, %1" : "+v"(a[k]) : "v"(b));
-            else asm volatile("v_perm_b32 %0, %0, %1, %2" : "+v"(a[k]) : "v"(b), "v"(sel));
+            else if (KIND == 5) asm volatile("v_perm_b32 %0, %0, %1, %2" : "+v"(a[k]) : "v"(b), "v"(sel));
+            else if (KIND == 6) asm volatile("v_mad_u64_u32 %0, vcc, %1, %1, %0" : "+v"(w[k]) : "v"(b) : "vcc");
+            else if (KIND == 7) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a[k]) : "v"(b));
+            else asm volatile("v_mad_u32_u24 %0, %0, %1, %0" : "+v"(a[k]) : "v"(b));
         }
     }
     uint32_t r = 0;
-    for (int k = 0; k < 8; k++) r ^= a[k];
+    for (int k = 0; k < 8; k++) r ^= a[k] ^ (uint32_t)w[k] ^ (uint32_t)(w[k] >> 32);
     out[t] = r;
 }
 
@@ -46,7 +50,8 @@ int ztprobe_copy(const void *src, void *dst, uint64_t bytes, void *stream) {
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-// kind 0 v_add_u32, 1 v_dot4_u32_u8, 2 v_fma_f32, 3 v_max_i32, 4 v_lshl_add_u32, 5 v_perm_b32;
+// kind 0 v_add_u32, 1 v_dot4_u32_u8, 2 v_fma_f32, 3 v_max_i32, 4 v_lshl_add_u32, 5 v_perm_b32, 6 v_mad_u64_u32,
+// 7 v_mul_lo_u32, 8 v_mad_u32_u24;
 // out: groups * 256 uint32 of device memory
 int ztprobe_valu(int kind, int groups, int iters, uint32_t *out, void *stream) {
     hipStream_t s = (hipStream_t)stream;
@@ -58,6 +63,9 @@ int ztprobe_valu(int kind, int groups, int iters, uint32_t *out, void *stream) {
     case 3: hipLaunchKernelGGL(zt_valu_probe<3>, dim3(groups), dim3(256), 0, s, out, iters, 12345u); break;
     case 4: hipLaunchKernelGGL(zt_valu_probe<4>, dim3(groups), dim3(256), 0, s, out, iters, 12345u); break;
     case 5: hipLaunchKernelGGL(zt_valu_probe<5>, dim3(groups), dim3(256), 0, s, out, iters, 12345u); break;
+    case 6: hipLaunchKernelGGL(zt_valu_probe<6>, dim3(groups), dim3(256), 0, s, out, iters, 12345u); break;
+    case 7: hipLaunchKernelGGL(zt_valu_probe<7>, dim3(groups), dim3(256), 0, s, out, iters, 12345u); break;
+    case 8: hipLaunchKernelGGL(zt_valu_probe<8>, dim3(groups), dim3(256), 0, s, out, iters, 12345u); break;
     default: return 1;
     }
     return hipGetLastError() == hipSuccess ? 0 : 2;
