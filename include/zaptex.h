@@ -49,8 +49,9 @@ enum {
     ZAPTEX_OPT_BATCH_GROUPS = 9,     /* persistent workgroups per CU, 0 (default: as many as fit) or 1..32 */
     ZAPTEX_OPT_MIP_TILE = 10,        /* level-1 tile side of the fused mip passes: 32 (default, 6 levels a pass) or 64
                                         (7 levels a pass); the bytes are the same either way */
-    ZAPTEX_OPT_CLIMB4_MAX = 11,      /* BC1 hill climb: a launch with at most this many BC1/BC3/BC4/BC5 blocks climbs with
-                                        four lanes per block (default 2^20; 0: always one lane per block) */
+    ZAPTEX_OPT_CLIMB4_MAX = 11,      /* a launch with at most this many BC1/BC3/BC4/BC5 blocks runs the BC1 starts with two
+                                        lanes per block and the hill climb with four (default 2^20; 0: always one lane
+                                        per block) */
     ZAPTEX_OPT_CLIMB4_RANGE = 12,    /* blocks per wave of the grouped climb, a multiple of 8 (of 64 / lanes per block),
                                         or 0 = automatic */
     ZAPTEX_OPT_CLIMB_GROUP = 13      /* lanes per block of the grouped climb: 4 (default) or 8 */

@@ -1,8 +1,9 @@
-"""The forms of the BC1 hill climb give the reference's bytes: one lane per block with lane refill (zt_bc1_climb,
-used for large launches such as the 512-texture batch) and four or eight lanes per block, each trying one of the
-next moves from the same endpoints (zt_bc1_climb4 / zt_bc1_climb8, used up to ZAPTEX_OPT_CLIMB4_MAX blocks).  Each
-is forced in turn on the reference's hashes of the 1024^2 BC1 workload, the adversarial block set and the
-near-singular fixture, and against the oracle on odd sizes; the grouped climbs also with other wave ranges."""
+"""The forms of the BC1-family encode give the reference's bytes: one lane per block (zt_bc14_encode, then the hill
+climb with lane refill, zt_bc1_climb; used for large launches such as the 512-texture batch) and, up to
+ZAPTEX_OPT_CLIMB4_MAX blocks, two lanes per block for the starts (zt_bc14_encode2) and four or eight per block for
+the climb, each lane trying one of the next moves from the same endpoints (zt_bc1_climb4 / zt_bc1_climb8).  Each is
+forced in turn on the reference's hashes of the 1024^2 workload (BC1, BC3, BC4, BC5), the adversarial block set and
+the near-singular fixture, and against the oracle on odd sizes; the grouped climbs also with other wave ranges."""
 import hashlib
 
 import numpy as np
@@ -33,10 +34,10 @@ def sha(a):
 
 def test_climbs_match_reference_hashes(zt, golden, climb):
     c1 = synth.c1_bc1(1024, 1)
-    for f in ("bc1", "bc3"):
+    for f in ("bc1", "bc3", "bc4", "bc5"):
         assert sha(zt.bc_encode(c1, f)) == golden["outputs"][f"c1_1024_{f}"], (climb, f)
     blocks = synth.blocks_to_image(synth.random_blocks(8192, 7))
-    for f in ("bc1", "bc3"):
+    for f in ("bc1", "bc3", "bc4", "bc5"):
         assert sha(zt.bc_encode(blocks, f)) == golden["outputs"][f"blocks_8192_{f}"], (climb, f)
     ent = golden["near_singular"]
     img = np.frombuffer(bytes.fromhex(ent["input"]), np.uint8).reshape(4, 4 * ent["blocks"], 4)
@@ -47,5 +48,5 @@ def test_climbs_match_reference_hashes(zt, golden, climb):
 @pytest.mark.parametrize("w,h,seed", [(1, 1, 1), (5, 3, 3), (129, 67, 4), (1000, 750, 6)])
 def test_climbs_match_oracle(zt, oracle, climb, w, h, seed):
     img = np.ascontiguousarray(synth.c1_bc1(max(w, h, 4), seed)[:h, :w])
-    for f in ("bc1", "bc3"):
+    for f in ("bc1", "bc3", "bc4", "bc5"):
         assert np.array_equal(zt.bc_encode(img, f), oracle.bc_encode(img, f)), (climb, f)

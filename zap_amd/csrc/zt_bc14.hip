@@ -99,18 +99,19 @@ ZT_DEV int bc1_prepare(const uint32_t px[16], uint32_t pm[16]) {
     return (int)spp;
 }
 
-// everything before the hill climb: PCA, two starts, least squares; returns the best SSE
-ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32_t &oidx) {
-    uint32_t pm[16];
-    const int spp = bc1_prepare(px, pm);
-    float mean[3] = { 0.0f, 0.0f, 0.0f }, cv[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+// the block's principal axis (zap_tex.h:99-124): mean, axis, |axis|^2 and the projections' range
+struct Bc1Axis { float mean[3], ax[3], n2, lo, hi; };
+ZT_DEV void bc1_axis(const uint32_t px[16], Bc1Axis &A) {
+    float cv[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int k = 0; k < 3; k++) A.mean[k] = 0.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) mean[k] += chanf(px[i], k) / 16.0f;
+        for (int k = 0; k < 3; k++) A.mean[k] += chanf(px[i], k) / 16.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        float r = chanf(px[i], 0) - mean[0], g = chanf(px[i], 1) - mean[1], b = chanf(px[i], 2) - mean[2];
+        float r = chanf(px[i], 0) - A.mean[0], g = chanf(px[i], 1) - A.mean[1], b = chanf(px[i], 2) - A.mean[2];
         cv[0] += r * r; cv[1] += r * g; cv[2] += r * b; cv[3] += g * g; cv[4] += g * b; cv[5] += b * b;
     }
     float ax[3] = { 1.0f, 1.0f, 1.0f };
@@ -124,42 +125,60 @@ ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32
         if (m < 1e-6f) break;
         ax[0] = x / m; ax[1] = y / m; ax[2] = z / m;
     }
-    float n2 = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2], lo = 1e9f, hi = -1e9f;
+    float lo = 1e9f, hi = -1e9f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        float t = (chanf(px[i], 0) - mean[0]) * ax[0] + (chanf(px[i], 1) - mean[1]) * ax[1] + (chanf(px[i], 2) - mean[2]) * ax[2];
+        float t = (chanf(px[i], 0) - A.mean[0]) * ax[0] + (chanf(px[i], 1) - A.mean[1]) * ax[1] + (chanf(px[i], 2) - A.mean[2]) * ax[2];
         if (t < lo) lo = t;
         if (t > hi) hi = t;
     }
+#pragma unroll
+    for (int k = 0; k < 3; k++) A.ax[k] = ax[k];
+    A.n2 = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]; A.lo = lo; A.hi = hi;
+}
+// start s (0: endpoints inset by 1/16 of the range, 1: not) with at most 3 least-squares refits (zap_tex.h:129-148)
+ZT_DEV int bc1_one_start(const uint32_t px[16], const uint32_t pm[16], int spp, const Bc1Axis &A, int s, uint32_t &oc0, uint32_t &oc1,
+                         uint32_t &oidx) {
+    float in = s ? 0.0f : (A.hi - A.lo) / 16.0f, l = (A.lo + in) / A.n2, u = (A.hi - in) / A.n2;
+    uint32_t c0 = to565(A.mean[0] + A.ax[0] * u, A.mean[1] + A.ax[1] * u, A.mean[2] + A.ax[2] * u);
+    uint32_t c1 = to565(A.mean[0] + A.ax[0] * l, A.mean[1] + A.ax[1] * l, A.mean[2] + A.ax[2] * l);
+    uint32_t idx = 0;
+    int e = bc1_attempt(pm, spp, c0, c1, idx);
+    for (int it = 0; it < 3 && e > 0; it++) {  // least squares on the chosen indices
+        const float W1 = 1.0f, W2 = 0.0f, W3 = 2.0f / 3.0f, W4 = 1.0f / 3.0f;
+        float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[3] = { 0.0f, 0.0f, 0.0f }, sb[3] = { 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            uint32_t k = (idx >> (2 * i)) & 3;
+            float w = k == 0 ? W1 : k == 1 ? W2 : k == 2 ? W3 : W4, v = 1.0f - w;
+            aa += w * w; ab += w * v; bb += v * v;
+#pragma unroll
+            for (int c = 0; c < 3; c++) { sa[c] += w * chanf(px[i], c); sb[c] += v * chanf(px[i], c); }
+        }
+        float det = aa * bb - ab * ab;
+        if (det < 1e-6f && det > -1e-6f) break;
+        float Av[3], Bv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) { Av[c] = (sa[c] * bb - sb[c] * ab) / det; Bv[c] = (sb[c] * aa - sa[c] * ab) / det; }
+        uint32_t n0 = to565(Av[0], Av[1], Av[2]), n1 = to565(Bv[0], Bv[1], Bv[2]), ni = 0;
+        int ne = bc1_attempt(pm, spp, n0, n1, ni);
+        if (ne >= e) break;
+        e = ne; c0 = n0; c1 = n1; idx = ni;
+    }
+    oc0 = c0; oc1 = c1; oidx = idx;
+    return e;
+}
+// everything before the hill climb: PCA, two starts, least squares; returns the best SSE (start 0 on ties)
+ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32_t &oidx) {
+    uint32_t pm[16];
+    const int spp = bc1_prepare(px, pm);
+    Bc1Axis A;
+    bc1_axis(px, A);
     uint32_t b0 = 0, b1 = 0, bidx = 0;
     int best = 1 << 30;
     for (int s = 0; s < 2; s++) {
-        float in = s ? 0.0f : (hi - lo) / 16.0f, l = (lo + in) / n2, u = (hi - in) / n2;
-        uint32_t c0 = to565(mean[0] + ax[0] * u, mean[1] + ax[1] * u, mean[2] + ax[2] * u);
-        uint32_t c1 = to565(mean[0] + ax[0] * l, mean[1] + ax[1] * l, mean[2] + ax[2] * l);
-        uint32_t idx = 0;
-        int e = bc1_attempt(pm, spp, c0, c1, idx);
-        for (int it = 0; it < 3 && e > 0; it++) {  // least squares on the chosen indices
-            const float W1 = 1.0f, W2 = 0.0f, W3 = 2.0f / 3.0f, W4 = 1.0f / 3.0f;
-            float aa = 0.0f, ab = 0.0f, bb = 0.0f, sa[3] = { 0.0f, 0.0f, 0.0f }, sb[3] = { 0.0f, 0.0f, 0.0f };
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                uint32_t k = (idx >> (2 * i)) & 3;
-                float w = k == 0 ? W1 : k == 1 ? W2 : k == 2 ? W3 : W4, v = 1.0f - w;
-                aa += w * w; ab += w * v; bb += v * v;
-#pragma unroll
-                for (int c = 0; c < 3; c++) { sa[c] += w * chanf(px[i], c); sb[c] += v * chanf(px[i], c); }
-            }
-            float det = aa * bb - ab * ab;
-            if (det < 1e-6f && det > -1e-6f) break;
-            float A[3], B[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) { A[c] = (sa[c] * bb - sb[c] * ab) / det; B[c] = (sb[c] * aa - sa[c] * ab) / det; }
-            uint32_t n0 = to565(A[0], A[1], A[2]), n1 = to565(B[0], B[1], B[2]), ni = 0;
-            int ne = bc1_attempt(pm, spp, n0, n1, ni);
-            if (ne >= e) break;
-            e = ne; c0 = n0; c1 = n1; idx = ni;
-        }
+        uint32_t c0, c1, idx;
+        const int e = bc1_one_start(px, pm, spp, A, s, c0, c1, idx);
         if (e < best) { best = e; b0 = c0; b1 = c1; bidx = idx; }
     }
     oc0 = b0; oc1 = b1; oidx = bidx;
@@ -292,6 +311,43 @@ ZT_DEV void bc14_encode_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, 
 }  // namespace
 extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb) {
     bc14_encode_body(jobs, total, climb, blockIdx.x, gridDim.x);
+}
+
+// The same with two lanes per block, for launches too small to fill the GPU one lane per block: lane 0 of a pair
+// runs start 0 (and BC5's R half), lane 1 start 1 (and BC3's BC4 alpha half, BC5's G half); the pair keeps start 1
+// when its SSE is strictly below start 0's, as the reference's loop over the starts does.
+namespace {
+ZT_DEV void bc14_encode2_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid(), q = tid & 1u;
+    const uint64_t step = (uint64_t)nblk * blockDim.x, n = 2ull * total;
+    for (uint64_t t = (uint64_t)bid * blockDim.x + tid; t < n; t += step) {   // both lanes of a pair take the same path
+        const uint32_t g = (uint32_t)(t >> 1);
+        const int ji = zt_find_job(jobs, g);
+        const zt_job J = job_at(jobs, ji);
+        uint32_t b = g - J.first;
+        int bx = (int)(b % (uint32_t)J.bw), by = (int)(b / (uint32_t)J.bw);
+        uint32_t px[16];
+        gather_block(J.src, J.w, J.h, J.stride, bx, by, px);
+        if (J.fmt == 0 || J.fmt == 1) {
+            if (J.fmt == 1 && q) store8(J.dst + (uint64_t)b * 16, bc4_block(px, 3));
+            uint32_t pm[16], c0, c1, ix;
+            const int spp = bc1_prepare(px, pm);
+            Bc1Axis A;
+            bc1_axis(px, A);
+            const int e = bc1_one_start(px, pm, spp, A, (int)q, c0, c1, ix);
+            const uint32_t oc = (uint32_t)__shfl_xor((int)(c0 | c1 << 16), 1), oix = (uint32_t)__shfl_xor((int)ix, 1);
+            const int oe = __shfl_xor(e, 1);
+            if (!q) climb[g] = oe < e ? make_uint4(oc, (uint32_t)oe, oix, (uint32_t)ji) : make_uint4(c0 | c1 << 16, (uint32_t)e, ix, (uint32_t)ji);
+        } else {
+            if (!q) climb[g] = make_uint4(0, 0, 0, (uint32_t)ji);  // every entry names its job: the climb reads it first
+            if (J.fmt == 2) { if (!q) store8(J.dst + (uint64_t)b * 8, bc4_block(px, 0)); }
+            else store8(J.dst + (uint64_t)b * 16 + 8 * q, bc4_block(px, (int)q));
+        }
+    }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc14_encode2(zt_jobs jobs, uint32_t total, uint4 *climb) {
+    bc14_encode2_body(jobs, total, climb, blockIdx.x, gridDim.x);
 }
 
 // The BC1 hill climb with lane refill.  Each wave owns ranges of `range` blocks (wave-strided; a multiple of 64, at

@@ -22,7 +22,8 @@ thread_local char g_err[512];
 std::mutex g_mu;     // per-device set-up, and the host slots' busy flags
 int g_m1k = 4;
 int g_climb_refill = 16;  // zt_bc1_climb refills once this many lanes of a wave are idle
-int g_climb4_max = 1 << 20;   // BC1-family blocks up to which a launch climbs with four lanes per block (zt_bc1_climb4)
+int g_climb4_max = 1 << 20;   // BC1-family blocks up to which a launch starts with two lanes per block (zt_bc14_encode2)
+                              // and climbs with four (zt_bc1_climb4)
 int g_climb4_range = 0;       // zt_bc1_climb4's blocks per wave (0: automatic)
 int g_climb_group = 4;        // lanes per block of the grouped climb (4 or 8)
 
@@ -312,10 +313,12 @@ int grid_for(uint64_t items, int per) {
 // BC1/BC3/BC4/BC5 over one job table: everything but the BC1 hill climb, then the hill climb (climb: 16 B/block)
 int enqueue_bc14(const zt_jobs &J, uint32_t tot, uint4 *climb, hipStream_t s) {
     const int wg = g_cfg[CFG_BC14].wg, wc = g_cfg[CFG_CLIMB].wg;
+    const bool small = tot <= (uint64_t)g_climb4_max;   // too few blocks to fill the GPU one lane per block
     if (int rc = launch("zt_bc14_encode", s, [&] {
-            hipLaunchKernelGGL(zt_bc14_encode, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, climb); }))
+            if (small) hipLaunchKernelGGL(zt_bc14_encode2, dim3((unsigned)((2ull * tot + 255) / 256)), dim3(256), 0, s, J, tot, climb);
+            else hipLaunchKernelGGL(zt_bc14_encode, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, climb); }))
         return rc;
-    if (tot <= (uint64_t)g_climb4_max) {
+    if (small) {
         // G lanes per block: ranges of g_climb4_range blocks (automatic: 64 / G blocks per wave up to 4096 waves)
         const uint32_t G = (uint32_t)g_climb_group, per = 64 / G;
         const uint32_t range = g_climb4_range ? (uint32_t)g_climb4_range

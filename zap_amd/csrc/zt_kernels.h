@@ -197,6 +197,7 @@ __global__ void zt_bc7_finish_learned(zt_jobs jobs, uint32_t total, int K, const
 // largest block range per wave of zt_bc1_climb (a wave's lanes refill from its range)
 #define ZT_CLIMB_RANGE 1024u
 __global__ void zt_bc14_encode(zt_jobs jobs, uint32_t total, uint4 *climb);
+__global__ void zt_bc14_encode2(zt_jobs jobs, uint32_t total, uint4 *climb);   // two lanes per block
 __global__ void zt_bc1_climb(zt_jobs jobs, uint32_t total, uint4 *climb, int refill_min, uint32_t range);
 __global__ void zt_bc1_climb4(zt_jobs jobs, uint32_t total, uint4 *climb, uint32_t range);   // four lanes per block
 __global__ void zt_bc1_climb8(zt_jobs jobs, uint32_t total, uint4 *climb, uint32_t range);   // eight lanes per block
