@@ -548,6 +548,11 @@ def main() -> int:
                      "clock_ghz_profiled": round(cycles / (s6c["_ns"] / 1e9) / 1e9, 3),
                      "issue_per_cu_clock": round(insts / N_CU / cycles, 4),
                      "counters_source": "rocprofv3 --pmc passes in this run: " + "; ".join(" ".join(p) for p in COUNTER_PASSES)})
+        # the other kernels of the step from the same passes: VALU instructions per launch and calibrated HBM bytes
+        roof["chain_kernels"] = {
+            k: {"us": round(v["_ns"] / 1e3, 2), "sq_insts_valu": int(v.get("SQ_INSTS_VALU", 0)), "sq_waves": int(v.get("SQ_WAVES", 0)),
+                "fetch_bytes": int(v.get("FETCH_SIZE", 0) * 1024 * scale_f), "write_bytes": int(v.get("WRITE_SIZE", 0) * 1024 * scale_w)}
+            for k, v in sorted(counters.items()) if k.startswith("zt_") and "_ns" in v}
     else:
         roof["counters_source"] = f"unavailable: {counters.get('error', 'stage-6 or copy counters missing')}"
     roof.update({"valu_probe": probe,
