@@ -1726,7 +1726,7 @@ BatchSizes batch_sizes(const zaptex_texture *t, int n, const MipPlan &P) {
     z.mipjobs = align_up(P.jobs.size() * sizeof(zt_mipjob), 256);
     z.jobs14 = align_up((size_t)z.n14 * sizeof(zt_job), 256);
     z.jobs7 = align_up((size_t)z.n7 * sizeof(zt_job), 256);
-    return z;
+    return z;   // (each family's table is held twice: whole, and its levels >= 1 rebased to block 0)
 }
 }  // namespace
 
@@ -1740,6 +1740,10 @@ struct zaptex_batch {
     zt_mipjob *mipjobs = nullptr;
     zt_jobs j14{}, j7{};
     uint32_t tot14 = 0, tot7 = 0;
+    // the graph's branches: each family's level-0 jobs (they read the bases only, so they run beside the mips) and
+    // its later levels (rebased to block 0, after the mips and the family's level-0 encode, sharing its workspace)
+    zt_jobs j14a{}, j14b{}, j7a{}, j7b{};
+    uint32_t tot14a = 0, tot14b = 0, tot7a = 0, tot7b = 0;
     void *bc7ws = nullptr;
     size_t bc7ws_bytes = 0;
     uint4 *climb = nullptr;
@@ -1784,7 +1788,7 @@ size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
     if (z.blocks14 >= (1ull << 31) || z.blocks7 >= (1ull << 31)) return 0;
     if (plan_queue(P, family_jobs(t, n, P, false), family_jobs(t, n, P, true), (uint32_t)z.blocks14, (uint32_t)z.blocks7, PqTables(), Q)) return 0;
     const QueueSizes qz = queue_sizes(Q);
-    return 256 + z.steps + z.items + z.mipjobs + z.jobs14 + z.jobs7 + qz.queue + qz.segs + qz.ctl + ws_total(z.blocks7, z.blocks14);
+    return 256 + z.steps + z.items + z.mipjobs + 2 * (z.jobs14 + z.jobs7) + qz.queue + qz.segs + qz.ctl + ws_total(z.blocks7, z.blocks14);
 }
 
 int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out) {
@@ -1813,6 +1817,8 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     b->mipjobs = (zt_mipjob *)p; p += z.mipjobs;
     zt_job *d14 = (zt_job *)p; p += z.jobs14;
     zt_job *d7 = (zt_job *)p; p += z.jobs7;
+    zt_job *d14b = (zt_job *)p; p += z.jobs14;
+    zt_job *d7b = (zt_job *)p; p += z.jobs7;
     uint32_t *dqueue = (uint32_t *)p; p += qz.queue;
     zt_seg *dsegs = (zt_seg *)p; p += qz.segs;
     unsigned *dctl = (unsigned *)p; p += qz.ctl;
@@ -1845,9 +1851,25 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     if (!P.jobs.empty()) HIP_TRY(hipMemcpyAsync(b->mipjobs, P.jobs.data(), P.jobs.size() * sizeof(zt_mipjob), hipMemcpyHostToDevice, s));
     if (!h14.empty()) HIP_TRY(hipMemcpyAsync(d14, h14.data(), h14.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
     if (!h7.empty()) HIP_TRY(hipMemcpyAsync(d7, h7.data(), h7.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
     b->j14.ext = d14; b->j14.n = (int)h14.size();
     b->j7.ext = d7; b->j7.n = (int)h7.size();
+    // the level-0 prefix of each family (family_jobs puts level 0 of every texture first) and the rest, rebased
+    std::vector<zt_job> r14, r7;
+    auto split = [&](const std::vector<FamJob> &f, const std::vector<zt_job> &h, zt_job *dall, zt_job *drest, std::vector<zt_job> &rest,
+                     zt_jobs &ja, zt_jobs &jb, uint32_t &ta, uint32_t &tb, uint32_t tot) {
+        size_t k = 0;
+        while (k < f.size() && f[k].lvl == 0) k++;
+        ta = k < h.size() ? h[k].first : tot;
+        tb = tot - ta;
+        for (size_t i = k; i < h.size(); i++) { zt_job j = h[i]; j.first -= ta; rest.push_back(j); }
+        ja.ext = dall; ja.n = (int)k;
+        jb.ext = drest; jb.n = (int)rest.size();
+    };
+    split(f14, h14, d14, d14b, r14, b->j14a, b->j14b, b->tot14a, b->tot14b, b->tot14);
+    split(f7, h7, d7, d7b, r7, b->j7a, b->j7b, b->tot7a, b->tot7b, b->tot7);
+    if (!r14.empty()) HIP_TRY(hipMemcpyAsync(d14b, r14.data(), r14.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
+    if (!r7.empty()) HIP_TRY(hipMemcpyAsync(d7b, r7.data(), r7.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
     // the persistent kernel (the mode-1 split it runs covers M1K <= 8, the default 4): queue, segments, counters
     b->persistent = g_batch_persistent && g_m1k <= 8;
     if (b->persistent) {
@@ -1874,11 +1896,21 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
         HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
         b->grid = (g_batch_groups > 0 ? g_batch_groups : std::max(1, per_cu)) * std::max(1, cus);
     }
-    // capture: mip levels in order, then the encoders (BC1-5, then BC7), as one graph
-    auto body = [&](hipStream_t cs) -> int {
+    // capture as one graph: the mip levels in order on the capturing stream; beside them, on two more streams, each
+    // family's level-0 encode (BC1-5 on one, BC7 on the other); then each family's later levels once the mips are done
+    hipStream_t side[2] = { nullptr, nullptr };
+    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };   // fork, mips done, side 0 done, side 1 done
+    auto cleanup = [&] {
+        for (hipEvent_t &e : ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+        for (hipStream_t &x : side) if (x) { (void)hipStreamDestroy(x); x = nullptr; }
+    };
+    const bool branches = !b->persistent && !P.launches.empty();
+    if (branches) {
+        for (hipStream_t &x : side) if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) { cleanup(); delete b; return fail(ZAPTEX_E_DEVICE, "stream create"); }
+        for (hipEvent_t &e : ev) if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { cleanup(); delete b; return fail(ZAPTEX_E_DEVICE, "event create"); }
+    }
+    auto mips_body = [&](hipStream_t cs) -> int {
         int rc;
-        if (b->persistent)
-            return launch("zt_batch_persistent", cs, [&] { hipLaunchKernelGGL(zt_batch_persistent, dim3(b->grid), dim3(256), 0, cs, b->args); });
         size_t tl = 0;  // kind-0 launches seen
         for (const MipLaunch &ml : b->mip_launches) {
             if (ml.kind == 0) {
@@ -1900,8 +1932,30 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
                     return rc;
             }
         }
-        if (b->tot14 && (rc = enqueue_bc14(b->j14, b->tot14, b->climb, cs))) return rc;
-        if (b->tot7 && (rc = enqueue_bc7(b->j7, b->tot7, b->bc7ws, b->bc7ws_bytes, cs))) return rc;
+        return ZAPTEX_OK;
+    };
+    auto body = [&](hipStream_t cs) -> int {
+        int rc;
+        if (b->persistent)
+            return launch("zt_batch_persistent", cs, [&] { hipLaunchKernelGGL(zt_batch_persistent, dim3(b->grid), dim3(256), 0, cs, b->args); });
+        if (!branches) {   // no mips: the encoders in order
+            if (b->tot14 && (rc = enqueue_bc14(b->j14, b->tot14, b->climb, cs))) return rc;
+            if (b->tot7 && (rc = enqueue_bc7(b->j7, b->tot7, b->bc7ws, b->bc7ws_bytes, cs))) return rc;
+            return ZAPTEX_OK;
+        }
+        HIP_TRY(hipEventRecord(ev[0], cs));
+        for (hipStream_t x : side) HIP_TRY(hipStreamWaitEvent(x, ev[0], 0));
+        if (b->tot14a && (rc = enqueue_bc14(b->j14a, b->tot14a, b->climb, side[0]))) return rc;
+        if (b->tot7a && (rc = enqueue_bc7(b->j7a, b->tot7a, b->bc7ws, b->bc7ws_bytes, side[1]))) return rc;
+        if ((rc = mips_body(cs))) return rc;
+        HIP_TRY(hipEventRecord(ev[1], cs));
+        for (hipStream_t x : side) HIP_TRY(hipStreamWaitEvent(x, ev[1], 0));
+        if (b->tot14b && (rc = enqueue_bc14(b->j14b, b->tot14b, b->climb, side[0]))) return rc;
+        if (b->tot7b && (rc = enqueue_bc7(b->j7b, b->tot7b, b->bc7ws, b->bc7ws_bytes, side[1]))) return rc;
+        HIP_TRY(hipEventRecord(ev[2], side[0]));
+        HIP_TRY(hipEventRecord(ev[3], side[1]));
+        HIP_TRY(hipStreamWaitEvent(cs, ev[2], 0));
+        HIP_TRY(hipStreamWaitEvent(cs, ev[3], 0));
         return ZAPTEX_OK;
     };
     {
@@ -1912,6 +1966,7 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
         g_capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
+        cleanup();
         if (rc) { if (g) (void)hipGraphDestroy(g); delete b; return rc; }
         if (e != hipSuccess) { delete b; return fail(ZAPTEX_E_DEVICE, "graph capture: %s", hipGetErrorString(e)); }
         b->graph = g;
