@@ -63,7 +63,7 @@ ZT_DEV int bc1_attempt(const uint32_t pm[16], int spp, uint32_t &c0, uint32_t &c
     return bc1_trial(pm, spp, c0, c1, idx);
 }
 // bc1_attempt's SSE alone (the hill climb compares errors; the winner's indices are recomputed once, in
-// zt_bc1_finish): the same keys and maxima without the index bits
+// zt_bc1_finish or at the end of the grouped climb's range): the same keys and maxima without the index bits
 ZT_DEV int bc1_attempt_err(const uint32_t pm[16], int spp, uint32_t &c0, uint32_t &c1) {
     if (c0 < c1) { uint32_t t = c0; c0 = c1; c1 = t; }
     const uint32_t a = rgb565_expand(c0), b = rgb565_expand(c1);
@@ -217,6 +217,34 @@ ZT_DEV void climb_apply(Climb &s, uint32_t n0, uint32_t n1, int ne) {
     if (ne < s.best) { s.best = ne; s.b0 = n0; s.b1 = n1; s.improved = 1; s.moved = 1; }
 }
 ZT_DEV uint64_t bc1_pack(uint32_t c0, uint32_t c1, uint32_t idx) { return (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)idx << 32; }
+// the 2-bit indices of row `row` of block b (pixels 4 row .. 4 row + 3, edge-clamped like gather_block) for the
+// canonical pair (c0 >= c1), at their places in the block's 32 index bits: bc1_trial's keys for four pixels
+ZT_DEV uint32_t bc1_row_indices(const zt_job &J, uint32_t b, int row, uint32_t c0, uint32_t c1) {
+    if (c0 == c1) return 0;
+    const int bx = (int)(b % (uint32_t)J.bw), by = (int)(b / (uint32_t)J.bw), y = min(by * 4 + row, J.h - 1);
+    const uint32_t a = rgb565_expand(c0), e = rgb565_expand(c1);
+    uint32_t p2 = 0, p3 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int x = chan(a, k), z = chan(e, k);
+        p2 |= (uint32_t)((2 * x + z + 1) / 3) << (8 * k);
+        p3 |= (uint32_t)((x + 2 * z + 1) / 3) << (8 * k);
+    }
+    const int K0 = no_fold(3 - 64 * (int)dot4(a, a)), K1 = no_fold(2 - 64 * (int)dot4(e, e)), K2 = no_fold(1 - 64 * (int)dot4(p2, p2)),
+              K3 = no_fold(-64 * (int)dot4(p3, p3));
+    uint32_t bits = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int x = min(bx * 4 + i, J.w - 1);
+        const uint8_t *p = J.src + (uint64_t)y * J.stride + (uint64_t)x * 4;
+        const uint32_t pm = (((uintptr_t)p & 3) == 0 ? *(const uint32_t *)p : (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16) & 0xFFFFFFu;
+        const int k0 = (int)(dot4(pm, a) << 7) + K0, k1 = (int)(dot4(pm, e) << 7) + K1;
+        const int k2 = (int)(dot4(pm, p2) << 7) + K2, k3 = (int)(dot4(pm, p3) << 7) + K3;
+        const int best = max(max(k0, k1), max(k2, k3));
+        bits |= (uint32_t)(3 - (best & 3)) << (2 * (4 * row + i));
+    }
+    return bits;
+}
 
 // BC4 palette of an endpoint pair with a0 > a1 (zap_tex.h:172-176, first branch): a0, a1 and the six interpolants
 // ((8 - i) a0 + (i - 1) a1 + 3) / 7.  The encoder never meets a0 <= a1 with a0 != a1 (a pair that would be one is
@@ -418,7 +446,8 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc1_climb(zt_jobs jobs, uin
 // and goes on from the move after it, or from mv + 4 when none improves.  A move tried from the endpoints the
 // reference would try it from gives the reference's result: the moves before the committed one saw no improvement
 // from those same endpoints, and the moves after it are tried again from the new ones.  Groups refill from their
-// wave's range like the lanes of zt_bc1_climb; the results go to the same places.
+// wave's range like the lanes of zt_bc1_climb.  When its range is done the wave writes the moved blocks' indices
+// itself (zt_bc1_finish's work, with four lanes per block), so a small launch needs no finish kernel.
 namespace {
 template <int G>
 ZT_DEV void bc1_climb_group_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, uint32_t range, uint32_t bid, uint32_t nblk) {
@@ -495,6 +524,22 @@ ZT_DEV void bc1_climb_group_body(const zt_jobs &jobs, uint32_t total, uint4 *cli
                     active = false;
                 }
             }
+        }
+        // the range's moved blocks take their indices here, as zt_bc1_finish would: four lanes per block, one row of
+        // four pixels each, the rows' index bits OR-ed over the quad.  The climb state they read was stored by this wave.
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        for (uint32_t c = (uint32_t)r0; c < end; c += 16) {
+            const uint32_t g = c + (uint32_t)(lane >> 2);
+            const int row = lane & 3;
+            const uint4 st = g < end ? climb[g] : make_uint4(0u, 0u, 0u, 0u);
+            const bool fin = g < end && (st.w >> 31);
+            uint32_t bits = 0, c0 = st.x & 0xFFFFu, c1 = st.x >> 16;
+            const zt_job J = job_at(jobs, fin ? (int)(st.w & 0x7FFFFFFFu) : 0);
+            const uint32_t b = g - (fin ? J.first : 0u);
+            if (fin) bits = bc1_row_indices(J, b, row, c0, c1);
+            bits |= (uint32_t)__shfl_xor((int)bits, 1);
+            bits |= (uint32_t)__shfl_xor((int)bits, 2);
+            if (fin && row == 0) store8(J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8), bc1_pack(c0, c1, bits));
         }
     }
 }

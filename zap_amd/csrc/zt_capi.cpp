@@ -336,6 +336,7 @@ int enqueue_bc14(const zt_jobs &J, uint32_t tot, uint4 *climb, hipStream_t s) {
                 hipLaunchKernelGGL(zt_bc1_climb, dim3(groups_for(CFG_CLIMB, waves, wc / 64)), dim3(wc), 0, s, J, tot, climb, g_climb_refill, range); }))
             return rc;
     }
+    if (small) return ZAPTEX_OK;   // the grouped climb gives its moved blocks their indices itself
     return launch("zt_bc1_finish", s, [&] {
         hipLaunchKernelGGL(zt_bc1_finish, dim3(groups_for(CFG_BC14, tot, wg)), dim3(wg), 0, s, J, tot, (const uint4 *)climb); });
 }
