@@ -205,9 +205,8 @@ ZT_DEV void ls_sums(const ChanWords &T, const uint32_t KW[4], const int P[NC], i
 }
 
 // ---- one subset: quantised endpoints + indices (zap_tex.h:466-486).  idx: 4 bits per pixel at 4*i.
-template <int NC, int N, int IB, bool SHARED>
-ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t snib, const float lo[4], const float hi[4], int q[2][4],
-                      int pb[2], uint64_t &idx) {
+// The quantised endpoints with the best p-bit pair (zap_tex.h:468-475)
+template <int NC, int N, bool SHARED> ZT_DEV void quant_pair(const float lo[4], const float hi[4], int q[2][4], int pb[2]) {
     int ql[2][NC], qh[2][NC], el[2], eh[2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
@@ -228,12 +227,15 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t sn
             if (e < best) { best = e; b0 = p0; b1 = p1; }
         }
     pb[0] = b0; pb[1] = b1;
+#pragma unroll
+    for (int c = 0; c < NC; c++) { q[0][c] = b0 ? ql[1][c] : ql[0][c]; q[1][c] = b1 ? qh[1][c] : qh[0][c]; }
+}
+// SSE and indices of quantised endpoints over the subset: the palette and its argmin (zap_tex.h:476-485)
+template <int NC, int N, int IB>
+ZT_DEV int fit_quant(const uint32_t px[16], uint32_t mask, int spp, uint64_t snib, const int q[2][4], const int pb[2], uint64_t &idx) {
     int e0[NC], e1[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-        q[0][c] = b0 ? ql[1][c] : ql[0][c]; q[1][c] = b1 ? qh[1][c] : qh[0][c];
-        e0[c] = expand_bits(q[0][c] << 1 | b0, N); e1[c] = expand_bits(q[1][c] << 1 | b1, N);
-    }
+    for (int c = 0; c < NC; c++) { e0[c] = expand_bits(q[0][c] << 1 | pb[0], N); e1[c] = expand_bits(q[1][c] << 1 | pb[1], N); }
     constexpr int NP = 1 << IB;
     uint32_t pal[NP];
     int K[NP];
@@ -251,6 +253,43 @@ ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t sn
     idx = bits ^ snib;
     return spp - ((sk - nibble_sum(bits)) >> 4);
 }
+template <int NC, int N, int IB, bool SHARED>
+ZT_DEV int fit_subset(const uint32_t px[16], uint32_t mask, int spp, uint64_t snib, const float lo[4], const float hi[4], int q[2][4],
+                      int pb[2], uint64_t &idx) {
+    quant_pair<NC, N, SHARED>(lo, hi, q, pb);
+    return fit_quant<NC, N, IB>(px, mask, spp, snib, q, pb, idx);
+}
+
+// ---- one least-squares step (zap_tex.h:495-507): endpoints for the indices idx, clamped to [0, 255]; false when
+// the system is singular (|det| < 1e-6), which ends the reference's refinement.  RT: the channel words again from
+// the pixels, so they are not live across the fits.
+template <int NC, int IB, bool RT, typename PX>
+ZT_DEV bool ls_step(PX *px, const ChanWords &T, const int P[NC], int cnt, uint64_t idx, float nlo[4], float nhi[4]) {
+    // pixels outside the subset have index 0, hence weight 0
+    uint32_t KW[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) KW[g] = weight_bytes<IB>(idx_bytes4(idx, g));
+    float aa, ab, bb, sa[NC], sb[NC];
+    if constexpr (RT) {
+        uint32_t pr[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) pr[k] = px[k];
+        refresh16(pr);
+        ChanWords T2;
+        chan_words(pr, T2);
+        ls_sums<NC, 0>(T2, KW, P, cnt, aa, ab, bb, sa, sb);
+    } else ls_sums<NC, 0>(T, KW, P, cnt, aa, ab, bb, sa, sb);
+    float det = aa * bb - ab * ab;
+    if (det < 1e-6f && det > -1e-6f) return false;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (c < NC) {
+            float l = (sa[c] * bb - sb[c] * ab) / det, h = (sb[c] * aa - sa[c] * ab) / det;
+            nlo[c] = l < 0 ? 0 : l > 255 ? 255 : l; nhi[c] = h < 0 ? 0 : h > 255 ? 255 : h;
+        } else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
+    }
+    return true;
+}
 
 // ---- PCA start, fit, at most 2 least-squares refinements (zap_tex.h:489-515)
 // R: refresh the (then non-const) pixel registers between the PCA passes (pca's REFRESH)
@@ -264,30 +303,8 @@ ZT_DEV int encode_subset(PX *px, const ChanWords &T, uint32_t mask, int cnt, int
     const uint64_t snib = FULL ? ~0ull : subset_nibbles(mask);
     int err = fit_subset<NC, N, IB, SHARED>(px, mask, spp, snib, lo, hi, q, pb, idx);
     for (int it = 0; it < 2 && err > 0; it++) {
-        // pixels outside the subset have index 0, hence weight 0
-        uint32_t KW[4];
-#pragma unroll
-        for (int g = 0; g < 4; g++) KW[g] = weight_bytes<IB>(idx_bytes4(idx, g));
-        float aa, ab, bb, sa[NC], sb[NC];
-        if constexpr (RT) {   // the channel words again, so they are not live across the fits
-            uint32_t pr[16];
-#pragma unroll
-            for (int k = 0; k < 16; k++) pr[k] = px[k];
-            refresh16(pr);
-            ChanWords T2;
-            chan_words(pr, T2);
-            ls_sums<NC, 0>(T2, KW, P, cnt, aa, ab, bb, sa, sb);
-        } else ls_sums<NC, 0>(T, KW, P, cnt, aa, ab, bb, sa, sb);
-        float det = aa * bb - ab * ab;
-        if (det < 1e-6f && det > -1e-6f) break;
         float nlo[4], nhi[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (c < NC) {
-                float l = (sa[c] * bb - sb[c] * ab) / det, h = (sb[c] * aa - sa[c] * ab) / det;
-                nlo[c] = l < 0 ? 0 : l > 255 ? 255 : l; nhi[c] = h < 0 ? 0 : h > 255 ? 255 : h;
-            } else { nlo[c] = 255.0f; nhi[c] = 255.0f; }
-        }
+        if (!ls_step<NC, IB, RT>(px, T, P, cnt, idx, nlo, nhi)) break;
         int nq[2][4], np[2];
         uint64_t ni;
         int ne = fit_subset<NC, N, IB, SHARED>(px, mask, spp, snib, nlo, nhi, nq, np, ni);
@@ -300,12 +317,8 @@ ZT_DEV int encode_subset(PX *px, const ChanWords &T, uint32_t mask, int cnt, int
 }
 
 // ---- mode 6 (zap_tex.h:582-593, pack :520-527)
-ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
-    int q[2][4], pb[2];
-    uint64_t idx;
-    ChanWords T;
-    chan_words(px, T);
-    int err = encode_subset<4, 8, 4, false, true, false, true>(px, T, 0xFFFFu, 16, q, pb, idx);
+// the block: endpoints, p-bits and indices as the reference stores them (anchor index MSB clear)
+ZT_DEV void pack6(int q[2][4], int pb[2], uint64_t idx, uint64_t &lo, uint64_t &hi) {
     if (idx & 8) {  // anchor MSB set: swap endpoints and p-bits, invert indices
 #pragma unroll
         for (int c = 0; c < 4; c++) { int t = q[0][c]; q[0][c] = q[1][c]; q[1][c] = t; }
@@ -321,9 +334,16 @@ ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
     uint64_t field = (idx & 7) | ((idx >> 1) & ~7ull);  // pixel 0 keeps 3 bits
     lo = l;
     hi = (uint64_t)pb[1] | field << 1;
+}
+ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
+    int q[2][4], pb[2];
+    uint64_t idx;
+    ChanWords T;
+    chan_words(px, T);
+    int err = encode_subset<4, 8, 4, false, true, false, true>(px, T, 0xFFFFu, 16, q, pb, idx);
+    pack6(q, pb, idx, lo, hi);
     return err;
 }
-
 // ---- mode 5 pieces (zap_tex.h:597-657); NC = 3 (colour, 7 bits) or 1 (alpha, 8 bits)
 // plain n-bit quantiser (zap_tex.h:613-624): candidates and keys as in quant_pbit1 (checked exhaustively for
 // every float |v| <= 2048 and n = 4..8)

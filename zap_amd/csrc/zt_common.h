@@ -86,16 +86,18 @@ ZT_DEV int no_fold(int v) {
 #endif
 }
 
-// 16 pixels of block (bx, by), edge-clamped like zap_tex.h:1482-1488, packed RGBA8 (R in the low byte)
+// 16 pixels of block (bx, by), edge-clamped like zap_tex.h:1482-1488, packed RGBA8 (R in the low byte).  Rows are
+// read as 16-byte vectors when the block lies inside the image and the base and stride are 16-byte aligned (one
+// test for the block, so no row pointer stays live across a branch), else byte by byte.
 ZT_DEV void gather_block(const uint8_t *src, int w, int h, uint64_t stride, int bx, int by, uint32_t px[16]) {
     int x0 = bx * 4;
-    bool full_x = x0 + 3 < w;
+    const bool fast = x0 + 3 < w && (((uintptr_t)src | stride) & 15) == 0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         int y = by * 4 + r;
         if (y >= h) y = h - 1;
         const uint8_t *row = src + (uint64_t)y * stride;
-        if (full_x && (((uintptr_t)(row + x0 * 4)) & 15) == 0) {
+        if (fast) {
             uint4 v = *(const uint4 *)(row + x0 * 4);
             px[r * 4 + 0] = v.x; px[r * 4 + 1] = v.y; px[r * 4 + 2] = v.z; px[r * 4 + 3] = v.w;
         } else {
@@ -160,6 +162,10 @@ ZT_DEV uint32_t shard_slot(const unsigned *counts, uint32_t cap, uint32_t e) {
     return 0xFFFFFFFFu;
 }
 
+// set bits of m below this lane (v_mbcnt: no lane mask is materialised)
+ZT_DEV uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 // wave-level stream compaction: returns this lane's slot in the list (or -1), one atomic per wave
 ZT_DEV int wave_append(bool pred, unsigned *counter) {
     uint64_t mask = __ballot(pred);
@@ -170,8 +176,7 @@ ZT_DEV int wave_append(bool pred, unsigned *counter) {
     if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
     base = __shfl(base, leader);
     if (!pred) return -1;
-    uint64_t below = lane ? (mask & ((1ull << lane) - 1)) : 0ull;
-    return (int)(base + __popcll(below));
+    return (int)(base + lanes_below(mask));
 }
 
 // LSB-first reader over a 128-bit block (zap_tex.h:367-371)
