@@ -187,18 +187,24 @@ ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32
 
 // hill-climb state of one block: +-1 on each 565 channel of each endpoint (order: endpoint, channel, -1 then +1),
 // passes of 12 moves while a pass improved, at most 8 (zap_tex.h:152-166)
+// The hill climb's state.  Within a pass (zap_tex.h:152-163) the moves after the pass's last improvement were tried
+// from the pass's final endpoints and failed; the next pass starts from those endpoints, so until something improves
+// it those moves fail again.  A pass that reaches them without an improvement therefore ends the climb there: `stop`
+// is the previous pass's last improving move (11 in the first pass), `last` this pass's.  The result is the
+// reference's; on textures it skips 16-21% of the trials.
 struct Climb {
     uint32_t b0, b1, bidx;   // bidx: the start's indices (valid while moved == 0)
-    int best, pass, mv, improved, moved;
+    int best, pass, mv, improved, moved, stop, last;
 };
 // the next valid move's endpoint pair (cheap; no trial); false when the climb is over.  Stopping as soon as the
 // SSE is 0 gives the reference's result: no later move can improve on 0.
 ZT_DEV bool climb_next(Climb &s, uint32_t &n0, uint32_t &n1) {
     for (int guard = 0; guard < 16; guard++) {  // >= 6 of any pass's 12 moves are valid, so this always ends early
         if (s.best == 0) return false;
+        if (!s.improved && s.mv > s.stop) return false;   // the rest of the pass fails again (with s.mv == 12 as well)
         if (s.mv == 12) {
-            if (!s.improved || s.pass == 7) return false;
-            s.pass++; s.mv = 0; s.improved = 0;
+            if (s.pass == 7) return false;
+            s.pass++; s.mv = 0; s.improved = 0; s.stop = s.last;
         }
         const int mv = s.mv, ep = mv / 6, ch = (mv >> 1) % 3, d = (mv & 1) ? 1 : -1;
         const int sh = ch == 0 ? 11 : ch == 1 ? 5 : 0, mx = ch == 1 ? 63 : 31;
@@ -212,9 +218,9 @@ ZT_DEV bool climb_next(Climb &s, uint32_t &n0, uint32_t &n1) {
     }
     return false;
 }
-// fold a trial (endpoints as bc1_attempt_err left them, canonical order) into the state
+// fold a trial (endpoints as bc1_attempt_err left them, canonical order; the move climb_next just gave) into the state
 ZT_DEV void climb_apply(Climb &s, uint32_t n0, uint32_t n1, int ne) {
-    if (ne < s.best) { s.best = ne; s.b0 = n0; s.b1 = n1; s.improved = 1; s.moved = 1; }
+    if (ne < s.best) { s.best = ne; s.b0 = n0; s.b1 = n1; s.improved = 1; s.moved = 1; s.last = s.mv - 1; }
 }
 ZT_DEV uint64_t bc1_pack(uint32_t c0, uint32_t c1, uint32_t idx) { return (uint64_t)c0 | (uint64_t)c1 << 16 | (uint64_t)idx << 32; }
 // the 2-bit indices of row `row` of block b (pixels 4 row .. 4 row + 3, edge-clamped like gather_block) for the
@@ -398,7 +404,7 @@ ZT_DEV void bc1_climb_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, in
         bool active = false;
         uint32_t px[16];  // alpha cleared (bc1_prepare)
         int spp = 0;
-        Climb s = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        Climb s = { 0, 0, 0, 0, 0, 0, 0, 0, 11, -1 };
         uint8_t *dst = nullptr;
         uint32_t cur = 0, job = 0;
         for (;;) {
@@ -414,6 +420,7 @@ ZT_DEV void bc1_climb_body(const zt_jobs &jobs, uint32_t total, uint4 *climb, in
                     const uint32_t b = mine - J.first;
                     dst = J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8);
                     s.b0 = st.x & 0xFFFFu; s.b1 = st.x >> 16; s.best = (int)st.y; s.bidx = st.z; s.pass = 0; s.mv = 0; s.improved = 0; s.moved = 0;
+                    s.stop = 11; s.last = -1;
                     cur = mine; job = st.w;
                     gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
                     spp = bc1_prepare(px, px);
@@ -460,7 +467,7 @@ ZT_DEV void bc1_climb_group_body(const zt_jobs &jobs, uint32_t total, uint4 *cli
         uint32_t next = (uint32_t)r0;  // same on every lane
         bool active = false;
         uint32_t px[16];  // alpha cleared (bc1_prepare)
-        int spp = 0, best = 0, pass = 0, mv = 0;
+        int spp = 0, best = 0, pass = 0, mv = 0, stop = 11, last = -1;   // stop, last: as in Climb
         bool improved = false, moved = false;
         uint32_t b0 = 0, b1 = 0, bidx = 0, cur = 0, job = 0;
         uint8_t *dst = nullptr;
@@ -476,6 +483,7 @@ ZT_DEV void bc1_climb_group_body(const zt_jobs &jobs, uint32_t total, uint4 *cli
                     const uint32_t b = mine - J.first;
                     dst = J.dst + (J.fmt == 0 ? (uint64_t)b * 8 : (uint64_t)b * 16 + 8);
                     b0 = st.x & 0xFFFFu; b1 = st.x >> 16; best = (int)st.y; bidx = st.z; pass = 0; mv = 0; improved = false; moved = false;
+                    stop = 11; last = -1;
                     cur = mine; job = st.w;
                     gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
                     spp = bc1_prepare(px, px);
@@ -508,12 +516,13 @@ ZT_DEV void bc1_climb_group_body(const zt_jobs &jobs, uint32_t total, uint4 *cli
             if (active) {
                 bool done = best == 0;
                 if (!done) {
-                    if (gb) { best = pbest; b0 = pair & 0xFFFFu; b1 = pair >> 16; improved = true; moved = true; mv += first + 1; }
+                    if (gb) { best = pbest; b0 = pair & 0xFFFFu; b1 = pair >> 16; improved = true; moved = true; last = mv + first; mv += first + 1; }
                     else mv += G;
                     if (best == 0) done = true;   // no later move can improve on 0
+                    else if (!improved && mv > stop) done = true;   // the rest of the pass fails again (mv >= 12 too)
                     else if (mv >= 12) {
-                        if (!improved || pass == 7) done = true;
-                        else { pass++; mv = 0; improved = false; }
+                        if (pass == 7) done = true;
+                        else { pass++; mv = 0; improved = false; stop = last; }
                     }
                 }
                 if (done) {

@@ -1929,7 +1929,9 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
                     return rc;
             } else {
                 const zt_mipstep *steps = b->steps + ml.first;
-                if ((rc = launch("zt_mip_tail_batch", cs, [&] { hipLaunchKernelGGL(zt_mip_tail_batch, dim3(ml.count), dim3(1024), 0, cs, steps); })))
+                // 256 lanes: in the graph the tail steps run beside the level-0 encoders, where a 1024-lane workgroup
+                // waits for a CU with 16 free wave slots (a round took 5 ms instead of 22 us alone), a 256-lane one for 4
+                if ((rc = launch("zt_mip_tail_batch", cs, [&] { hipLaunchKernelGGL(zt_mip_tail_batch, dim3(ml.count), dim3(256), 0, cs, steps); })))
                     return rc;
             }
         }

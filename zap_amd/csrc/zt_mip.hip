@@ -275,7 +275,7 @@ extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles_batch(const zt_mi
 extern "C" __global__ void __launch_bounds__(256) zt_mip_tiles32_batch(const zt_mipstep *steps, const uint2 *items) {
     tiles_item_body<32>(steps, items, blockIdx.x);
 }
-extern "C" __global__ void __launch_bounds__(1024) zt_mip_tail_batch(const zt_mipstep *steps) { tail_step_body(steps, blockIdx.x); }
+extern "C" __global__ void __launch_bounds__(256) zt_mip_tail_batch(const zt_mipstep *steps) { tail_step_body(steps, blockIdx.x); }
 
 // ---------------------------------------------------------------- batched level
 // One level of many textures in one launch: lane per destination pixel over all jobs (grid-stride); a job is one
