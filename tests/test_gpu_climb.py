@@ -50,3 +50,25 @@ def test_climbs_match_oracle(zt, oracle, climb, w, h, seed):
     img = np.ascontiguousarray(synth.c1_bc1(max(w, h, 4), seed)[:h, :w])
     for f in ("bc1", "bc3", "bc4", "bc5"):
         assert np.array_equal(zt.bc_encode(img, f), oracle.bc_encode(img, f)), (climb, f)
+
+
+def test_climbs_on_unaligned_strided_device_rows(zt, oracle, climb):
+    """Device sources only need 4-byte rows: a base 4 bytes past a 16-byte boundary and a row pitch of w * 4 + 4
+    send the pixel gathers (and the grouped climb's own index pass) down their unaligned paths."""
+    import torch
+    w, h = 203, 131
+    img = np.ascontiguousarray(synth.c1_bc1(256, 9)[:h, :w])
+    pitch = w * 4 + 4
+    buf = torch.zeros(4 + pitch * h, dtype=torch.uint8, device="cuda")
+    rows = torch.from_numpy(np.ascontiguousarray(np.pad(img.reshape(h, w * 4), ((0, 0), (0, pitch - w * 4))))).reshape(-1)
+    buf[4:].copy_(rows.cuda())
+    src = buf[4:]
+    assert src.data_ptr() % 16 == 4
+    for f in ("bc1", "bc3"):
+        from zap_amd import binding
+        out = torch.zeros(binding.bc_size(w, h, f), dtype=torch.uint8, device="cuda")
+        imgs = zt.images([(src, out, w, h, pitch, f)])
+        ws = torch.empty(max(1, zt.workspace_size(imgs)), dtype=torch.uint8, device="cuda")
+        zt.encode_dev(imgs, ws)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), oracle.bc_encode(img, f)), (climb, f)
