@@ -186,12 +186,11 @@ ZT_DEV int bc1_start(const uint32_t px[16], uint32_t &oc0, uint32_t &oc1, uint32
 }
 
 // hill-climb state of one block: +-1 on each 565 channel of each endpoint (order: endpoint, channel, -1 then +1),
-// passes of 12 moves while a pass improved, at most 8 (zap_tex.h:152-166)
-// The hill climb's state.  Within a pass (zap_tex.h:152-163) the moves after the pass's last improvement were tried
-// from the pass's final endpoints and failed; the next pass starts from those endpoints, so until something improves
-// it those moves fail again.  A pass that reaches them without an improvement therefore ends the climb there: `stop`
-// is the previous pass's last improving move (11 in the first pass), `last` this pass's.  The result is the
-// reference's; on textures it skips 16-21% of the trials.
+// passes of 12 moves while a pass improved, at most 8 (zap_tex.h:152-166).  Within a pass the moves after the pass's
+// last improvement were tried from the pass's final endpoints and failed; the next pass starts from those endpoints,
+// so until something improves they fail again.  A pass that reaches them without an improvement therefore ends the
+// climb there: `stop` is the previous pass's last improving move (11 in the first pass), `last` this pass's.  The
+// result is the reference's; on textures it skips 16-21% of the trials.
 struct Climb {
     uint32_t b0, b1, bidx;   // bidx: the start's indices (valid while moved == 0)
     int best, pass, mv, improved, moved, stop, last;
@@ -242,8 +241,8 @@ ZT_DEV uint32_t bc1_row_indices(const zt_job &J, uint32_t b, int row, uint32_t c
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int x = min(bx * 4 + i, J.w - 1);
-        const uint8_t *p = J.src + (uint64_t)y * J.stride + (uint64_t)x * 4;
-        const uint32_t pm = (((uintptr_t)p & 3) == 0 ? *(const uint32_t *)p : (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16) & 0xFFFFFFu;
+        // (source rows are 4-byte aligned: the entry points refuse other bases and strides)
+        const uint32_t pm = *(const uint32_t *)(J.src + (uint64_t)y * J.stride + (uint64_t)x * 4) & 0xFFFFFFu;
         const int k0 = (int)(dot4(pm, a) << 7) + K0, k1 = (int)(dot4(pm, e) << 7) + K1;
         const int k2 = (int)(dot4(pm, p2) << 7) + K2, k3 = (int)(dot4(pm, p3) << 7) + K3;
         const int best = max(max(k0, k1), max(k2, k3));
