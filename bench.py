@@ -482,7 +482,7 @@ def main() -> int:
     torch.cuda.synchronize()
     kern = {}
     for k in ["zt_mip_tiles", "zt_mip_tail", "zt_mip_level", "zt_zero_u32", "zt_bc7_stage6", "zt_bc7_stage5",
-              "zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_stage1"]:
+              "zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_s1_fitpick", "zt_bc7_stage1"]:
         t, n = lib.profile_get(k)
         if n:
             kern[k] = {"launches_per_step": n // args.profile_steps, "ms_per_launch": round(t / n, 5),
@@ -493,7 +493,7 @@ def main() -> int:
         return sum(kern[k]["ms_per_step"] for k in names if k in kern)
 
     stages = {"stage6": group_ms(["zt_bc7_stage6"]), "stage5": group_ms(["zt_bc7_stage5"]),
-              "stage1": group_ms(["zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_stage1"]),
+              "stage1": group_ms(["zt_bc7_s1_rank", "zt_bc7_s1_fit", "zt_bc7_s1_pick", "zt_bc7_s1_fitpick", "zt_bc7_stage1"]),
               "mips": group_ms(["zt_mip_tiles", "zt_mip_tail", "zt_mip_level"])}
     s6 = kern["zt_bc7_stage6"]["ms_per_launch"]
     # the mip chain alone, back to back without per-kernel events (which add a few microseconds to every launch):

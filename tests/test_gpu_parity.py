@@ -196,3 +196,18 @@ def test_split_merge_match_oracle(zt, oracle):
         assert np.array_equal(sp, exp), f
         assert np.array_equal(zt.bc_merge(sp, f), raw), f
 
+
+
+@pytest.mark.parametrize("m1k", [1, 2, 3, 8])
+def test_mode1_fit_and_pick_forms(zt, oracle, m1k):
+    """Mode 1's fits pick their winner in the same launch when a block's 2 * M1K fits make whole lane groups
+    (M1K = 1, 2, 4, 8; zt_bc7_s1_fitpick), else in a second launch (zt_bc7_s1_fit + zt_bc7_s1_pick, M1K = 3):
+    both against the oracle at the same M1K, on the adversarial blocks."""
+    blocks = synth.blocks_to_image(synth.random_blocks(8192, 7))
+    zt.set_m1k(m1k)
+    oracle.lib.zto_set_m1k(m1k)
+    try:
+        assert np.array_equal(zt.bc_encode(blocks, "bc7"), oracle.bc_encode(blocks, "bc7")), m1k
+    finally:
+        zt.set_m1k(4)
+        oracle.lib.zto_set_m1k(4)

@@ -256,6 +256,50 @@ extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fit(zt_bc7_lists L, 
 }
 
 namespace {
+// the mode-1 block of partition `part` from its two subsets' fits: indices (4 bits per pixel, the subset's pixels
+// only) and endpoint words (6 bits per value: endpoint e, channel c at bit 18e + 6c; the shared p-bit at bit 36),
+// anchors' index MSBs cleared by swapping endpoints (zap_tex.h:745-748 and the pack that follows)
+ZT_DEV void mode1_pack(int part, uint64_t idx0, uint64_t qs0, uint64_t idx1, uint64_t qs1, uint64_t &lo, uint64_t &hi) {
+    const int anc = kA2[part];
+    int q[2][2][3], pb[2] = { (int)((qs0 >> 36) & 1), (int)((qs1 >> 36) & 1) };
+    uint64_t full = idx0 | idx1;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            q[0][x][c] = (int)((qs0 >> (18 * x + 6 * c)) & 63);
+            q[1][x][c] = (int)((qs1 >> (18 * x + 6 * c)) & 63);
+        }
+    const uint32_t m1 = kP2[part], m0 = ~m1 & 0xFFFFu;
+    uint64_t sel0 = 0, sel1 = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) { if ((m0 >> i) & 1) sel0 |= 7ull << (4 * i); else sel1 |= 7ull << (4 * i); }
+    if (full & 4) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { int t = q[0][0][c]; q[0][0][c] = q[0][1][c]; q[0][1][c] = t; }
+        full ^= sel0;
+    }
+    if ((full >> (4 * anc)) & 4) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { int t = q[1][0][c]; q[1][0][c] = q[1][1][c]; q[1][1][c] = t; }
+        full ^= sel1;
+    }
+    Bits128 b{ 2ull | (uint64_t)part << 2, 0 };
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int x = 0; x < 2; x++) b.put(8 + 6 * (4 * c + 2 * s + x), (uint64_t)q[s][x][c], 6);
+    b.put(80, (uint64_t)pb[0], 1); b.put(81, (uint64_t)pb[1], 1);
+    int pos = 82;
+    for (int i = 0; i < 16; i++) {
+        const int nb = 3 - (i == 0 || i == anc);
+        b.put(pos, (full >> (4 * i)) & 7, nb);
+        pos += nb;
+    }
+    lo = b.lo; hi = b.hi;
+}
 ZT_DEV void s1_pick_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k, const uint8_t *cand, const int *sub_err, const uint4 *sub_state,
                          uint32_t bid, uint32_t nblk) {
     const uint32_t tid = zt_tid();
@@ -268,53 +312,65 @@ ZT_DEV void s1_pick_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k, co
             if (err < best) { best = err; bk = k; }
         }
         if (bk < 0) continue;
-        const int part = cand[(uint64_t)e * 8 + bk], anc = kA2[part];
+        const int part = cand[(uint64_t)e * 8 + bk];
         const uint4 s0 = sub_state[(uint64_t)e * 16 + 2 * bk], s1 = sub_state[(uint64_t)e * 16 + 2 * bk + 1];
-        const uint64_t qs0 = (uint64_t)s0.z | (uint64_t)s0.w << 32, qs1 = (uint64_t)s1.z | (uint64_t)s1.w << 32;
-        int q[2][2][3], pb[2] = { (int)((qs0 >> 36) & 1), (int)((qs1 >> 36) & 1) };
-        uint64_t full = ((uint64_t)s0.x | (uint64_t)s0.y << 32) | ((uint64_t)s1.x | (uint64_t)s1.y << 32);
-#pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                q[0][x][c] = (int)((qs0 >> (18 * x + 6 * c)) & 63);
-                q[1][x][c] = (int)((qs1 >> (18 * x + 6 * c)) & 63);
-            }
-        const uint32_t m1 = kP2[part], m0 = ~m1 & 0xFFFFu;
-        uint64_t sel0 = 0, sel1 = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) { if ((m0 >> i) & 1) sel0 |= 7ull << (4 * i); else sel1 |= 7ull << (4 * i); }
-        if (full & 4) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) { int t = q[0][0][c]; q[0][0][c] = q[0][1][c]; q[0][1][c] = t; }
-            full ^= sel0;
-        }
-        if ((full >> (4 * anc)) & 4) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) { int t = q[1][0][c]; q[1][0][c] = q[1][1][c]; q[1][1][c] = t; }
-            full ^= sel1;
-        }
-        Bits128 b{ 2ull | (uint64_t)part << 2, 0 };
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int s = 0; s < 2; s++)
-#pragma unroll
-                for (int x = 0; x < 2; x++) b.put(8 + 6 * (4 * c + 2 * s + x), (uint64_t)q[s][x][c], 6);
-        b.put(80, (uint64_t)pb[0], 1); b.put(81, (uint64_t)pb[1], 1);
-        int pos = 82;
-        for (int i = 0; i < 16; i++) {
-            int nb = 3 - (i == 0 || i == anc);
-            b.put(pos, (full >> (4 * i)) & 7, nb);
-            pos += nb;
-        }
+        uint64_t lo, hi;
+        mode1_pack(part, (uint64_t)s0.x | (uint64_t)s0.y << 32, (uint64_t)s0.z | (uint64_t)s0.w << 32,
+                   (uint64_t)s1.x | (uint64_t)s1.y << 32, (uint64_t)s1.z | (uint64_t)s1.w << 32, lo, hi);
         const uint4 meta = L.meta5[m.x];
         const zt_job J = job_at(jobs, (int)meta.z);
-        store16(J.dst + (uint64_t)(meta.x - J.first) * 16, b.lo, b.hi);
+        store16(J.dst + (uint64_t)(meta.x - J.first) * 16, lo, hi);
     }
 }
 }  // namespace
 extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err,
                                                                    const uint4 *sub_state) {
     s1_pick_body(jobs, L, m1k, cand, sub_err, sub_state, blockIdx.x, gridDim.x);
+}
+
+// The fits and the pick in one launch, when a block's 2 * M1K fits make whole lane groups of a wave (M1K = 1, 2, 4 or
+// 8): the group's lanes hand their SSEs and states to the winner by shuffles instead of a second kernel reading them
+// back.  The winner is the reference's: the first candidate in rank order whose two subsets' SSE sum is the smallest,
+// if that is below the block's best so far (zap_tex.h:771-783).
+namespace {
+ZT_DEV void s1_fitpick_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k, const uint8_t *cand, uint32_t bid, uint32_t nblk) {
+    const uint32_t tid = zt_tid();
+    const uint32_t n1 = shard_total(L.counters + ZT_BC7_LIST1), items = n1 * (uint32_t)m1k * 2, step = nblk * blockDim.x;
+    const int G = 2 * m1k;
+    // a lane group (2 M1K lanes, aligned in the wave) is one block's items, all in or all past `items`
+    for (uint32_t it = bid * blockDim.x + tid; it < items; it += step) {
+        const uint32_t e = it / (2u * m1k), k = (it >> 1) % (uint32_t)m1k, sb = it & 1;
+        const uint4 m = L.meta1[shard_slot(L.counters + ZT_BC7_LIST1, L.cap1, e)];
+        uint32_t px[16];
+        const uint4 *s = L.px5 + (uint64_t)m.x * 4;
+#pragma unroll
+        for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+        const int part = cand[(uint64_t)e * 8 + k];
+        const uint32_t m1 = kP2[part], mask = sb ? m1 : (~m1 & 0xFFFFu);
+        int q[2][4], pb[2];
+        uint64_t idx;
+        ChanWords T;
+        chan_words(px, T);
+        const int err = encode_subset<3, 7, 3, true, false>(px, T, mask, __popc(mask), q, pb, idx);
+        const uint64_t qs = (uint64_t)q[0][0] | (uint64_t)q[0][1] << 6 | (uint64_t)q[0][2] << 12 | (uint64_t)q[1][0] << 18 |
+                            (uint64_t)q[1][1] << 24 | (uint64_t)q[1][2] << 30 | (uint64_t)pb[0] << 36;
+        // candidate k's SSE (its subsets sit on lanes 2k and 2k + 1), then (SSE, k) smallest over the group
+        const int tot = err + __shfl_xor(err, 1);
+        uint32_t key = tot < (int)m.y ? (uint32_t)tot << 4 | k : 0xFFFFFFFFu;   // SSE < 2^22
+        for (int off = 2; off < G; off <<= 1) key = min(key, (uint32_t)__shfl_xor((int)key, off));
+        // the other subset's state, for the winner's first lane
+        const uint32_t o0 = (uint32_t)__shfl_down((int)(uint32_t)idx, 1), o1 = (uint32_t)__shfl_down((int)(uint32_t)(idx >> 32), 1);
+        const uint32_t o2 = (uint32_t)__shfl_down((int)(uint32_t)qs, 1), o3 = (uint32_t)__shfl_down((int)(uint32_t)(qs >> 32), 1);
+        if (key != 0xFFFFFFFFu && (key & 15u) == k && sb == 0) {
+            uint64_t lo, hi;
+            mode1_pack(part, idx, qs, (uint64_t)o0 | (uint64_t)o1 << 32, (uint64_t)o2 | (uint64_t)o3 << 32, lo, hi);
+            const uint4 meta = L.meta5[m.x];
+            const zt_job J = job_at(jobs, (int)meta.z);
+            store16(J.dst + (uint64_t)(meta.x - J.first) * 16, lo, hi);
+        }
+    }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(256) zt_bc7_s1_fitpick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand) {
+    s1_fitpick_body(jobs, L, m1k, cand, blockIdx.x, gridDim.x);
 }

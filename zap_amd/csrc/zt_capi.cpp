@@ -360,6 +360,9 @@ int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipSt
         const int gsz = groups_for(CFG_S1, (uint64_t)tot / 16 + 1, w1 / 64);
         if ((rc = launch("zt_bc7_s1_rank", s, [&] { hipLaunchKernelGGL(zt_bc7_s1_rank, dim3(gsz), dim3(w1), 0, s, L, g_m1k, L.cand1); })))
             return rc;
+        if (64 % (2 * g_m1k) == 0)   // a block's fits make whole lane groups: they pick the winner themselves
+            return launch("zt_bc7_s1_fitpick", s, [&] {
+                hipLaunchKernelGGL(zt_bc7_s1_fitpick, dim3(gsz), dim3(w1), 0, s, J, L, g_m1k, (const uint8_t *)L.cand1); });
         if ((rc = launch("zt_bc7_s1_fit", s, [&] {
                 hipLaunchKernelGGL(zt_bc7_s1_fit, dim3(gsz), dim3(w1), 0, s, L, g_m1k, (const uint8_t *)L.cand1, L.sub_err, L.sub_state); })))
             return rc;

@@ -151,6 +151,7 @@ __global__ void zt_zero_u32(unsigned *p, int n);
 __global__ void zt_bc7_s1_rank(zt_bc7_lists L, int m1k, uint8_t *cand);
 __global__ void zt_bc7_s1_fit(zt_bc7_lists L, int m1k, const uint8_t *cand, int *sub_err, uint4 *sub_state);
 __global__ void zt_bc7_s1_pick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand, const int *sub_err, const uint4 *sub_state);
+__global__ void zt_bc7_s1_fitpick(zt_jobs jobs, zt_bc7_lists L, int m1k, const uint8_t *cand);   // M1K = 1, 2, 4 or 8
 // exhaustive all-mode BC7 search: one launch per mode (mode 4: per index selector) over dense blocks, optionally a
 // list of them, folding (SSE << 9 | candidate) into key[block]; zt_bc7_finish encodes the winners
 #define ZT_EXH_DECL(NAME) __global__ void NAME(const uint4 *blocks, uint32_t total, const uint32_t *list, const unsigned *nlist, \
