@@ -1714,14 +1714,15 @@ int plan_queue(const MipPlan &P, const std::vector<FamJob> &f14, const std::vect
     return ZAPTEX_OK;
 }
 
-struct BatchSizes { size_t steps, items, mipjobs, jobs14, jobs7; uint64_t blocks14, blocks7; int n14, n7; };
+struct BatchSizes { size_t steps, items, mipjobs, jobs14, jobs7; uint64_t blocks14, blocks7, blocks14_0, blocks7_0; int n14, n7; };
 BatchSizes batch_sizes(const zaptex_texture *t, int n, const MipPlan &P) {
     BatchSizes z{};
     for (int i = 0; i < n; i++) {
         int cw = t[i].w, ch = t[i].h;
         for (int l = 0; l < t[i].levels; l++) {
             const uint64_t nb = (uint64_t)((cw + 3) / 4) * ((ch + 3) / 4);
-            if (t[i].fmt == ZAPTEX_BC7) { z.n7++; z.blocks7 += nb; } else { z.n14++; z.blocks14 += nb; }
+            if (t[i].fmt == ZAPTEX_BC7) { z.n7++; z.blocks7 += nb; z.blocks7_0 += l ? 0 : nb; }
+            else { z.n14++; z.blocks14 += nb; z.blocks14_0 += l ? 0 : nb; }
             cw = cw > 1 ? cw / 2 : 1; ch = ch > 1 ? ch / 2 : 1;
         }
     }
@@ -1731,6 +1732,13 @@ BatchSizes batch_sizes(const zaptex_texture *t, int n, const MipPlan &P) {
     z.jobs14 = align_up((size_t)z.n14 * sizeof(zt_job), 256);
     z.jobs7 = align_up((size_t)z.n7 * sizeof(zt_job), 256);
     return z;   // (each family's table is held twice: whole, and its levels >= 1 rebased to block 0)
+}
+// The encoders' workspace of a batch: the BC7 lists of the level-0 blocks, those of the later levels (the two run
+// at once, on their own streams), then 16 B of BC1 climb state per BC1-family block (level 0 first).  The two BC7
+// regions together hold at least ws_bytes_for(all BC7 blocks), what one list over every block needs.
+size_t batch_bc7_a_bytes(const BatchSizes &z) { return align_up(ws_bytes_for(z.blocks7_0), 256); }
+size_t batch_bc7_bytes(const BatchSizes &z) { return batch_bc7_a_bytes(z) + ws_bytes_for(z.blocks7 - z.blocks7_0); }
+size_t batch_enc_bytes(const BatchSizes &z) { return align_up(batch_bc7_bytes(z), 256) + z.blocks14 * 16;
 }
 }  // namespace
 
@@ -1748,8 +1756,10 @@ struct zaptex_batch {
     // its later levels (rebased to block 0, after the mips and the family's level-0 encode, sharing its workspace)
     zt_jobs j14a{}, j14b{}, j7a{}, j7b{};
     uint32_t tot14a = 0, tot14b = 0, tot7a = 0, tot7b = 0;
-    void *bc7ws = nullptr;
+    void *bc7ws = nullptr;     // the BC7 lists: the whole region (one list over every block) ...
     size_t bc7ws_bytes = 0;
+    void *bc7ws_b = nullptr;   // ... or the level-0 lists at its start and the later levels' lists from here
+    size_t bc7ws_a_bytes = 0, bc7ws_b_bytes = 0;
     uint4 *climb = nullptr;
     bool persistent = false;   // one zt_batch_persistent launch (else the per-stage kernels)
     zt_batch_args args;
@@ -1792,7 +1802,7 @@ size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n) {
     if (z.blocks14 >= (1ull << 31) || z.blocks7 >= (1ull << 31)) return 0;
     if (plan_queue(P, family_jobs(t, n, P, false), family_jobs(t, n, P, true), (uint32_t)z.blocks14, (uint32_t)z.blocks7, PqTables(), Q)) return 0;
     const QueueSizes qz = queue_sizes(Q);
-    return 256 + z.steps + z.items + z.mipjobs + 2 * (z.jobs14 + z.jobs7) + qz.queue + qz.segs + qz.ctl + ws_total(z.blocks7, z.blocks14);
+    return 256 + z.steps + z.items + z.mipjobs + 2 * (z.jobs14 + z.jobs7) + qz.queue + qz.segs + qz.ctl + batch_enc_bytes(z);
 }
 
 int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_bytes, zaptex_batch **out) {
@@ -1827,8 +1837,11 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     zt_seg *dsegs = (zt_seg *)p; p += qz.segs;
     unsigned *dctl = (unsigned *)p; p += qz.ctl;
     b->bc7ws = p;
-    b->bc7ws_bytes = ws_bytes_for(z.blocks7);
-    b->climb = (uint4 *)(p + ws_climb_offset(z.blocks7));
+    b->bc7ws_bytes = batch_bc7_bytes(z);
+    b->bc7ws_a_bytes = batch_bc7_a_bytes(z);
+    b->bc7ws_b = p + b->bc7ws_a_bytes;
+    b->bc7ws_b_bytes = b->bc7ws_bytes - b->bc7ws_a_bytes;
+    b->climb = (uint4 *)(p + align_up(b->bc7ws_bytes, 256));
     b->mip_launches = P.launches;
     b->tile_steps = P.tile_steps;
     b->mip_tile = P.tile;
@@ -1871,6 +1884,7 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     };
     split(f14, h14, d14, d14b, r14, b->j14a, b->j14b, b->tot14a, b->tot14b, b->tot14);
     split(f7, h7, d7, d7b, r7, b->j7a, b->j7b, b->tot7a, b->tot7b, b->tot7);
+    if (b->tot7a != z.blocks7_0 || b->tot14a != z.blocks14_0) { delete b; return fail(ZAPTEX_E_ARG, "batch plan: level-0 block counts disagree"); }
     if (!r14.empty()) HIP_TRY(hipMemcpyAsync(d14b, r14.data(), r14.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
     if (!r7.empty()) HIP_TRY(hipMemcpyAsync(d7b, r7.data(), r7.size() * sizeof(zt_job), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1902,8 +1916,11 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
     }
     // capture as one graph: the mip levels in order on the capturing stream; beside them, on two more streams, each
     // family's level-0 encode (BC1-5 on one, BC7 on the other); then each family's later levels once the mips are done
-    hipStream_t side[2] = { nullptr, nullptr };
-    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };   // fork, mips done, side 0 done, side 1 done
+    // streams: each family's level-0 encode (side 0: BC1-5, side 1: BC7) and its later levels (sides 2 and 3), the
+    // later levels in their own workspace (BC7 lists of their own, the BC1 climb state after level 0's), so they start
+    // as soon as the mips are done
+    hipStream_t side[4] = { nullptr, nullptr, nullptr, nullptr };
+    hipEvent_t ev[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };   // fork, mips done, sides 0..3 done
     auto cleanup = [&] {
         for (hipEvent_t &e : ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
         for (hipStream_t &x : side) if (x) { (void)hipStreamDestroy(x); x = nullptr; }
@@ -1950,18 +1967,18 @@ int zaptex_batch_create(const zaptex_texture *t, int n, void *ws, size_t ws_byte
             return ZAPTEX_OK;
         }
         HIP_TRY(hipEventRecord(ev[0], cs));
-        for (hipStream_t x : side) HIP_TRY(hipStreamWaitEvent(x, ev[0], 0));
+        for (int k = 0; k < 2; k++) HIP_TRY(hipStreamWaitEvent(side[k], ev[0], 0));
         if (b->tot14a && (rc = enqueue_bc14(b->j14a, b->tot14a, b->climb, side[0]))) return rc;
-        if (b->tot7a && (rc = enqueue_bc7(b->j7a, b->tot7a, b->bc7ws, b->bc7ws_bytes, side[1]))) return rc;
+        if (b->tot7a && (rc = enqueue_bc7(b->j7a, b->tot7a, b->bc7ws, b->bc7ws_a_bytes, side[1]))) return rc;
         if ((rc = mips_body(cs))) return rc;
         HIP_TRY(hipEventRecord(ev[1], cs));
-        for (hipStream_t x : side) HIP_TRY(hipStreamWaitEvent(x, ev[1], 0));
-        if (b->tot14b && (rc = enqueue_bc14(b->j14b, b->tot14b, b->climb, side[0]))) return rc;
-        if (b->tot7b && (rc = enqueue_bc7(b->j7b, b->tot7b, b->bc7ws, b->bc7ws_bytes, side[1]))) return rc;
-        HIP_TRY(hipEventRecord(ev[2], side[0]));
-        HIP_TRY(hipEventRecord(ev[3], side[1]));
-        HIP_TRY(hipStreamWaitEvent(cs, ev[2], 0));
-        HIP_TRY(hipStreamWaitEvent(cs, ev[3], 0));
+        for (int k = 2; k < 4; k++) HIP_TRY(hipStreamWaitEvent(side[k], ev[1], 0));
+        if (b->tot14b && (rc = enqueue_bc14(b->j14b, b->tot14b, b->climb + b->tot14a, side[2]))) return rc;
+        if (b->tot7b && (rc = enqueue_bc7(b->j7b, b->tot7b, b->bc7ws_b, b->bc7ws_b_bytes, side[3]))) return rc;
+        for (int k = 0; k < 4; k++) {
+            HIP_TRY(hipEventRecord(ev[2 + k], side[k]));
+            HIP_TRY(hipStreamWaitEvent(cs, ev[2 + k], 0));
+        }
         return ZAPTEX_OK;
     };
     {
