@@ -530,8 +530,10 @@ def main() -> int:
             counters = rocprof_counters(c2_path)
         finally:
             os.unlink(c2_path)
-    s6c, cpc = counters.get("zt_bc7_stage6"), counters.get("zt_copy_probe")
-    roof = {"bound": "valu", "kernel": "zt_bc7_stage6", "unit": "G VALU wave-instructions/s", "peak": ceiling,
+    # the mode-6 stage runs as zt_bc7_stage6q (refits queued per wave, the default) or zt_bc7_stage6 (one pass)
+    s6name = "zt_bc7_stage6q" if "zt_bc7_stage6q" in counters else "zt_bc7_stage6"
+    s6c, cpc = counters.get(s6name), counters.get("zt_copy_probe")
+    roof = {"bound": "valu", "kernel": s6name, "unit": "G VALU wave-instructions/s", "peak": ceiling,
             "peak_is": "the fastest issue rate of the VALU probe (zt_valu_probe, inline-asm chains) measured in this run",
             "ms_per_launch": s6, "achieved": None, "frac": None, "traffic": None}
     if s6c and cpc and "SQ_INSTS_VALU" in s6c:

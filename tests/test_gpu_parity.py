@@ -211,3 +211,34 @@ def test_mode1_fit_and_pick_forms(zt, oracle, m1k):
     finally:
         zt.set_m1k(4)
         oracle.lib.zto_set_m1k(4)
+
+
+@pytest.mark.parametrize("chunks", [0, 1, 2, 5])
+def test_mode6_chunks_per_wave_do_not_change_bytes(zt, golden, oracle, chunks):
+    """The BC7 mode-6 stage in one pass per block (0) or with its refits queued over k 64-block chunks per wave
+    (ZAPTEX_OPT_S6_CHUNKS): the reference's bytes either way, also where a wave's chunks cross from one level into
+    the next (a 10-job chain in one call) and where the queues are flushed part full."""
+    import torch
+    OPT_S6_CHUNKS = 14
+    assert zt.get_option(OPT_S6_CHUNKS) == 2
+    blocks = synth.blocks_to_image(synth.random_blocks(8192, 7))
+    c2 = synth.c2_bc7(512, 2)
+    base = torch.from_numpy(c2).cuda()
+    sizes = binding.mip_sizes(512, 512)
+    levels = [torch.empty((h, w, 4), dtype=torch.uint8, device="cuda") for (w, h) in sizes[1:]]
+    zt.mip_chain_dev(base, 512, 512, 512 * 4, levels, srgb=0)
+    torch.cuda.synchronize()
+    src = [base] + levels
+    zt.set_option(OPT_S6_CHUNKS, chunks)
+    try:
+        assert sha(zt.bc_encode(blocks, "bc7")) == golden["outputs"]["blocks_8192_bc7"]
+        outs = [torch.zeros(binding.bc_size(w, h, "bc7"), dtype=torch.uint8, device="cuda") for (w, h) in sizes]
+        imgs = zt.images((src[i], outs[i], w, h, w * 4, "bc7") for i, (w, h) in enumerate(sizes))
+        ws = torch.empty(zt.workspace_size(imgs), dtype=torch.uint8, device="cuda")
+        zt.encode_dev(imgs, ws)
+        torch.cuda.synchronize()
+        got = [o.cpu().numpy() for o in outs]
+    finally:
+        zt.set_option(OPT_S6_CHUNKS, 2)
+    for lv in range(len(sizes)):
+        assert np.array_equal(got[lv], oracle.bc_encode(src[lv].cpu().numpy(), "bc7")), (chunks, lv)

@@ -60,6 +60,105 @@ extern "C" __global__ void __launch_bounds__(256, 4) zt_bc7_stage6(zt_jobs jobs,
     stage6_body(jobs, total, L, blockIdx.x, gridDim.x);
 }
 
+// Stage 6 with the refinement steps queued (mode6_first / mode6_refit).  A step whose endpoints quantise to the ones
+// the block has ends the reference's loop without a fit, so on textures a block needs its second fit about 40% of
+// the time and its third about 10%, where a wave of 64 blocks one lane each runs all three for every lane.  Here a
+// wave (one per workgroup) runs the first pass over `kc` chunks of 64 blocks; the blocks that go on wait in the
+// wave's LDS queue, and whenever 64 are queued the wave runs the next pass over those 64 at once (a block that goes
+// on again waits in the second queue).  What is left when the wave's chunks are done is run before it exits, so
+// waves stay short and new ones start as others end.
+namespace {
+constexpr uint32_t kRefitQ = 128;   // queue slots: < 64 waiting + 64 more from a chunk or a pass
+struct RefitQueue { uint32_t g[kRefitQ], j[kRefitQ], err[kRefitQ], lo[kRefitQ], hi[kRefitQ]; };
+
+ZT_DEV void list5_put(const zt_bc7_lists &L, int slot, uint32_t g, int err, int j, const uint32_t px[16]) {
+    uint32_t amin = 255;
+#pragma unroll
+    for (int i = 0; i < 16; i++) amin = min(amin, px[i] >> 24);
+    L.meta5[slot] = make_uint4(g, (uint32_t)err, (uint32_t)j, amin == 255 ? 1u : 0u);
+    uint4 *d = L.px5 + (uint64_t)slot * 4;
+    d[0] = make_uint4(px[0], px[1], px[2], px[3]); d[1] = make_uint4(px[4], px[5], px[6], px[7]);
+    d[2] = make_uint4(px[8], px[9], px[10], px[11]); d[3] = make_uint4(px[12], px[13], px[14], px[15]);
+}
+// queue the lanes with pred at tail (wave-uniform); returns how many
+ZT_DEV uint32_t refit_push(RefitQueue &Q, uint32_t tail, bool pred, uint32_t g, int j, int err, uint64_t nqp) {
+    const uint64_t m = __ballot(pred);
+    if (pred) {
+        const uint32_t k = (tail + lanes_below(m)) % kRefitQ;
+        Q.g[k] = g; Q.j[k] = (uint32_t)j; Q.err[k] = (uint32_t)err; Q.lo[k] = (uint32_t)nqp; Q.hi[k] = (uint32_t)(nqp >> 32);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return __builtin_amdgcn_readfirstlane((uint32_t)__popcll(m));
+}
+// the next pass over count (<= 64) queued blocks from head; blocks that go on are queued in N at ntail.  A block
+// enters list 5 in the shard of its own index (g >> 6), as in the first pass.
+template <bool LAST>
+ZT_DEV void refit_round(const zt_jobs &jobs, const zt_bc7_lists &L, const RefitQueue &Q, uint32_t head, uint32_t count, RefitQueue &N,
+                        uint32_t &ntail) {
+    const uint32_t lane = (uint32_t)__lane_id(), k = (head + lane) % kRefitQ;
+    const bool act = lane < count;
+    uint32_t g = 0, px[16];
+    int j = 0, err = 0;
+    uint64_t nqp = 0;
+    bool more = false, improved = false;
+    if (act) {
+        g = Q.g[k]; j = (int)Q.j[k]; err = (int)Q.err[k]; nqp = (uint64_t)Q.lo[k] | (uint64_t)Q.hi[k] << 32;
+        const zt_job J = job_at(jobs, j);
+        const uint32_t b = g - J.first;
+        gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+        uint64_t lo, hi;
+        err = mode6_refit<LAST>(px, err, nqp, lo, hi, improved, more);
+        if (improved) store16(J.dst + (uint64_t)b * 16, lo, hi);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int slot = wave_append_sharded(act && !more && err > 64, L.counters, L.cap5, g);
+    if (slot >= 0) list5_put(L, slot, g, err, j, px);
+    if (!LAST) ntail += refit_push(N, ntail, more, g, j, err, nqp);
+}
+// blockDim.x == 64; the wave's chunks are bid * kc .. bid * kc + kc - 1
+ZT_DEV void stage6q_body(const zt_jobs &jobs, uint32_t total, const zt_bc7_lists &L, uint32_t bid, uint32_t kc) {
+    __shared__ RefitQueue Q2, Q3;
+    const uint32_t lane = zt_tid() & 63u, nch = (total + 63) / 64, c1 = min(nch, (bid + 1) * kc);
+    uint32_t h2 = 0, t2 = 0, h3 = 0, t3 = 0;
+    for (uint32_t c = bid * kc; c < c1; c++) {
+        const uint32_t g = c * 64 + lane;
+        const bool valid = g < total;
+        bool more = false;
+        int err = 0, j = 0;
+        uint64_t nqp = 0;
+        uint32_t px[16];
+        if (valid) {
+            j = zt_find_job(jobs, g);
+            const zt_job J = job_at(jobs, j);
+            const uint32_t b = g - J.first;
+            gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+            uint64_t lo, hi;
+            err = mode6_first(px, lo, hi, more, nqp);
+            store16(J.dst + (uint64_t)b * 16, lo, hi);
+        }
+        const uint32_t sh = c & (ZT_SHARDS - 1);
+        const int slot = wave_append(valid && !more && err > 64, L.counters + sh * ZT_SHARD_STRIDE);
+        if (slot >= 0) list5_put(L, slot + (int)(sh * L.cap5), g, err, j, px);
+        t2 += refit_push(Q2, t2, more, g, j, err, nqp);
+        if (t2 - h2 >= 64) { refit_round<false>(jobs, L, Q2, h2, 64, Q3, t3); h2 += 64; }
+        if (t3 - h3 >= 64) { refit_round<true>(jobs, L, Q3, h3, 64, Q3, t3); h3 += 64; }
+    }
+    while (t2 != h2) {
+        const uint32_t n = min(64u, t2 - h2);
+        refit_round<false>(jobs, L, Q2, h2, n, Q3, t3);
+        h2 += n;
+    }
+    while (t3 != h3) {
+        const uint32_t n = min(64u, t3 - h3);
+        refit_round<true>(jobs, L, Q3, h3, n, Q3, t3);
+        h3 += n;
+    }
+}
+}  // namespace
+extern "C" __global__ void __launch_bounds__(64, 4) zt_bc7_stage6q(zt_jobs jobs, uint32_t total, zt_bc7_lists L, uint32_t kc) {
+    stage6q_body(jobs, total, L, blockIdx.x, kc);
+}
+
 // Stage 5: lane = (list-5 entry, rotation). Keeps mode 5 when it beats mode 6; opaque blocks go to list 1.
 namespace {
 ZT_DEV void stage5_body(const zt_jobs &jobs, const zt_bc7_lists &L, uint32_t bid, uint32_t nblk) {

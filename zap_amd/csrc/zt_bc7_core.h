@@ -344,6 +344,77 @@ ZT_DEV int mode6(const uint32_t px[16], uint64_t &lo, uint64_t &hi) {
     pack6(q, pb, idx, lo, hi);
     return err;
 }
+// Mode 6 in passes.  A refinement step whose least-squares endpoints quantise to the endpoints and p-bits the
+// block already has refits nothing: the same palette gives the same SSE, which the reference's loop rejects
+// (ne >= err) and stops.  So each pass ends every block it can, and hands on only the blocks whose next fit has new
+// endpoints.  q and p-bits travel as one word: q[e][c] at 7 (2c + e), p-bits at 56 and 57.
+ZT_DEV uint64_t qp_word(const int q[2][4], const int pb[2]) {
+    uint64_t w = (uint64_t)pb[0] << 56 | (uint64_t)pb[1] << 57;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) w |= (uint64_t)q[e][c] << (7 * (2 * c + e));
+    return w;
+}
+ZT_DEV void qp_unword(uint64_t w, int q[2][4], int pb[2]) {
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) q[e][c] = (int)((w >> (7 * (2 * c + e))) & 127u);
+    pb[0] = (int)((w >> 56) & 1u); pb[1] = (int)((w >> 57) & 1u);
+}
+// the next refinement step after a fit with (q, pb, idx) and SSE err: true with its quantised endpoints in nqp when
+// they differ from (q, pb) (the step's fit may improve), false when the loop ends here
+ZT_DEV bool mode6_next(const uint32_t px[16], const ChanWords &T, const int P[4], int err, const int q[2][4], const int pb[2],
+                       uint64_t idx, uint64_t &nqp) {
+    if (err <= 0) return false;
+    float nlo[4], nhi[4];
+    if (!ls_step<4, 4, true>(px, T, P, 16, idx, nlo, nhi)) return false;
+    int nq[2][4], np[2];
+    quant_pair<4, 8, false>(nlo, nhi, nq, np);
+    nqp = qp_word(nq, np);
+    return nqp != qp_word(q, pb);
+}
+// first pass: the PCA start, its fit and the first step's endpoints; returns the SSE, the block in (lo, hi), and
+// whether the block goes on (nqp = the endpoints of its first refit)
+ZT_DEV int mode6_first(const uint32_t px[16], uint64_t &lo, uint64_t &hi, bool &more, uint64_t &nqp) {
+    ChanWords T;
+    chan_words(px, T);
+    int P[4];
+    chan_sums<4, 0, true>(T, 0xFFFFu, P);
+    float plo[4], phi[4];
+    pca<4, true, false>(px, 0xFFFFu, 16, plo, phi, P);
+    const int spp = subset_sq<4, 0>(px, 0xFFFFu);
+    int q[2][4], pb[2];
+    uint64_t idx;
+    const int err = fit_subset<4, 8, 4, false>(px, 0xFFFFu, spp, ~0ull, plo, phi, q, pb, idx);
+    more = mode6_next(px, T, P, err, q, pb, idx, nqp);
+    pack6(q, pb, idx, lo, hi);
+    return err;
+}
+// a later pass: the fit of the endpoints nqp against the block's SSE err.  An improvement is written to (lo, hi)
+// (improved = true) and, unless LAST, the next step's endpoints decide whether the block goes on (more = true, nqp =
+// those endpoints).
+template <bool LAST>
+ZT_DEV int mode6_refit(const uint32_t px[16], int err, uint64_t &nqp, uint64_t &lo, uint64_t &hi, bool &improved, bool &more) {
+    int q[2][4], pb[2];
+    qp_unword(nqp, q, pb);
+    uint64_t idx;
+    const int ne = fit_quant<4, 8, 4>(px, 0xFFFFu, subset_sq<4, 0>(px, 0xFFFFu), ~0ull, q, pb, idx);
+    improved = ne < err;
+    more = false;
+    if (!improved) return err;
+    if (!LAST) {
+        ChanWords T;
+        chan_words(px, T);
+        int P[4];
+        chan_sums<4, 0, true>(T, 0xFFFFu, P);
+        more = mode6_next(px, T, P, ne, q, pb, idx, nqp);
+    }
+    pack6(q, pb, idx, lo, hi);
+    return ne;
+}
+
 // ---- mode 5 pieces (zap_tex.h:597-657); NC = 3 (colour, 7 bits) or 1 (alpha, 8 bits)
 // plain n-bit quantiser (zap_tex.h:613-624): candidates and keys as in quant_pbit1 (checked exhaustively for
 // every float |v| <= 2048 and n = 4..8)

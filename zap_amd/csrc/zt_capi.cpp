@@ -26,6 +26,8 @@ int g_climb4_max = 1 << 20;   // BC1-family blocks up to which a launch starts w
                               // and climbs with four (zt_bc1_climb4)
 int g_climb4_range = 0;       // zt_bc1_climb4's blocks per wave (0: automatic)
 int g_climb_group = 4;        // lanes per block of the grouped climb (4 or 8)
+int g_s6_chunks = 2;          // BC7 mode-6 stage: 0 = one pass per block (zt_bc7_stage6), k > 0 = refits queued
+                              // over k 64-block chunks per wave (zt_bc7_stage6q; 2: the fastest on the 4096^2 chain)
 
 // launch shapes of the grid-stride kernels: workgroup size (64, 128 or 256) and the cap on workgroups;
 // zaptex_set_launch changes them (the autotuner's results), a batch captures those in force when it is built
@@ -350,8 +352,12 @@ int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipSt
             hipLaunchKernelGGL(zt_zero_u32, dim3(ZT_BC7_COUNTERS / 256), dim3(256), 0, s, L.counters, ZT_BC7_COUNTERS); })))
         return rc;
     const int w6 = g_cfg[CFG_S6].wg, w5 = g_cfg[CFG_S5].wg, w1 = g_cfg[CFG_S1].wg;
-    if ((rc = launch("zt_bc7_stage6", s, [&] {
-            hipLaunchKernelGGL(zt_bc7_stage6, dim3(groups_for(CFG_S6, tot, w6)), dim3(w6), 0, s, J, tot, L); })))
+    if (g_s6_chunks > 0) {
+        const uint32_t kc = (uint32_t)g_s6_chunks, waves = (uint32_t)((((uint64_t)tot + 63) / 64 + kc - 1) / kc);
+        if ((rc = launch("zt_bc7_stage6", s, [&] { hipLaunchKernelGGL(zt_bc7_stage6q, dim3(waves), dim3(64), 0, s, J, tot, L, kc); })))
+            return rc;
+    } else if ((rc = launch("zt_bc7_stage6", s, [&] {
+                   hipLaunchKernelGGL(zt_bc7_stage6, dim3(groups_for(CFG_S6, tot, w6)), dim3(w6), 0, s, J, tot, L); })))
         return rc;
     if ((rc = launch("zt_bc7_stage5", s, [&] {
             hipLaunchKernelGGL(zt_bc7_stage5, dim3(groups_for(CFG_S5, (uint64_t)tot * 4, w5)), dim3(w5), 0, s, J, L); })))
@@ -2069,6 +2075,11 @@ int zaptex_set_option(int key, int value) {
         g_climb_group = value;
         return ZAPTEX_OK;
     }
+    if (key == ZAPTEX_OPT_S6_CHUNKS) {
+        if (value < 0 || value > 64) return fail(ZAPTEX_E_ARG, "mode-6 chunks per wave must be 0..64");
+        g_s6_chunks = value;
+        return ZAPTEX_OK;
+    }
     if (key == ZAPTEX_OPT_MIP_TILE) {
         if (value != 32 && value != 64) return fail(ZAPTEX_E_ARG, "mip tile side must be 32 or 64");
         g_mip_tile = value;
@@ -2101,7 +2112,8 @@ int zaptex_get_option(int key) {
          : key == ZAPTEX_OPT_BATCH_PERSISTENT ? g_batch_persistent : key == ZAPTEX_OPT_BATCH_MIX ? g_batch_mix
          : key == ZAPTEX_OPT_BATCH_CHUNKS ? g_batch_per : key == ZAPTEX_OPT_BATCH_GROUPS ? g_batch_groups
          : key == ZAPTEX_OPT_MIP_TILE ? g_mip_tile : key == ZAPTEX_OPT_CLIMB4_MAX ? g_climb4_max
-         : key == ZAPTEX_OPT_CLIMB4_RANGE ? g_climb4_range : key == ZAPTEX_OPT_CLIMB_GROUP ? g_climb_group : -1;
+         : key == ZAPTEX_OPT_CLIMB4_RANGE ? g_climb4_range : key == ZAPTEX_OPT_CLIMB_GROUP ? g_climb_group
+         : key == ZAPTEX_OPT_S6_CHUNKS ? g_s6_chunks : -1;
 }
 
 int zaptex_profile_collect(void) {

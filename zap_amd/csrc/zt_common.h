@@ -179,6 +179,26 @@ ZT_DEV int wave_append(bool pred, unsigned *counter) {
     return (int)(base + lanes_below(mask));
 }
 
+// the same for lanes whose items lie in different shards (item i in shard (i >> 6) mod ZT_SHARDS): one atomic per
+// shard present in the wave; returns the lane's slot in the sharded list (shard * cap + position) or -1
+ZT_DEV int wave_append_sharded(bool pred, unsigned *counts, uint32_t cap, uint32_t item) {
+    const uint32_t sh = (item >> 6) & (ZT_SHARDS - 1);
+    const int lane = __lane_id();
+    int slot = -1;
+#pragma unroll
+    for (uint32_t k = 0; k < ZT_SHARDS; k++) {
+        const bool mine = pred && sh == k;
+        const uint64_t mask = __ballot(mine);
+        if (!mask) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(counts + k * ZT_SHARD_STRIDE, (unsigned)__popcll(mask));
+        base = __shfl(base, leader);
+        if (mine) slot = (int)(k * cap + base + lanes_below(mask));
+    }
+    return slot;
+}
+
 // LSB-first reader over a 128-bit block (zap_tex.h:367-371)
 struct BitReader {
     uint64_t lo, hi;

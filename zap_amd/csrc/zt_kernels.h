@@ -145,6 +145,7 @@ struct zt_batch_args {
 extern "C" {
 __global__ void zt_batch_persistent(zt_batch_args A);
 __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
+__global__ void zt_bc7_stage6q(zt_jobs jobs, uint32_t total, zt_bc7_lists L, uint32_t kc);   // 64 lanes per workgroup
 __global__ void zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L);
 __global__ void zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k);
 __global__ void zt_zero_u32(unsigned *p, int n);
