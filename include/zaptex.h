@@ -55,9 +55,9 @@ enum {
     ZAPTEX_OPT_CLIMB4_RANGE = 12,    /* blocks per wave of the grouped climb, a multiple of 8 (of 64 / lanes per block),
                                         or 0 = automatic */
     ZAPTEX_OPT_CLIMB_GROUP = 13,     /* lanes per block of the grouped climb: 4 (default) or 8 */
-    ZAPTEX_OPT_S6_CHUNKS = 14        /* BC7 mode-6 stage: 0 = one pass per block, k = 1..64: each wave takes k chunks of 64
-                                        blocks and runs the least-squares refits that can change a block over 64 queued
-                                        blocks at a time (default 2) */
+    ZAPTEX_OPT_S6_CHUNKS = 14        /* BC7 mode-6 stage: 0 = one pass per block, k = 1..64: the waves take k chunks of 64
+                                        blocks at a time until none are left, and run the least-squares refits that can
+                                        change a block over 64 queued blocks at a time (default 2) */
 };
 
 /* ---------------- host entry points (drop-in bodies of zap_tex.h) */

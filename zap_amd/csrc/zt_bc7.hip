@@ -63,10 +63,13 @@ extern "C" __global__ void __launch_bounds__(256, 4) zt_bc7_stage6(zt_jobs jobs,
 // Stage 6 with the refinement steps queued (mode6_first / mode6_refit).  A step whose endpoints quantise to the ones
 // the block has ends the reference's loop without a fit, so on textures a block needs its second fit about 40% of
 // the time and its third about 10%, where a wave of 64 blocks one lane each runs all three for every lane.  Here a
-// wave (one per workgroup) runs the first pass over `kc` chunks of 64 blocks; the blocks that go on wait in the
-// wave's LDS queue, and whenever 64 are queued the wave runs the next pass over those 64 at once (a block that goes
-// on again waits in the second queue).  What is left when the wave's chunks are done is run before it exits, so
-// waves stay short and new ones start as others end.
+// wave (one per workgroup) runs the first pass over chunks of 64 blocks; the blocks that go on wait in the wave's
+// LDS queue, and whenever 64 are queued the wave runs the next pass over those 64 at once (a block that goes on
+// again waits in the second queue).  The waves take `kc` chunks at a time from a cursor until none are left, so they
+// all run to the end of the stage (a fixed share per wave left the chip part idle while the last waves ran), then
+// run what is left in their queues.  There is a cursor per shard (chunk c is in shard c mod ZT_SHARDS, wave b takes
+// from shard b mod ZT_SHARDS), each on its own cache line: atomics on one address serialise, and one cursor for the
+// whole stage cost more than the idle tail.
 namespace {
 constexpr uint32_t kRefitQ = 128;   // queue slots: < 64 waiting + 64 more from a chunk or a pass
 struct RefitQueue { uint32_t g[kRefitQ], j[kRefitQ], err[kRefitQ], lo[kRefitQ], hi[kRefitQ]; };
@@ -115,33 +118,43 @@ ZT_DEV void refit_round(const zt_jobs &jobs, const zt_bc7_lists &L, const RefitQ
     if (slot >= 0) list5_put(L, slot, g, err, j, px);
     if (!LAST) ntail += refit_push(N, ntail, more, g, j, err, nqp);
 }
-// blockDim.x == 64; the wave's chunks are bid * kc .. bid * kc + kc - 1
+// blockDim.x == 64, gridDim.x >= min(chunks, ZT_SHARDS) (every shard with chunks has a wave); the cursors
+// L.counters[ZT_BC7_S6_CURSOR + s * ZT_SHARD_STRIDE] start at 0
 ZT_DEV void stage6q_body(const zt_jobs &jobs, uint32_t total, const zt_bc7_lists &L, uint32_t bid, uint32_t kc) {
     __shared__ RefitQueue Q2, Q3;
-    const uint32_t lane = zt_tid() & 63u, nch = (total + 63) / 64, c1 = min(nch, (bid + 1) * kc);
+    const uint32_t lane = zt_tid() & 63u, nch = (total + 63) / 64, sh = bid & (ZT_SHARDS - 1);
+    // the shard's chunks are sh, sh + ZT_SHARDS, ...: n of them
+    const uint32_t n = nch > sh ? (nch - sh + ZT_SHARDS - 1) / ZT_SHARDS : 0u;
+    unsigned *cursor = L.counters + ZT_BC7_S6_CURSOR + sh * ZT_SHARD_STRIDE;
     uint32_t h2 = 0, t2 = 0, h3 = 0, t3 = 0;
-    for (uint32_t c = bid * kc; c < c1; c++) {
-        const uint32_t g = c * 64 + lane;
-        const bool valid = g < total;
-        bool more = false;
-        int err = 0, j = 0;
-        uint64_t nqp = 0;
-        uint32_t px[16];
-        if (valid) {
-            j = zt_find_job(jobs, g);
-            const zt_job J = job_at(jobs, j);
-            const uint32_t b = g - J.first;
-            gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
-            uint64_t lo, hi;
-            err = mode6_first(px, lo, hi, more, nqp);
-            store16(J.dst + (uint64_t)b * 16, lo, hi);
+    for (;;) {
+        uint32_t i0 = 0;
+        if (lane == 0) i0 = __hip_atomic_fetch_add(cursor, kc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        i0 = __builtin_amdgcn_readfirstlane(__shfl(i0, 0));
+        if (i0 >= n) break;
+        for (uint32_t i = i0, i1 = min(n, i0 + kc); i < i1; i++) {
+            const uint32_t c = i * ZT_SHARDS + sh;
+            const uint32_t g = c * 64 + lane;
+            const bool valid = g < total;
+            bool more = false;
+            int err = 0, j = 0;
+            uint64_t nqp = 0;
+            uint32_t px[16];
+            if (valid) {
+                j = zt_find_job(jobs, g);
+                const zt_job J = job_at(jobs, j);
+                const uint32_t b = g - J.first;
+                gather_block(J.src, J.w, J.h, J.stride, (int)(b % (uint32_t)J.bw), (int)(b / (uint32_t)J.bw), px);
+                uint64_t lo, hi;
+                err = mode6_first(px, lo, hi, more, nqp);
+                store16(J.dst + (uint64_t)b * 16, lo, hi);
+            }
+            const int slot = wave_append(valid && !more && err > 64, L.counters + sh * ZT_SHARD_STRIDE);
+            if (slot >= 0) list5_put(L, slot + (int)(sh * L.cap5), g, err, j, px);
+            t2 += refit_push(Q2, t2, more, g, j, err, nqp);
+            if (t2 - h2 >= 64) { refit_round<false>(jobs, L, Q2, h2, 64, Q3, t3); h2 += 64; }
+            if (t3 - h3 >= 64) { refit_round<true>(jobs, L, Q3, h3, 64, Q3, t3); h3 += 64; }
         }
-        const uint32_t sh = c & (ZT_SHARDS - 1);
-        const int slot = wave_append(valid && !more && err > 64, L.counters + sh * ZT_SHARD_STRIDE);
-        if (slot >= 0) list5_put(L, slot + (int)(sh * L.cap5), g, err, j, px);
-        t2 += refit_push(Q2, t2, more, g, j, err, nqp);
-        if (t2 - h2 >= 64) { refit_round<false>(jobs, L, Q2, h2, 64, Q3, t3); h2 += 64; }
-        if (t3 - h3 >= 64) { refit_round<true>(jobs, L, Q3, h3, 64, Q3, t3); h3 += 64; }
     }
     while (t2 != h2) {
         const uint32_t n = min(64u, t2 - h2);

@@ -26,8 +26,8 @@ int g_climb4_max = 1 << 20;   // BC1-family blocks up to which a launch starts w
                               // and climbs with four (zt_bc1_climb4)
 int g_climb4_range = 0;       // zt_bc1_climb4's blocks per wave (0: automatic)
 int g_climb_group = 4;        // lanes per block of the grouped climb (4 or 8)
-int g_s6_chunks = 2;          // BC7 mode-6 stage: 0 = one pass per block (zt_bc7_stage6), k > 0 = refits queued
-                              // over k 64-block chunks per wave (zt_bc7_stage6q; 2: the fastest on the 4096^2 chain)
+int g_s6_chunks = 2;          // BC7 mode-6 stage: 0 = one pass per block (zt_bc7_stage6), k > 0 = refits queued per
+                              // wave, which takes k 64-block chunks at a time (zt_bc7_stage6q)
 
 // launch shapes of the grid-stride kernels: workgroup size (64, 128 or 256) and the cap on workgroups;
 // zaptex_set_launch changes them (the autotuner's results), a batch captures those in force when it is built
@@ -353,7 +353,14 @@ int enqueue_bc7(const zt_jobs &J, uint32_t tot, void *ws, size_t ws_bytes, hipSt
         return rc;
     const int w6 = g_cfg[CFG_S6].wg, w5 = g_cfg[CFG_S5].wg, w1 = g_cfg[CFG_S1].wg;
     if (g_s6_chunks > 0) {
-        const uint32_t kc = (uint32_t)g_s6_chunks, waves = (uint32_t)((((uint64_t)tot + 63) / 64 + kc - 1) / kc);
+        // the waves take chunks until none are left: as many as the chip holds at once, no more than there are grabs
+        int dev = 0, cus = 256, per_cu = 16;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)zt_bc7_stage6q, 64, 0) != hipSuccess || per_cu < 1) per_cu = 16;
+        const uint32_t kc = (uint32_t)g_s6_chunks;
+        const uint64_t nch = ((uint64_t)tot + 63) / 64;   // (a wave for every shard that has chunks)
+        const uint32_t waves = (uint32_t)std::max<uint64_t>(std::min<uint64_t>((nch + kc - 1) / kc, (uint64_t)cus * (uint64_t)per_cu),
+                                                            std::min<uint64_t>(nch, ZT_SHARDS));
         if ((rc = launch("zt_bc7_stage6", s, [&] { hipLaunchKernelGGL(zt_bc7_stage6q, dim3(waves), dim3(64), 0, s, J, tot, L, kc); })))
             return rc;
     } else if ((rc = launch("zt_bc7_stage6", s, [&] {

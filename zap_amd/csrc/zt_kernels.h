@@ -9,6 +9,8 @@ struct zt_jobs;
 // BC7 compaction lists (device memory, capacity = total blocks of the launch)
 #define ZT_BC7_LIST1 (ZT_SHARDS * ZT_SHARD_STRIDE)
 #define ZT_BC7_COUNTERS (2 * ZT_SHARDS * ZT_SHARD_STRIDE)
+#define ZT_BC7_S6_CURSOR (ZT_BC7_LIST1 + 32)   // zt_bc7_stage6q's cursors, one per shard: a word of each of list 1's lines,
+                                               // which stage 6 does not use
 struct zt_bc7_lists {
     unsigned *counters;  // sharded lengths (ZT_SHARDS x ZT_SHARD_STRIDE): list 5 from 0, list 1 from ZT_BC7_LIST1
     uint32_t cap5, cap1; // slots per shard
@@ -145,7 +147,7 @@ struct zt_batch_args {
 extern "C" {
 __global__ void zt_batch_persistent(zt_batch_args A);
 __global__ void zt_bc7_stage6(zt_jobs jobs, uint32_t total, zt_bc7_lists L);
-__global__ void zt_bc7_stage6q(zt_jobs jobs, uint32_t total, zt_bc7_lists L, uint32_t kc);   // 64 lanes per workgroup
+__global__ void zt_bc7_stage6q(zt_jobs jobs, uint32_t total, zt_bc7_lists L, uint32_t kc);   // 64 lanes per workgroup, any grid
 __global__ void zt_bc7_stage5(zt_jobs jobs, zt_bc7_lists L);
 __global__ void zt_bc7_stage1(zt_jobs jobs, zt_bc7_lists L, int m1k);
 __global__ void zt_zero_u32(unsigned *p, int n);
