@@ -205,6 +205,8 @@ typedef struct {
 typedef struct zaptex_batch zaptex_batch;
 size_t zaptex_batch_mip_bytes(int w, int h, int levels);
 size_t zaptex_batch_out_bytes(int w, int h, int fmt, int levels);
+/* the workspace a plan of these textures needs under the launch shapes now in force (zaptex_set_launch): a plan
+ * made after the shapes change can need more */
 size_t zaptex_batch_workspace_size(const zaptex_texture *t, int n);
 /* Builds the plan: device job tables (uploaded once into the caller's workspace) and one captured graph:
  * the mip chains of every texture (each chain cut into fused tile passes, single levels and a <= 128x128 tail;

@@ -54,3 +54,21 @@ def test_options_are_checked(zt):
     assert zt.lib.zaptex_set_option(OPTS["chunks"], 0) == 1
     assert zt.lib.zaptex_set_option(OPTS["groups"], 33) == 1
     assert zt.get_option(OPTS["persistent"]) == 0 and zt.get_option(OPTS["mix"]) == 2 and zt.get_option(OPTS["chunks"]) == 1
+
+
+def test_workspace_size_follows_the_launch_shapes(zt):
+    """The task queue is planned from the launch shapes, so the workspace a plan needs changes with
+    zaptex_set_launch: Batch.replan sizes it again (the GPU test of a re-plan relies on this growth)."""
+    specs = [(64, 64, "bc1", 7), (100, 36, "bc7", 7), (257, 129, "bc3", 9), (1024, 512, "bc7", 11), (512, 512, "bc1", 10)]
+    names = ["zt_bc14_encode", "zt_bc1_climb", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1", "zt_mip_batch"]
+    saved = {k: zt.get_launch(k) for k in names}
+    try:
+        for k in names:
+            zt.set_launch(k, 256, 1)
+        small = plan_ok(zt, specs)
+        for k in names:
+            zt.set_launch(k, 64, 1 << 20)
+        assert plan_ok(zt, specs) > small
+    finally:
+        for k, (wg, cap) in saved.items():
+            zt.set_launch(k, wg, cap)

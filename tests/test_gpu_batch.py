@@ -154,6 +154,36 @@ def test_launch_shapes_do_not_change_bytes(zt):
             zt.set_launch(k, wg, cap)
 
 
+def test_replan_sizes_the_workspace_again(zt):
+    """A re-plan under launch shapes whose task queue is larger than the first plan's (tests/test_batch_plan.py pins
+    the growth): Batch.replan allocates the larger workspace, and the bytes stay the same."""
+    import torch
+    names = ["zt_bc14_encode", "zt_bc1_climb", "zt_bc7_stage6", "zt_bc7_stage5", "zt_bc7_s1", "zt_mip_batch"]
+    saved = {k: zt.get_launch(k) for k in names}
+    try:
+        for k in names:
+            zt.set_launch(k, 256, 1)
+        b, _ = _make(zt, SMALL[:10], seed0=700)
+        first = b.ws.numel()
+        b.run()
+        torch.cuda.synchronize()
+        ref = _snapshot(b)
+        for k in names:
+            zt.set_launch(k, 64, 1 << 20)
+        b.replan()
+        b.run()
+        torch.cuda.synchronize()
+        got = _snapshot(b)
+        grown = b.ws.numel()
+        b.close()
+        assert grown >= zt.batch_workspace_size(b.texs) and grown > first
+        for (m0, o0), (m1, o1) in zip(ref, got):
+            assert all(np.array_equal(x, y) for x, y in zip(m0, m1)) and all(np.array_equal(x, y) for x, y in zip(o0, o1))
+    finally:
+        for k, (wg, cap) in saved.items():
+            zt.set_launch(k, wg, cap)
+
+
 @pytest.mark.skip(reason="the persistent batch kernel's first GPU run faulted the GPU; it stays off the GPU (and off by "
                          "default) until the cause is found")
 @pytest.mark.parametrize("mix,chunks,groups", [(2, 1, 0), (0, 1, 1), (64, 8, 0), (3, 64, 2)])

@@ -64,8 +64,14 @@ class Batch:
         self.lib.batch_run(self.handle, stream)
 
     def replan(self) -> None:
-        """Rebuild the plan (and its graph) over the same buffers, e.g. after zaptex_set_launch."""
+        """Rebuild the plan (and its graph) over the same textures, e.g. after zaptex_set_launch.  The workspace is
+        sized again first: its task queue is planned from the launch shapes, so new shapes can need a larger one."""
+        import torch
         self.close()
+        need = self.lib.batch_workspace_size(self.texs)
+        if need > self.ws.numel():
+            self.ws = None
+            self.ws = torch.empty(need, dtype=torch.uint8, device="cuda")
         self.handle = self.lib.batch_create(self.texs, self.ws)
 
     def level(self, i: int, lv: int):
