@@ -457,6 +457,11 @@ ZT_DEV void s1_fitpick_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k,
         const uint4 *s = L.px5 + (uint64_t)m.x * 4;
 #pragma unroll
         for (int r = 0; r < 4; r++) { uint4 v = s[r]; px[4 * r] = v.x; px[4 * r + 1] = v.y; px[4 * r + 2] = v.z; px[4 * r + 3] = v.w; }
+        // the block's destination, loaded now so the two dependent loads overlap the fit (this launch is about one
+        // wave per SIMD: nothing else hides their latency)
+        const uint4 meta = L.meta5[m.x];
+        const zt_job J = job_at(jobs, (int)meta.z);
+        uint8_t *const dst = J.dst + (uint64_t)(meta.x - J.first) * 16;
         const int part = cand[(uint64_t)e * 8 + k];
         const uint32_t m1 = kP2[part], mask = sb ? m1 : (~m1 & 0xFFFFu);
         int q[2][4], pb[2];
@@ -476,9 +481,7 @@ ZT_DEV void s1_fitpick_body(const zt_jobs &jobs, const zt_bc7_lists &L, int m1k,
         if (key != 0xFFFFFFFFu && (key & 15u) == k && sb == 0) {
             uint64_t lo, hi;
             mode1_pack(part, idx, qs, (uint64_t)o0 | (uint64_t)o1 << 32, (uint64_t)o2 | (uint64_t)o3 << 32, lo, hi);
-            const uint4 meta = L.meta5[m.x];
-            const zt_job J = job_at(jobs, (int)meta.z);
-            store16(J.dst + (uint64_t)(meta.x - J.first) * 16, lo, hi);
+            store16(dst, lo, hi);
         }
     }
 }
